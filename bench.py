@@ -1,0 +1,204 @@
+#!/usr/bin/env python3
+"""PBSM3D blowing-snow step throughput on one MI355X.
+
+Workload: the production-size synthetic case -- 500,000 seeded uniform points
+in a 10 x 10 km square -> Delaunay (about 1M triangles), 10 suspension layers
+(about 10M unknowns), per-face lognormal 2 m wind in [4, 20] m/s, direction
+N(270, 20) deg, T ~ U(-25, -2) C, one 900 s step solved to relative residual
+1e-8.  Forcing and outputs stay resident in HBM; a step is one full
+pbsm3d_step (coefficients, suspension assembly + solve, Qsusp/Qsubl,
+deposition assembly + solve, drift mass), as the reference's run(mesh&).
+
+Two variants are timed back to back on the same mesh and forcing:
+  plain  : FP64 GMRES(30), column-tridiagonal preconditioner, x0 = 0
+  mixed  : FP64 refinement around FP32 GMRES(30) cycles (+ learned warm start
+           when a trained network is supplied)
+`value` is the mixed variant's steps/s; the plain variant is reported beside it.
+
+Prints ONE JSON line.  --gpus must be 1 (one GPU per box).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "PBSM3D steps/s @1M tri x 10 layers, rel-resid 1e-8: plain vs mixed+learned"
+
+
+def spmv_bytes(nnz: int, n: int, s: int) -> int:
+    """Algorithmic bytes of one CSR SpMV: values + int32 columns + row_ptr + x read once + y written."""
+    return nnz * (s + 4) + 4 * (n + 1) + 2 * n * s
+
+
+def log(msg: str):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def run_variant(pb, abi, g, dev_forcing, dev_outs, cfg, steps, warmup, torch):
+    ctx = pb.Context(g, cfg)
+    for _ in range(warmup):
+        ctx.step(dev_forcing, outs=dev_outs, on_device=True)
+    torch.cuda.synchronize()
+    ctx.profile(True)
+    ctx.profile_read(reset=True)
+    stats = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        _, st = ctx.step(dev_forcing, outs=dev_outs, on_device=True)
+        stats.append(st)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    prof = ctx.profile_read(reset=True)
+    ctx.profile(False)
+    nnz = pb.lib().pbsm3d_system_nnz(ctx._h, 0)
+    ctx.close()
+    return (t1 - t0), stats, prof, nnz
+
+
+def cpu_baseline(g_full, forcing, n_layer, faces, threads):
+    """Times the CPU oracle (GMRES(30) + ILUT(1e-4, 3), as the reference's solver
+    setup) on a Hilbert-contiguous patch of the same mesh; steps/s is scaled
+    to the full face count by the face ratio."""
+    import numpy as np
+
+    from chm_amd import abi
+    from oracle import oracle_py
+
+    F = g_full.n_faces
+    fs = min(faces, F)
+    nb = g_full.neigh.reshape(F, 3)[:fs].copy()
+    nb[nb >= fs] = -1
+    sl3 = slice(0, 3 * fs)
+    gs = abi.Geometry(nb, g_full.normal_x[sl3], g_full.normal_y[sl3], g_full.edge_length[sl3], g_full.area[:fs],
+                      g_full.center_x[:fs], g_full.center_y[:fs], g_full.center_z[:fs])
+    fo = {k: np.ascontiguousarray(v[:fs]) for k, v in forcing.items()}
+    orc = oracle_py.Oracle(gs, oracle_py.oracle_config(nLayer=n_layer), threads=threads)
+    t0 = time.perf_counter()
+    _, st = orc.step(fo, want_fields=False)
+    dt = time.perf_counter() - t0
+    orc.close()
+    return {
+        "value": (1.0 / dt) * (fs / F),
+        "unit": "steps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"one 900 s step of the CPU FP64 restatement (GMRES(30) + ILUT(1e-4, fill 3), tol 1e-8) on a "
+                   f"{fs}-face Hilbert-contiguous patch of the same mesh x {n_layer} layers, {dt:.1f} s, "
+                   f"scaled by faces {fs}/{F}; susp/dep iterations {st['susp_iters']}/{st['dep_iters']}"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--points", type=int, default=500000)
+    ap.add_argument("--layers", type=int, default=10)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--variants", default="plain,mixed")
+    ap.add_argument("--cpu-faces", type=int, default=100000)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU design is described in "
+              f"DESIGN.md and no multi-GPU figure is produced.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from chm_amd import abi, meshgen
+    from chm_amd import pbsm3d as pb
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    pb.lib()
+    t = time.perf_counter()
+    mesh = meshgen.synthetic_mesh(args.points, seed=args.seed)
+    F, L = mesh.n_faces, args.layers
+    g = pb.face_geometry(mesh)
+    forcing = meshgen.forcing_c2(F, seed=args.seed)
+    log(f"[bench] mesh F={F} L={L} N={F * L} built in {time.perf_counter() - t:.1f}s")
+    dev_forcing = {k: torch.tensor(v, device="cuda:0") for k, v in forcing.items()}
+    dev_outs = {k: torch.zeros(F, dtype=torch.float64, device="cuda:0") for k in abi.OUT_FIELDS}
+
+    variants = {}
+    for name in args.variants.split(","):
+        if name == "plain":
+            cfg = pb.default_config(nLayer=L, precision=abi.FP64, precond=abi.PC_COLUMN)
+        elif name == "mixed":
+            cfg = pb.default_config(nLayer=L, precision=abi.MIXED, precond=abi.PC_COLUMN)
+        else:
+            raise SystemExit(f"unknown variant {name}")
+        log(f"[bench] variant {name}: warmup {args.warmup}, steps {args.steps}")
+        secs, stats, prof, nnz = run_variant(pb, abi, g, dev_forcing, dev_outs, cfg, args.steps, args.warmup, torch)
+        st = stats[-1]
+        variants[name] = {
+            "steps_per_s": args.steps / secs,
+            "ms_per_step": 1000.0 * secs / args.steps,
+            "susp_iters": st["susp_iters"],
+            "dep_iters": st["dep_iters"],
+            "susp_resid": st["susp_resid"],
+            "dep_resid": st["dep_resid"],
+            "refinements": st["susp_refinements"],
+            "ms_phase": {k: round(st[k], 3) for k in ("ms_assembly", "ms_susp_solve", "ms_post", "ms_dep_solve")},
+            "kernels": {k: {"launches": n, "ms_avg": (ms / n if n else 0.0), "ms_total": ms}
+                        for k, (n, ms) in prof.items() if n},
+            "nnz": nnz,
+        }
+        log(f"[bench] {name}: {variants[name]['steps_per_s']:.2f} steps/s, iters {st['susp_iters']}/{st['dep_iters']}")
+
+    N = F * L
+    head = variants.get("mixed") or variants.get("plain")
+    head_name = "mixed" if "mixed" in variants else "plain"
+    # roofline of the dominant kernel (the suspension SpMV of the headline variant)
+    kname, s = ("spmv32", 4) if head_name == "mixed" else ("spmv64", 8)
+    k = head["kernels"].get(kname, {"ms_avg": 0.0, "launches": 0})
+    algo = spmv_bytes(head["nnz"], N, s)
+    achieved = algo / (k["ms_avg"] * 1e-3) / 1e9 if k["ms_avg"] > 0 else 0.0
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": f"{kname} (CSR SpMV)",
+                "algorithmic_bytes": algo, "ms_avg": k["ms_avg"]}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        log(f"[bench] CPU baseline: oracle on {args.cpu_faces} faces, {threads} threads")
+        cpu = cpu_baseline(g, forcing, L, args.cpu_faces, threads)
+
+    out = {
+        "metric": METRIC,
+        "value": round(head["steps_per_s"], 4),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(head["ms_per_step"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32+fp64" if head_name == "mixed" else "f64",
+        "data": "synthetic (seeded Delaunay mesh and forcing)",
+        "config": {"workload": f"PBSM3D C2: {args.points} points -> {F} triangles x {L} layers, 900 s step, tol 1e-8",
+                   "faces": F, "layers": L, "unknowns": N, "headline_variant": head_name},
+        "variants": variants,
+        "speedup_mixed_vs_plain": (round(variants["mixed"]["steps_per_s"] / variants["plain"]["steps_per_s"], 3)
+                                   if "mixed" in variants and "plain" in variants else None),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

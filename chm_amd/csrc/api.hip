@@ -1,0 +1,652 @@
+// C ABI of the MI355X PBSM3D step (include/pbsm3d.h): context creation
+// (PBSM3D::init + NearestNeighborProblem), the timestep driver (PBSM3D::run)
+// and the parity / checkpoint hooks.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+
+#include "krylov.hpp"
+
+using namespace pb;
+
+struct pbsm3d_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[6] = {};
+    int64_t F = 0;
+    int L = 0;
+    pbsm3d_config cfg{};
+    double dz = 0;
+    bool enable_veg = false;
+
+    DevBuf<double> nx, ny, elen, area, cz, canopy, lai, stalk_n, stalk_d, dep_w;
+    DevBuf<int32_t> neigh;
+    DevBuf<uint16_t> slots;
+    DevBuf<uint8_t> water;
+    System susp, dep;
+    DevBuf<double> u_z, csubl, Qsalt, Qsusp, Qsubl, Qsubl_mass, sum_subl, drift_mass, sum_drift, more;
+    DevBuf<uint8_t> salt_flag;
+    DevBuf<double> forcing_stage;
+    ColumnFactors colf;
+    DevBuf<double> invdiag_s, invdiag_d, r_s, r_d;
+    DevBuf<float> val32, r32, d32;
+    GmresWork<double> ws64, wd64;
+    GmresWork<float> ws32;
+    DevBuf<double> red_partial, red_out;
+    HostBuf<double> red_host;
+    Profiler prof;
+    std::string err;
+
+    ~pbsm3d_ctx() {
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace pb {
+void throw_hip(hipError_t e, const char* what, const char* file, int line) {
+    char buf[512];
+    std::snprintf(buf, sizeof buf, "HIP error %d (%s) in %s at %s:%d", (int)e, hipGetErrorString(e), what, file, line);
+    throw Error(PBSM3D_EHIP, buf);
+}
+}  // namespace pb
+
+namespace {
+
+std::mutex g_err_mu;
+std::string g_err;
+
+void set_global_error(const std::string& m) {
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    g_err = m;
+}
+
+template <typename F>
+int guarded(pbsm3d_ctx* c, F&& fn) {
+    try {
+        fn();
+        return PBSM3D_OK;
+    } catch (const Error& e) {
+        if (c) c->err = e.what();
+        set_global_error(e.what());
+        return e.code;
+    } catch (const std::exception& e) {
+        if (c) c->err = e.what();
+        set_global_error(e.what());
+        return PBSM3D_EINVAL;
+    }
+}
+
+template <typename T>
+void upload(DevBuf<T>& d, const T* h, size_t n, hipStream_t s) {
+    d.alloc(n);
+    PB_HIP(hipMemcpyAsync(d.p, h, n * sizeof(T), hipMemcpyHostToDevice, s));
+    PB_HIP(hipStreamSynchronize(s));
+}
+
+template <typename T>
+void fill(DevBuf<T>& d, size_t n, T v, hipStream_t s) {
+    std::vector<T> h(n, v);
+    upload(d, h.data(), n, s);
+}
+
+// Lateral block of face i: sorted {i, existing neighbours}; slot nibbles give
+// the position of self (bits 0-3) and of neighbour f (bits 4(f+1)..), 0xF if absent.
+uint16_t lateral_block(int64_t i, const int32_t* nb3, int32_t* sorted, int* count) {
+    int32_t ids[4];
+    int k = 0;
+    ids[k++] = (int32_t)i;
+    for (int f = 0; f < 3; ++f)
+        if (nb3[f] >= 0) ids[k++] = nb3[f];
+    std::sort(ids, ids + k);
+    uint16_t slot = 0xFFFF;
+    auto pos_of = [&](int32_t v) {
+        for (int q = 0; q < k; ++q)
+            if (ids[q] == v) return q;
+        return 15;
+    };
+    slot = (uint16_t)((slot & ~0xFu) | (unsigned)pos_of((int32_t)i));
+    for (int f = 0; f < 3; ++f) {
+        const unsigned p = nb3[f] >= 0 ? (unsigned)pos_of(nb3[f]) : 0xFu;
+        slot = (uint16_t)((slot & ~(0xFu << (4 * (f + 1)))) | (p << (4 * (f + 1))));
+    }
+    std::copy(ids, ids + k, sorted);
+    *count = k;
+    return slot;
+}
+
+// Extruded CSR pattern of LinearAlgebra.cpp:43-149 with sorted columns:
+// row z*F+i = [z*F+i-F if z>0] [z*F + sorted lateral block] [z*F+i+F if z<L-1].
+void build_pattern(int64_t F, int L, const int32_t* neigh_fm, std::vector<int32_t>& rp, std::vector<int32_t>& col,
+                   std::vector<uint16_t>* slots) {
+    const int64_t N = F * L;
+    std::vector<int32_t> blk(4 * F);
+    std::vector<int8_t> cnt(F);
+    if (slots) slots->resize(F);
+    for (int64_t i = 0; i < F; ++i) {
+        int k = 0;
+        const uint16_t s = lateral_block(i, neigh_fm + 3 * i, blk.data() + 4 * i, &k);
+        cnt[i] = (int8_t)k;
+        if (slots) (*slots)[i] = s;
+    }
+    rp.assign(N + 1, 0);
+    int64_t acc = 0;
+    for (int z = 0; z < L; ++z)
+        for (int64_t i = 0; i < F; ++i) {
+            rp[z * F + i] = (int32_t)acc;
+            acc += cnt[i] + (z > 0) + (z < L - 1);
+            if (acc > INT32_MAX) throw Error(PBSM3D_EINVAL, "system too large for 32-bit CSR indices");
+        }
+    rp[N] = (int32_t)acc;
+    col.resize(acc);
+    for (int z = 0; z < L; ++z)
+        for (int64_t i = 0; i < F; ++i) {
+            const int64_t r = z * F + i;
+            int32_t* c = col.data() + rp[r];
+            if (z > 0) *c++ = (int32_t)(r - F);
+            for (int q = 0; q < cnt[i]; ++q) *c++ = (int32_t)(z * F + blk[4 * i + q]);
+            if (z < L - 1) *c++ = (int32_t)(r + F);
+        }
+}
+
+void validate(const pbsm3d_mesh* m, const pbsm3d_config* c) {
+    if (!m || !c) throw Error(PBSM3D_EINVAL, "null mesh or config");
+    if (m->n_faces <= 0) throw Error(PBSM3D_EINVAL, "mesh has no faces");
+    if (!m->neigh || !m->normal_x || !m->normal_y || !m->edge_length || !m->area || !m->center_x || !m->center_y ||
+        !m->center_z)
+        throw Error(PBSM3D_EINVAL, "mesh geometry pointer is null");
+    if (c->nLayer < 2 || c->nLayer > 64) throw Error(PBSM3D_EINVAL, "nLayer must be in [2, 64]");
+    if (c->settling_velocity < 0) throw Error(PBSM3D_EINVAL, "PBSM3D settling velocity must be positive");
+    if (c->use_exp_fetch && c->use_tanh_fetch) throw Error(PBSM3D_EINVAL, "Cannot specify both exp_fetch and tanh_fetch");
+    if (c->restart < 1 || c->restart > 31) throw Error(PBSM3D_EINVAL, "restart must be in [1, 31]");
+    if (c->max_iters < 1) throw Error(PBSM3D_EINVAL, "max_iters must be >= 1");
+    if (!(c->tol > 0)) throw Error(PBSM3D_EINVAL, "tol must be > 0");
+    if (c->precision != PBSM3D_FP64 && c->precision != PBSM3D_MIXED) throw Error(PBSM3D_EINVAL, "unknown precision");
+    if (c->precond != PBSM3D_PC_NONE && c->precond != PBSM3D_PC_JACOBI && c->precond != PBSM3D_PC_COLUMN)
+        throw Error(PBSM3D_EINVAL, "preconditioner not available on the GPU path");
+    if (c->dep_precond != PBSM3D_PC_NONE && c->dep_precond != PBSM3D_PC_JACOBI)
+        throw Error(PBSM3D_EINVAL, "deposition preconditioner not available on the GPU path");
+    const int64_t F = m->n_faces;
+    for (int64_t k = 0; k < 3 * F; ++k)
+        if (m->neigh[k] < -1 || m->neigh[k] >= F) throw Error(PBSM3D_EINVAL, "neighbour index out of range");
+}
+
+// edge arrays: face-major [3*i+j] (ABI) -> j-major [j*F+i] (device)
+std::vector<double> to_jmajor(const double* a, int64_t F) {
+    std::vector<double> o(3 * F);
+    for (int64_t i = 0; i < F; ++i)
+        for (int j = 0; j < 3; ++j) o[j * F + i] = a[3 * i + j];
+    return o;
+}
+
+double max_abs(pbsm3d_ctx* c, const double* v, int64_t n) {
+    const int nblk = (int)std::min<int64_t>(kReduceBlocks, (n + kBlock - 1) / kBlock);
+    absmax(c->stream, n, v, c->red_partial.p, nblk);
+    reduce_max(c->stream, c->red_partial.p, nblk, c->red_out.p);
+    PB_HIP(hipMemcpyAsync(c->red_host.p, c->red_out.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    PB_HIP(hipStreamSynchronize(c->stream));
+    return c->red_host.p[0];
+}
+
+float ms_between(hipEvent_t a, hipEvent_t b) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0;
+    return ms;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pbsm3d_abi_version(void) { return PBSM3D_ABI_VERSION; }
+
+void pbsm3d_config_default(pbsm3d_config* c) {
+    std::memset(c, 0, sizeof(*c));
+    c->nLayer = 10;
+    c->do_fixed_settling = 0;
+    c->settling_velocity = 0.5;
+    c->do_sublimation = 1;
+    c->do_lateral_diff = 1;
+    c->smooth_coeff = 820;
+    c->min_sd_trans = 0.1;
+    c->cutoff = 0.3;
+    c->snow_diffusion_const = 0.3;
+    c->enable_veg = 1;
+    c->use_tanh_fetch = 1;
+    c->use_exp_fetch = 0;
+    c->use_R94_lambda = 1;
+    c->z0_snow = 0.01;
+    c->settling_scale = 1.0;
+    c->tol = 1e-8;
+    c->max_iters = 1000;
+    c->restart = 30;
+    c->precision = PBSM3D_FP64;
+    c->precond = PBSM3D_PC_COLUMN;
+    c->warm_start = PBSM3D_WS_ZERO;
+    c->dep_precond = PBSM3D_PC_JACOBI;
+}
+
+int pbsm3d_face_geometry(int64_t F, const double* v, const int32_t* elem, const double* area_param, double* nx,
+                         double* ny, double* elen, double* area, double* cx, double* cy, double* cz) {
+    if (F <= 0 || !v || !elem) return PBSM3D_EINVAL;
+    for (int64_t i = 0; i < F; ++i) {
+        const double* p0 = v + 3 * (int64_t)elem[3 * i + 0];
+        const double* p1 = v + 3 * (int64_t)elem[3 * i + 1];
+        const double* p2 = v + 3 * (int64_t)elem[3 * i + 2];
+        const double* P[3] = {p0, p1, p2};
+        // edge j runs from vertex (j+1)%3 to vertex (j+2)%3 (CGAL ccw/cw of j)
+        double ex[3], ey[3];
+        for (int j = 0; j < 3; ++j) {
+            ex[j] = P[(j + 2) % 3][0] - P[(j + 1) % 3][0];
+            ey[j] = P[(j + 2) % 3][1] - P[(j + 1) % 3][1];
+        }
+        for (int j = 0; j < 3; ++j) {
+            // candidate normal (e.y, -e.x), flipped to point away from the next edge
+            double qx = ey[j], qy = -ex[j];
+            const int jn = (j + 1) % 3;
+            if (ex[jn] * qx + ey[jn] * qy > 0) {
+                qx = -qx;
+                qy = -qy;
+            }
+            const double len = std::sqrt(qx * qx + qy * qy);
+            nx[3 * i + j] = qx / len;
+            ny[3 * i + j] = qy / len;
+            elen[3 * i + j] = std::sqrt(ex[j] * ex[j] + ey[j] * ey[j]);
+        }
+        if (area_param) {
+            area[i] = area_param[i];
+        } else {
+            const double ax = p1[0] - p0[0], ay = p1[1] - p0[1];
+            const double bx = p2[0] - p0[0], by = p2[1] - p0[1];
+            area[i] = (ax * by - bx * ay) / 2;
+        }
+        cx[i] = (p0[0] + p1[0] + p2[0]) / 3;
+        cy[i] = (p0[1] + p1[1] + p2[1]) / 3;
+        cz[i] = (p0[2] + p1[2] + p2[2]) / 3;
+    }
+    return PBSM3D_OK;
+}
+
+int pbsm3d_create(pbsm3d_ctx** out, const pbsm3d_mesh* m, const pbsm3d_config* cfg, int device) {
+    if (!out) return PBSM3D_EINVAL;
+    *out = nullptr;
+    std::unique_ptr<pbsm3d_ctx> c(new pbsm3d_ctx());
+    const int rc = guarded(nullptr, [&] {
+        validate(m, cfg);
+        c->device = device;
+        PB_HIP(hipSetDevice(device));
+        PB_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        for (auto& e : c->ev) PB_HIP(hipEventCreate(&e));
+        const int64_t F = m->n_faces;
+        c->F = F;
+        c->L = cfg->nLayer;
+        c->cfg = *cfg;
+        c->dz = 5.0 / (double)cfg->nLayer;  // susp_depth / nLayer, PBSM3D.cpp:225-226
+        hipStream_t s = c->stream;
+
+        // geometry
+        std::vector<int32_t> nbj(3 * F);
+        for (int64_t i = 0; i < F; ++i)
+            for (int j = 0; j < 3; ++j) nbj[j * F + i] = m->neigh[3 * i + j];
+        upload(c->neigh, nbj.data(), 3 * F, s);
+        auto nxj = to_jmajor(m->normal_x, F), nyj = to_jmajor(m->normal_y, F), elj = to_jmajor(m->edge_length, F);
+        upload(c->nx, nxj.data(), 3 * F, s);
+        upload(c->ny, nyj.data(), 3 * F, s);
+        upload(c->elen, elj.data(), 3 * F, s);
+        upload(c->area, m->area, F, s);
+        upload(c->cz, m->center_z, F, s);
+        // deposition edge weights eps*E/dx (PBSM3D.cpp:1609-1628)
+        std::vector<double> w(3 * F, 0.0);
+        for (int64_t i = 0; i < F; ++i)
+            for (int j = 0; j < 3; ++j) {
+                const int32_t n = m->neigh[3 * i + j];
+                if (n < 0) continue;
+                const double ddx = m->center_x[i] - m->center_x[n], ddy = m->center_y[i] - m->center_y[n];
+                const double dx = std::sqrt(ddx * ddx + ddy * ddy);
+                w[j * F + i] = cfg->smooth_coeff * m->edge_length[3 * i + j] / dx;
+            }
+        upload(c->dep_w, w.data(), 3 * F, s);
+        // vegetation (PBSM3D.cpp:284-324): faces without veg parameters -> zeros, veg off
+        c->enable_veg = cfg->enable_veg && m->canopy_height != nullptr;
+        if (c->enable_veg) {
+            upload(c->canopy, m->canopy_height, F, s);
+            if (cfg->use_R94_lambda) {
+                if (m->lai) upload(c->lai, m->lai, F, s); else fill(c->lai, F, 0.0, s);
+            } else {
+                if (m->stalk_number) upload(c->stalk_n, m->stalk_number, F, s); else fill(c->stalk_n, F, 1.0, s);
+                if (m->stalk_diameter) upload(c->stalk_d, m->stalk_diameter, F, s); else fill(c->stalk_d, F, 0.8, s);
+            }
+        }
+        if (m->is_water) upload(c->water, m->is_water, F, s);
+
+        // CSR patterns
+        std::vector<uint16_t> slots;
+        build_pattern(F, c->L, m->neigh, c->susp.h_rp, c->susp.h_col, &slots);
+        build_pattern(F, 1, m->neigh, c->dep.h_rp, c->dep.h_col, nullptr);
+        upload(c->slots, slots.data(), F, s);
+        for (System* S : {&c->susp, &c->dep}) {
+            S->n = (int64_t)S->h_rp.size() - 1;
+            S->nnz = S->h_rp.back();
+            upload(S->rp, S->h_rp.data(), S->h_rp.size(), s);
+            upload(S->col, S->h_col.data(), S->h_col.size(), s);
+            S->val.alloc(S->nnz);
+            S->rhs.alloc(S->n);
+            S->x.alloc(S->n);
+            PB_HIP(hipMemsetAsync(S->val.p, 0, S->val.bytes(), s));
+            PB_HIP(hipMemsetAsync(S->rhs.p, 0, S->rhs.bytes(), s));
+            PB_HIP(hipMemsetAsync(S->x.p, 0, S->x.bytes(), s));
+        }
+        const int64_t N = F * c->L;
+        c->u_z.alloc(N);
+        c->csubl.alloc(N);
+        for (auto* b : {&c->Qsalt, &c->Qsusp, &c->Qsubl, &c->Qsubl_mass, &c->drift_mass, &c->more}) fill(*b, F, -9999.0, s);
+        fill(c->sum_subl, F, 0.0, s);
+        fill(c->sum_drift, F, 0.0, s);  // PBSM3D.cpp:391
+        c->salt_flag.alloc(F);
+        c->forcing_stage.alloc(8 * F);
+        if (cfg->precond == PBSM3D_PC_COLUMN) {
+            c->colf.lo.alloc(N);
+            c->colf.cp.alloc(N);
+            c->colf.im.alloc(N);
+            if (cfg->precision == PBSM3D_MIXED) {
+                c->colf.lo32.alloc(N);
+                c->colf.cp32.alloc(N);
+                c->colf.im32.alloc(N);
+            }
+        }
+        if (cfg->precond == PBSM3D_PC_JACOBI) c->invdiag_s.alloc(N);
+        c->invdiag_d.alloc(F);
+        c->r_s.alloc(N);
+        c->r_d.alloc(F);
+        if (cfg->precision == PBSM3D_MIXED) {
+            c->val32.alloc(c->susp.nnz);
+            c->r32.alloc(N);
+            c->d32.alloc(N);
+            c->ws32.ensure(N, cfg->restart);
+        } else {
+            c->ws64.ensure(N, cfg->restart);
+        }
+        c->wd64.ensure(F, cfg->restart);
+        c->red_partial.alloc(kReduceBlocks);
+        c->red_out.alloc(4);
+        c->red_host.alloc(4);
+        PB_HIP(hipStreamSynchronize(s));
+    });
+    if (rc != PBSM3D_OK) return rc;
+    *out = c.release();
+    return PBSM3D_OK;
+}
+
+int pbsm3d_step(pbsm3d_ctx* c, const pbsm3d_forcing* fo, int32_t dt_s, pbsm3d_out* out, pbsm3d_stats* st) {
+    if (!c || !fo) return PBSM3D_EINVAL;
+    return guarded(c, [&] {
+        PB_HIP(hipSetDevice(c->device));
+        hipStream_t s = c->stream;
+        const int64_t F = c->F;
+        const int L = c->L;
+        const int64_t N = F * L;
+        const pbsm3d_config& cfg = c->cfg;
+        pbsm3d_stats stats{};
+        if (!fo->U_R || !fo->U_2m_above_srf || !fo->vw_dir || !fo->swe || !fo->t || !fo->rh || !fo->snowdepthavg)
+            throw Error(PBSM3D_EINVAL, "forcing pointer is null");
+        if ((cfg.use_exp_fetch || cfg.use_tanh_fetch) && !fo->fetch)
+            throw Error(PBSM3D_EINVAL, "fetch forcing required by the fetch mode");
+
+        const double* fin[8] = {fo->U_R, fo->U_2m_above_srf, fo->vw_dir, fo->swe, fo->t, fo->rh, fo->snowdepthavg, fo->fetch};
+        const double* fdev[8];
+        for (int k = 0; k < 8; ++k) {
+            if (!fin[k]) {
+                fdev[k] = nullptr;
+            } else if (fo->on_device) {
+                fdev[k] = fin[k];
+            } else {
+                double* dst = c->forcing_stage.p + (size_t)k * F;
+                PB_HIP(hipMemcpyAsync(dst, fin[k], F * sizeof(double), hipMemcpyHostToDevice, s));
+                fdev[k] = dst;
+            }
+        }
+        PB_HIP(hipEventRecord(c->ev[0], s));
+
+        // ---- coefficients + suspension assembly (+ fused preconditioner set-up)
+        AssembleArgs a{};
+        a.F = F;
+        a.L = L;
+        a.dz = c->dz;
+        a.dt = (double)dt_s;
+        a.do_fixed_settling = cfg.do_fixed_settling;
+        a.do_sublimation = cfg.do_sublimation;
+        a.do_lateral_diff = cfg.do_lateral_diff;
+        a.enable_veg = c->enable_veg ? 1 : 0;
+        a.use_tanh_fetch = cfg.use_tanh_fetch;
+        a.use_exp_fetch = cfg.use_exp_fetch;
+        a.use_R94_lambda = cfg.use_R94_lambda;
+        a.settling_velocity = cfg.settling_velocity;
+        a.settling_scale = cfg.settling_scale;
+        a.min_sd_trans = cfg.min_sd_trans;
+        a.cutoff = cfg.cutoff;
+        a.snow_diffusion_const = cfg.snow_diffusion_const;
+        a.z0_snow = cfg.z0_snow;
+        a.nx = c->nx.p;
+        a.ny = c->ny.p;
+        a.elen = c->elen.p;
+        a.area = c->area.p;
+        a.cz = c->cz.p;
+        a.neigh = c->neigh.p;
+        a.slots = c->slots.p;
+        a.canopy = c->canopy.p;
+        a.lai = c->lai.p;
+        a.stalk_n = c->stalk_n.p;
+        a.stalk_d = c->stalk_d.p;
+        a.is_water = c->water.p;
+        a.U_R = fdev[0];
+        a.U_2m = fdev[1];
+        a.vw_dir = fdev[2];
+        a.swe = fdev[3];
+        a.t = fdev[4];
+        a.rh = fdev[5];
+        a.snowdepthavg = fdev[6];
+        a.fetch = fdev[7];
+        a.rp = c->susp.rp.p;
+        a.val = c->susp.val.p;
+        a.rhs = c->susp.rhs.p;
+        a.x = c->susp.x.p;
+        a.u_z = c->u_z.p;
+        a.csubl = c->csubl.p;
+        a.Qsalt = c->Qsalt.p;
+        a.salt_flag = c->salt_flag.p;
+        a.pc_lo = c->colf.lo.p;
+        a.pc_cp = c->colf.cp.p;
+        a.pc_im = c->colf.im.p;
+        a.pc_lo32 = c->colf.lo32.p;
+        a.pc_cp32 = c->colf.cp32.p;
+        a.pc_im32 = c->colf.im32.p;
+        a.pc_invdiag = c->invdiag_s.p;
+        {
+            ProfScope ps(&c->prof, s, PBSM3D_K_ASSEMBLE);
+            hipLaunchKernelGGL(k_assemble, dim3(grid_for(F)), dim3(kBlock), 0, s, a);
+        }
+        PB_HIP(hipGetLastError());
+        PB_HIP(hipEventRecord(c->ev[1], s));
+
+        // ---- suspension presence + solve (PBSM3D.cpp:1424-1459)
+        stats.susp_rhs_max = max_abs(c, c->susp.rhs.p, N);
+        stats.suspension_present = stats.susp_rhs_max > 1e-12;
+        if (stats.suspension_present) {
+            SolveInfo si;
+            CsrOp<double> A{N, c->susp.rp.p, c->susp.col.p, c->susp.val.p, &c->prof};
+            if (cfg.precision == PBSM3D_FP64) {
+                PcFn<double> M;
+                if (cfg.precond == PBSM3D_PC_COLUMN)
+                    M = [c, F, L](hipStream_t st2, const double* r, double* z) {
+                        pc_column<double>(st2, F, L, c->colf.lo.p, c->colf.cp.p, c->colf.im.p, r, z);
+                    };
+                else if (cfg.precond == PBSM3D_PC_JACOBI)
+                    M = [c, N](hipStream_t st2, const double* r, double* z) { pc_jacobi<double>(st2, N, c->invdiag_s.p, r, z); };
+                else
+                    M = [N](hipStream_t st2, const double* r, double* z) {
+                        PB_HIP(hipMemcpyAsync(z, r, N * sizeof(double), hipMemcpyDeviceToDevice, st2));
+                    };
+                si = gmres_solve(s, c->ws64, A, M, c->susp.rhs.p, c->susp.x.p, c->r_s.p, cfg.restart, cfg.max_iters,
+                                 cfg.tol);
+            } else {
+                to_f32(s, c->susp.nnz, c->susp.val.p, c->val32.p);
+                CsrOp<float> A32{N, c->susp.rp.p, c->susp.col.p, c->val32.p, &c->prof};
+                PcFn<float> M;
+                if (cfg.precond == PBSM3D_PC_COLUMN)
+                    M = [c, F, L](hipStream_t st2, const float* r, float* z) {
+                        pc_column<float>(st2, F, L, c->colf.lo32.p, c->colf.cp32.p, c->colf.im32.p, r, z);
+                    };
+                else if (cfg.precond == PBSM3D_PC_JACOBI)
+                    M = [c, N](hipStream_t st2, const float* r, float* z) { pc_jacobi<float>(st2, N, c->invdiag_s.p, r, z); };
+                else
+                    M = [N](hipStream_t st2, const float* r, float* z) {
+                        PB_HIP(hipMemcpyAsync(z, r, N * sizeof(float), hipMemcpyDeviceToDevice, st2));
+                    };
+                si = mixed_solve(s, c->ws32, A, A32, M, c->susp.rhs.p, c->susp.x.p, c->r_s.p, c->r32.p, c->d32.p,
+                                 cfg.restart, cfg.max_iters, cfg.tol, 1e-4);
+            }
+            stats.susp_iters = si.iters;
+            stats.susp_resid = si.resid;
+            stats.susp_refinements = si.refinements;
+            if (!si.converged) throw Error(PBSM3D_ENOCONV, "suspension solver failed to converge");
+        }
+        PB_HIP(hipEventRecord(c->ev[2], s));
+
+        // ---- Qsusp / Qsubl and deposition assembly
+        PostArgs p{F, L, c->dz, (double)dt_s, c->susp.x.p, c->u_z.p, c->csubl.p,
+                   c->Qsusp.p, c->Qsubl.p, c->Qsubl_mass.p, c->sum_subl.p};
+        DepositionArgs d{F, fdev[2], c->nx.p, c->ny.p, c->elen.p, c->area.p, c->dep_w.p, c->Qsusp.p, c->Qsalt.p,
+                         c->neigh.p, c->dep.rp.p, c->slots.p, c->dep.val.p, c->dep.rhs.p, c->dep.x.p, c->invdiag_d.p};
+        {
+            ProfScope ps(&c->prof, s, PBSM3D_K_DEPOSITION);
+            hipLaunchKernelGGL(k_post, dim3(grid_for(F)), dim3(kBlock), 0, s, p);
+            hipLaunchKernelGGL(k_deposition, dim3(grid_for(F)), dim3(kBlock), 0, s, d);
+        }
+        PB_HIP(hipGetLastError());
+        stats.dep_rhs_max = max_abs(c, c->dep.rhs.p, F);
+        stats.deposition_present = stats.suspension_present && stats.dep_rhs_max > 1e-12;
+        PB_HIP(hipEventRecord(c->ev[3], s));
+
+        // ---- deposition solve + drift (PBSM3D.cpp:1661-1740)
+        if (stats.deposition_present) {
+            CsrOp<double> D{F, c->dep.rp.p, c->dep.col.p, c->dep.val.p};
+            PcFn<double> M;
+            if (cfg.dep_precond == PBSM3D_PC_JACOBI)
+                M = [c, F](hipStream_t st2, const double* r, double* z) { pc_jacobi<double>(st2, F, c->invdiag_d.p, r, z); };
+            else
+                M = [F](hipStream_t st2, const double* r, double* z) {
+                    PB_HIP(hipMemcpyAsync(z, r, F * sizeof(double), hipMemcpyDeviceToDevice, st2));
+                };
+            SolveInfo si = gmres_solve(s, c->wd64, D, M, c->dep.rhs.p, c->dep.x.p, c->r_d.p, cfg.restart,
+                                       cfg.max_iters, cfg.tol);
+            stats.dep_iters = si.iters;
+            stats.dep_resid = si.resid;
+            if (!si.converged) throw Error(PBSM3D_ENOCONV, "deposition solver failed to converge");
+            DriftArgs dr{F, (double)dt_s, c->dep.x.p, fdev[3], c->salt_flag.p, c->drift_mass.p, c->sum_drift.p, c->more.p};
+            hipLaunchKernelGGL(k_drift, dim3(grid_for(F)), dim3(kBlock), 0, s, dr);
+            PB_HIP(hipGetLastError());
+        }
+        PB_HIP(hipEventRecord(c->ev[4], s));
+
+        // ---- outputs
+        if (out) {
+            const hipMemcpyKind kind = out->on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+            auto cp = [&](double* dst, const DevBuf<double>& src, size_t n) {
+                if (dst) PB_HIP(hipMemcpyAsync(dst, src.p, n * sizeof(double), kind, s));
+            };
+            cp(out->Qsalt, c->Qsalt, F);
+            cp(out->Qsusp, c->Qsusp, F);
+            cp(out->Qsubl, c->Qsubl, F);
+            cp(out->Qsubl_mass, c->Qsubl_mass, F);
+            cp(out->sum_subl, c->sum_subl, F);
+            cp(out->drift_mass, c->drift_mass, F);
+            cp(out->sum_drift, c->sum_drift, F);
+            cp(out->pbsm_more_than_avail, c->more, F);
+            cp(out->conc, c->susp.x, N);
+            cp(out->qdep, c->dep.x, F);
+        }
+        PB_HIP(hipEventRecord(c->ev[5], s));
+        PB_HIP(hipStreamSynchronize(s));
+        stats.ms_assembly = ms_between(c->ev[0], c->ev[1]);
+        stats.ms_susp_solve = ms_between(c->ev[1], c->ev[2]);
+        stats.ms_post = ms_between(c->ev[2], c->ev[3]);
+        stats.ms_dep_solve = ms_between(c->ev[3], c->ev[4]);
+        stats.ms_total = ms_between(c->ev[0], c->ev[4]);
+        c->prof.collect();
+        if (st) *st = stats;
+    });
+}
+
+int pbsm3d_profile_enable(pbsm3d_ctx* c, int enable) {
+    if (!c) return PBSM3D_EINVAL;
+    c->prof.on = enable != 0;
+    return PBSM3D_OK;
+}
+
+int pbsm3d_profile_read(pbsm3d_ctx* c, int kernel, int64_t* launches, double* ms_total, int reset) {
+    if (!c || kernel < 0 || kernel >= PBSM3D_K_COUNT) return PBSM3D_EINVAL;
+    if (launches) *launches = c->prof.count[kernel];
+    if (ms_total) *ms_total = c->prof.ms[kernel];
+    if (reset) {
+        c->prof.count[kernel] = 0;
+        c->prof.ms[kernel] = 0;
+    }
+    return PBSM3D_OK;
+}
+
+int64_t pbsm3d_system_nnz(const pbsm3d_ctx* c, int which) {
+    if (!c) return -1;
+    return which == 0 ? c->susp.nnz : c->dep.nnz;
+}
+
+int pbsm3d_export_system(pbsm3d_ctx* c, int which, int32_t* rp, int32_t* col, double* val, double* rhs) {
+    if (!c || (which != 0 && which != 1)) return PBSM3D_EINVAL;
+    return guarded(c, [&] {
+        PB_HIP(hipSetDevice(c->device));
+        System& S = which == 0 ? c->susp : c->dep;
+        if (rp) std::memcpy(rp, S.h_rp.data(), S.h_rp.size() * sizeof(int32_t));
+        if (col) std::memcpy(col, S.h_col.data(), S.h_col.size() * sizeof(int32_t));
+        if (val) PB_HIP(hipMemcpyAsync(val, S.val.p, S.val.bytes(), hipMemcpyDeviceToHost, c->stream));
+        if (rhs) PB_HIP(hipMemcpyAsync(rhs, S.rhs.p, S.rhs.bytes(), hipMemcpyDeviceToHost, c->stream));
+        PB_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int pbsm3d_get_state(pbsm3d_ctx* c, double* sum_drift, double* sum_subl) {
+    if (!c) return PBSM3D_EINVAL;
+    return guarded(c, [&] {
+        PB_HIP(hipSetDevice(c->device));
+        if (sum_drift) PB_HIP(hipMemcpyAsync(sum_drift, c->sum_drift.p, c->sum_drift.bytes(), hipMemcpyDeviceToHost, c->stream));
+        if (sum_subl) PB_HIP(hipMemcpyAsync(sum_subl, c->sum_subl.p, c->sum_subl.bytes(), hipMemcpyDeviceToHost, c->stream));
+        PB_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int pbsm3d_set_state(pbsm3d_ctx* c, const double* sum_drift, const double* sum_subl) {
+    if (!c) return PBSM3D_EINVAL;
+    return guarded(c, [&] {
+        PB_HIP(hipSetDevice(c->device));
+        if (sum_drift) PB_HIP(hipMemcpyAsync(c->sum_drift.p, sum_drift, c->sum_drift.bytes(), hipMemcpyHostToDevice, c->stream));
+        if (sum_subl) PB_HIP(hipMemcpyAsync(c->sum_subl.p, sum_subl, c->sum_subl.bytes(), hipMemcpyHostToDevice, c->stream));
+        PB_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+void pbsm3d_destroy(pbsm3d_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    delete c;
+}
+
+const char* pbsm3d_last_error(const pbsm3d_ctx* c) {
+    if (c) return c->err.c_str();
+    return g_err.c_str();
+}
+
+}  // extern "C"

@@ -1,0 +1,109 @@
+// Kernel argument structs and kernel declarations of the PBSM3D library.
+#pragma once
+
+#include "pbsm3d_internal.hpp"
+
+namespace pb {
+
+struct AssembleArgs {
+    int64_t F;
+    int L;
+    double dz, dt;
+    // config
+    int do_fixed_settling, do_sublimation, do_lateral_diff, enable_veg;
+    int use_tanh_fetch, use_exp_fetch, use_R94_lambda;
+    double settling_velocity, settling_scale, min_sd_trans, cutoff, snow_diffusion_const, z0_snow;
+    // geometry, edges j-major ([j*F + i])
+    const double *nx, *ny, *elen, *area, *cz;
+    const int32_t* neigh;
+    const uint16_t* slots;  // 4-bit CSR positions of self / neighbour 0..2 in the lateral block
+    const double *canopy, *lai, *stalk_n, *stalk_d;
+    const uint8_t* is_water;
+    // forcing
+    const double *U_R, *U_2m, *vw_dir, *swe, *t, *rh, *snowdepthavg, *fetch;
+    // suspension system
+    const int32_t* rp;
+    double *val, *rhs, *x;
+    // per-cell / per-face outputs
+    double *u_z, *csubl, *Qsalt;
+    uint8_t* salt_flag;
+    // fused preconditioner set-up (nullable)
+    double *pc_lo, *pc_cp, *pc_im, *pc_invdiag;
+    float *pc_lo32, *pc_cp32, *pc_im32;
+};
+
+struct PostArgs {
+    int64_t F;
+    int L;
+    double dz, dt;
+    const double *x, *u_z, *csubl;
+    double *Qsusp, *Qsubl, *Qsubl_mass, *sum_subl;
+};
+
+struct DepositionArgs {
+    int64_t F;
+    const double *vw_dir, *nx, *ny, *elen, *area, *dep_w, *Qsusp, *Qsalt;
+    const int32_t *neigh, *rp;
+    const uint16_t* slots;
+    double *val, *rhs, *x, *invdiag;
+};
+
+struct DriftArgs {
+    int64_t F;
+    double dt;
+    const double *qdep, *swe;
+    const uint8_t* salt_flag;
+    double *drift_mass, *sum_drift, *more;
+};
+
+__global__ void k_assemble(AssembleArgs a);
+__global__ void k_post(PostArgs a);
+__global__ void k_deposition(DepositionArgs a);
+__global__ void k_drift(DriftArgs a);
+
+// ---------------------------------------------------------------------------
+// linear algebra (linalg.hip)
+// ---------------------------------------------------------------------------
+// y = A x for a CSR matrix, value type T (double / float), vector type T.
+template <typename T>
+void spmv(hipStream_t s, int64_t n, const int32_t* rp, const int32_t* col, const T* val, const T* x, T* y);
+// r = b - A x  and per-block partial sums of r.r into partial[block]
+template <typename T>
+void residual(hipStream_t s, int64_t n, const int32_t* rp, const int32_t* col, const T* val, const T* x,
+              const T* b, T* r, double* partial, int nblk);
+
+// partial[block*(nv+1) + c] = sum over the block's rows of V_c . w (c < nv), w . w (c == nv)
+template <typename T>
+void mdot(hipStream_t s, int64_t n, const T* V, int nv, const T* w, double* partial, int nblk);
+// out[c] = sum_b partial[b*stride + c] for c < ncols (fixed order, deterministic)
+void reduce_cols(hipStream_t s, const double* partial, int nblk, int stride, int ncols, double* out);
+// w -= sum_c h[c] V_c ; partial[block] = sum w.w (after the update)
+template <typename T>
+void gs_update(hipStream_t s, int64_t n, const T* V, int nv, const double* h, T* w, double* partial, int nblk);
+// v = w * (1 / *nrm)   (nrm on device)
+template <typename T>
+void scale_by_inv(hipStream_t s, int64_t n, const T* w, const double* nrm, T* v);
+// u = sum_c y[c] V_c  (y on device)
+template <typename T>
+void combine(hipStream_t s, int64_t n, const T* V, int nv, const double* y, T* u);
+// max |b| per block into partial
+void absmax(hipStream_t s, int64_t n, const double* b, double* partial, int nblk);
+void reduce_max(hipStream_t s, const double* partial, int nblk, double* out);
+
+// preconditioners (z = M^-1 r)
+template <typename T>
+void pc_column(hipStream_t s, int64_t F, int L, const T* lo, const T* cp, const T* im, const T* r, T* z);
+template <typename T>
+void pc_jacobi(hipStream_t s, int64_t n, const double* invdiag, const T* r, T* z);
+
+// conversions / vector updates
+template <typename T>
+void scale_host(hipStream_t s, int64_t n, const T* a, double alpha, T* b);  // b = alpha * a
+void to_f32(hipStream_t s, int64_t n, const double* a, float* b);
+void axpy_f32_into_f64(hipStream_t s, int64_t n, double alpha, const float* x, double* y);  // y += alpha*x
+void scale_f64_to_f32_host(hipStream_t s, int64_t n, const double* r, double alpha, float* out);  // out = alpha*r
+void axpy_dev_scaled_f32(hipStream_t s, int64_t n, const double* scale, const float* x, double* y);  // y += (*scale) x
+void axpy_f64(hipStream_t s, int64_t n, double alpha, const double* x, double* y);
+void copy_f64(hipStream_t s, int64_t n, const double* a, double* b);
+
+}  // namespace pb

@@ -1,0 +1,232 @@
+// GMRES(m) with classical Gram-Schmidt and a conditional second pass (the
+// DGKS test ||w_after|| < ||w_before|| / sqrt(2)), as iterated classical
+// Gram-Schmidt does.  The basis lives in HBM; each Arnoldi step is
+//   z = M^-1 v_j ; v_{j+1} = A z ; h = V^T v_{j+1} (one batched pass) ;
+//   v_{j+1} -= V h (one pass, fused norm) ; v_{j+1} /= ||v_{j+1}||
+// and only the (j+2) Hessenberg entries travel to the host, where the Givens
+// rotations and the least-squares solve run in FP64.
+#include <cmath>
+#include <vector>
+
+#include "krylov.hpp"
+
+namespace pb {
+namespace {
+
+// device scalar slots
+inline int off_h() { return 0; }
+inline int off_ss(int m) { return m + 2; }
+inline int off_y(int m) { return m + 4; }
+
+inline int reduce_blocks(int64_t n) {
+    int64_t b = (n + kBlock - 1) / kBlock;
+    if (b > kReduceBlocks) b = kReduceBlocks;
+    return (int)(b < 1 ? 1 : b);
+}
+
+}  // namespace
+
+template <typename T>
+void GmresWork<T>::ensure(int64_t n_, int m_) {
+    if (n == n_ && m == m_ && V.p) return;
+    n = n_;
+    m = m_;
+    V.alloc((size_t)(m + 1) * n);
+    z.alloc(n);
+    u.alloc(n);
+    const size_t spmv_blocks = (size_t)((n + kBlock - 1) / kBlock) + 1;
+    size_t pbytes = (size_t)kReduceBlocks * (m + 2);
+    if (pbytes < spmv_blocks) pbytes = spmv_blocks;
+    partial.alloc(pbytes);
+    dsc.alloc(2 * m + 8);
+    hsc.alloc(2 * m + 8);
+}
+
+template <typename T>
+int gmres_cycle(hipStream_t s, GmresWork<T>& w, const CsrOp<T>& A, const PcFn<T>& M, const T* r0, double beta,
+                double stop_abs, int max_steps, T* out, double* resid_est) {
+    const int64_t n = A.n;
+    const int m = w.m;
+    const int nblk = reduce_blocks(n);
+    T* V = w.V.p;
+    double* d_h = w.dsc.p + off_h();
+    double* d_ss = w.dsc.p + off_ss(m);
+    double* d_y = w.dsc.p + off_y(m);
+    double* hs = w.hsc.p;
+    if (max_steps > m) max_steps = m;
+
+    std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m, 0.0), sn(m, 0.0), g(m + 1, 0.0), h(m + 2, 0.0), y(m, 0.0);
+    scale_host<T>(s, n, r0, 1.0 / beta, V);
+    g[0] = beta;
+    int k = 0;
+    for (int j = 0; j < max_steps; ++j) {
+        T* vj = V + (size_t)j * n;
+        T* vj1 = V + (size_t)(j + 1) * n;
+        {
+            ProfScope ps(A.prof, s, PBSM3D_K_PRECOND);
+            M(s, vj, w.z.p);
+        }
+        {
+            ProfScope ps(A.prof, s, sizeof(T) == 8 ? PBSM3D_K_SPMV64 : PBSM3D_K_SPMV32);
+            spmv<T>(s, n, A.rp, A.col, A.val, w.z.p, vj1);
+        }
+        for (int i = 0; i <= j; ++i) h[i] = 0.0;
+        double ss_before = 0, ss_after = 0;
+        for (int pass = 0; pass < 2; ++pass) {
+            {
+                ProfScope ps(A.prof, s, PBSM3D_K_ORTHO);
+                mdot<T>(s, n, V, j + 1, vj1, w.partial.p, nblk);
+                reduce_cols(s, w.partial.p, nblk, j + 2, j + 2, d_h);
+                gs_update<T>(s, n, V, j + 1, d_h, vj1, w.partial.p, nblk);
+                reduce_cols(s, w.partial.p, nblk, 1, 1, d_ss);
+            }
+            PB_HIP(hipMemcpyAsync(hs, w.dsc.p, sizeof(double) * (m + 3), hipMemcpyDeviceToHost, s));
+            PB_HIP(hipStreamSynchronize(s));
+            for (int i = 0; i <= j; ++i) h[i] += hs[off_h() + i];
+            if (pass == 0) ss_before = hs[off_h() + j + 1];
+            ss_after = hs[off_ss(m)];
+            if (!(ss_after < 0.5 * ss_before)) break;  // no re-orthogonalisation needed
+        }
+        const double hn = std::sqrt(ss_after);
+        if (hn > 0) scale_by_inv<T>(s, n, vj1, d_ss, vj1);
+        for (int i = 0; i <= j; ++i) H[(size_t)i * m + j] = h[i];
+        H[(size_t)(j + 1) * m + j] = hn;
+        for (int i = 0; i < j; ++i) {
+            const double a = H[(size_t)i * m + j], b = H[(size_t)(i + 1) * m + j];
+            H[(size_t)i * m + j] = cs[i] * a + sn[i] * b;
+            H[(size_t)(i + 1) * m + j] = -sn[i] * a + cs[i] * b;
+        }
+        const double a = H[(size_t)j * m + j], b = H[(size_t)(j + 1) * m + j];
+        const double rr = std::hypot(a, b);
+        cs[j] = rr > 0 ? a / rr : 1.0;
+        sn[j] = rr > 0 ? b / rr : 0.0;
+        H[(size_t)j * m + j] = rr;
+        H[(size_t)(j + 1) * m + j] = 0.0;
+        g[j + 1] = -sn[j] * g[j];
+        g[j] = cs[j] * g[j];
+        k = j + 1;
+        if (std::fabs(g[j + 1]) <= stop_abs || hn == 0.0) break;
+    }
+    for (int i = k - 1; i >= 0; --i) {
+        double sacc = g[i];
+        for (int q = i + 1; q < k; ++q) sacc -= H[(size_t)i * m + q] * y[q];
+        y[i] = sacc / H[(size_t)i * m + i];
+    }
+    for (int i = 0; i < k; ++i) hs[off_y(m) + i] = y[i];
+    PB_HIP(hipMemcpyAsync(d_y, hs + off_y(m), sizeof(double) * k, hipMemcpyHostToDevice, s));
+    combine<T>(s, n, V, k, d_y, w.u.p);
+    {
+        ProfScope ps(A.prof, s, PBSM3D_K_PRECOND);
+        M(s, w.u.p, out);
+    }
+    // the pinned staging buffer is reused by the next cycle: wait for the copy
+    PB_HIP(hipStreamSynchronize(s));
+    *resid_est = std::fabs(g[k]);
+    return k;
+}
+
+double dev_norm2(hipStream_t s, GmresWork<double>& w, int64_t n, const double* v) {
+    const int nblk = reduce_blocks(n);
+    mdot<double>(s, n, nullptr, 0, v, w.partial.p, nblk);
+    reduce_cols(s, w.partial.p, nblk, 1, 1, w.dsc.p + off_ss(w.m) + 1);
+    PB_HIP(hipMemcpyAsync(w.hsc.p, w.dsc.p + off_ss(w.m) + 1, sizeof(double), hipMemcpyDeviceToHost, s));
+    PB_HIP(hipStreamSynchronize(s));
+    return std::sqrt(w.hsc.p[0]);
+}
+
+namespace {
+template <typename T>
+double residual_norm(hipStream_t s, GmresWork<T>& w, const CsrOp<double>& A, const double* x, const double* b,
+                     double* r) {
+    const int nblk = (int)((A.n + kBlock - 1) / kBlock);
+    {
+        ProfScope ps(A.prof, s, PBSM3D_K_RESIDUAL);
+        residual<double>(s, A.n, A.rp, A.col, A.val, x, b, r, w.partial.p, nblk);
+    }
+    reduce_cols(s, w.partial.p, nblk, 1, 1, w.dsc.p + off_ss(w.m) + 1);
+    PB_HIP(hipMemcpyAsync(w.hsc.p, w.dsc.p + off_ss(w.m) + 1, sizeof(double), hipMemcpyDeviceToHost, s));
+    PB_HIP(hipStreamSynchronize(s));
+    return std::sqrt(w.hsc.p[0]);
+}
+
+template <typename T>
+double norm_of(hipStream_t s, GmresWork<T>& w, int64_t n, const double* v) {
+    const int nblk = reduce_blocks(n);
+    mdot<double>(s, n, nullptr, 0, v, w.partial.p, nblk);
+    reduce_cols(s, w.partial.p, nblk, 1, 1, w.dsc.p + off_ss(w.m) + 1);
+    PB_HIP(hipMemcpyAsync(w.hsc.p, w.dsc.p + off_ss(w.m) + 1, sizeof(double), hipMemcpyDeviceToHost, s));
+    PB_HIP(hipStreamSynchronize(s));
+    return std::sqrt(w.hsc.p[0]);
+}
+}  // namespace
+
+SolveInfo gmres_solve(hipStream_t s, GmresWork<double>& w, const CsrOp<double>& A, const PcFn<double>& M,
+                      const double* b, double* x, double* r, int m, int maxit, double tol) {
+    SolveInfo info;
+    w.ensure(A.n, m);
+    const double nb = norm_of(s, w, A.n, b);
+    if (nb == 0.0) {
+        PB_HIP(hipMemsetAsync(x, 0, sizeof(double) * A.n, s));
+        info.converged = true;
+        return info;
+    }
+    double beta = residual_norm(s, w, A, x, b, r);
+    while (true) {
+        info.resid = beta / nb;
+        if (info.resid <= tol) {
+            info.converged = true;
+            break;
+        }
+        if (info.iters >= maxit) break;
+        double est = 0;
+        const int steps = gmres_cycle<double>(s, w, A, M, r, beta, tol * nb, maxit - info.iters, r, &est);
+        axpy_f64(s, A.n, 1.0, r, x);
+        info.iters += steps;
+        info.cycles += 1;
+        beta = residual_norm(s, w, A, x, b, r);
+    }
+    return info;
+}
+
+SolveInfo mixed_solve(hipStream_t s, GmresWork<float>& w, const CsrOp<double>& A64, const CsrOp<float>& A32,
+                      const PcFn<float>& M32, const double* b, double* x, double* r, float* r32, float* d32, int m,
+                      int maxit, double tol, double inner_rtol) {
+    SolveInfo info;
+    w.ensure(A64.n, m);
+    const double nb = norm_of(s, w, A64.n, b);
+    if (nb == 0.0) {
+        PB_HIP(hipMemsetAsync(x, 0, sizeof(double) * A64.n, s));
+        info.converged = true;
+        return info;
+    }
+    double beta = residual_norm(s, w, A64, x, b, r);
+    while (true) {
+        info.resid = beta / nb;
+        if (info.resid <= tol) {
+            info.converged = true;
+            break;
+        }
+        if (info.iters >= maxit) break;
+        // inner FP32 cycle on the normalised FP64 residual
+        scale_f64_to_f32_host(s, A64.n, r, 1.0 / beta, r32);
+        double est = 0;
+        double stop = 0.5 * tol * nb / beta;
+        if (stop < inner_rtol) stop = inner_rtol;
+        const int steps = gmres_cycle<float>(s, w, A32, M32, r32, 1.0, stop, maxit - info.iters, d32, &est);
+        axpy_f32_into_f64(s, A64.n, beta, d32, x);
+        info.iters += steps;
+        info.cycles += 1;
+        info.refinements += 1;
+        beta = residual_norm(s, w, A64, x, b, r);
+    }
+    return info;
+}
+
+template struct GmresWork<double>;
+template struct GmresWork<float>;
+template int gmres_cycle<double>(hipStream_t, GmresWork<double>&, const CsrOp<double>&, const PcFn<double>&,
+                                 const double*, double, double, int, double*, double*);
+template int gmres_cycle<float>(hipStream_t, GmresWork<float>&, const CsrOp<float>&, const PcFn<float>&, const float*,
+                                double, double, int, float*, double*);
+
+}  // namespace pb

@@ -1,0 +1,69 @@
+// Restarted, right-preconditioned GMRES on the device, and FP64 iterative
+// refinement around FP32 GMRES cycles (the mixed-precision solve).
+//
+// Replaces Belos "GMRES" (Num Blocks 30, Maximum Iterations 1000, Convergence
+// Tolerance 1e-8, right preconditioner) of src/math/LinearAlgebra.cpp:164-195,
+// 228-252.  Convergence is ||b - A x||_2 / ||b||_2 <= tol, the denominator
+// being ||b|| whatever the initial guess, so a warm start never loosens the
+// target.
+#pragma once
+
+#include <functional>
+
+#include "kernels.hpp"
+
+namespace pb {
+
+template <typename T>
+struct CsrOp {
+    int64_t n = 0;
+    const int32_t* rp = nullptr;
+    const int32_t* col = nullptr;
+    const T* val = nullptr;
+    Profiler* prof = nullptr;  // optional launch timing
+};
+
+// z = M^-1 r on the stream
+template <typename T>
+using PcFn = std::function<void(hipStream_t, const T*, T*)>;
+
+struct SolveInfo {
+    int iters = 0;
+    int cycles = 0;
+    int refinements = 0;
+    double resid = 0;  // explicit ||b - A x|| / ||b||
+    bool converged = false;
+};
+
+// Device workspace of one GMRES instance of storage type T.
+template <typename T>
+struct GmresWork {
+    int64_t n = 0;
+    int m = 0;
+    DevBuf<T> V, z, u;
+    DevBuf<double> partial;  // kReduceBlocks * (m + 2)
+    DevBuf<double> dsc;      // device scalars
+    HostBuf<double> hsc;     // pinned mirror
+    void ensure(int64_t n_, int m_);
+};
+
+// One Arnoldi cycle of at most m steps from v0 = r0 / beta; accumulates
+// x_update = M^-1 V y.  Stops early when |g_{j+1}| <= stop_abs.
+// Returns the number of steps taken; *resid_est = |g_k| (implicit residual).
+template <typename T>
+int gmres_cycle(hipStream_t s, GmresWork<T>& w, const CsrOp<T>& A, const PcFn<T>& M, const T* r0, double beta,
+                double stop_abs, int max_steps, T* x_update, double* resid_est);
+
+// Full FP64 restarted GMRES: x holds the initial guess on entry.
+SolveInfo gmres_solve(hipStream_t s, GmresWork<double>& w, const CsrOp<double>& A, const PcFn<double>& M,
+                      const double* b, double* x, double* r, int m, int maxit, double tol);
+
+// FP64 iterative refinement around FP32 GMRES cycles.
+SolveInfo mixed_solve(hipStream_t s, GmresWork<float>& w, const CsrOp<double>& A64, const CsrOp<float>& A32,
+                      const PcFn<float>& M32, const double* b, double* x, double* r, float* r32, float* d32, int m,
+                      int maxit, double tol, double inner_rtol);
+
+// device norm helpers (synchronous: return to host)
+double dev_norm2(hipStream_t s, GmresWork<double>& w, int64_t n, const double* v);
+
+}  // namespace pb

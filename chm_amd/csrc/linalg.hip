@@ -1,0 +1,404 @@
+// Sparse and dense vector kernels of the Krylov solves: CSR SpMV with the
+// row block's values and column indices staged through LDS, fused residual,
+// batched Gram-Schmidt dot products / updates with wavefront-shuffle
+// reductions (FP64 accumulation for both FP64 and FP32 storage), and the
+// Jacobi and per-column tridiagonal preconditioners.
+//
+// Reductions are two-stage and deterministic: every block writes its partial
+// sums, then reduce_cols adds the partials in a fixed order.
+#include "kernels.hpp"
+
+namespace pb {
+namespace {
+
+constexpr int kRowsPerBlock = kBlock;  // one row per thread
+constexpr int kMaxRowLen = 6;          // self + 3 lateral + below + above
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// block-wide sum; result valid in thread 0
+__device__ __forceinline__ double block_sum(double v, double* lds) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) lds[wid] = v;
+    __syncthreads();
+    double s = 0;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) s += lds[k];
+    __syncthreads();
+    return s;
+}
+
+// ---------------------------------------------------------------------------
+// SpMV: a block of 256 rows copies its contiguous slice of val/col into LDS
+// with coalesced loads, then each thread forms its row's dot product.
+// Requires every row to have at most kMaxRowLen entries (checked on the host).
+// ---------------------------------------------------------------------------
+template <typename T, bool RESID>
+__global__ void __launch_bounds__(kBlock) k_spmv_lds(int64_t n, const int32_t* __restrict__ rp,
+                                                     const int32_t* __restrict__ col, const T* __restrict__ val,
+                                                     const T* __restrict__ x, const T* __restrict__ b, T* __restrict__ y,
+                                                     double* __restrict__ partial) {
+    __shared__ T s_val[kRowsPerBlock * kMaxRowLen];
+    __shared__ int32_t s_col[kRowsPerBlock * kMaxRowLen];
+    __shared__ int32_t s_rp[kRowsPerBlock + 1];
+    __shared__ double s_red[kBlock / 64];
+    const int64_t row0 = (int64_t)blockIdx.x * kRowsPerBlock;
+    const int64_t r = row0 + threadIdx.x;
+    const int64_t last = min(row0 + kRowsPerBlock, n);
+    if (r <= last) s_rp[threadIdx.x] = rp[r];
+    if (threadIdx.x == 0 && row0 + kRowsPerBlock <= n) s_rp[kRowsPerBlock] = rp[row0 + kRowsPerBlock];
+    __syncthreads();
+    const int32_t k0 = s_rp[0];
+    const int32_t cnt = s_rp[last - row0] - k0;
+    for (int k = threadIdx.x; k < cnt; k += kBlock) {
+        s_val[k] = val[k0 + k];
+        s_col[k] = col[k0 + k];
+    }
+    __syncthreads();
+    double rr = 0.0;
+    if (r < n) {
+        const int b0 = s_rp[threadIdx.x] - k0, b1 = s_rp[threadIdx.x + 1] - k0;
+        T acc = 0;
+        for (int k = b0; k < b1; ++k) acc = fma(s_val[k], x[s_col[k]], acc);
+        if (RESID) {
+            const T res = b[r] - acc;
+            y[r] = res;
+            rr = (double)res * (double)res;
+        } else {
+            y[r] = acc;
+        }
+    }
+    if (RESID) {
+        const double s = block_sum(rr, s_red);
+        if (threadIdx.x == 0) partial[blockIdx.x] = s;
+    }
+}
+
+// Generic CSR SpMV (any row length), one row per thread.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_spmv_plain(int64_t n, const int32_t* __restrict__ rp,
+                                                       const int32_t* __restrict__ col, const T* __restrict__ val,
+                                                       const T* __restrict__ x, T* __restrict__ y) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    T acc = 0;
+    for (int32_t k = rp[r]; k < rp[r + 1]; ++k) acc = fma(val[k], x[col[k]], acc);
+    y[r] = acc;
+}
+
+// Batched dot products of w against nv <= kNV basis vectors plus w.w: w is
+// read once, each basis vector once; FP64 accumulation.  Each wavefront
+// shuffle-reduces all nv+1 sums, then the block combines its wavefronts in LDS.
+constexpr int kNV = 32;
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_mdot(int64_t n, const T* __restrict__ V, int nv, const T* __restrict__ w,
+                                                 double* __restrict__ partial) {
+    __shared__ double s_acc[kBlock / 64][kNV + 1];
+    double acc[kNV + 1];
+#pragma unroll
+    for (int c = 0; c <= kNV; ++c) acc[c] = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+        const double wk = (double)w[k];
+        acc[kNV] = fma(wk, wk, acc[kNV]);
+#pragma unroll
+        for (int c = 0; c < kNV; ++c)
+            if (c < nv) acc[c] = fma((double)V[(int64_t)c * n + k], wk, acc[c]);
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int c = 0; c <= kNV; ++c) {
+        if (c < nv || c == kNV) {
+            const double v = wave_sum(acc[c]);
+            if (lane == 0) s_acc[wid][c] = v;
+        }
+    }
+    __syncthreads();
+    const int t = threadIdx.x;
+    if (t <= nv) {
+        const int src = t == nv ? kNV : t;
+        double s = 0;
+        for (int q = 0; q < (int)(blockDim.x >> 6); ++q) s += s_acc[q][src];
+        partial[(int64_t)blockIdx.x * (nv + 1) + t] = s;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_reduce_cols(const double* __restrict__ partial, int nblk, int stride,
+                                                        double* __restrict__ out) {
+    __shared__ double s_red[kBlock / 64];
+    const int c = blockIdx.x;
+    double s = 0;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) s += partial[(int64_t)b * stride + c];
+    s = block_sum(s, s_red);
+    if (threadIdx.x == 0) out[c] = s;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_gs_update(int64_t n, const T* __restrict__ V, int nv,
+                                                      const double* __restrict__ h, T* __restrict__ w,
+                                                      double* __restrict__ partial) {
+    __shared__ double s_red[kBlock / 64];
+    double hc[kNV];
+#pragma unroll
+    for (int c = 0; c < kNV; ++c) hc[c] = c < nv ? h[c] : 0.0;
+    double acc = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+        double wk = (double)w[k];
+#pragma unroll
+        for (int c = 0; c < kNV; ++c)
+            if (c < nv) wk = fma(-hc[c], (double)V[(int64_t)c * n + k], wk);
+        const T wt = (T)wk;
+        w[k] = wt;
+        acc = fma((double)wt, (double)wt, acc);
+    }
+    const double s = block_sum(acc, s_red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_scale_by_inv(int64_t n, const T* __restrict__ w, const double* nrm,
+                                                         T* __restrict__ v) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const double inv = 1.0 / sqrt(*nrm);
+    v[k] = (T)((double)w[k] * inv);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_combine(int64_t n, const T* __restrict__ V, int nv,
+                                                    const double* __restrict__ y, T* __restrict__ u) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    double s = 0.0;
+    for (int c = 0; c < nv; ++c) s = fma(y[c], (double)V[(int64_t)c * n + k], s);
+    u[k] = (T)s;
+}
+
+__global__ void __launch_bounds__(kBlock) k_absmax(int64_t n, const double* __restrict__ b, double* __restrict__ partial) {
+    __shared__ double s_red[kBlock / 64];
+    double m = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) m = fmax(m, fabs(b[k]));
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) s_red[wid] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = 0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) r = fmax(r, s_red[k]);
+        partial[blockIdx.x] = r;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_reduce_max(const double* __restrict__ partial, int nblk, double* out) {
+    __shared__ double s_red[kBlock / 64];
+    double m = 0;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) m = fmax(m, partial[b]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) s_red[wid] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = 0;
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) r = fmax(r, s_red[k]);
+        *out = r;
+    }
+}
+
+// Per-face vertical tridiagonal solve with precomputed Thomas factors
+// (m_z = d_z - l_z c'_{z-1}, c'_z = u_z / m_z): forward d_z = (r_z - l_z d_{z-1}) / m_z,
+// backward z_k = d_k - c'_k z_{k+1}.  LL > 0: column held in registers.
+template <typename T, int LL>
+__global__ void __launch_bounds__(kBlock) k_pc_column(int64_t F, int Ldyn, const T* __restrict__ lo,
+                                                      const T* __restrict__ cp, const T* __restrict__ im,
+                                                      const T* __restrict__ r, T* __restrict__ z) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= F) return;
+    if constexpr (LL > 0) {
+        T d[LL];
+        T prev = 0;
+#pragma unroll
+        for (int k = 0; k < LL; ++k) {
+            const int64_t q = (int64_t)k * F + i;
+            prev = (r[q] - lo[q] * prev) * im[q];
+            d[k] = prev;
+        }
+        T next = d[LL - 1];
+        z[(int64_t)(LL - 1) * F + i] = next;
+#pragma unroll
+        for (int k = LL - 2; k >= 0; --k) {
+            next = d[k] - cp[(int64_t)k * F + i] * next;
+            z[(int64_t)k * F + i] = next;
+        }
+    } else {
+        T prev = 0;
+        for (int k = 0; k < Ldyn; ++k) {
+            const int64_t q = (int64_t)k * F + i;
+            prev = (r[q] - lo[q] * prev) * im[q];
+            z[q] = prev;
+        }
+        T next = z[(int64_t)(Ldyn - 1) * F + i];
+        for (int k = Ldyn - 2; k >= 0; --k) {
+            const int64_t q = (int64_t)k * F + i;
+            next = z[q] - cp[q] * next;
+            z[q] = next;
+        }
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_pc_jacobi(int64_t n, const double* __restrict__ inv, const T* __restrict__ r,
+                                                      T* __restrict__ z) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) z[k] = (T)((double)r[k] * inv[k]);
+}
+
+template <typename T>
+__global__ void k_scale_host(int64_t n, const T* a, double alpha, T* b) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) b[k] = (T)((double)a[k] * alpha);
+}
+
+__global__ void k_to_f32(int64_t n, const double* a, float* b) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) b[k] = (float)a[k];
+}
+__global__ void k_axpy_f32_f64(int64_t n, double alpha, const float* x, double* y) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) y[k] = fma(alpha, (double)x[k], y[k]);
+}
+__global__ void k_scale_f64_f32(int64_t n, const double* r, double alpha, float* out) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) out[k] = (float)(r[k] * alpha);
+}
+__global__ void k_axpy_dev_f32(int64_t n, const double* scale, const float* x, double* y) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) y[k] = fma(*scale, (double)x[k], y[k]);
+}
+__global__ void k_axpy_f64(int64_t n, double alpha, const double* x, double* y) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) y[k] = fma(alpha, x[k], y[k]);
+}
+__global__ void k_copy_f64(int64_t n, const double* a, double* b) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) b[k] = a[k];
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+template <typename T>
+void spmv(hipStream_t s, int64_t n, const int32_t* rp, const int32_t* col, const T* val, const T* x, T* y) {
+    const unsigned g = (unsigned)((n + kRowsPerBlock - 1) / kRowsPerBlock);
+    hipLaunchKernelGGL((k_spmv_lds<T, false>), dim3(g), dim3(kBlock), 0, s, n, rp, col, val, x, (const T*)nullptr, y,
+                       (double*)nullptr);
+}
+
+template <typename T>
+void residual(hipStream_t s, int64_t n, const int32_t* rp, const int32_t* col, const T* val, const T* x, const T* b,
+              T* r, double* partial, int nblk) {
+    const unsigned g = (unsigned)((n + kRowsPerBlock - 1) / kRowsPerBlock);
+    (void)nblk;
+    hipLaunchKernelGGL((k_spmv_lds<T, true>), dim3(g), dim3(kBlock), 0, s, n, rp, col, val, x, b, r, partial);
+}
+
+template <typename T>
+void mdot(hipStream_t s, int64_t n, const T* V, int nv, const T* w, double* partial, int nblk) {
+    hipLaunchKernelGGL((k_mdot<T>), dim3(nblk), dim3(kBlock), 0, s, n, V, nv, w, partial);
+}
+
+void reduce_cols(hipStream_t s, const double* partial, int nblk, int stride, int ncols, double* out) {
+    hipLaunchKernelGGL(k_reduce_cols, dim3(ncols), dim3(kBlock), 0, s, partial, nblk, stride, out);
+}
+
+template <typename T>
+void gs_update(hipStream_t s, int64_t n, const T* V, int nv, const double* h, T* w, double* partial, int nblk) {
+    hipLaunchKernelGGL((k_gs_update<T>), dim3(nblk), dim3(kBlock), 0, s, n, V, nv, h, w, partial);
+}
+
+template <typename T>
+void scale_by_inv(hipStream_t s, int64_t n, const T* w, const double* nrm, T* v) {
+    hipLaunchKernelGGL((k_scale_by_inv<T>), dim3(grid_for(n)), dim3(kBlock), 0, s, n, w, nrm, v);
+}
+
+template <typename T>
+void combine(hipStream_t s, int64_t n, const T* V, int nv, const double* y, T* u) {
+    hipLaunchKernelGGL((k_combine<T>), dim3(grid_for(n)), dim3(kBlock), 0, s, n, V, nv, y, u);
+}
+
+void absmax(hipStream_t s, int64_t n, const double* b, double* partial, int nblk) {
+    hipLaunchKernelGGL(k_absmax, dim3(nblk), dim3(kBlock), 0, s, n, b, partial);
+}
+void reduce_max(hipStream_t s, const double* partial, int nblk, double* out) {
+    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(kBlock), 0, s, partial, nblk, out);
+}
+
+template <typename T>
+void pc_column(hipStream_t s, int64_t F, int L, const T* lo, const T* cp, const T* im, const T* r, T* z) {
+    const dim3 g(grid_for(F)), b(kBlock);
+    switch (L) {
+#define PB_COL_CASE(LL) \
+    case LL: hipLaunchKernelGGL((k_pc_column<T, LL>), g, b, 0, s, F, L, lo, cp, im, r, z); break;
+        PB_COL_CASE(2) PB_COL_CASE(3) PB_COL_CASE(4) PB_COL_CASE(5) PB_COL_CASE(6) PB_COL_CASE(7) PB_COL_CASE(8)
+        PB_COL_CASE(9) PB_COL_CASE(10) PB_COL_CASE(11) PB_COL_CASE(12) PB_COL_CASE(16) PB_COL_CASE(20)
+#undef PB_COL_CASE
+        default: hipLaunchKernelGGL((k_pc_column<T, 0>), g, b, 0, s, F, L, lo, cp, im, r, z); break;
+    }
+}
+
+template <typename T>
+void pc_jacobi(hipStream_t s, int64_t n, const double* invdiag, const T* r, T* z) {
+    hipLaunchKernelGGL((k_pc_jacobi<T>), dim3(grid_for(n)), dim3(kBlock), 0, s, n, invdiag, r, z);
+}
+
+template <typename T>
+void scale_host(hipStream_t s, int64_t n, const T* a, double alpha, T* b) {
+    hipLaunchKernelGGL((k_scale_host<T>), dim3(grid_for(n)), dim3(kBlock), 0, s, n, a, alpha, b);
+}
+
+void to_f32(hipStream_t s, int64_t n, const double* a, float* b) {
+    hipLaunchKernelGGL(k_to_f32, dim3(grid_for(n)), dim3(kBlock), 0, s, n, a, b);
+}
+void axpy_f32_into_f64(hipStream_t s, int64_t n, double alpha, const float* x, double* y) {
+    hipLaunchKernelGGL(k_axpy_f32_f64, dim3(grid_for(n)), dim3(kBlock), 0, s, n, alpha, x, y);
+}
+void scale_f64_to_f32_host(hipStream_t s, int64_t n, const double* r, double alpha, float* out) {
+    hipLaunchKernelGGL(k_scale_f64_f32, dim3(grid_for(n)), dim3(kBlock), 0, s, n, r, alpha, out);
+}
+void axpy_dev_scaled_f32(hipStream_t s, int64_t n, const double* scale, const float* x, double* y) {
+    hipLaunchKernelGGL(k_axpy_dev_f32, dim3(grid_for(n)), dim3(kBlock), 0, s, n, scale, x, y);
+}
+void axpy_f64(hipStream_t s, int64_t n, double alpha, const double* x, double* y) {
+    hipLaunchKernelGGL(k_axpy_f64, dim3(grid_for(n)), dim3(kBlock), 0, s, n, alpha, x, y);
+}
+void copy_f64(hipStream_t s, int64_t n, const double* a, double* b) {
+    hipLaunchKernelGGL(k_copy_f64, dim3(grid_for(n)), dim3(kBlock), 0, s, n, a, b);
+}
+
+#define PB_INST(T)                                                                                                 \
+    template void spmv<T>(hipStream_t, int64_t, const int32_t*, const int32_t*, const T*, const T*, T*);           \
+    template void residual<T>(hipStream_t, int64_t, const int32_t*, const int32_t*, const T*, const T*, const T*,   \
+                              T*, double*, int);                                                                   \
+    template void mdot<T>(hipStream_t, int64_t, const T*, int, const T*, double*, int);                            \
+    template void gs_update<T>(hipStream_t, int64_t, const T*, int, const double*, T*, double*, int);              \
+    template void scale_by_inv<T>(hipStream_t, int64_t, const T*, const double*, T*);                              \
+    template void combine<T>(hipStream_t, int64_t, const T*, int, const double*, T*);                              \
+    template void pc_column<T>(hipStream_t, int64_t, int, const T*, const T*, const T*, const T*, T*);             \
+    template void pc_jacobi<T>(hipStream_t, int64_t, const double*, const T*, T*);                                \
+    template void scale_host<T>(hipStream_t, int64_t, const T*, double, T*);
+PB_INST(double)
+PB_INST(float)
+#undef PB_INST
+
+}  // namespace pb

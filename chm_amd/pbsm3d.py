@@ -1,0 +1,180 @@
+"""Python binding of the MI355X PBSM3D library (ctypes over include/pbsm3d.h).
+
+The library is the only compute path: if libpbsm3d.so is missing this module
+raises instead of falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpbsm3d.so")
+
+EXPORTED = (
+    "pbsm3d_abi_version",
+    "pbsm3d_config_default",
+    "pbsm3d_face_geometry",
+    "pbsm3d_create",
+    "pbsm3d_step",
+    "pbsm3d_system_nnz",
+    "pbsm3d_export_system",
+    "pbsm3d_get_state",
+    "pbsm3d_set_state",
+    "pbsm3d_destroy",
+    "pbsm3d_last_error",
+    "pbsm3d_profile_enable",
+    "pbsm3d_profile_read",
+)
+
+KERNEL_CLASSES = ("assemble", "spmv64", "spmv32", "precond", "ortho", "residual", "deposition", "warmstart")
+
+_lib = None
+
+
+class Pbsm3dError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"PBSM3D error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    """Load libpbsm3d.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: run __graft_entry__.build() (or python -m chm_amd.build)")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        dp, ip = C.POINTER(C.c_double), C.POINTER(C.c_int32)
+        L.pbsm3d_abi_version.restype = C.c_int
+        L.pbsm3d_config_default.argtypes = [C.POINTER(abi.Config)]
+        L.pbsm3d_config_default.restype = None
+        L.pbsm3d_face_geometry.argtypes = [C.c_int64, dp, ip, dp] + [dp] * 7
+        L.pbsm3d_create.argtypes = [C.POINTER(vp), C.POINTER(abi.Mesh), C.POINTER(abi.Config), C.c_int]
+        L.pbsm3d_step.argtypes = [vp, C.POINTER(abi.Forcing), C.c_int32, C.POINTER(abi.Out), C.POINTER(abi.Stats)]
+        L.pbsm3d_system_nnz.argtypes = [vp, C.c_int]
+        L.pbsm3d_system_nnz.restype = C.c_int64
+        L.pbsm3d_export_system.argtypes = [vp, C.c_int, ip, ip, dp, dp]
+        L.pbsm3d_get_state.argtypes = [vp, dp, dp]
+        L.pbsm3d_set_state.argtypes = [vp, dp, dp]
+        L.pbsm3d_destroy.argtypes = [vp]
+        L.pbsm3d_destroy.restype = None
+        L.pbsm3d_last_error.argtypes = [vp]
+        L.pbsm3d_last_error.restype = C.c_char_p
+        L.pbsm3d_profile_enable.argtypes = [vp, C.c_int]
+        L.pbsm3d_profile_read.argtypes = [vp, C.c_int, C.POINTER(C.c_int64), dp, C.c_int]
+        _lib = L
+    return _lib
+
+
+def _check(rc: int, h=None):
+    if rc != 0:
+        msg = lib().pbsm3d_last_error(h).decode(errors="replace")
+        raise Pbsm3dError(rc, msg)
+
+
+def face_geometry(mesh) -> abi.Geometry:
+    """Per-face geometry of a :class:`chm_amd.meshgen.Mesh` through the library's face API."""
+    return abi.geometry_args(lib().pbsm3d_face_geometry, mesh)
+
+
+def default_config(**overrides) -> abi.Config:
+    c = abi.Config()
+    lib().pbsm3d_config_default(C.byref(c))
+    for k, v in overrides.items():
+        if not hasattr(c, k):
+            raise KeyError(k)
+        setattr(c, k, v)
+    return c
+
+
+class Context:
+    """One PBSM3D instance on one GPU: ``init`` in the constructor, ``run`` in :meth:`step`."""
+
+    def __init__(self, geom: abi.Geometry, cfg: abi.Config | None = None, device: int = 0):
+        self.cfg = cfg if cfg is not None else default_config()
+        self.F = geom.n_faces
+        self.L = int(self.cfg.nLayer)
+        keep = abi.Keep()
+        mesh = geom.struct(keep)
+        self._h = C.c_void_p()
+        _check(lib().pbsm3d_create(C.byref(self._h), C.byref(mesh), C.byref(self.cfg), device))
+        self.last_stats: dict = {}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pbsm3d_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, forcing: dict, dt: int = 900, outs: dict | None = None, want_fields: bool = False,
+             on_device: bool = False) -> tuple[dict, dict]:
+        """Run one timestep.
+
+        ``forcing`` maps the reference's variable names to arrays of F values
+        (numpy host arrays, or torch device tensors with ``on_device=True``).
+        Returns (outputs, stats); outputs are numpy arrays unless ``outs``
+        supplies device tensors (then written in place).
+        """
+        keep = abi.Keep()
+        f = abi.forcing_struct(forcing, keep, on_device=on_device)
+        out_on_dev = False
+        if outs is None:
+            outs = {k: np.zeros(self.F) for k in abi.OUT_FIELDS}
+            if want_fields:
+                outs["conc"] = np.zeros(self.F * self.L)
+                outs["qdep"] = np.zeros(self.F)
+        else:
+            out_on_dev = any(hasattr(v, "data_ptr") for v in outs.values())
+        o = abi.out_struct(outs, keep, on_device=out_on_dev)
+        st = abi.Stats()
+        _check(lib().pbsm3d_step(self._h, C.byref(f), int(dt), C.byref(o), C.byref(st)), self._h)
+        self.last_stats = st.as_dict()
+        return outs, self.last_stats
+
+    def system(self, which: int = 0):
+        """(row_ptr, col, val, rhs) of the last assembled suspension (0) or deposition (1) system."""
+        L = lib()
+        nnz = L.pbsm3d_system_nnz(self._h, which)
+        n = self.F * (self.L if which == 0 else 1)
+        rp = np.zeros(n + 1, dtype=np.int32)
+        col = np.zeros(nnz, dtype=np.int32)
+        val = np.zeros(nnz)
+        rhs = np.zeros(n)
+        keep = abi.Keep()
+        _check(L.pbsm3d_export_system(self._h, which, keep.iptr(rp), keep.iptr(col), keep.dptr(val), keep.dptr(rhs)),
+               self._h)
+        return rp, col, val, rhs
+
+    def profile(self, enable: bool = True):
+        """Turn HIP-event timing of the kernel classes on or off."""
+        _check(lib().pbsm3d_profile_enable(self._h, 1 if enable else 0), self._h)
+
+    def profile_read(self, reset: bool = True) -> dict:
+        """{class: (launches, device ms)} since the last reset."""
+        out = {}
+        for k, name in enumerate(KERNEL_CLASSES):
+            n, ms = C.c_int64(0), C.c_double(0)
+            _check(lib().pbsm3d_profile_read(self._h, k, C.byref(n), C.byref(ms), 1 if reset else 0), self._h)
+            out[name] = (int(n.value), float(ms.value))
+        return out
+
+    def get_state(self):
+        a, b = np.zeros(self.F), np.zeros(self.F)
+        keep = abi.Keep()
+        _check(lib().pbsm3d_get_state(self._h, keep.dptr(a), keep.dptr(b)), self._h)
+        return a, b
+
+    def set_state(self, sum_drift, sum_subl):
+        keep = abi.Keep()
+        _check(lib().pbsm3d_set_state(self._h, keep.dptr(sum_drift), keep.dptr(sum_subl)), self._h)

@@ -1,0 +1,211 @@
+/*
+ * pbsm3d.h -- C ABI of the MI355X PBSM3D blowing-snow step.
+ *
+ * One context = one mesh on one GPU, one HIP stream, all device memory owned by
+ * the context.  A context is not thread-safe; separate contexts are independent.
+ *
+ * The entry points replace, for the C++ PBSM3D module, the Trilinos-backed pieces
+ * of the reference step:
+ *   pbsm3d_create          PBSM3D::init            src/modules/PBSM3D.cpp:221-398
+ *                          + NearestNeighborProblem ctor  src/math/LinearAlgebra.cpp:31-197
+ *   pbsm3d_step            PBSM3D::run             src/modules/PBSM3D.cpp:400-1748
+ *                          (zeroSystem/sumInto/Solve  src/math/LinearAlgebra.cpp:199-252)
+ *   pbsm3d_export_system   NearestNeighborProblem::writeSystemMatrixMarket
+ *                                                  src/math/LinearAlgebra.cpp:277-287
+ *   pbsm3d_get_state /     PBSM3D::checkpoint / load_checkpoint
+ *   pbsm3d_set_state                               src/modules/PBSM3D.cpp:1753-1773
+ *   pbsm3d_destroy         PBSM3D::~PBSM3D         src/modules/PBSM3D.cpp:1750-1751
+ *
+ * Conventions: 0 = success, negative = error (pbsm3d_last_error gives text).
+ * Per-face arrays have length F; per-cell arrays have length nLayer*F in the
+ * reference's layer-major order (row r = z*F + face, LinearAlgebra.cpp:43-63).
+ * Per-edge arrays have length 3*F, face-major: [3*face + j] for edge j.
+ * All floating point is IEEE double unless a name says otherwise.
+ */
+#ifndef PBSM3D_H
+#define PBSM3D_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PBSM3D_ABI_VERSION 1
+
+/* return codes */
+#define PBSM3D_OK 0
+#define PBSM3D_EINVAL (-1)   /* invalid argument / config (reference: module_error on bad config) */
+#define PBSM3D_EHIP (-2)     /* HIP runtime error */
+#define PBSM3D_ENOCONV (-3)  /* Krylov solve did not converge (reference: module_error, PBSM3D.cpp:1447,1704) */
+#define PBSM3D_ENAN (-4)     /* non-finite value detected */
+#define PBSM3D_ENOMEM (-5)   /* device allocation failed */
+
+/* arithmetic of the linear solves */
+#define PBSM3D_FP64 0   /* FP64 SpMV, preconditioner and Krylov */
+#define PBSM3D_MIXED 1  /* FP32 SpMV/preconditioner inner solves, FP64 refinement, FP64 dots/residuals */
+
+/* preconditioner of the suspension solve */
+#define PBSM3D_PC_NONE 0
+#define PBSM3D_PC_JACOBI 1
+#define PBSM3D_PC_COLUMN 2   /* per-face vertical tridiagonal block (Thomas, nLayer unknowns) */
+#define PBSM3D_PC_ILU0 3     /* ILU(0) on the CSR, level-scheduled */
+#define PBSM3D_PC_ILUT 4     /* ILUT(1e-4, fill 3) as the reference's Ifpack2 setup (CPU oracle only) */
+
+/* initial guess of the suspension solve */
+#define PBSM3D_WS_ZERO 0     /* x0 = 0, as zeroSystem (LinearAlgebra.cpp:199-206) */
+#define PBSM3D_WS_LEARNED 1  /* per-face network prediction (see pbsm3d_mlp_*) */
+
+typedef struct pbsm3d_mesh {
+    int64_t n_faces;
+    const int32_t* neigh;       /* [3F] face::neighbor(j) index, -1 = none          */
+    const double* normal_x;     /* [3F] face::edge_unit_normal(j).x                 */
+    const double* normal_y;     /* [3F] face::edge_unit_normal(j).y                 */
+    const double* edge_length;  /* [3F] face::edge_length(j)                        */
+    const double* area;         /* [F]  face::get_area()                            */
+    const double* center_x;     /* [F]  face::center().x                            */
+    const double* center_y;     /* [F]  face::center().y                            */
+    const double* center_z;     /* [F]  face::get_z()                               */
+    const double* canopy_height;  /* [F] veg_attribute("CanopyHeight") or NULL     */
+    const double* lai;            /* [F] veg_attribute("LAI") or NULL              */
+    const double* stalk_number;   /* [F] veg_attribute("stalk_number") or NULL     */
+    const double* stalk_diameter; /* [F] veg_attribute("stalk_diameter") or NULL   */
+    const uint8_t* is_water;      /* [F] module_base::is_water(face) or NULL       */
+} pbsm3d_mesh;
+
+/* config.PBSM3D keys (PBSM3D.cpp:123-145, 223-258) plus solver settings that the
+ * reference hard-codes (LinearAlgebra.cpp:165-185) and extension keys whose
+ * defaults reproduce the reference. */
+typedef struct pbsm3d_config {
+    int32_t nLayer;               /* 10  (>= 2)                                    */
+    int32_t do_fixed_settling;    /* 0                                             */
+    double settling_velocity;     /* 0.5 m/s, must be >= 0                         */
+    int32_t do_sublimation;       /* 1                                             */
+    int32_t do_lateral_diff;      /* 1                                             */
+    double smooth_coeff;          /* 820                                           */
+    double min_sd_trans;          /* 0.1 m                                         */
+    double cutoff;                /* 0.3 m                                         */
+    double snow_diffusion_const;  /* 0.3                                           */
+    int32_t enable_veg;           /* 1 (turned off when the mesh has no veg data)  */
+    int32_t use_tanh_fetch;       /* 1                                             */
+    int32_t use_exp_fetch;        /* 0                                             */
+    int32_t use_R94_lambda;       /* 1                                             */
+    /* extension keys */
+    double z0_snow;               /* 0.01 = Snow::Z0_SNOW                          */
+    double settling_scale;        /* 1.0: multiplies the settling velocity         */
+    /* solver */
+    double tol;                   /* 1e-8: ||b - A x||_2 / ||b||_2                 */
+    int32_t max_iters;            /* 1000                                          */
+    int32_t restart;              /* 30                                            */
+    int32_t precision;            /* PBSM3D_FP64                                   */
+    int32_t precond;              /* PBSM3D_PC_COLUMN                              */
+    int32_t warm_start;           /* PBSM3D_WS_ZERO                                */
+    int32_t dep_precond;          /* PBSM3D_PC_JACOBI (deposition system)          */
+} pbsm3d_config;
+
+typedef struct pbsm3d_forcing {
+    const double* U_R;            /* [F] m/s at Z_U_R = 50 m                       */
+    const double* U_2m_above_srf; /* [F] m/s                                       */
+    const double* vw_dir;         /* [F] degrees (bearing the wind blows from)     */
+    const double* swe;            /* [F] kg m-2 (-9999/NaN -> 0)                   */
+    const double* t;              /* [F] deg C                                     */
+    const double* rh;             /* [F] %                                         */
+    const double* snowdepthavg;   /* [F] m (-9999/NaN -> 0)                        */
+    const double* fetch;          /* [F] m, or NULL when no fetch mode is on       */
+    int32_t on_device;            /* 1: all pointers are device (HBM) pointers     */
+} pbsm3d_forcing;
+
+typedef struct pbsm3d_out {
+    /* per-face module outputs, NULL to skip (the context keeps its own copy)      */
+    double* Qsalt;
+    double* Qsusp;
+    double* Qsubl;
+    double* Qsubl_mass;
+    double* sum_subl;
+    double* drift_mass;
+    double* sum_drift;
+    double* pbsm_more_than_avail;
+    /* solver fields for parity checks, NULL to skip                               */
+    double* conc;   /* [nLayer*F] suspension solution (after no clamping)          */
+    double* qdep;   /* [F] deposition solution                                     */
+    int32_t on_device;
+} pbsm3d_out;
+
+typedef struct pbsm3d_stats {
+    int32_t suspension_present;
+    int32_t deposition_present;
+    int32_t susp_iters;           /* total Krylov iterations (all restarts/cycles)  */
+    int32_t dep_iters;
+    double susp_resid;            /* final ||b - A x|| / ||b|| (explicit)           */
+    double dep_resid;
+    double susp_rhs_max;          /* ||b||_inf (getRhsMax)                          */
+    double dep_rhs_max;
+    int32_t susp_refinements;     /* outer FP64 refinement steps (MIXED)            */
+    int32_t reserved;
+    double ms_assembly;           /* HIP-event times of the last step, ms           */
+    double ms_susp_solve;
+    double ms_post;
+    double ms_dep_solve;
+    double ms_total;
+} pbsm3d_stats;
+
+typedef struct pbsm3d_ctx pbsm3d_ctx;
+
+void pbsm3d_config_default(pbsm3d_config* cfg);
+
+/* Face geometry from vertices, restating the face API the module reads in init():
+ * edge_unit_normal, edge_length, get_area (signed 2-D area, or the "area"
+ * parameter when given), center (triangulation.hpp:1443-1498, 1577-1589,
+ * 1774-1780, 1830-1856).  Host arrays; outputs sized as in pbsm3d_mesh. */
+int pbsm3d_face_geometry(int64_t n_faces, const double* vertex_xyz, const int32_t* elem,
+                         const double* area_param, double* normal_x, double* normal_y,
+                         double* edge_length, double* area, double* center_x, double* center_y,
+                         double* center_z);
+
+/* init(): copies geometry to HBM, builds both CSR patterns (bit-exact with
+ * LinearAlgebra.cpp:67-149), allocates solver workspaces on device `device`. */
+int pbsm3d_create(pbsm3d_ctx** out, const pbsm3d_mesh* mesh, const pbsm3d_config* cfg, int device);
+
+/* run(): one full timestep of dt_s seconds; outputs synchronised on return. */
+int pbsm3d_step(pbsm3d_ctx* ctx, const pbsm3d_forcing* forcing, int32_t dt_s, pbsm3d_out* out,
+                pbsm3d_stats* stats);
+
+/* nnz of the suspension (which=0) or deposition (which=1) system. */
+int64_t pbsm3d_system_nnz(const pbsm3d_ctx* ctx, int which);
+
+/* Copy the last assembled system (pattern + values + RHS) to host buffers;
+ * any pointer may be NULL.  row_ptr has n_rows+1 entries. */
+int pbsm3d_export_system(pbsm3d_ctx* ctx, int which, int32_t* row_ptr, int32_t* col, double* val,
+                         double* rhs);
+
+/* Checkpoint state: sum_drift (checkpointed by the reference) and sum_subl
+ * (module state the reference does not checkpoint), host arrays of F. */
+int pbsm3d_get_state(pbsm3d_ctx* ctx, double* sum_drift, double* sum_subl);
+int pbsm3d_set_state(pbsm3d_ctx* ctx, const double* sum_drift, const double* sum_subl);
+
+void pbsm3d_destroy(pbsm3d_ctx* ctx);
+const char* pbsm3d_last_error(const pbsm3d_ctx* ctx); /* ctx may be NULL: last global error */
+int pbsm3d_abi_version(void);
+
+/* Instrumentation (no counterpart in the reference, whose only timing is the
+ * per-timestep timer of core.cpp:2411-2437): with profiling on, each launch of
+ * the kernel classes below is bracketed by HIP events recorded on the
+ * context's own stream, and pbsm3d_profile_read returns the launch count and
+ * the summed device time since the last reset. */
+#define PBSM3D_K_ASSEMBLE 0    /* coefficients + suspension CSR assembly       */
+#define PBSM3D_K_SPMV64 1      /* FP64 suspension SpMV (Arnoldi)               */
+#define PBSM3D_K_SPMV32 2      /* FP32 suspension SpMV (mixed inner cycles)    */
+#define PBSM3D_K_PRECOND 3     /* suspension preconditioner apply              */
+#define PBSM3D_K_ORTHO 4       /* batched Gram-Schmidt dots + update           */
+#define PBSM3D_K_RESIDUAL 5    /* FP64 residual b - A x (fused norm)           */
+#define PBSM3D_K_DEPOSITION 6  /* post-processing + deposition assembly + solve */
+#define PBSM3D_K_WARMSTART 7   /* learned warm start (network forward)         */
+#define PBSM3D_K_COUNT 8
+int pbsm3d_profile_enable(pbsm3d_ctx* ctx, int enable);
+int pbsm3d_profile_read(pbsm3d_ctx* ctx, int kernel, int64_t* launches, double* ms_total, int reset);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PBSM3D_H */

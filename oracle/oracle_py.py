@@ -1,0 +1,191 @@
+"""ctypes wrapper of the CPU oracle (liboracle_pbsm3d.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke check and the CPU
+baseline leg of bench.py, never by the product package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from chm_amd import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle_pbsm3d.so")
+
+_lib = None
+
+
+def build(force: bool = False) -> str:
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-s", "-C", HERE, "-B" if force else "all"], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        dp, ip = C.POINTER(C.c_double), C.POINTER(C.c_int32)
+        L.oracle_create.argtypes = [C.POINTER(vp), C.POINTER(abi.Mesh), C.POINTER(abi.Config)]
+        L.oracle_step.argtypes = [vp, C.POINTER(abi.Forcing), C.c_int32, C.POINTER(abi.Out), C.POINTER(abi.Stats)]
+        L.oracle_destroy.argtypes = [vp]
+        L.oracle_destroy.restype = None
+        L.oracle_set_solver.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.oracle_system_nnz.argtypes = [vp, C.c_int]
+        L.oracle_system_nnz.restype = C.c_int64
+        L.oracle_export_system.argtypes = [vp, C.c_int, ip, ip, dp, dp]
+        L.oracle_export_face_trace.argtypes = [vp] + [dp] * 6
+        L.oracle_export_cell_trace.argtypes = [vp, dp, dp]
+        L.oracle_get_state.argtypes = [vp, dp, dp]
+        L.oracle_set_state.argtypes = [vp, dp, dp]
+        L.oracle_last_error.argtypes = [vp]
+        L.oracle_last_error.restype = C.c_char_p
+        L.oracle_face_geometry.argtypes = [C.c_int64, dp, ip, dp] + [dp] * 7
+        L.oracle_csr_nnz.argtypes = [C.c_int64, C.c_int, ip]
+        L.oracle_csr_nnz.restype = C.c_int64
+        L.oracle_csr_pattern.argtypes = [C.c_int64, C.c_int, ip, ip, ip]
+        L.oracle_solve_csr.argtypes = [C.c_int64, ip, ip, dp, dp, dp, C.c_int, C.c_int, C.c_int, C.c_double,
+                                       C.c_int64, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
+        L.oracle_dense_solve.argtypes = [C.c_int64, ip, ip, dp, dp, dp]
+        L.oracle_precond_apply.argtypes = [C.c_int64, ip, ip, dp, C.c_int, C.c_int64, C.c_int, dp, dp]
+        L.oracle_config_default.argtypes = [C.POINTER(abi.Config)]
+        L.oracle_config_default.restype = None
+        L.oracle_max_threads.restype = C.c_int
+        L.oracle_set_ilut.argtypes = [C.c_double, C.c_double]
+        L.oracle_set_ilut.restype = None
+        _lib = L
+    return _lib
+
+
+def face_geometry(mesh) -> abi.Geometry:
+    return abi.geometry_args(lib().oracle_face_geometry, mesh)
+
+
+def csr_pattern(n_faces: int, n_layer: int, neigh: np.ndarray):
+    keep = abi.Keep()
+    nb = np.ascontiguousarray(neigh, dtype=np.int32).reshape(-1)
+    nnz = lib().oracle_csr_nnz(n_faces, n_layer, keep.iptr(nb))
+    rp = np.zeros(n_faces * n_layer + 1, dtype=np.int32)
+    col = np.zeros(nnz, dtype=np.int32)
+    lib().oracle_csr_pattern(n_faces, n_layer, keep.iptr(nb), keep.iptr(rp), keep.iptr(col))
+    return rp, col
+
+
+def oracle_config(**overrides) -> abi.Config:
+    """Reference defaults with the reference's solver (GMRES(30) + ILUT)."""
+    c = abi.Config()
+    lib().oracle_config_default(C.byref(c))
+    for k, v in overrides.items():
+        setattr(c, k, v)
+    return c
+
+
+class Oracle:
+    """One PBSM3D module instance on the CPU (init in the constructor, run in step())."""
+
+    def __init__(self, geom: abi.Geometry, cfg: abi.Config | None = None, direct: bool = False, threads: int = 1):
+        self.cfg = cfg if cfg is not None else oracle_config()
+        self.F = geom.n_faces
+        self.L = int(self.cfg.nLayer)
+        self._keep = abi.Keep()
+        mesh = geom.struct(self._keep)
+        self._h = C.c_void_p()
+        rc = lib().oracle_create(C.byref(self._h), C.byref(mesh), C.byref(self.cfg))
+        if rc != 0:
+            raise ValueError(f"oracle_create failed ({rc})")
+        lib().oracle_set_solver(self._h, 1 if direct else 0, threads, -1, -1)
+
+    def close(self):
+        if self._h:
+            lib().oracle_destroy(self._h)
+            self._h = C.c_void_p()
+
+    __del__ = close
+
+    def set_solver(self, direct=None, threads=None, precond=-1, dep_precond=-1):
+        lib().oracle_set_solver(self._h, int(bool(direct)), int(threads or 1), precond, dep_precond)
+
+    def step(self, forcing: dict, dt: int = 900, want_fields: bool = True):
+        keep = abi.Keep()
+        f = abi.forcing_struct(forcing, keep)
+        outs = {k: np.zeros(self.F) for k in abi.OUT_FIELDS}
+        if want_fields:
+            outs["conc"] = np.zeros(self.F * self.L)
+            outs["qdep"] = np.zeros(self.F)
+        o = abi.out_struct(outs, keep)
+        st = abi.Stats()
+        rc = lib().oracle_step(self._h, C.byref(f), dt, C.byref(o), C.byref(st))
+        if rc != 0:
+            raise RuntimeError(f"oracle_step failed ({rc}): {lib().oracle_last_error(self._h).decode()}")
+        return outs, st.as_dict()
+
+    def system(self, which: int = 0):
+        L = lib()
+        nnz = L.oracle_system_nnz(self._h, which)
+        n = self.F * (self.L if which == 0 else 1)
+        rp = np.zeros(n + 1, dtype=np.int32)
+        col = np.zeros(nnz, dtype=np.int32)
+        val = np.zeros(nnz)
+        rhs = np.zeros(n)
+        keep = abi.Keep()
+        L.oracle_export_system(self._h, which, keep.iptr(rp), keep.iptr(col), keep.dptr(val), keep.dptr(rhs))
+        return rp, col, val, rhs
+
+    def face_trace(self) -> dict:
+        names = ("ustar", "hs", "c_salt", "z0", "saltation", "uth")
+        out = {k: np.zeros(self.F) for k in names}
+        keep = abi.Keep()
+        lib().oracle_export_face_trace(self._h, *(keep.dptr(out[k]) for k in names))
+        return out
+
+    def cell_trace(self) -> dict:
+        out = {k: np.zeros(self.F * self.L) for k in ("u_z", "csubl")}
+        keep = abi.Keep()
+        lib().oracle_export_cell_trace(self._h, keep.dptr(out["u_z"]), keep.dptr(out["csubl"]))
+        return out
+
+    def get_state(self):
+        a, b = np.zeros(self.F), np.zeros(self.F)
+        keep = abi.Keep()
+        lib().oracle_get_state(self._h, keep.dptr(a), keep.dptr(b))
+        return a, b
+
+    def set_state(self, sum_drift, sum_subl):
+        keep = abi.Keep()
+        lib().oracle_set_state(self._h, keep.dptr(sum_drift), keep.dptr(sum_subl))
+
+
+def solve_csr(rp, col, val, b, precond=abi.PC_ILUT, restart=30, maxit=1000, tol=1e-8, F=0, L=1, x0=None):
+    keep = abi.Keep()
+    n = len(rp) - 1
+    x = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64)
+    it, res = C.c_int(0), C.c_double(0)
+    rc = lib().oracle_solve_csr(n, keep.iptr(rp), keep.iptr(col), keep.dptr(val), keep.dptr(b), keep.dptr(x),
+                                precond, restart, maxit, tol, F, L, C.byref(it), C.byref(res))
+    return x, it.value, res.value, rc
+
+
+def dense_solve(rp, col, val, b):
+    keep = abi.Keep()
+    n = len(rp) - 1
+    x = np.zeros(n)
+    rc = lib().oracle_dense_solve(n, keep.iptr(rp), keep.iptr(col), keep.dptr(val), keep.dptr(b), keep.dptr(x))
+    if rc != 0:
+        raise RuntimeError("singular")
+    return x
+
+
+def precond_apply(rp, col, val, r, precond, F=0, L=1):
+    keep = abi.Keep()
+    n = len(rp) - 1
+    z = np.zeros(n)
+    lib().oracle_precond_apply(n, keep.iptr(rp), keep.iptr(col), keep.dptr(val), precond, F, L,
+                               keep.dptr(np.ascontiguousarray(r, dtype=np.float64)), keep.dptr(z))
+    return z

@@ -1,0 +1,77 @@
+"""The C-ABI library loads and exports what include/pbsm3d.h declares (no GPU needed)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from chm_amd import abi, meshgen
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    txt = open(os.path.join(ROOT, "include", "pbsm3d.h")).read()
+    return sorted(set(re.findall(r"\b(pbsm3d_[a-z_0-9]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    from chm_amd import pbsm3d
+
+    L = pbsm3d.lib()
+    names = _declared()
+    assert len(names) >= 10
+    for n in names:
+        assert hasattr(L, n), n
+    assert L.pbsm3d_abi_version() == 1
+    assert set(pbsm3d.EXPORTED) <= set(names)
+
+
+def test_struct_layouts_match_header():
+    # field counts of the ctypes mirror equal the C struct members
+    txt = open(os.path.join(ROOT, "include", "pbsm3d.h")).read()
+    for cname, py in (("pbsm3d_mesh", abi.Mesh), ("pbsm3d_config", abi.Config), ("pbsm3d_forcing", abi.Forcing),
+                      ("pbsm3d_out", abi.Out), ("pbsm3d_stats", abi.Stats)):
+        body = re.search(r"typedef struct %s \{(.*?)\} %s;" % (cname, cname), txt, re.S).group(1)
+        body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+        members = re.findall(r"([A-Za-z_][A-Za-z_0-9]*)\s*;", body)
+        assert [f[0] for f in py._fields_] == members, cname
+
+
+def test_config_defaults_match_reference():
+    from chm_amd import pbsm3d
+
+    c = pbsm3d.default_config()
+    # PBSM3D.cpp:123-145, 223-258 and LinearAlgebra.cpp:165-185
+    assert (c.nLayer, c.settling_velocity, c.smooth_coeff, c.min_sd_trans, c.cutoff, c.snow_diffusion_const) == (
+        10, 0.5, 820.0, 0.1, 0.3, 0.3)
+    assert (c.do_fixed_settling, c.do_sublimation, c.do_lateral_diff, c.enable_veg) == (0, 1, 1, 1)
+    assert (c.use_tanh_fetch, c.use_exp_fetch, c.use_R94_lambda) == (1, 0, 1)
+    assert (c.tol, c.max_iters, c.restart) == (1e-8, 1000, 30)
+    assert c.z0_snow == 0.01 and c.settling_scale == 1.0
+    d = abi.default_config()
+    for name, _ in abi.Config._fields_:
+        assert getattr(c, name) == getattr(d, name), name
+
+
+def test_host_face_geometry_matches_oracle(oracle, mesh_c1):
+    from chm_amd import pbsm3d
+
+    g1 = pbsm3d.face_geometry(mesh_c1)
+    g0 = oracle.face_geometry(mesh_c1)
+    for k in ("normal_x", "normal_y", "edge_length", "area", "center_x", "center_y", "center_z"):
+        assert np.array_equal(getattr(g1, k), getattr(g0, k)), k
+
+
+def test_create_rejects_bad_config_without_gpu():
+    """Argument validation happens before any device call."""
+    from chm_amd import pbsm3d
+
+    m = meshgen.synthetic_mesh(30, seed=1)
+    g = pbsm3d.face_geometry(m)
+    for bad in (dict(nLayer=1), dict(settling_velocity=-1.0), dict(use_exp_fetch=1, use_tanh_fetch=1),
+                dict(restart=0), dict(precond=abi.PC_ILUT)):
+        with pytest.raises(pbsm3d.Pbsm3dError) as e:
+            pbsm3d.Context(g, pbsm3d.default_config(**bad))
+        assert e.value.code == abi.PBSM3D_EINVAL
