@@ -135,9 +135,9 @@ def main():
     variants = {}
     for name in args.variants.split(","):
         if name == "plain":
-            cfg = pb.default_config(nLayer=L, precision=abi.FP64, precond=abi.PC_COLUMN)
+            cfg = pb.default_config(nLayer=L, precision=abi.FP64, precond=abi.PC_MCILU0)
         elif name == "mixed":
-            cfg = pb.default_config(nLayer=L, precision=abi.MIXED, precond=abi.PC_COLUMN)
+            cfg = pb.default_config(nLayer=L, precision=abi.MIXED, precond=abi.PC_MCILU0)
         else:
             raise SystemExit(f"unknown variant {name}")
         log(f"[bench] variant {name}: warmup {args.warmup}, steps {args.steps}")

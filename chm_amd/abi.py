@@ -20,7 +20,7 @@ PBSM3D_ENOMEM = -5
 FP64 = 0
 MIXED = 1
 
-PC_NONE, PC_JACOBI, PC_COLUMN, PC_ILU0, PC_ILUT = range(5)
+PC_NONE, PC_JACOBI, PC_COLUMN, PC_ILU0, PC_ILUT, PC_MCILU0 = range(6)
 WS_ZERO, WS_LEARNED = 0, 1
 
 _dp = C.POINTER(C.c_double)
@@ -153,9 +153,9 @@ def default_config(**overrides) -> Config:
         max_iters=1000,
         restart=30,
         precision=FP64,
-        precond=PC_COLUMN,
+        precond=PC_MCILU0,
         warm_start=WS_ZERO,
-        dep_precond=PC_JACOBI,
+        dep_precond=PC_MCILU0,
     )
     for k, v in overrides.items():
         if not hasattr(c, k):

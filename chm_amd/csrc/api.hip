@@ -32,6 +32,9 @@ struct pbsm3d_ctx {
     DevBuf<uint8_t> salt_flag;
     DevBuf<double> forcing_stage;
     ColumnFactors colf;
+    ColorOrder co;
+    DevBuf<double> dinv_s, dinv_d;
+    DevBuf<float> dinv32;
     DevBuf<double> invdiag_s, invdiag_d, r_s, r_d;
     DevBuf<float> val32, r32, d32;
     GmresWork<double> ws64, wd64;
@@ -167,9 +170,10 @@ void validate(const pbsm3d_mesh* m, const pbsm3d_config* c) {
     if (c->max_iters < 1) throw Error(PBSM3D_EINVAL, "max_iters must be >= 1");
     if (!(c->tol > 0)) throw Error(PBSM3D_EINVAL, "tol must be > 0");
     if (c->precision != PBSM3D_FP64 && c->precision != PBSM3D_MIXED) throw Error(PBSM3D_EINVAL, "unknown precision");
-    if (c->precond != PBSM3D_PC_NONE && c->precond != PBSM3D_PC_JACOBI && c->precond != PBSM3D_PC_COLUMN)
+    if (c->precond != PBSM3D_PC_NONE && c->precond != PBSM3D_PC_JACOBI && c->precond != PBSM3D_PC_COLUMN &&
+        c->precond != PBSM3D_PC_MCILU0)
         throw Error(PBSM3D_EINVAL, "preconditioner not available on the GPU path");
-    if (c->dep_precond != PBSM3D_PC_NONE && c->dep_precond != PBSM3D_PC_JACOBI)
+    if (c->dep_precond != PBSM3D_PC_NONE && c->dep_precond != PBSM3D_PC_JACOBI && c->dep_precond != PBSM3D_PC_MCILU0)
         throw Error(PBSM3D_EINVAL, "deposition preconditioner not available on the GPU path");
     const int64_t F = m->n_faces;
     for (int64_t k = 0; k < 3 * F; ++k)
@@ -226,9 +230,9 @@ void pbsm3d_config_default(pbsm3d_config* c) {
     c->max_iters = 1000;
     c->restart = 30;
     c->precision = PBSM3D_FP64;
-    c->precond = PBSM3D_PC_COLUMN;
+    c->precond = PBSM3D_PC_MCILU0;
     c->warm_start = PBSM3D_WS_ZERO;
-    c->dep_precond = PBSM3D_PC_JACOBI;
+    c->dep_precond = PBSM3D_PC_MCILU0;
 }
 
 int pbsm3d_face_geometry(int64_t F, const double* v, const int32_t* elem, const double* area_param, double* nx,
@@ -360,6 +364,15 @@ int pbsm3d_create(pbsm3d_ctx** out, const pbsm3d_mesh* m, const pbsm3d_config* c
             }
         }
         if (cfg->precond == PBSM3D_PC_JACOBI) c->invdiag_s.alloc(N);
+        if (cfg->precond == PBSM3D_PC_MCILU0 || cfg->dep_precond == PBSM3D_PC_MCILU0)
+            build_color_order(c->co, F, m->neigh, s);
+        if (cfg->precond == PBSM3D_PC_MCILU0) {
+            if (cfg->precision == PBSM3D_MIXED)
+                c->dinv32.alloc(N);
+            else
+                c->dinv_s.alloc(N);
+        }
+        if (cfg->dep_precond == PBSM3D_PC_MCILU0) c->dinv_d.alloc(F);
         c->invdiag_d.alloc(F);
         c->r_s.alloc(N);
         c->r_d.alloc(F);
@@ -481,7 +494,17 @@ int pbsm3d_step(pbsm3d_ctx* c, const pbsm3d_forcing* fo, int32_t dt_s, pbsm3d_ou
             CsrOp<double> A{N, c->susp.rp.p, c->susp.col.p, c->susp.val.p, &c->prof};
             if (cfg.precision == PBSM3D_FP64) {
                 PcFn<double> M;
-                if (cfg.precond == PBSM3D_PC_COLUMN)
+                if (cfg.precond == PBSM3D_PC_MCILU0) {
+                    {
+                        ProfScope ps(&c->prof, s, PBSM3D_K_PRECOND);
+                        dilu_factor<double>(s, c->co, F, L, c->susp.rp.p, c->susp.val.p, c->neigh.p, c->slots.p,
+                                            c->dinv_s.p);
+                    }
+                    M = [c, F, L](hipStream_t st2, const double* r, double* z) {
+                        dilu_apply<double>(st2, c->co, F, L, c->susp.rp.p, c->susp.val.p, c->neigh.p, c->slots.p,
+                                           c->dinv_s.p, r, z);
+                    };
+                } else if (cfg.precond == PBSM3D_PC_COLUMN)
                     M = [c, F, L](hipStream_t st2, const double* r, double* z) {
                         pc_column<double>(st2, F, L, c->colf.lo.p, c->colf.cp.p, c->colf.im.p, r, z);
                     };
@@ -497,7 +520,17 @@ int pbsm3d_step(pbsm3d_ctx* c, const pbsm3d_forcing* fo, int32_t dt_s, pbsm3d_ou
                 to_f32(s, c->susp.nnz, c->susp.val.p, c->val32.p);
                 CsrOp<float> A32{N, c->susp.rp.p, c->susp.col.p, c->val32.p, &c->prof};
                 PcFn<float> M;
-                if (cfg.precond == PBSM3D_PC_COLUMN)
+                if (cfg.precond == PBSM3D_PC_MCILU0) {
+                    {
+                        ProfScope ps(&c->prof, s, PBSM3D_K_PRECOND);
+                        dilu_factor<float>(s, c->co, F, L, c->susp.rp.p, c->val32.p, c->neigh.p, c->slots.p,
+                                           c->dinv32.p);
+                    }
+                    M = [c, F, L](hipStream_t st2, const float* r, float* z) {
+                        dilu_apply<float>(st2, c->co, F, L, c->susp.rp.p, c->val32.p, c->neigh.p, c->slots.p,
+                                          c->dinv32.p, r, z);
+                    };
+                } else if (cfg.precond == PBSM3D_PC_COLUMN)
                     M = [c, F, L](hipStream_t st2, const float* r, float* z) {
                         pc_column<float>(st2, F, L, c->colf.lo32.p, c->colf.cp32.p, c->colf.im32.p, r, z);
                     };
@@ -536,7 +569,13 @@ int pbsm3d_step(pbsm3d_ctx* c, const pbsm3d_forcing* fo, int32_t dt_s, pbsm3d_ou
         if (stats.deposition_present) {
             CsrOp<double> D{F, c->dep.rp.p, c->dep.col.p, c->dep.val.p};
             PcFn<double> M;
-            if (cfg.dep_precond == PBSM3D_PC_JACOBI)
+            if (cfg.dep_precond == PBSM3D_PC_MCILU0) {
+                dilu_factor<double>(s, c->co, F, 1, c->dep.rp.p, c->dep.val.p, c->neigh.p, c->slots.p, c->dinv_d.p);
+                M = [c, F](hipStream_t st2, const double* r, double* z) {
+                    dilu_apply<double>(st2, c->co, F, 1, c->dep.rp.p, c->dep.val.p, c->neigh.p, c->slots.p,
+                                       c->dinv_d.p, r, z);
+                };
+            } else if (cfg.dep_precond == PBSM3D_PC_JACOBI)
                 M = [c, F](hipStream_t st2, const double* r, double* z) { pc_jacobi<double>(st2, F, c->invdiag_d.p, r, z); };
             else
                 M = [F](hipStream_t st2, const double* r, double* z) {

@@ -96,6 +96,26 @@ void pc_column(hipStream_t s, int64_t F, int L, const T* lo, const T* cp, const 
 template <typename T>
 void pc_jacobi(hipStream_t s, int64_t n, const double* invdiag, const T* r, T* z);
 
+// Multicolour D-ILU(0) (precond.hip).  Faces are greedily coloured (no two
+// neighbours share a colour); rows are ordered (colour, layer, face).  In that
+// order ILU(0) only modifies the diagonal, so the factors are the strict lower
+// and upper parts of A plus d~_r = a_rr - sum_{k<r} a_rk a_kr / d~_k.
+struct ColorOrder {
+    int ncolor = 0;
+    std::vector<int64_t> off;  // faces of colour c: faces[off[c] .. off[c+1])
+    DevBuf<int32_t> faces;
+    DevBuf<uint8_t> rev;       // [3F] j-major: edge of neighbour j that points back to the face
+    DevBuf<uint8_t> lowmask;   // [F] bit f: neighbour f has a lower colour
+    std::vector<int32_t> h_color;
+};
+void build_color_order(ColorOrder& co, int64_t F, const int32_t* neigh_fm, hipStream_t s);
+template <typename T>
+void dilu_factor(hipStream_t s, const ColorOrder& co, int64_t F, int L, const int32_t* rp, const T* val,
+                 const int32_t* neigh, const uint16_t* slots, T* dinv);
+template <typename T>
+void dilu_apply(hipStream_t s, const ColorOrder& co, int64_t F, int L, const int32_t* rp, const T* val,
+                const int32_t* neigh, const uint16_t* slots, const T* dinv, const T* r, T* z);
+
 // conversions / vector updates
 template <typename T>
 void scale_host(hipStream_t s, int64_t n, const T* a, double alpha, T* b);  // b = alpha * a

@@ -48,6 +48,13 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is missing: run __graft_entry__.build() (or python -m chm_amd.build)")
+        # One HIP runtime per process: torch ships its own libamdhip64.so.7, and a
+        # process that uses both torch device tensors and this library must resolve
+        # the library's libamdhip64.so.7 to that same copy, so torch loads first.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         vp = C.c_void_p
         dp, ip = C.POINTER(C.c_double), C.POINTER(C.c_int32)

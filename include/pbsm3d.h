@@ -51,6 +51,9 @@ extern "C" {
 #define PBSM3D_PC_COLUMN 2   /* per-face vertical tridiagonal block (Thomas, nLayer unknowns) */
 #define PBSM3D_PC_ILU0 3     /* ILU(0) on the CSR, level-scheduled */
 #define PBSM3D_PC_ILUT 4     /* ILUT(1e-4, fill 3) as the reference's Ifpack2 setup (CPU oracle only) */
+#define PBSM3D_PC_MCILU0 5   /* D-ILU(0) in the multicolour order (face colour, layer, face): faces are
+                                greedily coloured so that no two neighbours share a colour; each colour is
+                                one parallel phase in which a thread factors / solves one face column */
 
 /* initial guess of the suspension solve */
 #define PBSM3D_WS_ZERO 0     /* x0 = 0, as zeroSystem (LinearAlgebra.cpp:199-206) */
@@ -98,9 +101,9 @@ typedef struct pbsm3d_config {
     int32_t max_iters;            /* 1000                                          */
     int32_t restart;              /* 30                                            */
     int32_t precision;            /* PBSM3D_FP64                                   */
-    int32_t precond;              /* PBSM3D_PC_COLUMN                              */
+    int32_t precond;              /* PBSM3D_PC_MCILU0                              */
     int32_t warm_start;           /* PBSM3D_WS_ZERO                                */
-    int32_t dep_precond;          /* PBSM3D_PC_JACOBI (deposition system)          */
+    int32_t dep_precond;          /* PBSM3D_PC_MCILU0 (deposition system)          */
 } pbsm3d_config;
 
 typedef struct pbsm3d_forcing {

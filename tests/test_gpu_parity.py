@@ -67,23 +67,53 @@ def test_pattern_and_assembly_c1(pb, oracle, mesh_c1):
     assert gst["susp_resid"] <= 1e-8 and gst["dep_resid"] <= 1e-8
 
 
-@pytest.mark.parametrize("precision,precond", [(abi.FP64, abi.PC_COLUMN), (abi.FP64, abi.PC_JACOBI),
+def test_dilu_matches_oracle_dilu(pb, oracle, mesh_c1):
+    """The multicolour D-ILU(0) kernels and the oracle's D-ILU(0) (written on the
+    generic CSR) are the same preconditioner: GMRES(30) takes the same number of
+    iterations on both, for the 3-D suspension and the 2-D deposition systems."""
+    _print_case("dilu_vs_oracle")
+    F = mesh_c1.n_faces
+    fo = meshgen.forcing_c1(F, seed=0)
+    g = pb.face_geometry(mesh_c1)
+    for L in (5, 10):
+        ctx = pb.Context(g, pb.default_config(nLayer=L, tol=1e-10, precond=abi.PC_MCILU0,
+                                              dep_precond=abi.PC_MCILU0))
+        orc = oracle.Oracle(g, oracle.oracle_config(nLayer=L, tol=1e-10, precond=abi.PC_MCILU0,
+                                                    dep_precond=abi.PC_MCILU0))
+        og, ost = orc.step(fo)
+        gg, gst = ctx.step(fo, want_fields=True)
+        assert abs(gst["susp_iters"] - ost["susp_iters"]) <= 1, (gst["susp_iters"], ost["susp_iters"])
+        assert abs(gst["dep_iters"] - ost["dep_iters"]) <= 1, (gst["dep_iters"], ost["dep_iters"])
+        for k in ("conc", "qdep"):
+            assert rel_err(gg[k], og[k]) < 1e-6, k
+
+
+@pytest.mark.parametrize("precision,precond", [(abi.FP64, abi.PC_MCILU0), (abi.MIXED, abi.PC_MCILU0),
+                                               (abi.FP64, abi.PC_COLUMN), (abi.FP64, abi.PC_JACOBI),
                                                (abi.MIXED, abi.PC_COLUMN)])
 def test_fields_c1(pb, oracle, mesh_c1, precision, precond):
     _print_case("fields_c1", precision=precision, precond=precond)
     F = mesh_c1.n_faces
-    for tol, bound in ((1e-12, 1e-8), (1e-8, 2e-5)):
-        g, ctx, orc = _pair(pb, oracle, mesh_c1, L=5, precision=precision, precond=precond, tol=tol)
-        fo = meshgen.forcing_c1(F, seed=0)
-        og, _ = orc.step(fo)
-        gg, gst = ctx.step(fo, want_fields=True)
-        if precision == abi.MIXED and tol < 1e-10:
-            continue  # FP32 inner cycles: the FP64 refinement still reaches 1e-12 but slowly; covered below
-        assert gst["susp_resid"] <= tol and gst["dep_resid"] <= tol
-        # forward error of a solve to relative residual tol is <= kappa*tol (kappa ~ 1e3 here)
-        for k in ("conc", "Qsusp", "Qsubl", "qdep", "drift_mass", "sum_drift", "sum_subl"):
-            assert rel_err(gg[k], og[k]) < bound, (k, tol, rel_err(gg[k], og[k]))
-        np.testing.assert_array_equal(gg["pbsm_more_than_avail"], og["pbsm_more_than_avail"])
+    fo = meshgen.forcing_c1(F, seed=0)
+    # arithmetic parity: both sides solved far below the production tolerance
+    g, ctx, orc = _pair(pb, oracle, mesh_c1, L=5, precision=precision, precond=precond, tol=1e-12)
+    og, _ = orc.step(fo)
+    gg, gst = ctx.step(fo, want_fields=True)
+    assert gst["susp_resid"] <= 1e-12 and gst["dep_resid"] <= 1e-12
+    for k in ("conc", "Qsusp", "Qsubl", "qdep", "drift_mass", "sum_drift", "sum_subl"):
+        assert rel_err(gg[k], og[k]) < 1e-8, (k, rel_err(gg[k], og[k]))
+    np.testing.assert_array_equal(gg["pbsm_more_than_avail"], og["pbsm_more_than_avail"])
+    # production tolerance: the explicit residuals meet 1e-8 and the fields sit within the
+    # forward-error band kappa*tol of the tight solution (kappa_dep ~ 1e4 on this mesh)
+    ctx8 = pb.Context(g, pb.default_config(nLayer=5, precision=precision, precond=precond, tol=1e-8))
+    g8, st8 = ctx8.step(fo, want_fields=True)
+    assert st8["susp_resid"] <= 1e-8 and st8["dep_resid"] <= 1e-8
+    for k, bound in (("conc", 1e-4), ("Qsusp", 1e-4), ("qdep", 1e-3), ("drift_mass", 1e-3)):
+        assert rel_err(g8[k], og[k]) < bound, (k, rel_err(g8[k], og[k]))
+    # deposition mass balance sum_i area_i qdep_i = sum_i rhs_i holds to the solve tolerance
+    _, _, _, rhs = ctx8.system(1)
+    # (|1^T r| <= sqrt(F) ||r|| <= sqrt(F) tol ||b||)
+    assert abs((g.area * g8["qdep"]).sum() - rhs.sum()) <= 2 * np.sqrt(F) * 1e-8 * np.linalg.norm(rhs)
 
 
 def test_tiny_golden(pb):
@@ -137,7 +167,8 @@ def test_multistep_state_and_checkpoint(pb, oracle, mesh_small):
         for name in abi.OUT_FIELDS:
             assert rel_err(gg[name], og[name]) < 1e-7, (k, name)
     sd, ss = ctx.get_state()
-    np.testing.assert_allclose(sd, og["sum_drift"], rtol=1e-7, atol=1e-12)
+    np.testing.assert_array_equal(sd, gg["sum_drift"])
+    assert rel_err(sd, og["sum_drift"]) < 1e-7 and rel_err(ss, og["sum_subl"]) < 1e-7
     # restore into a fresh context (checkpoint/resume) and continue identically
     ctx2 = pb.Context(g, pb.default_config(nLayer=4, tol=1e-12))
     ctx2.set_state(sd, ss)

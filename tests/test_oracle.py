@@ -311,6 +311,44 @@ def test_column_preconditioner_is_exact_on_columns(oracle, mesh_small):
     assert np.linalg.norm(A @ z - r) / np.linalg.norm(r) < 1e-13
 
 
+def _colour_key(neigh, F, L):
+    colr = -np.ones(F, int)
+    for i in range(F):
+        used = {colr[n] for n in neigh[i] if 0 <= n < i}
+        c = 0
+        while c in used:
+            c += 1
+        colr[i] = c
+    r = np.arange(F * L)
+    return (colr[r % F] * L + r // F) * F + r % F, colr
+
+
+def test_dilu_exact_on_triangular_parts(oracle, mesh_small):
+    """D-ILU(0) in the multicolour order is the exact inverse when A has no
+    strictly-upper (resp. strictly-lower) part in that order, and reduces to
+    the column Thomas solve when lateral couplings are absent."""
+    F, L = mesh_small.n_faces, 4
+    fo = meshgen.forcing_c1(F, seed=0)
+    o, _, _, _ = _run(oracle, mesh_small, fo, nLayer=L)
+    rp, col, val, _ = o.system(0)
+    key, colr = _colour_key(mesh_small.neigh, F, L)
+    assert colr.max() <= 3 and all((colr[mesh_small.neigh[i][mesh_small.neigh[i] >= 0]] != colr[i]).all()
+                                   for i in range(F))
+    rows = np.repeat(np.arange(F * L), np.diff(rp))
+    import scipy.sparse as sp
+
+    r = np.random.default_rng(3).standard_normal(F * L)
+    for keep in (key[col] <= key[rows], key[col] >= key[rows]):
+        v = val * keep
+        A = sp.csr_matrix((v, col, rp), shape=(F * L, F * L))
+        z = oracle.precond_apply(rp, col, v, r, abi.PC_MCILU0, F, L)
+        assert np.linalg.norm(A @ z - r) / np.linalg.norm(r) < 1e-13
+    vert = (col == rows) | (np.abs(col - rows) == F)
+    z1 = oracle.precond_apply(rp, col, val * vert, r, abi.PC_MCILU0, F, L)
+    z2 = oracle.precond_apply(rp, col, val * vert, r, abi.PC_COLUMN, F, L)
+    assert rel_err(z1, z2) < 1e-13
+
+
 def test_tiny_golden_fields(oracle):
     """Dense-solve outputs committed in tests/golden/tiny_fields.npz."""
     d = np.load(os.path.join(GOLD, "tiny_fields.npz"))
