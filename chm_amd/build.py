@@ -59,9 +59,27 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-4000:]}")
     os.replace(tmp, LIB)
+    build_host()
     if verbose:
-        print("built", LIB)
+        print("built", LIB, "and", HOST_BIN)
     return LIB
+
+
+HOST_DIR = os.path.join(HERE, "host")
+HOST_BIN = os.path.join(HERE, "bin", "pbsm3d_host")
+
+
+def build_host() -> str:
+    """The C++ PBSM3D module and its driver, linked against libpbsm3d.so."""
+    os.makedirs(os.path.dirname(HOST_BIN), exist_ok=True)
+    srcs = sorted(glob.glob(os.path.join(HOST_DIR, "*.cpp")))
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-Wextra", f"-I{os.path.join(ROOT, 'include')}", f"-I{HOST_DIR}",
+           *srcs, f"-L{HERE}", "-lpbsm3d", "-Wl,-rpath,$ORIGIN/..", "-o", HOST_BIN + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"host build failed:\n{r.stderr[-4000:]}")
+    os.replace(HOST_BIN + ".tmp", HOST_BIN)
+    return HOST_BIN
 
 
 if __name__ == "__main__":

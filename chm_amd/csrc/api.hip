@@ -9,47 +9,9 @@
 #include <mutex>
 #include <string>
 
-#include "krylov.hpp"
+#include "context.hpp"
 
 using namespace pb;
-
-struct pbsm3d_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev[6] = {};
-    int64_t F = 0;
-    int L = 0;
-    pbsm3d_config cfg{};
-    double dz = 0;
-    bool enable_veg = false;
-
-    DevBuf<double> nx, ny, elen, area, cz, canopy, lai, stalk_n, stalk_d, dep_w;
-    DevBuf<int32_t> neigh;
-    DevBuf<uint16_t> slots;
-    DevBuf<uint8_t> water;
-    System susp, dep;
-    DevBuf<double> u_z, csubl, Qsalt, Qsusp, Qsubl, Qsubl_mass, sum_subl, drift_mass, sum_drift, more;
-    DevBuf<uint8_t> salt_flag;
-    DevBuf<double> forcing_stage;
-    ColumnFactors colf;
-    ColorOrder co;
-    DevBuf<double> dinv_s, dinv_d;
-    DevBuf<float> dinv32;
-    DevBuf<double> invdiag_s, invdiag_d, r_s, r_d;
-    DevBuf<float> val32, r32, d32;
-    GmresWork<double> ws64, wd64;
-    GmresWork<float> ws32;
-    DevBuf<double> red_partial, red_out;
-    HostBuf<double> red_host;
-    Profiler prof;
-    std::string err;
-
-    ~pbsm3d_ctx() {
-        for (auto& e : ev)
-            if (e) (void)hipEventDestroy(e);
-        if (stream) (void)hipStreamDestroy(stream);
-    }
-};
 
 namespace pb {
 void throw_hip(hipError_t e, const char* what, const char* file, int line) {
@@ -368,11 +330,11 @@ int pbsm3d_create(pbsm3d_ctx** out, const pbsm3d_mesh* m, const pbsm3d_config* c
             build_color_order(c->co, F, m->neigh, s);
         if (cfg->precond == PBSM3D_PC_MCILU0) {
             if (cfg->precision == PBSM3D_MIXED)
-                c->dinv32.alloc(N);
+                c->dl32.alloc(N);
             else
-                c->dinv_s.alloc(N);
+                c->dl_s.alloc(N);
         }
-        if (cfg->dep_precond == PBSM3D_PC_MCILU0) c->dinv_d.alloc(F);
+        if (cfg->dep_precond == PBSM3D_PC_MCILU0) c->dl_d.alloc(F);
         c->invdiag_d.alloc(F);
         c->r_s.alloc(N);
         c->r_d.alloc(F);
@@ -497,12 +459,10 @@ int pbsm3d_step(pbsm3d_ctx* c, const pbsm3d_forcing* fo, int32_t dt_s, pbsm3d_ou
                 if (cfg.precond == PBSM3D_PC_MCILU0) {
                     {
                         ProfScope ps(&c->prof, s, PBSM3D_K_PRECOND);
-                        dilu_factor<double>(s, c->co, F, L, c->susp.rp.p, c->susp.val.p, c->neigh.p, c->slots.p,
-                                            c->dinv_s.p);
+                        dilu_factor<double>(s, c->co, F, L, c->susp.rp.p, c->susp.val.p, c->slots.p, c->dl_s);
                     }
                     M = [c, F, L](hipStream_t st2, const double* r, double* z) {
-                        dilu_apply<double>(st2, c->co, F, L, c->susp.rp.p, c->susp.val.p, c->neigh.p, c->slots.p,
-                                           c->dinv_s.p, r, z);
+                        dilu_apply<double>(st2, c->co, F, L, c->dl_s, r, z);
                     };
                 } else if (cfg.precond == PBSM3D_PC_COLUMN)
                     M = [c, F, L](hipStream_t st2, const double* r, double* z) {
@@ -523,12 +483,10 @@ int pbsm3d_step(pbsm3d_ctx* c, const pbsm3d_forcing* fo, int32_t dt_s, pbsm3d_ou
                 if (cfg.precond == PBSM3D_PC_MCILU0) {
                     {
                         ProfScope ps(&c->prof, s, PBSM3D_K_PRECOND);
-                        dilu_factor<float>(s, c->co, F, L, c->susp.rp.p, c->val32.p, c->neigh.p, c->slots.p,
-                                           c->dinv32.p);
+                        dilu_factor<float>(s, c->co, F, L, c->susp.rp.p, c->val32.p, c->slots.p, c->dl32);
                     }
                     M = [c, F, L](hipStream_t st2, const float* r, float* z) {
-                        dilu_apply<float>(st2, c->co, F, L, c->susp.rp.p, c->val32.p, c->neigh.p, c->slots.p,
-                                          c->dinv32.p, r, z);
+                        dilu_apply<float>(st2, c->co, F, L, c->dl32, r, z);
                     };
                 } else if (cfg.precond == PBSM3D_PC_COLUMN)
                     M = [c, F, L](hipStream_t st2, const float* r, float* z) {
@@ -570,10 +528,9 @@ int pbsm3d_step(pbsm3d_ctx* c, const pbsm3d_forcing* fo, int32_t dt_s, pbsm3d_ou
             CsrOp<double> D{F, c->dep.rp.p, c->dep.col.p, c->dep.val.p};
             PcFn<double> M;
             if (cfg.dep_precond == PBSM3D_PC_MCILU0) {
-                dilu_factor<double>(s, c->co, F, 1, c->dep.rp.p, c->dep.val.p, c->neigh.p, c->slots.p, c->dinv_d.p);
+                dilu_factor<double>(s, c->co, F, 1, c->dep.rp.p, c->dep.val.p, c->slots.p, c->dl_d);
                 M = [c, F](hipStream_t st2, const double* r, double* z) {
-                    dilu_apply<double>(st2, c->co, F, 1, c->dep.rp.p, c->dep.val.p, c->neigh.p, c->slots.p,
-                                       c->dinv_d.p, r, z);
+                    dilu_apply<double>(st2, c->co, F, 1, c->dl_d, r, z);
                 };
             } else if (cfg.dep_precond == PBSM3D_PC_JACOBI)
                 M = [c, F](hipStream_t st2, const double* r, double* z) { pc_jacobi<double>(st2, F, c->invdiag_d.p, r, z); };

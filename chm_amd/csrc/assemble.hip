@@ -124,6 +124,11 @@ __global__ void __launch_bounds__(kBlock) k_assemble(AssembleArgs a) {
     }
     a.Qsalt[i] = Qsalt;
     a.salt_flag[i] = salt ? 1 : 0;
+    if (a.fscal) {
+        a.fscal[i] = c_salt;
+        a.fscal[F + i] = hs;
+        a.fscal[2 * F + i] = ustar;
+    }
 
     // ---------------- per-layer coefficients + column assembly (PBSM3D.cpp:927-1404)
     const double rh = a.rh[i] / 100.;

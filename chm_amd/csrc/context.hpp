@@ -1,0 +1,59 @@
+// The context behind a pbsm3d_ctx handle: device copies of the mesh, the two
+// linear systems, per-step fields, solver workspaces and instrumentation.
+#pragma once
+
+#include <string>
+
+#include "krylov.hpp"
+#include "mlp.hpp"
+
+struct pbsm3d_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[6] = {};
+    int64_t F = 0;
+    int L = 0;
+    pbsm3d_config cfg{};
+    double dz = 0;
+    bool enable_veg = false;
+
+    pb::DevBuf<double> nx, ny, elen, area, cz, canopy, lai, stalk_n, stalk_d, dep_w;
+    pb::DevBuf<int32_t> neigh;
+    pb::DevBuf<uint16_t> slots;
+    pb::DevBuf<uint8_t> water;
+    pb::System susp, dep;
+    pb::DevBuf<double> u_z, csubl, Qsalt, Qsusp, Qsubl, Qsubl_mass, sum_subl, drift_mass, sum_drift, more;
+    pb::DevBuf<double> fscal;  // [3F] per-face c_salt, hs, u* of the last step
+    pb::DevBuf<uint8_t> salt_flag;
+    pb::DevBuf<double> forcing_stage;
+    pb::ColumnFactors colf;
+    pb::ColorOrder co;
+    pb::DiluFactors<double> dl_s, dl_d;
+    pb::DiluFactors<float> dl32;
+    pb::DevBuf<double> invdiag_s, invdiag_d, r_s, r_d;
+    pb::DevBuf<float> val32, r32, d32;
+    pb::GmresWork<double> ws64, wd64;
+    pb::GmresWork<float> ws32;
+    pb::DevBuf<double> red_partial, red_out;
+    pb::HostBuf<double> red_host;
+    // learned warm start
+    pb::Mlp* warm = nullptr;          // not owned
+    pb::DevBuf<float> feat, pred;     // [F x 16], [F x 11]
+    const double* last_vw_dir = nullptr;  // forcing pointer of the current step (device)
+    const double* last_t = nullptr;
+    pb::Profiler prof;
+    std::string err;
+
+    ~pbsm3d_ctx() {
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace pb {
+// features of the current step into c->feat (needs the assembled step)
+void warm_features(pbsm3d_ctx* c, const double* vw_dir, const double* t);
+// x0 of both solves from the network prediction in c->pred
+void warm_apply(pbsm3d_ctx* c);
+}  // namespace pb

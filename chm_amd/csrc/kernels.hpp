@@ -27,6 +27,7 @@ struct AssembleArgs {
     // per-cell / per-face outputs
     double *u_z, *csubl, *Qsalt;
     uint8_t* salt_flag;
+    double* fscal;  // [3F] c_salt, hs, u* (features of the learned warm start), may be null
     // fused preconditioner set-up (nullable)
     double *pc_lo, *pc_cp, *pc_im, *pc_invdiag;
     float *pc_lo32, *pc_cp32, *pc_im32;
@@ -100,21 +101,38 @@ void pc_jacobi(hipStream_t s, int64_t n, const double* invdiag, const T* r, T* z
 // neighbours share a colour); rows are ordered (colour, layer, face).  In that
 // order ILU(0) only modifies the diagonal, so the factors are the strict lower
 // and upper parts of A plus d~_r = a_rr - sum_{k<r} a_rk a_kr / d~_k.
+// Faces get a colour-major position k (colour by colour, ascending face id
+// inside a colour); the factors live in position order so that every colour
+// phase reads and writes contiguous memory.
 struct ColorOrder {
     int ncolor = 0;
-    std::vector<int64_t> off;  // faces of colour c: faces[off[c] .. off[c+1])
-    DevBuf<int32_t> faces;
-    DevBuf<uint8_t> rev;       // [3F] j-major: edge of neighbour j that points back to the face
-    DevBuf<uint8_t> lowmask;   // [F] bit f: neighbour f has a lower colour
+    std::vector<int64_t> off;  // positions of colour c: [off[c], off[c+1])
+    DevBuf<int32_t> faces;     // [F] position -> face id
+    DevBuf<int32_t> nbp;       // [3F] j-major over positions: position of neighbour j, -1 if none
+    DevBuf<uint8_t> rev;       // [3F] j-major over positions: edge of neighbour j that points back
+    DevBuf<uint8_t> lowmask;   // [F] over positions: bit j = neighbour j has a lower colour
     std::vector<int32_t> h_color;
+};
+// D-ILU factors in position order; index q = z*F + k.
+template <typename T>
+struct DiluFactors {
+    DevBuf<T> lat;   // [3N] coupling to neighbour j (row's own lateral entries)
+    DevBuf<T> vert;  // [2N] coupling below (0) / above (1)
+    DevBuf<T> dinv;  // [N]  1 / d~
+    DevBuf<T> y;     // [N]  intermediate of the apply
+    void alloc(int64_t N) {
+        lat.alloc(3 * N);
+        vert.alloc(2 * N);
+        dinv.alloc(N);
+        y.alloc(N);
+    }
 };
 void build_color_order(ColorOrder& co, int64_t F, const int32_t* neigh_fm, hipStream_t s);
 template <typename T>
 void dilu_factor(hipStream_t s, const ColorOrder& co, int64_t F, int L, const int32_t* rp, const T* val,
-                 const int32_t* neigh, const uint16_t* slots, T* dinv);
+                 const uint16_t* slots, DiluFactors<T>& d);
 template <typename T>
-void dilu_apply(hipStream_t s, const ColorOrder& co, int64_t F, int L, const int32_t* rp, const T* val,
-                const int32_t* neigh, const uint16_t* slots, const T* dinv, const T* r, T* z);
+void dilu_apply(hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFactors<T>& d, const T* r, T* z);
 
 // conversions / vector updates
 template <typename T>

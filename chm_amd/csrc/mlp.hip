@@ -1,0 +1,512 @@
+// Per-face network of the learned warm start: 16 -> 128 -> 128 -> 128 -> 11,
+// ReLU, BF16 operands with FP32 accumulation on the matrix cores
+// (v_mfma_f32_16x16x16_bf16), trained on the GPU by its own forward and
+// backward kernels with Adam.
+//
+// Every layer is a GEMM with a fused epilogue.  Activations are stored
+// [samples x features] row-major; for a layer with weights W [out x in]
+//   forward   Z = A_in W^T + b,   A_out = relu(Z)        (M=samples, N=out, K=in)
+//   backward  dW = dZ^T A_in,     db = colsum(dZ)        (M=out, N=in, K=samples)
+//             dA_in = dZ W,       dZ_in = dA_in * [Z_in > 0]  (M=samples, N=in, K=out)
+// The GEMM stages 64x16 A and 16x64 B tiles of BF16 in LDS; each of the four
+// wavefronts owns a 32x32 quarter of the 64x64 output tile, i.e. 2x2 MFMA
+// 16x16 blocks, with the operand fragments read from LDS in the MFMA lane
+// layout (A: row = lane%16, k = 4*(lane/16)+j; B: col = lane%16, same k;
+// C: col = lane%16, row = 4*(lane/16)+j).
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#include "mlp.hpp"
+
+namespace pb {
+namespace {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int TM = 64, TN = 64, TK = 16;
+
+__device__ __forceinline__ uint16_t to_bf16(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0;  // NaN
+    u += 0x7fffu + ((u >> 16) & 1u);                    // round to nearest even
+    return (uint16_t)(u >> 16);
+}
+
+struct GemmArgs {
+    int M, N, K;
+    const uint16_t* A;
+    int lda, transA;
+    const uint16_t* B;
+    int ldb, transB;
+    float* C;  // fp32 output (pre-activation), may be null
+    int ldc;
+    const float* bias;  // per output column, may be null
+    const float* mask;  // relu' mask source (same shape as output, ld = ldc), may be null
+    int relu;
+    uint16_t* Cb;  // bf16 copy of the (activated) output, may be null
+    int ldcb;
+    int atomic;  // split-K: atomically add into C
+    int ksplit;  // K range per blockIdx.z
+};
+
+__global__ void __launch_bounds__(256) k_gemm_bf16(GemmArgs g) {
+    __shared__ uint16_t As[TM][TK + 4];
+    __shared__ uint16_t Bs[TK][TN + 4];
+    const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
+    const int kb = blockIdx.z * g.ksplit;
+    const int ke = min(g.K, kb + g.ksplit);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    f32x4 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int k0 = kb; k0 < ke; k0 += TK) {
+        for (int e = tid; e < TM * TK; e += 256) {
+            int r, c;
+            if (g.transA) {  // stored [K x M]: walk m fastest for coalescing
+                r = e % TM;
+                c = e / TM;
+            } else {
+                r = e / TK;
+                c = e % TK;
+            }
+            const int gm = m0 + r, gk = k0 + c;
+            uint16_t v = 0;
+            if (gm < g.M && gk < ke)
+                v = g.transA ? g.A[(size_t)gk * g.lda + gm] : g.A[(size_t)gm * g.lda + gk];
+            As[r][c] = v;
+        }
+        for (int e = tid; e < TK * TN; e += 256) {
+            int r, c;
+            if (g.transB) {  // stored [N x K]: walk k fastest
+                r = e % TK;
+                c = e / TK;
+            } else {
+                r = e / TN;
+                c = e % TN;
+            }
+            const int gk = k0 + r, gn = n0 + c;
+            uint16_t v = 0;
+            if (gk < ke && gn < g.N)
+                v = g.transB ? g.B[(size_t)gn * g.ldb + gk] : g.B[(size_t)gk * g.ldb + gn];
+            Bs[r][c] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int tm = 0; tm < 2; ++tm) {
+            bf16x4 a;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) a[j] = (short)As[wm + tm * 16 + (lane & 15)][4 * (lane >> 4) + j];
+#pragma unroll
+            for (int tn = 0; tn < 2; ++tn) {
+                bf16x4 b;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) b[j] = (short)Bs[4 * (lane >> 4) + j][wn + tn * 16 + (lane & 15)];
+                acc[tm][tn] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[tm][tn], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int tm = 0; tm < 2; ++tm)
+#pragma unroll
+        for (int tn = 0; tn < 2; ++tn)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int row = m0 + wm + tm * 16 + 4 * (lane >> 4) + j;
+                const int col = n0 + wn + tn * 16 + (lane & 15);
+                if (row >= g.M || col >= g.N) continue;
+                float v = acc[tm][tn][j];
+                if (g.atomic) {
+                    atomicAdd(&g.C[(size_t)row * g.ldc + col], v);
+                    continue;
+                }
+                if (g.bias) v += g.bias[col];
+                if (g.mask && !(g.mask[(size_t)row * g.ldc + col] > 0.f)) v = 0.f;
+                if (g.C) g.C[(size_t)row * g.ldc + col] = v;
+                if (g.relu) v = fmaxf(v, 0.f);
+                if (g.Cb) g.Cb[(size_t)row * g.ldcb + col] = to_bf16(v);
+            }
+}
+
+void gemm(hipStream_t s, GemmArgs g, int splits = 1) {
+    if (splits < 1) splits = 1;
+    g.ksplit = ((g.K + splits - 1) / splits + TK - 1) / TK * TK;
+    splits = (g.K + g.ksplit - 1) / g.ksplit;
+    g.atomic = splits > 1 ? 1 : g.atomic;
+    dim3 grid((g.N + TN - 1) / TN, (g.M + TM - 1) / TM, splits);
+    hipLaunchKernelGGL(k_gemm_bf16, grid, dim3(256), 0, s, g);
+}
+
+// x (rows x 16, f32) -> normalised bf16 rows; optional row gather idx
+__global__ void k_prep_inputs(int64_t n, const float* __restrict__ X, const int64_t* __restrict__ idx,
+                              const float* __restrict__ mu, const float* __restrict__ sd, uint16_t* __restrict__ Xb) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * MlpDims::IN) return;
+    const int64_t r = t / MlpDims::IN;
+    const int c = (int)(t % MlpDims::IN);
+    const int64_t src = idx ? idx[r] : r;
+    Xb[t] = to_bf16((X[src * MlpDims::IN + c] - mu[c]) / sd[c]);
+}
+
+// targets -> normalised f32 rows (OUTP wide, padding zero)
+__global__ void k_prep_targets(int64_t n, const float* __restrict__ Y, const int64_t* __restrict__ idx,
+                               const float* __restrict__ mu, const float* __restrict__ sd, float* __restrict__ T) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * MlpDims::OUTP) return;
+    const int64_t r = t / MlpDims::OUTP;
+    const int c = (int)(t % MlpDims::OUTP);
+    const int64_t src = idx ? idx[r] : r;
+    T[t] = c < MlpDims::OUT ? (Y[src * MlpDims::OUT + c] - mu[c]) / sd[c] : 0.f;
+}
+
+// dY = 2 (Yp - T) / (n * OUT) on the real outputs, and per-block sums of the squared error
+__global__ void k_mse_grad(int64_t n, const float* __restrict__ Yp, const float* __restrict__ T,
+                           uint16_t* __restrict__ dYb, float* __restrict__ dY, float* __restrict__ partial) {
+    __shared__ float red[256];
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    float se = 0.f;
+    if (t < n * MlpDims::OUTP) {
+        const int c = (int)(t % MlpDims::OUTP);
+        float g = 0.f;
+        if (c < MlpDims::OUT) {
+            const float d = Yp[t] - T[t];
+            se = d * d;
+            g = 2.f * d / (float)(n * MlpDims::OUT);
+        }
+        dYb[t] = to_bf16(g);
+        if (dY) dY[t] = g;
+    }
+    red[threadIdx.x] = se;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// column sums of a bf16 matrix (rows x cols) into f32 (bias gradients)
+__global__ void k_colsum(int64_t rows, int cols, const uint16_t* __restrict__ D, float* __restrict__ out) {
+    __shared__ float red[256];
+    const int c = blockIdx.x;
+    float s = 0.f;
+    for (int64_t r = threadIdx.x; r < rows; r += blockDim.x) s += __uint_as_float((uint32_t)D[r * cols + c] << 16);
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[c] = red[0];
+}
+
+__global__ void k_adam(int64_t n, float* __restrict__ p, float* __restrict__ m, float* __restrict__ v,
+                       const float* __restrict__ g, float lr, float b1, float b2, float eps, float c1, float c2,
+                       uint16_t* __restrict__ pb, const uint8_t* __restrict__ keep) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    if (keep && !keep[t]) return;
+    const float gt = g[t];
+    const float mt = b1 * m[t] + (1.f - b1) * gt;
+    const float vt = b2 * v[t] + (1.f - b2) * gt * gt;
+    m[t] = mt;
+    v[t] = vt;
+    const float pt = p[t] - lr * (mt / c1) / (sqrtf(vt / c2) + eps);
+    p[t] = pt;
+    if (pb) pb[t] = to_bf16(pt);
+}
+
+__global__ void k_to_bf16(int64_t n, const float* __restrict__ a, uint16_t* __restrict__ b) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) b[t] = to_bf16(a[t]);
+}
+
+// de-normalise network outputs (OUTP wide) into OUT-wide rows
+__global__ void k_denorm(int64_t n, const float* __restrict__ Yp, const float* __restrict__ mu,
+                         const float* __restrict__ sd, float* __restrict__ Y) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * MlpDims::OUT) return;
+    const int64_t r = t / MlpDims::OUT;
+    const int c = (int)(t % MlpDims::OUT);
+    Y[t] = Yp[r * MlpDims::OUTP + c] * sd[c] + mu[c];
+}
+
+__global__ void k_sample_idx(int64_t nb, int64_t n, uint64_t key, int64_t* idx) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nb) return;
+    uint64_t z = key + (uint64_t)t * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    idx[t] = (int64_t)(z % (uint64_t)n);
+}
+
+unsigned g1(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+}  // namespace
+
+// layer l: W[l] is [dout(l) x din(l)] row-major
+int MlpDims::din(int l) { return l == 0 ? IN : H; }
+int MlpDims::dout(int l) { return l == 3 ? OUTP : H; }
+
+Mlp::Mlp(int device_, uint64_t seed) : device(device_) {
+    PB_HIP(hipSetDevice(device));
+    PB_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    std::mt19937_64 rng(seed);
+    std::normal_distribution<double> nd(0.0, 1.0);
+    for (int l = 0; l < 4; ++l) {
+        const int di = MlpDims::din(l), dn = MlpDims::dout(l);
+        std::vector<float> w((size_t)di * dn), b(dn, 0.f);
+        const double sc = std::sqrt(2.0 / di);  // He initialisation
+        for (int o = 0; o < dn; ++o)
+            for (int i = 0; i < di; ++i) w[(size_t)o * di + i] = (l == 3 && o >= MlpDims::OUT) ? 0.f : (float)(nd(rng) * sc);
+        W[l].alloc(w.size());
+        Wb[l].alloc(w.size());
+        bvec[l].alloc(dn);
+        gW[l].alloc(w.size());
+        gb[l].alloc(dn);
+        mW[l].alloc(w.size());
+        vW[l].alloc(w.size());
+        mb[l].alloc(dn);
+        vb[l].alloc(dn);
+        PB_HIP(hipMemcpyAsync(W[l].p, w.data(), w.size() * 4, hipMemcpyHostToDevice, stream));
+        PB_HIP(hipMemcpyAsync(bvec[l].p, b.data(), b.size() * 4, hipMemcpyHostToDevice, stream));
+        for (auto* d : {&mW[l], &vW[l]}) PB_HIP(hipMemsetAsync(d->p, 0, d->bytes(), stream));
+        for (auto* d : {&mb[l], &vb[l]}) PB_HIP(hipMemsetAsync(d->p, 0, d->bytes(), stream));
+        hipLaunchKernelGGL(k_to_bf16, dim3(g1(w.size())), dim3(256), 0, stream, (int64_t)w.size(), W[l].p, Wb[l].p);
+    }
+    // output rows >= OUT stay zero: mask their updates
+    std::vector<uint8_t> keep((size_t)MlpDims::OUTP * MlpDims::H, 1), keepb(MlpDims::OUTP, 1);
+    for (int o = MlpDims::OUT; o < MlpDims::OUTP; ++o) {
+        keepb[o] = 0;
+        for (int i = 0; i < MlpDims::H; ++i) keep[(size_t)o * MlpDims::H + i] = 0;
+    }
+    keepW4.alloc(keep.size());
+    keepb4.alloc(keepb.size());
+    PB_HIP(hipMemcpyAsync(keepW4.p, keep.data(), keep.size(), hipMemcpyHostToDevice, stream));
+    PB_HIP(hipMemcpyAsync(keepb4.p, keepb.data(), keepb.size(), hipMemcpyHostToDevice, stream));
+    std::vector<float> one(32, 1.f), zero(32, 0.f);
+    norm.alloc(2 * MlpDims::IN + 2 * MlpDims::OUT);
+    std::vector<float> nrm(norm.n, 0.f);
+    for (int c = 0; c < MlpDims::IN; ++c) nrm[MlpDims::IN + c] = 1.f;
+    for (int c = 0; c < MlpDims::OUT; ++c) nrm[2 * MlpDims::IN + MlpDims::OUT + c] = 1.f;
+    PB_HIP(hipMemcpyAsync(norm.p, nrm.data(), nrm.size() * 4, hipMemcpyHostToDevice, stream));
+    PB_HIP(hipStreamSynchronize(stream));
+}
+
+Mlp::~Mlp() {
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+void Mlp::ensure(int64_t n) {
+    if (n <= cap) return;
+    cap = n;
+    Xb.alloc(n * MlpDims::IN);
+    for (int l = 0; l < 3; ++l) {
+        Z[l].alloc(n * MlpDims::H);
+        Ab[l].alloc(n * MlpDims::H);
+        dZb[l].alloc(n * MlpDims::H);
+    }
+    Yp.alloc(n * MlpDims::OUTP);
+    Tn.alloc(n * MlpDims::OUTP);
+    dYb.alloc(n * MlpDims::OUTP);
+    idx.alloc(n);
+    partial.alloc(g1(n * MlpDims::OUTP) + 1);
+}
+
+const float* Mlp::xmu() const { return norm.p; }
+const float* Mlp::xsd() const { return norm.p + MlpDims::IN; }
+const float* Mlp::ymu() const { return norm.p + 2 * MlpDims::IN; }
+const float* Mlp::ysd() const { return norm.p + 2 * MlpDims::IN + MlpDims::OUT; }
+
+void Mlp::forward_prepared(int64_t n) {
+    hipStream_t s = stream;
+    const uint16_t* in = Xb.p;
+    for (int l = 0; l < 4; ++l) {
+        GemmArgs g{};
+        g.M = (int)n;
+        g.N = MlpDims::dout(l);
+        g.K = MlpDims::din(l);
+        g.A = in;
+        g.lda = MlpDims::din(l);
+        g.B = Wb[l].p;
+        g.ldb = MlpDims::din(l);
+        g.transB = 1;
+        g.bias = bvec[l].p;
+        if (l < 3) {
+            g.C = Z[l].p;
+            g.ldc = MlpDims::H;
+            g.relu = 1;
+            g.Cb = Ab[l].p;
+            g.ldcb = MlpDims::H;
+            in = Ab[l].p;
+        } else {
+            g.C = Yp.p;
+            g.ldc = MlpDims::OUTP;
+        }
+        gemm(s, g);
+    }
+}
+
+double Mlp::backward_prepared(int64_t n, float* dY_f32) {
+    hipStream_t s = stream;
+    const unsigned nb = g1(n * MlpDims::OUTP);
+    hipLaunchKernelGGL(k_mse_grad, dim3(nb), dim3(256), 0, s, n, Yp.p, Tn.p, dYb.p, dY_f32, partial.p);
+    const int splits = (int)std::min<int64_t>(64, std::max<int64_t>(1, n / 2048));
+    const uint16_t* dcur = dYb.p;  // gradient w.r.t. this layer's pre-activation (bf16)
+    for (int l = 3; l >= 0; --l) {
+        const int di = MlpDims::din(l), dn = MlpDims::dout(l);
+        const uint16_t* ain = l == 0 ? Xb.p : Ab[l - 1].p;
+        PB_HIP(hipMemsetAsync(gW[l].p, 0, gW[l].bytes(), s));
+        GemmArgs gw{};  // gW = dZ^T A_in  (M=dout, N=din, K=n)
+        gw.M = dn;
+        gw.N = di;
+        gw.K = (int)n;
+        gw.A = dcur;
+        gw.lda = dn;
+        gw.transA = 1;
+        gw.B = ain;
+        gw.ldb = di;
+        gw.C = gW[l].p;
+        gw.ldc = di;
+        gw.atomic = 1;
+        gemm(s, gw, splits);
+        hipLaunchKernelGGL(k_colsum, dim3(dn), dim3(256), 0, s, n, dn, dcur, gb[l].p);
+        if (l > 0) {  // dZ_{l-1} = (dZ W) * [Z_{l-1} > 0]   (M=n, N=din, K=dout)
+            GemmArgs gd{};
+            gd.M = (int)n;
+            gd.N = di;
+            gd.K = dn;
+            gd.A = dcur;
+            gd.lda = dn;
+            gd.B = Wb[l].p;
+            gd.ldb = di;
+            gd.mask = Z[l - 1].p;
+            gd.ldc = di;
+            gd.Cb = dZb[l - 1].p;
+            gd.ldcb = di;
+            gemm(s, gd);
+            dcur = dZb[l - 1].p;
+        }
+    }
+    std::vector<float> hp(nb);
+    PB_HIP(hipMemcpyAsync(hp.data(), partial.p, nb * sizeof(float), hipMemcpyDeviceToHost, s));
+    PB_HIP(hipStreamSynchronize(s));
+    double se = 0;
+    for (float v : hp) se += v;
+    return se / (double)(n * MlpDims::OUT);
+}
+
+void Mlp::adam_step(float lr) {
+    ++t_adam;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    const float c1 = 1.f - std::pow(b1, (float)t_adam), c2 = 1.f - std::pow(b2, (float)t_adam);
+    for (int l = 0; l < 4; ++l) {
+        const int64_t nw = (int64_t)MlpDims::din(l) * MlpDims::dout(l), nbias = MlpDims::dout(l);
+        hipLaunchKernelGGL(k_adam, dim3(g1(nw)), dim3(256), 0, stream, nw, W[l].p, mW[l].p, vW[l].p, gW[l].p, lr, b1,
+                           b2, eps, c1, c2, Wb[l].p, l == 3 ? keepW4.p : (const uint8_t*)nullptr);
+        hipLaunchKernelGGL(k_adam, dim3(g1(nbias)), dim3(256), 0, stream, nbias, bvec[l].p, mb[l].p, vb[l].p, gb[l].p,
+                           lr, b1, b2, eps, c1, c2, (uint16_t*)nullptr, l == 3 ? keepb4.p : (const uint8_t*)nullptr);
+    }
+}
+
+void Mlp::prepare(const float* X, const float* Y, int64_t n, const int64_t* rows) {
+    ensure(n);
+    hipLaunchKernelGGL(k_prep_inputs, dim3(g1(n * MlpDims::IN)), dim3(256), 0, stream, n, X, rows, xmu(), xsd(), Xb.p);
+    if (Y)
+        hipLaunchKernelGGL(k_prep_targets, dim3(g1(n * MlpDims::OUTP)), dim3(256), 0, stream, n, Y, rows, ymu(), ysd(),
+                           Tn.p);
+}
+
+double Mlp::train(const float* X, const float* Y, int64_t n, int epochs, int batch, float lr, uint64_t seed) {
+    ensure(batch);
+    const int64_t per_epoch = (n + batch - 1) / batch;
+    double loss = 0;
+    for (int e = 0; e < epochs; ++e) {
+        double acc = 0;
+        for (int64_t b = 0; b < per_epoch; ++b) {
+            const uint64_t key = seed * 0x9E3779B97F4A7C15ull + (uint64_t)(e * per_epoch + b) * 0xD1B54A32D192ED03ull;
+            hipLaunchKernelGGL(k_sample_idx, dim3(g1(batch)), dim3(256), 0, stream, (int64_t)batch, n, key, idx.p);
+            prepare(X, Y, batch, idx.p);
+            forward_prepared(batch);
+            acc += backward_prepared(batch, nullptr);
+            adam_step(lr);
+        }
+        loss = acc / (double)per_epoch;
+    }
+    PB_HIP(hipStreamSynchronize(stream));
+    return loss;
+}
+
+void Mlp::predict(const float* X, float* Y, int64_t n, hipStream_t s) {
+    // runs on the caller's stream, chunked so that the workspace stays bounded
+    const hipStream_t own = stream;
+    stream = s;
+    const int64_t chunk = 1 << 20;
+    for (int64_t o = 0; o < n; o += chunk) {
+        const int64_t m = std::min(chunk, n - o);
+        ensure(m);
+        prepare(X + o * MlpDims::IN, nullptr, m, nullptr);
+        forward_prepared(m);
+        hipLaunchKernelGGL(k_denorm, dim3(g1(m * MlpDims::OUT)), dim3(256), 0, stream, m, Yp.p, ymu(), ysd(),
+                           Y + o * MlpDims::OUT);
+    }
+    stream = own;
+}
+
+int64_t Mlp::param_count() {
+    int64_t c = 0;
+    for (int l = 0; l < 4; ++l) c += (int64_t)MlpDims::din(l) * MlpDims::dout(l) + MlpDims::dout(l);
+    return c;
+}
+
+void Mlp::get_params(float* host, bool grads) {
+    int64_t o = 0;
+    for (int l = 0; l < 4; ++l) {
+        const int64_t nw = (int64_t)MlpDims::din(l) * MlpDims::dout(l), nb = MlpDims::dout(l);
+        PB_HIP(hipMemcpyAsync(host + o, grads ? gW[l].p : W[l].p, nw * 4, hipMemcpyDeviceToHost, stream));
+        o += nw;
+        PB_HIP(hipMemcpyAsync(host + o, grads ? gb[l].p : bvec[l].p, nb * 4, hipMemcpyDeviceToHost, stream));
+        o += nb;
+    }
+    PB_HIP(hipStreamSynchronize(stream));
+}
+
+void Mlp::set_params(const float* host) {
+    int64_t o = 0;
+    for (int l = 0; l < 4; ++l) {
+        const int64_t nw = (int64_t)MlpDims::din(l) * MlpDims::dout(l), nb = MlpDims::dout(l);
+        PB_HIP(hipMemcpyAsync(W[l].p, host + o, nw * 4, hipMemcpyHostToDevice, stream));
+        o += nw;
+        PB_HIP(hipMemcpyAsync(bvec[l].p, host + o, nb * 4, hipMemcpyHostToDevice, stream));
+        o += nb;
+        hipLaunchKernelGGL(k_to_bf16, dim3(g1(nw)), dim3(256), 0, stream, nw, W[l].p, Wb[l].p);
+    }
+    PB_HIP(hipStreamSynchronize(stream));
+}
+
+void Mlp::set_normalization(const float* host) {
+    PB_HIP(hipMemcpyAsync(norm.p, host, norm.bytes(), hipMemcpyHostToDevice, stream));
+    PB_HIP(hipStreamSynchronize(stream));
+}
+
+double Mlp::gradients(const float* Xh, const float* Yh, int64_t n, float* grads_host) {
+    DevBuf<float> X, Y;
+    X.alloc(n * MlpDims::IN);
+    Y.alloc(n * MlpDims::OUT);
+    PB_HIP(hipMemcpyAsync(X.p, Xh, X.bytes(), hipMemcpyHostToDevice, stream));
+    PB_HIP(hipMemcpyAsync(Y.p, Yh, Y.bytes(), hipMemcpyHostToDevice, stream));
+    prepare(X.p, Y.p, n, nullptr);
+    forward_prepared(n);
+    const double loss = backward_prepared(n, nullptr);
+    if (grads_host) get_params(grads_host, true);
+    return loss;
+}
+
+}  // namespace pb

@@ -1,0 +1,124 @@
+// Driver of the PBSM3D module the way the reference's core drives a module
+// (construct from config, init(mesh), run(mesh) per timestep, checkpoint):
+//
+//   pbsm3d_host check <mesh.json>
+//       host-only checks: depends/provides, config errors, face geometry of the
+//       module's face API against pbsm3d_face_geometry.  Prints one JSON line.
+//   pbsm3d_host run <mesh.json> <forcing.bin> <nsteps> <out.bin> [key=value ...]
+//       forcing.bin holds 8*F doubles (U_R, U_2m_above_srf, vw_dir, swe, t, rh,
+//       snowdepthavg, fetch); runs nsteps timesteps and writes the 8 output
+//       fields (Qsalt, Qsusp, Qsubl, Qsubl_mass, sum_subl, drift_mass, sum_drift,
+//       pbsm_more_than_avail) as 8*F doubles; checkpoints, restores into a fresh
+//       module and runs one more step on both.  Prints one JSON line.
+#include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <iostream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "PBSM3D.hpp"
+
+using namespace chm;
+
+static const char* kForcing[8] = {"U_R", "U_2m_above_srf", "vw_dir", "swe", "t", "rh", "snowdepthavg", "fetch"};
+static const char* kOut[8] = {"Qsalt", "Qsusp", "Qsubl", "Qsubl_mass", "sum_subl", "drift_mass", "sum_drift",
+                              "pbsm_more_than_avail"};
+
+static std::string join(const std::vector<std::string>& v) {
+    std::string s = "[";
+    for (size_t i = 0; i < v.size(); ++i) s += (i ? ",\"" : "\"") + v[i] + "\"";
+    return s + "]";
+}
+
+static int check(const std::string& mesh_path) {
+    mesh m = triangulation::from_json_file(mesh_path);
+    PBSM3D mod{config_file()};
+    bool threw = false;
+    try {
+        config_file bad;
+        bad.put("use_exp_fetch", "true");
+        PBSM3D b{bad};
+    } catch (const module_error&) {
+        threw = true;
+    }
+    const size_t F = m->size_faces();
+    std::vector<double> nx(3 * F), ny(3 * F), el(3 * F), ar(F), cx(F), cy(F), cz(F);
+    int rc = pbsm3d_face_geometry((int64_t)F, m->vertex().data(), m->elem().data(), nullptr, nx.data(), ny.data(),
+                                  el.data(), ar.data(), cx.data(), cy.data(), cz.data());
+    double worst = 0;
+    for (size_t i = 0; i < F; ++i) {
+        face* f = m->face_at(i);
+        for (int j = 0; j < 3; ++j) {
+            worst = std::max(worst, std::fabs(f->edge_unit_normal(j).x() - nx[3 * i + j]));
+            worst = std::max(worst, std::fabs(f->edge_unit_normal(j).y() - ny[3 * i + j]));
+            worst = std::max(worst, std::fabs(f->edge_length(j) - el[3 * i + j]));
+        }
+        worst = std::max(worst, std::fabs(f->get_area() - ar[i]));
+        worst = std::max(worst, std::fabs(f->get_z() - cz[i]));
+    }
+    std::printf("{\"faces\": %zu, \"depends\": %s, \"provides\": %s, \"both_fetch_throws\": %s, "
+                "\"geometry_rc\": %d, \"geometry_max_abs_diff\": %.17g, \"default_nan\": %s}\n",
+                F, join(mod.depends_list()).c_str(), join(mod.provides_list()).c_str(), threw ? "true" : "false", rc,
+                worst, (*m->face_at(0))["t"] == -9999.0 ? "true" : "false");
+    return 0;
+}
+
+static int run(int argc, char** argv) {
+    const std::string mesh_path = argv[2], forcing_path = argv[3], out_path = argv[5];
+    const int nsteps = std::atoi(argv[4]);
+    config_file cfg;
+    for (int a = 6; a < argc; ++a) {
+        std::string kv = argv[a];
+        const size_t eq = kv.find('=');
+        if (eq != std::string::npos) cfg.put(kv.substr(0, eq), kv.substr(eq + 1));
+    }
+    mesh m = triangulation::from_json_file(mesh_path);
+    const size_t F = m->size_faces();
+    std::vector<double> buf(8 * F);
+    std::ifstream in(forcing_path, std::ios::binary);
+    in.read(reinterpret_cast<char*>(buf.data()), buf.size() * sizeof(double));
+    if (!in) throw std::runtime_error("short forcing file");
+    for (int k = 0; k < 8; ++k) m->variable(kForcing[k]).assign(buf.begin() + k * F, buf.begin() + (k + 1) * F);
+
+    PBSM3D mod{cfg};
+    mod.init(m);
+    for (int s = 0; s < nsteps; ++s) mod.run(m);
+    const pbsm3d_stats st = mod.last_stats();
+    std::ofstream out(out_path, std::ios::binary);
+    for (int k = 0; k < 8; ++k)
+        out.write(reinterpret_cast<const char*>(m->variable(kOut[k]).data()), F * sizeof(double));
+
+    // checkpoint -> restore into a fresh module -> one more step on both
+    checkpoint_store chk;
+    mod.checkpoint(m, chk);
+    mesh m2 = triangulation::from_json_file(mesh_path);
+    for (int k = 0; k < 8; ++k) m2->variable(kForcing[k]) = m->variable(kForcing[k]);
+    PBSM3D mod2{cfg};
+    mod2.init(m2);
+    mod2.load_checkpoint(m2, chk);
+    mod.run(m);
+    mod2.run(m2);
+    const bool same = m->variable("sum_drift") == m2->variable("sum_drift");
+    std::printf("{\"faces\": %zu, \"susp_iters\": %d, \"dep_iters\": %d, \"susp_resid\": %.3e, \"dep_resid\": %.3e, "
+                "\"suspension_present\": %d, \"deposition_present\": %d, \"restart_identical\": %s}\n",
+                F, st.susp_iters, st.dep_iters, st.susp_resid, st.dep_resid, st.suspension_present,
+                st.deposition_present, same ? "true" : "false");
+    return 0;
+}
+
+int main(int argc, char** argv) {
+    try {
+        if (argc >= 3 && std::string(argv[1]) == "check") return check(argv[2]);
+        if (argc >= 6 && std::string(argv[1]) == "run") return run(argc, argv);
+        std::fprintf(stderr, "usage: pbsm3d_host check <mesh.json> | run <mesh.json> <forcing.bin> <nsteps> <out.bin> [k=v...]\n");
+        return 2;
+    } catch (const module_error& e) {
+        std::fprintf(stderr, "module_error (%d): %s\n", e.code, e.what());
+        return 3;
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "error: %s\n", e.what());
+        return 1;
+    }
+}
