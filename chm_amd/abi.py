@@ -71,6 +71,8 @@ class Config(C.Structure):
         ("precond", C.c_int32),
         ("warm_start", C.c_int32),
         ("dep_precond", C.c_int32),
+        ("ortho_passes", C.c_int32),
+        ("block_size", C.c_int32),
     ]
 
 
@@ -156,6 +158,8 @@ def default_config(**overrides) -> Config:
         precond=PC_MCILU0,
         warm_start=WS_ZERO,
         dep_precond=PC_MCILU0,
+        ortho_passes=2,
+        block_size=256,
     )
     for k, v in overrides.items():
         if not hasattr(c, k):

@@ -129,6 +129,7 @@ void validate(const pbsm3d_mesh* m, const pbsm3d_config* c) {
     if (c->settling_velocity < 0) throw Error(PBSM3D_EINVAL, "PBSM3D settling velocity must be positive");
     if (c->use_exp_fetch && c->use_tanh_fetch) throw Error(PBSM3D_EINVAL, "Cannot specify both exp_fetch and tanh_fetch");
     if (c->restart < 1 || c->restart > 31) throw Error(PBSM3D_EINVAL, "restart must be in [1, 31]");
+    if (c->ortho_passes != 1 && c->ortho_passes != 2) throw Error(PBSM3D_EINVAL, "ortho_passes must be 1 or 2");
     if (c->max_iters < 1) throw Error(PBSM3D_EINVAL, "max_iters must be >= 1");
     if (!(c->tol > 0)) throw Error(PBSM3D_EINVAL, "tol must be > 0");
     if (c->precision != PBSM3D_FP64 && c->precision != PBSM3D_MIXED) throw Error(PBSM3D_EINVAL, "unknown precision");
@@ -195,6 +196,8 @@ void pbsm3d_config_default(pbsm3d_config* c) {
     c->precond = PBSM3D_PC_MCILU0;
     c->warm_start = PBSM3D_WS_ZERO;
     c->dep_precond = PBSM3D_PC_MCILU0;
+    c->ortho_passes = 2;
+    c->block_size = 256;
 }
 
 int pbsm3d_face_geometry(int64_t F, const double* v, const int32_t* elem, const double* area_param, double* nx,
@@ -347,6 +350,8 @@ int pbsm3d_create(pbsm3d_ctx** out, const pbsm3d_mesh* m, const pbsm3d_config* c
             c->ws64.ensure(N, cfg->restart);
         }
         c->wd64.ensure(F, cfg->restart);
+        c->ws64.passes = c->ws32.passes = c->wd64.passes = cfg->ortho_passes;
+        c->fscal.alloc(3 * F);
         c->red_partial.alloc(kReduceBlocks);
         c->red_out.alloc(4);
         c->red_host.alloc(4);
@@ -441,6 +446,9 @@ int pbsm3d_step(pbsm3d_ctx* c, const pbsm3d_forcing* fo, int32_t dt_s, pbsm3d_ou
         a.pc_cp32 = c->colf.cp32.p;
         a.pc_im32 = c->colf.im32.p;
         a.pc_invdiag = c->invdiag_s.p;
+        a.fscal = c->fscal.p;
+        c->last_vw_dir = fdev[2];
+        c->last_t = fdev[4];
         {
             ProfScope ps(&c->prof, s, PBSM3D_K_ASSEMBLE);
             hipLaunchKernelGGL(k_assemble, dim3(grid_for(F)), dim3(kBlock), 0, s, a);
@@ -451,6 +459,16 @@ int pbsm3d_step(pbsm3d_ctx* c, const pbsm3d_forcing* fo, int32_t dt_s, pbsm3d_ou
         // ---- suspension presence + solve (PBSM3D.cpp:1424-1459)
         stats.susp_rhs_max = max_abs(c, c->susp.rhs.p, N);
         stats.suspension_present = stats.susp_rhs_max > 1e-12;
+        const bool warm = cfg.warm_start == PBSM3D_WS_LEARNED && c->warm != nullptr;
+        if (warm && stats.suspension_present) {
+            // x0 of the suspension solve from the network (x of the deposition after its assembly)
+            ProfScope ps(&c->prof, s, PBSM3D_K_WARMSTART);
+            c->feat.alloc(F * MlpDims::IN);
+            c->pred.alloc(F * MlpDims::OUT);
+            warm_features(c, fdev[2], fdev[4]);
+            c->warm->predict(c->feat.p, c->pred.p, F, s);
+            warm_apply(c, 0);
+        }
         if (stats.suspension_present) {
             SolveInfo si;
             CsrOp<double> A{N, c->susp.rp.p, c->susp.col.p, c->susp.val.p, &c->prof};
@@ -521,6 +539,7 @@ int pbsm3d_step(pbsm3d_ctx* c, const pbsm3d_forcing* fo, int32_t dt_s, pbsm3d_ou
         PB_HIP(hipGetLastError());
         stats.dep_rhs_max = max_abs(c, c->dep.rhs.p, F);
         stats.deposition_present = stats.suspension_present && stats.dep_rhs_max > 1e-12;
+        if (warm && stats.deposition_present) warm_apply(c, 1);
         PB_HIP(hipEventRecord(c->ev[3], s));
 
         // ---- deposition solve + drift (PBSM3D.cpp:1661-1740)

@@ -54,6 +54,6 @@ struct pbsm3d_ctx {
 namespace pb {
 // features of the current step into c->feat (needs the assembled step)
 void warm_features(pbsm3d_ctx* c, const double* vw_dir, const double* t);
-// x0 of both solves from the network prediction in c->pred
-void warm_apply(pbsm3d_ctx* c);
+// x0 of the suspension (which = 0) or deposition (which = 1) solve from c->pred
+void warm_apply(pbsm3d_ctx* c, int which);
 }  // namespace pb

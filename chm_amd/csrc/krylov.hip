@@ -85,7 +85,7 @@ int gmres_cycle(hipStream_t s, GmresWork<T>& w, const CsrOp<T>& A, const PcFn<T>
             for (int i = 0; i <= j; ++i) h[i] += hs[off_h() + i];
             if (pass == 0) ss_before = hs[off_h() + j + 1];
             ss_after = hs[off_ss(m)];
-            if (!(ss_after < 0.5 * ss_before)) break;  // no re-orthogonalisation needed
+            if (w.passes < 2 || !(ss_after < 0.5 * ss_before)) break;  // no re-orthogonalisation needed
         }
         const double hn = std::sqrt(ss_after);
         if (hn > 0) scale_by_inv<T>(s, n, vj1, d_ss, vj1);

@@ -40,6 +40,7 @@ template <typename T>
 struct GmresWork {
     int64_t n = 0;
     int m = 0;
+    int passes = 2;  // Gram-Schmidt passes allowed per Arnoldi step
     DevBuf<T> V, z, u;
     DevBuf<double> partial;  // kReduceBlocks * (m + 2)
     DevBuf<double> dsc;      // device scalars

@@ -96,7 +96,32 @@ __global__ void __launch_bounds__(kBlock) k_spmv_plain(int64_t n, const int32_t*
 // read once, each basis vector once; FP64 accumulation.  Each wavefront
 // shuffle-reduces all nv+1 sums, then the block combines its wavefronts in LDS.
 constexpr int kNV = 32;
-template <typename T>
+
+// four consecutive elements per thread: one 16-byte (float) or two 16-byte
+// (double) loads per lane, so each wavefront load moves 1-2 KB
+__device__ __forceinline__ void load4(const double* p, double (&o)[4]) {
+    const double4 v = *reinterpret_cast<const double4*>(p);
+    o[0] = v.x;
+    o[1] = v.y;
+    o[2] = v.z;
+    o[3] = v.w;
+}
+__device__ __forceinline__ void load4(const float* p, double (&o)[4]) {
+    const float4 v = *reinterpret_cast<const float4*>(p);
+    o[0] = v.x;
+    o[1] = v.y;
+    o[2] = v.z;
+    o[3] = v.w;
+}
+__device__ __forceinline__ void store4(double* p, const double (&o)[4]) {
+    *reinterpret_cast<double4*>(p) = double4{o[0], o[1], o[2], o[3]};
+}
+__device__ __forceinline__ void store4(float* p, const double (&o)[4]) {
+    *reinterpret_cast<float4*>(p) = float4{(float)o[0], (float)o[1], (float)o[2], (float)o[3]};
+}
+
+// VEC: n % 4 == 0, every vector 4-element aligned
+template <typename T, bool VEC>
 __global__ void __launch_bounds__(kBlock) k_mdot(int64_t n, const T* __restrict__ V, int nv, const T* __restrict__ w,
                                                  double* __restrict__ partial) {
     __shared__ double s_acc[kBlock / 64][kNV + 1];
@@ -104,12 +129,30 @@ __global__ void __launch_bounds__(kBlock) k_mdot(int64_t n, const T* __restrict_
 #pragma unroll
     for (int c = 0; c <= kNV; ++c) acc[c] = 0.0;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
-        const double wk = (double)w[k];
-        acc[kNV] = fma(wk, wk, acc[kNV]);
+    if constexpr (VEC) {
+        const int64_t n4 = n / 4;
+        for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < n4; g += stride) {
+            double wv[4];
+            load4(w + 4 * g, wv);
 #pragma unroll
-        for (int c = 0; c < kNV; ++c)
-            if (c < nv) acc[c] = fma((double)V[(int64_t)c * n + k], wk, acc[c]);
+            for (int q = 0; q < 4; ++q) acc[kNV] = fma(wv[q], wv[q], acc[kNV]);
+#pragma unroll
+            for (int c = 0; c < kNV; ++c)
+                if (c < nv) {
+                    double vv[4];
+                    load4(V + (int64_t)c * n + 4 * g, vv);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) acc[c] = fma(vv[q], wv[q], acc[c]);
+                }
+        }
+    } else {
+        for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+            const double wk = (double)w[k];
+            acc[kNV] = fma(wk, wk, acc[kNV]);
+#pragma unroll
+            for (int c = 0; c < kNV; ++c)
+                if (c < nv) acc[c] = fma((double)V[(int64_t)c * n + k], wk, acc[c]);
+        }
     }
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
@@ -139,7 +182,7 @@ __global__ void __launch_bounds__(kBlock) k_reduce_cols(const double* __restrict
     if (threadIdx.x == 0) out[c] = s;
 }
 
-template <typename T>
+template <typename T, bool VEC>
 __global__ void __launch_bounds__(kBlock) k_gs_update(int64_t n, const T* __restrict__ V, int nv,
                                                       const double* __restrict__ h, T* __restrict__ w,
                                                       double* __restrict__ partial) {
@@ -149,14 +192,37 @@ __global__ void __launch_bounds__(kBlock) k_gs_update(int64_t n, const T* __rest
     for (int c = 0; c < kNV; ++c) hc[c] = c < nv ? h[c] : 0.0;
     double acc = 0.0;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
-        double wk = (double)w[k];
+    if constexpr (VEC) {
+        const int64_t n4 = n / 4;
+        for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < n4; g += stride) {
+            double wv[4];
+            load4(w + 4 * g, wv);
 #pragma unroll
-        for (int c = 0; c < kNV; ++c)
-            if (c < nv) wk = fma(-hc[c], (double)V[(int64_t)c * n + k], wk);
-        const T wt = (T)wk;
-        w[k] = wt;
-        acc = fma((double)wt, (double)wt, acc);
+            for (int c = 0; c < kNV; ++c)
+                if (c < nv) {
+                    double vv[4];
+                    load4(V + (int64_t)c * n + 4 * g, vv);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) wv[q] = fma(-hc[c], vv[q], wv[q]);
+                }
+            double wr[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                wr[q] = (double)(T)wv[q];  // the value actually stored
+                acc = fma(wr[q], wr[q], acc);
+            }
+            store4(w + 4 * g, wr);
+        }
+    } else {
+        for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+            double wk = (double)w[k];
+#pragma unroll
+            for (int c = 0; c < kNV; ++c)
+                if (c < nv) wk = fma(-hc[c], (double)V[(int64_t)c * n + k], wk);
+            const T wt = (T)wk;
+            w[k] = wt;
+            acc = fma((double)wt, (double)wt, acc);
+        }
     }
     const double s = block_sum(acc, s_red);
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
@@ -313,9 +379,18 @@ void residual(hipStream_t s, int64_t n, const int32_t* rp, const int32_t* col, c
     hipLaunchKernelGGL((k_spmv_lds<T, true>), dim3(g), dim3(kBlock), 0, s, n, rp, col, val, x, b, r, partial);
 }
 
+// 4-wide loads need n % 4 == 0 and 32-byte aligned vector starts
+template <typename T>
+static bool vec4_ok(int64_t n, const void* a, const void* b) {
+    return n % 4 == 0 && ((uintptr_t)a % 32) == 0 && ((uintptr_t)b % 32) == 0;
+}
+
 template <typename T>
 void mdot(hipStream_t s, int64_t n, const T* V, int nv, const T* w, double* partial, int nblk) {
-    hipLaunchKernelGGL((k_mdot<T>), dim3(nblk), dim3(kBlock), 0, s, n, V, nv, w, partial);
+    if (vec4_ok<T>(n, V, w))
+        hipLaunchKernelGGL((k_mdot<T, true>), dim3(nblk), dim3(kBlock), 0, s, n, V, nv, w, partial);
+    else
+        hipLaunchKernelGGL((k_mdot<T, false>), dim3(nblk), dim3(kBlock), 0, s, n, V, nv, w, partial);
 }
 
 void reduce_cols(hipStream_t s, const double* partial, int nblk, int stride, int ncols, double* out) {
@@ -324,7 +399,10 @@ void reduce_cols(hipStream_t s, const double* partial, int nblk, int stride, int
 
 template <typename T>
 void gs_update(hipStream_t s, int64_t n, const T* V, int nv, const double* h, T* w, double* partial, int nblk) {
-    hipLaunchKernelGGL((k_gs_update<T>), dim3(nblk), dim3(kBlock), 0, s, n, V, nv, h, w, partial);
+    if (vec4_ok<T>(n, V, w))
+        hipLaunchKernelGGL((k_gs_update<T, true>), dim3(nblk), dim3(kBlock), 0, s, n, V, nv, h, w, partial);
+    else
+        hipLaunchKernelGGL((k_gs_update<T, false>), dim3(nblk), dim3(kBlock), 0, s, n, V, nv, h, w, partial);
 }
 
 template <typename T>

@@ -83,8 +83,9 @@ __global__ void __launch_bounds__(kBlock) k_apply(int64_t F, int L, const float*
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= F) return;
     const float* y = Y + i * 11;
-    for (int z = 0; z < L; ++z) xs[(int64_t)z * F + i] = fmax(exp((double)y[z]) - kFloor, 0.0);
-    xd[i] = sinh((double)y[10]) / kQscale;
+    if (xs)
+        for (int z = 0; z < L; ++z) xs[(int64_t)z * F + i] = fmax(exp((double)y[z]) - kFloor, 0.0);
+    if (xd) xd[i] = sinh((double)y[10]) / kQscale;
 }
 
 }  // namespace
@@ -94,9 +95,9 @@ void warm_features(pbsm3d_ctx* c, const double* vw_dir, const double* t) {
                        c->csubl.p, c->area.p, c->elen.p, c->nx.p, c->ny.p, c->neigh.p, vw_dir, t, c->feat.p);
 }
 
-void warm_apply(pbsm3d_ctx* c) {
-    hipLaunchKernelGGL(k_apply, dim3(grid_for(c->F)), dim3(kBlock), 0, c->stream, c->F, c->L, c->pred.p, c->susp.x.p,
-                       c->dep.x.p);
+void warm_apply(pbsm3d_ctx* c, int which) {
+    hipLaunchKernelGGL(k_apply, dim3(grid_for(c->F)), dim3(kBlock), 0, c->stream, c->F, c->L, c->pred.p,
+                       which == 0 ? c->susp.x.p : nullptr, which == 1 ? c->dep.x.p : nullptr);
 }
 
 }  // namespace pb

@@ -29,7 +29,21 @@ EXPORTED = (
     "pbsm3d_last_error",
     "pbsm3d_profile_enable",
     "pbsm3d_profile_read",
+    "pbsm3d_mlp_create",
+    "pbsm3d_mlp_destroy",
+    "pbsm3d_mlp_param_count",
+    "pbsm3d_mlp_get_params",
+    "pbsm3d_mlp_set_params",
+    "pbsm3d_mlp_set_normalization",
+    "pbsm3d_mlp_train",
+    "pbsm3d_mlp_predict",
+    "pbsm3d_mlp_gradients",
+    "pbsm3d_mlp_last_error",
+    "pbsm3d_training_pairs",
+    "pbsm3d_set_warm_start",
 )
+
+N_FEATURES, N_TARGETS = 16, 11
 
 KERNEL_CLASSES = ("assemble", "spmv64", "spmv32", "precond", "ortho", "residual", "deposition", "warmstart")
 
@@ -75,6 +89,21 @@ def lib():
         L.pbsm3d_last_error.restype = C.c_char_p
         L.pbsm3d_profile_enable.argtypes = [vp, C.c_int]
         L.pbsm3d_profile_read.argtypes = [vp, C.c_int, C.POINTER(C.c_int64), dp, C.c_int]
+        fp = C.POINTER(C.c_float)
+        L.pbsm3d_mlp_create.argtypes = [C.POINTER(vp), C.c_int, C.c_uint64]
+        L.pbsm3d_mlp_destroy.argtypes = [vp]
+        L.pbsm3d_mlp_destroy.restype = None
+        L.pbsm3d_mlp_param_count.restype = C.c_int64
+        L.pbsm3d_mlp_get_params.argtypes = [vp, fp]
+        L.pbsm3d_mlp_set_params.argtypes = [vp, fp]
+        L.pbsm3d_mlp_set_normalization.argtypes = [vp, fp]
+        L.pbsm3d_mlp_train.argtypes = [vp, vp, vp, C.c_int64, C.c_int32, C.c_int32, C.c_float, C.c_uint64, dp]
+        L.pbsm3d_mlp_predict.argtypes = [vp, vp, vp, C.c_int64]
+        L.pbsm3d_mlp_gradients.argtypes = [vp, fp, fp, C.c_int64, fp, dp]
+        L.pbsm3d_mlp_last_error.argtypes = [vp]
+        L.pbsm3d_mlp_last_error.restype = C.c_char_p
+        L.pbsm3d_training_pairs.argtypes = [vp, vp, vp]
+        L.pbsm3d_set_warm_start.argtypes = [vp, vp]
         _lib = L
     return _lib
 
@@ -176,6 +205,15 @@ class Context:
             out[name] = (int(n.value), float(ms.value))
         return out
 
+    def training_pairs(self, X, Y):
+        """Write features [F x 16] and solved targets [F x 11] of the last step into
+        float32 device tensors X, Y (needs nLayer = 10)."""
+        _check(lib().pbsm3d_training_pairs(self._h, C.c_void_p(X.data_ptr()), C.c_void_p(Y.data_ptr())), self._h)
+
+    def set_warm_start(self, net: "Mlp | None"):
+        self._warm = net  # keep the network alive while attached
+        _check(lib().pbsm3d_set_warm_start(self._h, net._h if net is not None else None), self._h)
+
     def get_state(self):
         a, b = np.zeros(self.F), np.zeros(self.F)
         keep = abi.Keep()
@@ -185,3 +223,67 @@ class Context:
     def set_state(self, sum_drift, sum_subl):
         keep = abi.Keep()
         _check(lib().pbsm3d_set_state(self._h, keep.dptr(sum_drift), keep.dptr(sum_subl)), self._h)
+
+
+class Mlp:
+    """The warm-start network (16 -> 128 -> 128 -> 128 -> 11) living on one GPU."""
+
+    def __init__(self, device: int = 0, seed: int = 0):
+        self._h = C.c_void_p()
+        rc = lib().pbsm3d_mlp_create(C.byref(self._h), device, seed)
+        if rc != 0:
+            raise Pbsm3dError(rc, "pbsm3d_mlp_create failed")
+
+    def _ck(self, rc):
+        if rc != 0:
+            raise Pbsm3dError(rc, lib().pbsm3d_mlp_last_error(self._h).decode(errors="replace"))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pbsm3d_mlp_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def param_count() -> int:
+        return int(lib().pbsm3d_mlp_param_count())
+
+    def get_params(self) -> np.ndarray:
+        p = np.zeros(self.param_count(), dtype=np.float32)
+        self._ck(lib().pbsm3d_mlp_get_params(self._h, p.ctypes.data_as(C.POINTER(C.c_float))))
+        return p
+
+    def set_params(self, p: np.ndarray):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        self._ck(lib().pbsm3d_mlp_set_params(self._h, p.ctypes.data_as(C.POINTER(C.c_float))))
+
+    def set_normalization(self, x_mean, x_std, y_mean, y_std):
+        v = np.concatenate([x_mean, x_std, y_mean, y_std]).astype(np.float32)
+        assert v.size == 2 * N_FEATURES + 2 * N_TARGETS
+        self._ck(lib().pbsm3d_mlp_set_normalization(self._h, v.ctypes.data_as(C.POINTER(C.c_float))))
+
+    def train(self, X, Y, epochs: int = 4, batch: int = 4096, lr: float = 1e-3, seed: int = 0) -> float:
+        """X [n x 16], Y [n x 11] float32 device tensors."""
+        loss = C.c_double(0)
+        self._ck(lib().pbsm3d_mlp_train(self._h, C.c_void_p(X.data_ptr()), C.c_void_p(Y.data_ptr()), X.shape[0],
+                                        epochs, batch, lr, seed, C.byref(loss)))
+        return loss.value
+
+    def predict(self, X, Y):
+        self._ck(lib().pbsm3d_mlp_predict(self._h, C.c_void_p(X.data_ptr()), C.c_void_p(Y.data_ptr()), X.shape[0]))
+
+    def gradients(self, X: np.ndarray, Y: np.ndarray):
+        """(loss, flat gradient) of the normalised MSE on one batch of host arrays."""
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.ascontiguousarray(Y, dtype=np.float32)
+        g = np.zeros(self.param_count(), dtype=np.float32)
+        loss = C.c_double(0)
+        fp = C.POINTER(C.c_float)
+        self._ck(lib().pbsm3d_mlp_gradients(self._h, X.ctypes.data_as(fp), Y.ctypes.data_as(fp), X.shape[0],
+                                            g.ctypes.data_as(fp), C.byref(loss)))
+        return loss.value, g

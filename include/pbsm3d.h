@@ -104,6 +104,10 @@ typedef struct pbsm3d_config {
     int32_t precond;              /* PBSM3D_PC_MCILU0                              */
     int32_t warm_start;           /* PBSM3D_WS_ZERO                                */
     int32_t dep_precond;          /* PBSM3D_PC_MCILU0 (deposition system)          */
+    int32_t ortho_passes;         /* 2: classical Gram-Schmidt with a second pass when the
+                                     norm drops below 1/sqrt(2) (DGKS); 1: one pass  */
+    int32_t block_size;           /* faces per block of the block-multicolour ILU(0)
+                                     (PBSM3D_PC_BMCILU0); 256                       */
 } pbsm3d_config;
 
 typedef struct pbsm3d_forcing {
@@ -206,6 +210,36 @@ int pbsm3d_abi_version(void);
 #define PBSM3D_K_COUNT 8
 int pbsm3d_profile_enable(pbsm3d_ctx* ctx, int enable);
 int pbsm3d_profile_read(pbsm3d_ctx* ctx, int kernel, int64_t* launches, double* ms_total, int reset);
+
+/* Learned warm start (an extension; the reference always starts from x0 = 0,
+ * LinearAlgebra.cpp:199-206).  A per-face network 16 -> 128 -> 128 -> 128 -> 11
+ * (ReLU, BF16 operands, FP32 accumulation on the matrix cores) maps features of
+ * a face and its three neighbours to its 10-layer concentration profile and
+ * deposition flux; with cfg.warm_start = PBSM3D_WS_LEARNED and a network
+ * attached, both solves start from the prediction and still iterate to
+ * ||b - A x|| <= tol ||b||.  The network is trained on the GPU by its own
+ * forward/backward kernels on (features, solution) pairs of FP64 solves.
+ * Parameters are flat f32: W1[128x16], b1, W2[128x128], b2, W3[128x128], b3,
+ * W4[16x128] (rows >= 11 zero), b4[16].  Device arrays unless stated. */
+typedef struct pbsm3d_mlp pbsm3d_mlp;
+int pbsm3d_mlp_create(pbsm3d_mlp** out, int device, uint64_t seed);
+void pbsm3d_mlp_destroy(pbsm3d_mlp* net);
+int64_t pbsm3d_mlp_param_count(void);
+int pbsm3d_mlp_get_params(pbsm3d_mlp* net, float* host_params);
+int pbsm3d_mlp_set_params(pbsm3d_mlp* net, const float* host_params);
+/* host: x_mean[16], x_std[16], y_mean[11], y_std[11] */
+int pbsm3d_mlp_set_normalization(pbsm3d_mlp* net, const float* host_norm);
+/* X [n x 16], Y [n x 11]; mini-batch Adam with MSE on normalised targets */
+int pbsm3d_mlp_train(pbsm3d_mlp* net, const float* X, const float* Y, int64_t n, int32_t epochs, int32_t batch,
+                     float lr, uint64_t seed, double* final_loss);
+int pbsm3d_mlp_predict(pbsm3d_mlp* net, const float* X, float* Y, int64_t n);
+/* host arrays: MSE loss and its gradient w.r.t. all parameters on one batch */
+int pbsm3d_mlp_gradients(pbsm3d_mlp* net, const float* X, const float* Y, int64_t n, float* host_grads, double* loss);
+const char* pbsm3d_mlp_last_error(const pbsm3d_mlp* net);
+/* features [F x 16] and solved targets [F x 11] of the context's last step (nLayer 10) */
+int pbsm3d_training_pairs(pbsm3d_ctx* ctx, float* X, float* Y);
+/* attach (or detach with NULL) the network used when cfg.warm_start = PBSM3D_WS_LEARNED */
+int pbsm3d_set_warm_start(pbsm3d_ctx* ctx, pbsm3d_mlp* net);
 
 #ifdef __cplusplus
 }

@@ -698,6 +698,8 @@ void oracle_config_default(pbsm3d_config* c) {
     c->precond = PBSM3D_PC_ILUT;
     c->warm_start = PBSM3D_WS_ZERO;
     c->dep_precond = PBSM3D_PC_ILUT;
+    c->ortho_passes = 2;
+    c->block_size = 256;
 }
 
 int oracle_create(oracle_ctx** out, const pbsm3d_mesh* m, const pbsm3d_config* cfg) {

@@ -1,0 +1,123 @@
+"""Learned warm start: the network kernels against the FP64 reference MLP, the
+training loop, and what the trained network gains over a cold start."""
+import numpy as np
+import pytest
+
+from chm_amd import abi, meshgen
+from oracle import mlp_oracle
+
+
+def _norm(rng):
+    return (rng.normal(0, 1, 16), rng.uniform(0.5, 2, 16), rng.normal(0, 1, 11), rng.uniform(0.5, 2, 11))
+
+
+def test_reference_mlp_gradient_by_finite_differences():
+    """Pins the FP64 reference: its analytic gradient equals central differences."""
+    rng = np.random.default_rng(0)
+    p = (rng.normal(0, 0.2, mlp_oracle.param_count())).astype(np.float32)
+    p[-16 + 11:] = 0
+    X = rng.normal(0, 1, (8, 16))
+    Y = rng.normal(0, 1, (8, 11))
+    norm = _norm(rng)
+    loss, g = mlp_oracle.loss_and_grad(p, X, Y, norm, bf16=False)
+    p64 = p.astype(np.float64)
+    for k in rng.choice(p.size - 16, 40, replace=False):
+        e = 1e-4
+
+        def f(v):
+            q = p64.copy()
+            q[k] = v
+            return mlp_oracle.loss_and_grad(q, X, Y, norm, bf16=False)[0]
+
+        fd = (f(p64[k] + e) - f(p64[k] - e)) / (2 * e)
+        assert abs(fd - g[k]) <= 1e-6 + 1e-4 * abs(fd), (k, fd, g[k])
+
+
+def test_bf16_rounding_is_nearest_even():
+    v = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, -2.5, 3.14159265], dtype=np.float32)
+    r = mlp_oracle.bf16_round(v)
+    assert r[0] == 1.0 and r[1] == 1.0 and r[2] == 1.0 + 2 ** -6 and r[3] == -2.5
+    assert abs(r[4] - 3.140625) < 1e-12
+
+
+@pytest.mark.gpu
+def test_network_gradients_match_reference():
+    from chm_amd import pbsm3d as pb
+
+    print("[case] mlp gradients", flush=True)
+    rng = np.random.default_rng(1)
+    net = pb.Mlp(seed=3)
+    p = net.get_params()
+    assert p.size == mlp_oracle.param_count() == pb.Mlp.param_count()
+    assert np.all(p[-5:] == 0)
+    norm = _norm(rng)
+    net.set_normalization(*norm)
+    n = 777  # not a tile multiple: exercises the edge handling
+    X = rng.normal(0, 1.5, (n, 16)).astype(np.float32) * norm[1] + norm[0]
+    Y = rng.normal(0, 1, (n, 11)).astype(np.float32)
+    loss, g = net.gradients(X, Y)
+    rloss, rg = mlp_oracle.loss_and_grad(p, X, Y, norm, bf16=True)
+    assert abs(loss - rloss) <= 1e-3 * rloss
+    o = 0
+    for (r, c) in mlp_oracle.SHAPES:
+        for size in (r * c, r):
+            a, b = g[o:o + size], rg[o:o + size]
+            assert np.linalg.norm(a - b) <= 2e-2 * np.linalg.norm(b) + 1e-7, (o, size)
+            o += size
+    # forward agrees with the reference
+    import torch
+
+    Xd = torch.tensor(X, device="cuda:0")
+    P = torch.empty((n, 11), dtype=torch.float32, device="cuda:0")
+    net.predict(Xd, P)
+    ref = mlp_oracle.predict(p, X, norm, bf16=True)
+    assert np.abs(P.cpu().numpy() - ref).max() <= 1e-2 * np.abs(ref).max()
+
+
+@pytest.mark.gpu
+def test_training_reduces_loss_on_a_known_function():
+    """Fits y = smooth function of x; the loss must fall by an order of magnitude."""
+    import torch
+
+    from chm_amd import pbsm3d as pb
+
+    print("[case] mlp training", flush=True)
+    g = torch.Generator(device="cuda:0").manual_seed(0)
+    X = torch.randn((65536, 16), device="cuda:0", generator=g)
+    W = torch.randn((16, 11), device="cuda:0", generator=g) / 4
+    Y = torch.tanh(X @ W) + 0.1 * (X[:, :11] ** 2)
+    net = pb.Mlp(seed=1)
+    net.set_normalization(np.zeros(16), np.ones(16), np.zeros(11), np.ones(11))
+    l0, _ = net.gradients(X[:4096].cpu().numpy(), Y[:4096].cpu().numpy())
+    loss = net.train(X, Y, epochs=8, batch=2048, lr=2e-3, seed=2)
+    assert loss < 0.1 * l0, (l0, loss)
+
+
+@pytest.mark.gpu
+def test_warm_start_gain_c1():
+    """C1 mesh at 10 layers: train on FP64 solves of seeded forcing draws, then
+    compare iterations against the cold start on a held-out draw; both reach
+    the same tolerance and the fields agree."""
+    from chm_amd import pbsm3d as pb
+    from chm_amd import warmstart
+
+    print("[case] warm start C1", flush=True)
+    mesh = meshgen.synthetic_mesh(2500, seed=0)
+    geom = pb.face_geometry(mesh)
+    F = mesh.n_faces
+    draws = [meshgen.forcing_c2(F, seed=100 + k) for k in range(48)]
+    X, Y = warmstart.collect_pairs(geom, draws)
+    net, info = warmstart.fit(X, Y, epochs=10, batch=2048, lr=2e-3)
+    print("[warm] train", info, flush=True)
+    held = meshgen.forcing_c2(F, seed=7)
+    cold = pb.Context(geom, pb.default_config(nLayer=10))
+    warm = pb.Context(geom, pb.default_config(nLayer=10, warm_start=abi.WS_LEARNED))
+    warm.set_warm_start(net)
+    oc, sc = cold.step(held, want_fields=True)
+    ow, sw = warm.step(held, want_fields=True)
+    print("[warm] iters cold", sc["susp_iters"], sc["dep_iters"], "warm", sw["susp_iters"], sw["dep_iters"], flush=True)
+    assert sw["susp_resid"] <= 1e-8 and sw["dep_resid"] <= 1e-8
+    assert sw["susp_iters"] < sc["susp_iters"]
+    for k in ("Qsusp", "drift_mass"):
+        d = np.abs(ow[k] - oc[k]).max() / np.abs(oc[k]).max()
+        assert d < 1e-3, (k, d)
