@@ -22,6 +22,7 @@ MIXED = 1
 
 PC_NONE, PC_JACOBI, PC_COLUMN, PC_ILU0, PC_ILUT, PC_MCILU0 = range(6)
 WS_ZERO, WS_LEARNED = 0, 1
+KRYLOV_GMRES, KRYLOV_BICGSTAB = 0, 1
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -73,6 +74,7 @@ class Config(C.Structure):
         ("dep_precond", C.c_int32),
         ("ortho_passes", C.c_int32),
         ("block_size", C.c_int32),
+        ("krylov", C.c_int32),
     ]
 
 
@@ -160,6 +162,7 @@ def default_config(**overrides) -> Config:
         dep_precond=PC_MCILU0,
         ortho_passes=2,
         block_size=256,
+        krylov=KRYLOV_GMRES,
     )
     for k, v in overrides.items():
         if not hasattr(c, k):

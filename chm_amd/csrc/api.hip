@@ -130,6 +130,8 @@ void validate(const pbsm3d_mesh* m, const pbsm3d_config* c) {
     if (c->use_exp_fetch && c->use_tanh_fetch) throw Error(PBSM3D_EINVAL, "Cannot specify both exp_fetch and tanh_fetch");
     if (c->restart < 1 || c->restart > 31) throw Error(PBSM3D_EINVAL, "restart must be in [1, 31]");
     if (c->ortho_passes != 1 && c->ortho_passes != 2) throw Error(PBSM3D_EINVAL, "ortho_passes must be 1 or 2");
+    if (c->krylov != PBSM3D_KRYLOV_GMRES && c->krylov != PBSM3D_KRYLOV_BICGSTAB)
+        throw Error(PBSM3D_EINVAL, "unknown Krylov method");
     if (c->max_iters < 1) throw Error(PBSM3D_EINVAL, "max_iters must be >= 1");
     if (!(c->tol > 0)) throw Error(PBSM3D_EINVAL, "tol must be > 0");
     if (c->precision != PBSM3D_FP64 && c->precision != PBSM3D_MIXED) throw Error(PBSM3D_EINVAL, "unknown precision");
@@ -198,6 +200,7 @@ void pbsm3d_config_default(pbsm3d_config* c) {
     c->dep_precond = PBSM3D_PC_MCILU0;
     c->ortho_passes = 2;
     c->block_size = 256;
+    c->krylov = PBSM3D_KRYLOV_GMRES;
 }
 
 int pbsm3d_face_geometry(int64_t F, const double* v, const int32_t* elem, const double* area_param, double* nx,
@@ -341,13 +344,16 @@ int pbsm3d_create(pbsm3d_ctx** out, const pbsm3d_mesh* m, const pbsm3d_config* c
         c->invdiag_d.alloc(F);
         c->r_s.alloc(N);
         c->r_d.alloc(F);
+        const bool bicg = cfg->krylov == PBSM3D_KRYLOV_BICGSTAB;
         if (cfg->precision == PBSM3D_MIXED) {
             c->val32.alloc(c->susp.nnz);
             c->r32.alloc(N);
             c->d32.alloc(N);
-            c->ws32.ensure(N, cfg->restart);
+            c->ws32.ensure(N, bicg ? 1 : cfg->restart);  // BiCGStab uses it for residuals only
+            if (bicg) c->bw32.ensure(N);
         } else {
-            c->ws64.ensure(N, cfg->restart);
+            c->ws64.ensure(N, bicg ? 1 : cfg->restart);
+            if (bicg) c->bw64.ensure(N);
         }
         c->wd64.ensure(F, cfg->restart);
         c->ws64.passes = c->ws32.passes = c->wd64.passes = cfg->ortho_passes;
@@ -492,8 +498,11 @@ int pbsm3d_step(pbsm3d_ctx* c, const pbsm3d_forcing* fo, int32_t dt_s, pbsm3d_ou
                     M = [N](hipStream_t st2, const double* r, double* z) {
                         PB_HIP(hipMemcpyAsync(z, r, N * sizeof(double), hipMemcpyDeviceToDevice, st2));
                     };
-                si = gmres_solve(s, c->ws64, A, M, c->susp.rhs.p, c->susp.x.p, c->r_s.p, cfg.restart, cfg.max_iters,
-                                 cfg.tol);
+                if (cfg.krylov == PBSM3D_KRYLOV_BICGSTAB)
+                    si = bicgstab_solve(s, c->bw64, c->ws64, A, M, c->susp.rhs.p, c->susp.x.p, cfg.max_iters, cfg.tol);
+                else
+                    si = gmres_solve(s, c->ws64, A, M, c->susp.rhs.p, c->susp.x.p, c->r_s.p, cfg.restart,
+                                     cfg.max_iters, cfg.tol);
             } else {
                 to_f32(s, c->susp.nnz, c->susp.val.p, c->val32.p);
                 CsrOp<float> A32{N, c->susp.rp.p, c->susp.col.p, c->val32.p, &c->prof};
@@ -516,8 +525,12 @@ int pbsm3d_step(pbsm3d_ctx* c, const pbsm3d_forcing* fo, int32_t dt_s, pbsm3d_ou
                     M = [N](hipStream_t st2, const float* r, float* z) {
                         PB_HIP(hipMemcpyAsync(z, r, N * sizeof(float), hipMemcpyDeviceToDevice, st2));
                     };
-                si = mixed_solve(s, c->ws32, A, A32, M, c->susp.rhs.p, c->susp.x.p, c->r_s.p, c->r32.p, c->d32.p,
-                                 cfg.restart, cfg.max_iters, cfg.tol, 1e-4);
+                if (cfg.krylov == PBSM3D_KRYLOV_BICGSTAB)
+                    si = bicgstab_mixed(s, c->bw32, c->ws32, A, A32, M, c->susp.rhs.p, c->susp.x.p, c->r_s.p, c->r32.p,
+                                        c->d32.p, cfg.max_iters, cfg.tol, 1e-5);
+                else
+                    si = mixed_solve(s, c->ws32, A, A32, M, c->susp.rhs.p, c->susp.x.p, c->r_s.p, c->r32.p, c->d32.p,
+                                     cfg.restart, cfg.max_iters, cfg.tol, 1e-4);
             }
             stats.susp_iters = si.iters;
             stats.susp_resid = si.resid;

@@ -34,6 +34,8 @@ struct pbsm3d_ctx {
     pb::DevBuf<float> val32, r32, d32;
     pb::GmresWork<double> ws64, wd64;
     pb::GmresWork<float> ws32;
+    pb::BicgWork<double> bw64;
+    pb::BicgWork<float> bw32;
     pb::DevBuf<double> red_partial, red_out;
     pb::HostBuf<double> red_host;
     // learned warm start

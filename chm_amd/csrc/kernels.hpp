@@ -134,6 +134,16 @@ void dilu_factor(hipStream_t s, const ColorOrder& co, int64_t F, int L, const in
 template <typename T>
 void dilu_apply(hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFactors<T>& d, const T* r, T* z);
 
+// BiCGStab vector updates (FP64 arithmetic on T storage)
+template <typename T>
+void bicg_p(hipStream_t s, int64_t n, const T* r, const T* v, double beta, double omega, T* p);  // p = r + b(p - w v)
+// out = y - a x; partial[2*blk] = rh.out (if rh), partial[2*blk+1] = out.out
+template <typename T>
+void axpy_dots(hipStream_t s, int64_t n, double a, const T* x, const T* y, T* out, const T* rh, double* partial,
+               int nblk);
+template <typename T>
+void x_update(hipStream_t s, int64_t n, double alpha, const T* ph, double omega, const T* sh, T* x);  // x += a ph + w sh
+
 // conversions / vector updates
 template <typename T>
 void scale_host(hipStream_t s, int64_t n, const T* a, double alpha, T* b);  // b = alpha * a

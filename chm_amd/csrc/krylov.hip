@@ -222,6 +222,184 @@ SolveInfo mixed_solve(hipStream_t s, GmresWork<float>& w, const CsrOp<double>& A
     return info;
 }
 
+// ---------------------------------------------------------------------------
+// BiCGStab
+// ---------------------------------------------------------------------------
+template <typename T>
+void BicgWork<T>::ensure(int64_t n_) {
+    if (n == n_ && r.p) return;
+    n = n_;
+    for (DevBuf<T>* b : {&r, &rh, &p, &v, &s, &t, &ph, &sh}) b->alloc(n);
+    partial.alloc(4 * (size_t)kReduceBlocks);
+    dsc.alloc(8);
+    hsc.alloc(8);
+}
+
+namespace {
+// one BiCGStab run on A x = b from the x given; stops when the implicit
+// residual drops to stop_abs or after maxit iterations; returns iterations
+template <typename T>
+int bicg_run(hipStream_t s, BicgWork<T>& w, const CsrOp<T>& A, const PcFn<T>& M, const T* b, T* x, bool x_zero,
+             double stop_abs, int maxit, double* res_out) {
+    const int64_t n = A.n;
+    const int nblk = reduce_blocks(n);
+    const int spmv_id = sizeof(T) == 8 ? PBSM3D_K_SPMV64 : PBSM3D_K_SPMV32;
+    double* d = w.dsc.p;
+    double* h = w.hsc.p;
+    auto fetch = [&](int cnt) {
+        PB_HIP(hipMemcpyAsync(h, d, sizeof(double) * cnt, hipMemcpyDeviceToHost, s));
+        PB_HIP(hipStreamSynchronize(s));
+    };
+    // r = b - A x, rh = r
+    if (x_zero) {
+        PB_HIP(hipMemcpyAsync(w.r.p, b, n * sizeof(T), hipMemcpyDeviceToDevice, s));
+    } else {
+        {
+            ProfScope ps(A.prof, s, spmv_id);
+            spmv<T>(s, n, A.rp, A.col, A.val, x, w.v.p);
+        }
+        axpy_dots<T>(s, n, 1.0, w.v.p, b, w.r.p, nullptr, w.partial.p, nblk);
+    }
+    PB_HIP(hipMemcpyAsync(w.rh.p, w.r.p, n * sizeof(T), hipMemcpyDeviceToDevice, s));
+    mdot<T>(s, n, w.rh.p, 1, w.r.p, w.partial.p, nblk);  // (rh.r, r.r)
+    reduce_cols(s, w.partial.p, nblk, 2, 2, d);
+    fetch(2);
+    double rho_new = h[0], rr = std::sqrt(h[1]);
+    *res_out = rr;
+    if (rr <= stop_abs) return 0;
+    double rho = 1, alpha = 1, omega = 1;
+    int it = 0;
+    while (it < maxit) {
+        ++it;
+        if (it == 1) {
+            PB_HIP(hipMemcpyAsync(w.p.p, w.r.p, n * sizeof(T), hipMemcpyDeviceToDevice, s));
+        } else {
+            const double beta = (rho_new / rho) * (alpha / omega);
+            ProfScope ps(A.prof, s, PBSM3D_K_ORTHO);
+            bicg_p<T>(s, n, w.r.p, w.v.p, beta, omega, w.p.p);
+        }
+        rho = rho_new;
+        {
+            ProfScope ps(A.prof, s, PBSM3D_K_PRECOND);
+            M(s, w.p.p, w.ph.p);
+        }
+        {
+            ProfScope ps(A.prof, s, spmv_id);
+            spmv<T>(s, n, A.rp, A.col, A.val, w.ph.p, w.v.p);
+        }
+        {
+            ProfScope ps(A.prof, s, PBSM3D_K_ORTHO);
+            mdot<T>(s, n, w.rh.p, 1, w.v.p, w.partial.p, nblk);  // (rh.v, v.v)
+            reduce_cols(s, w.partial.p, nblk, 2, 1, d);
+        }
+        fetch(1);
+        const double rhv = h[0];
+        if (rhv == 0.0 || !std::isfinite(rhv)) break;  // breakdown: caller restarts
+        alpha = rho / rhv;
+        {
+            ProfScope ps(A.prof, s, PBSM3D_K_ORTHO);
+            axpy_dots<T>(s, n, alpha, w.v.p, w.r.p, w.s.p, nullptr, w.partial.p, nblk);  // s = r - a v
+            reduce_cols(s, w.partial.p, nblk, 2, 2, d + 2);                                // d[3] = s.s
+        }
+        {
+            ProfScope ps(A.prof, s, PBSM3D_K_PRECOND);
+            M(s, w.s.p, w.sh.p);
+        }
+        {
+            ProfScope ps(A.prof, s, spmv_id);
+            spmv<T>(s, n, A.rp, A.col, A.val, w.sh.p, w.t.p);
+        }
+        {
+            ProfScope ps(A.prof, s, PBSM3D_K_ORTHO);
+            mdot<T>(s, n, w.s.p, 1, w.t.p, w.partial.p, nblk);  // (s.t, t.t)
+            reduce_cols(s, w.partial.p, nblk, 2, 2, d);
+        }
+        fetch(4);
+        const double st = h[0], tt = h[1], ss = std::sqrt(h[3]);
+        if (ss <= stop_abs || tt == 0.0) {
+            x_update<T>(s, n, alpha, w.ph.p, 0.0, w.sh.p, x);
+            *res_out = ss;
+            return it;
+        }
+        omega = st / tt;
+        {
+            ProfScope ps(A.prof, s, PBSM3D_K_ORTHO);
+            x_update<T>(s, n, alpha, w.ph.p, omega, w.sh.p, x);
+            axpy_dots<T>(s, n, omega, w.t.p, w.s.p, w.r.p, w.rh.p, w.partial.p, nblk);  // r = s - w t
+            reduce_cols(s, w.partial.p, nblk, 2, 2, d);
+        }
+        fetch(2);
+        rho_new = h[0];
+        rr = std::sqrt(h[1]);
+        *res_out = rr;
+        if (rr <= stop_abs || omega == 0.0 || !std::isfinite(rr)) return it;
+    }
+    return it;
+}
+}  // namespace
+
+SolveInfo bicgstab_solve(hipStream_t s, BicgWork<double>& w, GmresWork<double>& gw, const CsrOp<double>& A,
+                         const PcFn<double>& M, const double* b, double* x, int maxit, double tol) {
+    SolveInfo info;
+    w.ensure(A.n);
+    GmresWork<double>& dummy = gw;
+    const double nb = norm_of(s, dummy, A.n, b);
+    if (nb == 0.0) {
+        PB_HIP(hipMemsetAsync(x, 0, sizeof(double) * A.n, s));
+        info.converged = true;
+        return info;
+    }
+    double beta = residual_norm(s, dummy, A, x, b, w.r.p);
+    while (true) {
+        info.resid = beta / nb;
+        if (info.resid <= tol) {
+            info.converged = true;
+            break;
+        }
+        if (info.iters >= maxit) break;
+        double est = 0;
+        info.iters += bicg_run<double>(s, w, A, M, b, x, false, 0.9 * tol * nb, maxit - info.iters, &est);
+        info.cycles += 1;
+        beta = residual_norm(s, dummy, A, x, b, w.r.p);
+    }
+    return info;
+}
+
+SolveInfo bicgstab_mixed(hipStream_t s, BicgWork<float>& w, GmresWork<float>& gw, const CsrOp<double>& A64,
+                         const CsrOp<float>& A32, const PcFn<float>& M32, const double* b, double* x, double* r,
+                         float* r32, float* d32, int maxit, double tol, double inner_rtol) {
+    SolveInfo info;
+    w.ensure(A64.n);
+    const double nb = norm_of(s, gw, A64.n, b);
+    if (nb == 0.0) {
+        PB_HIP(hipMemsetAsync(x, 0, sizeof(double) * A64.n, s));
+        info.converged = true;
+        return info;
+    }
+    double beta = residual_norm(s, gw, A64, x, b, r);
+    while (true) {
+        info.resid = beta / nb;
+        if (info.resid <= tol) {
+            info.converged = true;
+            break;
+        }
+        if (info.iters >= maxit) break;
+        scale_f64_to_f32_host(s, A64.n, r, 1.0 / beta, r32);
+        PB_HIP(hipMemsetAsync(d32, 0, A64.n * sizeof(float), s));
+        double stop = 0.5 * tol * nb / beta;
+        if (stop < inner_rtol) stop = inner_rtol;
+        double est = 0;
+        info.iters += bicg_run<float>(s, w, A32, M32, r32, d32, true, stop, maxit - info.iters, &est);
+        axpy_f32_into_f64(s, A64.n, beta, d32, x);
+        info.cycles += 1;
+        info.refinements += 1;
+        beta = residual_norm(s, gw, A64, x, b, r);
+    }
+    return info;
+}
+
+template struct BicgWork<double>;
+template struct BicgWork<float>;
 template struct GmresWork<double>;
 template struct GmresWork<float>;
 template int gmres_cycle<double>(hipStream_t, GmresWork<double>&, const CsrOp<double>&, const PcFn<double>&,

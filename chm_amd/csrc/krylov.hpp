@@ -67,4 +67,23 @@ SolveInfo mixed_solve(hipStream_t s, GmresWork<float>& w, const CsrOp<double>& A
 // device norm helpers (synchronous: return to host)
 double dev_norm2(hipStream_t s, GmresWork<double>& w, int64_t n, const double* v);
 
+// BiCGStab (van der Vorst 1992), right-preconditioned, as an alternative
+// Krylov method: two SpMV + two preconditioner applications per iteration and
+// a fixed set of vectors, so no Gram-Schmidt cost growing with the iteration.
+template <typename T>
+struct BicgWork {
+    int64_t n = 0;
+    DevBuf<T> r, rh, p, v, s, t, ph, sh;
+    DevBuf<double> partial, dsc;
+    HostBuf<double> hsc;
+    void ensure(int64_t n_);
+};
+
+SolveInfo bicgstab_solve(hipStream_t s, BicgWork<double>& w, GmresWork<double>& gw, const CsrOp<double>& A,
+                         const PcFn<double>& M, const double* b, double* x, int maxit, double tol);
+// FP64 refinement around FP32 BiCGStab runs (FP64 residuals, FP64 dot accumulation)
+SolveInfo bicgstab_mixed(hipStream_t s, BicgWork<float>& w, GmresWork<float>& gw, const CsrOp<double>& A64,
+                         const CsrOp<float>& A32, const PcFn<float>& M32, const double* b, double* x, double* r,
+                         float* r32, float* d32, int maxit, double tol, double inner_rtol);
+
 }  // namespace pb

@@ -329,6 +329,42 @@ __global__ void __launch_bounds__(kBlock) k_pc_jacobi(int64_t n, const double* _
 }
 
 template <typename T>
+__global__ void __launch_bounds__(kBlock) k_bicg_p(int64_t n, const T* __restrict__ r, const T* __restrict__ v,
+                                                   double beta, double omega, T* __restrict__ p) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = (T)fma(beta, fma(-omega, (double)v[k], (double)p[k]), (double)r[k]);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_axpy_dots(int64_t n, double a, const T* __restrict__ x,
+                                                      const T* __restrict__ y, T* __restrict__ out,
+                                                      const T* __restrict__ rh, double* __restrict__ partial) {
+    __shared__ double s_red[kBlock / 64];
+    double d1 = 0, d2 = 0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+        const T o = (T)fma(-a, (double)x[k], (double)y[k]);
+        out[k] = o;
+        const double od = (double)o;
+        if (rh) d1 = fma((double)rh[k], od, d1);
+        d2 = fma(od, od, d2);
+    }
+    const double s1 = block_sum(d1, s_red);
+    const double s2 = block_sum(d2, s_red);
+    if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x] = s1;
+        partial[2 * blockIdx.x + 1] = s2;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_x_update(int64_t n, double alpha, const T* __restrict__ ph, double omega,
+                                                     const T* __restrict__ sh, T* __restrict__ x) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) x[k] = (T)fma(omega, (double)sh[k], fma(alpha, (double)ph[k], (double)x[k]));
+}
+
+template <typename T>
 __global__ void k_scale_host(int64_t n, const T* a, double alpha, T* b) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n) b[k] = (T)((double)a[k] * alpha);
@@ -441,6 +477,20 @@ void pc_jacobi(hipStream_t s, int64_t n, const double* invdiag, const T* r, T* z
 }
 
 template <typename T>
+void bicg_p(hipStream_t s, int64_t n, const T* r, const T* v, double beta, double omega, T* p) {
+    hipLaunchKernelGGL((k_bicg_p<T>), dim3(grid_for(n)), dim3(kBlock), 0, s, n, r, v, beta, omega, p);
+}
+template <typename T>
+void axpy_dots(hipStream_t s, int64_t n, double a, const T* x, const T* y, T* out, const T* rh, double* partial,
+               int nblk) {
+    hipLaunchKernelGGL((k_axpy_dots<T>), dim3(nblk), dim3(kBlock), 0, s, n, a, x, y, out, rh, partial);
+}
+template <typename T>
+void x_update(hipStream_t s, int64_t n, double alpha, const T* ph, double omega, const T* sh, T* x) {
+    hipLaunchKernelGGL((k_x_update<T>), dim3(grid_for(n)), dim3(kBlock), 0, s, n, alpha, ph, omega, sh, x);
+}
+
+template <typename T>
 void scale_host(hipStream_t s, int64_t n, const T* a, double alpha, T* b) {
     hipLaunchKernelGGL((k_scale_host<T>), dim3(grid_for(n)), dim3(kBlock), 0, s, n, a, alpha, b);
 }
@@ -474,7 +524,10 @@ void copy_f64(hipStream_t s, int64_t n, const double* a, double* b) {
     template void combine<T>(hipStream_t, int64_t, const T*, int, const double*, T*);                              \
     template void pc_column<T>(hipStream_t, int64_t, int, const T*, const T*, const T*, const T*, T*);             \
     template void pc_jacobi<T>(hipStream_t, int64_t, const double*, const T*, T*);                                \
-    template void scale_host<T>(hipStream_t, int64_t, const T*, double, T*);
+    template void scale_host<T>(hipStream_t, int64_t, const T*, double, T*);                                       \
+    template void bicg_p<T>(hipStream_t, int64_t, const T*, const T*, double, double, T*);                         \
+    template void axpy_dots<T>(hipStream_t, int64_t, double, const T*, const T*, T*, const T*, double*, int);      \
+    template void x_update<T>(hipStream_t, int64_t, double, const T*, double, const T*, T*);
 PB_INST(double)
 PB_INST(float)
 #undef PB_INST

@@ -106,9 +106,13 @@ typedef struct pbsm3d_config {
     int32_t dep_precond;          /* PBSM3D_PC_MCILU0 (deposition system)          */
     int32_t ortho_passes;         /* 2: classical Gram-Schmidt with a second pass when the
                                      norm drops below 1/sqrt(2) (DGKS); 1: one pass  */
-    int32_t block_size;           /* faces per block of the block-multicolour ILU(0)
-                                     (PBSM3D_PC_BMCILU0); 256                       */
+    int32_t block_size;           /* reserved (256)                                */
+    int32_t krylov;               /* PBSM3D_KRYLOV_GMRES (reference) or _BICGSTAB  */
 } pbsm3d_config;
+
+/* Krylov method of the suspension solve (the deposition solve is GMRES(restart)) */
+#define PBSM3D_KRYLOV_GMRES 0     /* restarted GMRES(restart), as Belos "GMRES"     */
+#define PBSM3D_KRYLOV_BICGSTAB 1  /* BiCGStab: short recurrences, 2 SpMV/iteration  */
 
 typedef struct pbsm3d_forcing {
     const double* U_R;            /* [F] m/s at Z_U_R = 50 m                       */

@@ -88,15 +88,17 @@ def test_dilu_matches_oracle_dilu(pb, oracle, mesh_c1):
             assert rel_err(gg[k], og[k]) < 1e-6, k
 
 
-@pytest.mark.parametrize("precision,precond", [(abi.FP64, abi.PC_MCILU0), (abi.MIXED, abi.PC_MCILU0),
-                                               (abi.FP64, abi.PC_COLUMN), (abi.FP64, abi.PC_JACOBI),
-                                               (abi.MIXED, abi.PC_COLUMN)])
-def test_fields_c1(pb, oracle, mesh_c1, precision, precond):
-    _print_case("fields_c1", precision=precision, precond=precond)
+@pytest.mark.parametrize("precision,precond,krylov", [
+    (abi.FP64, abi.PC_MCILU0, abi.KRYLOV_GMRES), (abi.MIXED, abi.PC_MCILU0, abi.KRYLOV_GMRES),
+    (abi.FP64, abi.PC_MCILU0, abi.KRYLOV_BICGSTAB), (abi.MIXED, abi.PC_MCILU0, abi.KRYLOV_BICGSTAB),
+    (abi.FP64, abi.PC_COLUMN, abi.KRYLOV_GMRES), (abi.FP64, abi.PC_JACOBI, abi.KRYLOV_GMRES),
+    (abi.MIXED, abi.PC_COLUMN, abi.KRYLOV_GMRES)])
+def test_fields_c1(pb, oracle, mesh_c1, precision, precond, krylov):
+    _print_case("fields_c1", precision=precision, precond=precond, krylov=krylov)
     F = mesh_c1.n_faces
     fo = meshgen.forcing_c1(F, seed=0)
     # arithmetic parity: both sides solved far below the production tolerance
-    g, ctx, orc = _pair(pb, oracle, mesh_c1, L=5, precision=precision, precond=precond, tol=1e-12)
+    g, ctx, orc = _pair(pb, oracle, mesh_c1, L=5, precision=precision, precond=precond, tol=1e-12, krylov=krylov)
     og, _ = orc.step(fo)
     gg, gst = ctx.step(fo, want_fields=True)
     assert gst["susp_resid"] <= 1e-12 and gst["dep_resid"] <= 1e-12
@@ -105,7 +107,7 @@ def test_fields_c1(pb, oracle, mesh_c1, precision, precond):
     np.testing.assert_array_equal(gg["pbsm_more_than_avail"], og["pbsm_more_than_avail"])
     # production tolerance: the explicit residuals meet 1e-8 and the fields sit within the
     # forward-error band kappa*tol of the tight solution (kappa_dep ~ 1e4 on this mesh)
-    ctx8 = pb.Context(g, pb.default_config(nLayer=5, precision=precision, precond=precond, tol=1e-8))
+    ctx8 = pb.Context(g, pb.default_config(nLayer=5, precision=precision, precond=precond, tol=1e-8, krylov=krylov))
     g8, st8 = ctx8.step(fo, want_fields=True)
     assert st8["susp_resid"] <= 1e-8 and st8["dep_resid"] <= 1e-8
     for k, bound in (("conc", 1e-4), ("Qsusp", 1e-4), ("qdep", 1e-3), ("drift_mass", 1e-3)):
