@@ -2,18 +2,21 @@
 """PBSM3D blowing-snow step throughput on one MI355X.
 
 Workload: the production-size synthetic case -- 500,000 seeded uniform points
-in a 10 x 10 km square -> Delaunay (about 1M triangles), 10 suspension layers
-(about 10M unknowns), per-face lognormal 2 m wind in [4, 20] m/s, direction
-N(270, 20) deg, T ~ U(-25, -2) C, one 900 s step solved to relative residual
+in a 10 x 10 km square -> Delaunay (999,968 triangles), 10 suspension layers
+(9,999,680 unknowns), per-face lognormal 2 m wind in [4, 20] m/s, direction
+N(270, 20) deg, T ~ U(-25, -2) C; one 900 s step solved to relative residual
 1e-8.  Forcing and outputs stay resident in HBM; a step is one full
 pbsm3d_step (coefficients, suspension assembly + solve, Qsusp/Qsubl,
 deposition assembly + solve, drift mass), as the reference's run(mesh&).
 
-Two variants are timed back to back on the same mesh and forcing:
-  plain  : FP64 GMRES(30), column-tridiagonal preconditioner, x0 = 0
-  mixed  : FP64 refinement around FP32 GMRES(30) cycles (+ learned warm start
-           when a trained network is supplied)
-`value` is the mixed variant's steps/s; the plain variant is reported beside it.
+Variants, timed back to back on the same mesh and forcing:
+  plain          FP64 GMRES(30) + multicolour D-ILU(0), x0 = 0 (the reference's
+                 algorithm on the GPU)
+  mixed          FP32 SpMV / preconditioner / Krylov vectors inside FP64
+                 refinement (FP64 residuals and dot accumulation)
+  mixed_learned  mixed + the learned warm start; the network is trained during
+                 set-up (not timed) on FP64 solves of other seeded forcing draws
+`value` is mixed_learned's steps/s; `speedup_vs_plain` divides it by plain's.
 
 Prints ONE JSON line.  --gpus must be 1 (one GPU per box).
 """
@@ -41,8 +44,24 @@ def log(msg: str):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_variant(pb, abi, g, dev_forcing, dev_outs, cfg, steps, warmup, torch):
+def variant_config(pb, abi, name: str, L: int, krylov_mixed: int):
+    if name == "plain":
+        return pb.default_config(nLayer=L, precision=abi.FP64), False
+    if name == "plain_bicgstab":
+        return pb.default_config(nLayer=L, precision=abi.FP64, krylov=abi.KRYLOV_BICGSTAB), False
+    if name == "mixed":
+        return pb.default_config(nLayer=L, precision=abi.MIXED, krylov=krylov_mixed), False
+    if name == "mixed_learned":
+        return pb.default_config(nLayer=L, precision=abi.MIXED, krylov=krylov_mixed, warm_start=abi.WS_LEARNED), True
+    if name == "learned":
+        return pb.default_config(nLayer=L, precision=abi.FP64, warm_start=abi.WS_LEARNED), True
+    raise SystemExit(f"unknown variant {name}")
+
+
+def run_variant(pb, g, dev_forcing, dev_outs, cfg, net, steps, warmup, torch):
     ctx = pb.Context(g, cfg)
+    if net is not None:
+        ctx.set_warm_start(net)
     for _ in range(warmup):
         ctx.step(dev_forcing, outs=dev_outs, on_device=True)
     torch.cuda.synchronize()
@@ -64,7 +83,7 @@ def run_variant(pb, abi, g, dev_forcing, dev_outs, cfg, steps, warmup, torch):
 
 
 def cpu_baseline(g_full, forcing, n_layer, faces, threads):
-    """Times the CPU oracle (GMRES(30) + ILUT(1e-4, 3), as the reference's solver
+    """Times the CPU oracle (GMRES(30) + ILUT(1e-4, 3), the reference's solver
     setup) on a Hilbert-contiguous patch of the same mesh; steps/s is scaled
     to the full face count by the face ratio."""
     import numpy as np
@@ -91,9 +110,25 @@ def cpu_baseline(g_full, forcing, n_layer, faces, threads):
         "cores": threads,
         "kind": "port",
         "sample": (f"one 900 s step of the CPU FP64 restatement (GMRES(30) + ILUT(1e-4, fill 3), tol 1e-8) on a "
-                   f"{fs}-face Hilbert-contiguous patch of the same mesh x {n_layer} layers, {dt:.1f} s, "
-                   f"scaled by faces {fs}/{F}; susp/dep iterations {st['susp_iters']}/{st['dep_iters']}"),
+                   f"{fs}-face Hilbert-contiguous patch of the same mesh x {n_layer} layers, {dt:.1f} s on {threads} "
+                   f"threads, scaled by faces {fs}/{F}; susp/dep iterations {st['susp_iters']}/{st['dep_iters']}"),
     }
+
+
+def train_network(pb, abi, meshgen, warmstart, g, F, draws, samples, epochs, torch):
+    """FP64 GPU solves of seeded forcing draws (seeds 1..draws, disjoint from the
+    timed forcing's seed 0) -> (features, solution) pairs -> GPU training."""
+    t0 = time.perf_counter()
+    forcings = [meshgen.forcing_c2(F, seed=1 + k) for k in range(draws)]
+    X, Y = warmstart.collect_pairs(g, forcings, pb.default_config(nLayer=10))
+    t_collect = time.perf_counter() - t0
+    if samples and X.shape[0] > samples:
+        gen = torch.Generator(device=X.device).manual_seed(1)
+        sel = torch.randperm(X.shape[0], device=X.device, generator=gen)[:samples]
+        X, Y = X[sel].contiguous(), Y[sel].contiguous()
+    net, info = warmstart.fit(X, Y, epochs=epochs, batch=8192, lr=2e-3, seed=0)
+    info.update({"draws": draws, "collect_s": round(t_collect, 2)})
+    return net, info
 
 
 def main():
@@ -104,8 +139,12 @@ def main():
     ap.add_argument("--points", type=int, default=500000)
     ap.add_argument("--layers", type=int, default=10)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--variants", default="plain,mixed")
-    ap.add_argument("--cpu-faces", type=int, default=100000)
+    ap.add_argument("--variants", default="plain,mixed,mixed_learned")
+    ap.add_argument("--krylov-mixed", default="bicgstab", choices=["gmres", "bicgstab"])
+    ap.add_argument("--train-draws", type=int, default=4)
+    ap.add_argument("--train-samples", type=int, default=2000000)
+    ap.add_argument("--train-epochs", type=int, default=3)
+    ap.add_argument("--cpu-faces", type=int, default=200000)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -115,10 +154,9 @@ def main():
               f"DESIGN.md and no multi-GPU figure is produced.", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
     import torch
 
-    from chm_amd import abi, meshgen
+    from chm_amd import abi, meshgen, warmstart
     from chm_amd import pbsm3d as pb
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
@@ -131,21 +169,29 @@ def main():
     log(f"[bench] mesh F={F} L={L} N={F * L} built in {time.perf_counter() - t:.1f}s")
     dev_forcing = {k: torch.tensor(v, device="cuda:0") for k, v in forcing.items()}
     dev_outs = {k: torch.zeros(F, dtype=torch.float64, device="cuda:0") for k in abi.OUT_FIELDS}
+    krylov_mixed = abi.KRYLOV_BICGSTAB if args.krylov_mixed == "bicgstab" else abi.KRYLOV_GMRES
+
+    names = args.variants.split(",")
+    net, train_info = None, None
+    if any(n in ("mixed_learned", "learned") for n in names):
+        log(f"[bench] training the warm-start network ({args.train_draws} FP64 draws)")
+        net, train_info = train_network(pb, abi, meshgen, warmstart, g, F, args.train_draws, args.train_samples,
+                                        args.train_epochs, torch)
+        log(f"[bench] trained: {train_info}")
 
     variants = {}
-    for name in args.variants.split(","):
-        if name == "plain":
-            cfg = pb.default_config(nLayer=L, precision=abi.FP64, precond=abi.PC_MCILU0)
-        elif name == "mixed":
-            cfg = pb.default_config(nLayer=L, precision=abi.MIXED, precond=abi.PC_MCILU0)
-        else:
-            raise SystemExit(f"unknown variant {name}")
+    for name in names:
+        cfg, learned = variant_config(pb, abi, name, L, krylov_mixed)
         log(f"[bench] variant {name}: warmup {args.warmup}, steps {args.steps}")
-        secs, stats, prof, nnz = run_variant(pb, abi, g, dev_forcing, dev_outs, cfg, args.steps, args.warmup, torch)
+        secs, stats, prof, nnz = run_variant(pb, g, dev_forcing, dev_outs, cfg, net if learned else None, args.steps,
+                                             args.warmup, torch)
         st = stats[-1]
         variants[name] = {
             "steps_per_s": args.steps / secs,
             "ms_per_step": 1000.0 * secs / args.steps,
+            "krylov": "bicgstab" if cfg.krylov == abi.KRYLOV_BICGSTAB else f"gmres({cfg.restart})",
+            "precision": "fp64" if cfg.precision == abi.FP64 else "fp32-inner/fp64-refinement",
+            "warm_start": bool(learned),
             "susp_iters": st["susp_iters"],
             "dep_iters": st["dep_iters"],
             "susp_resid": st["susp_resid"],
@@ -159,16 +205,16 @@ def main():
         log(f"[bench] {name}: {variants[name]['steps_per_s']:.2f} steps/s, iters {st['susp_iters']}/{st['dep_iters']}")
 
     N = F * L
-    head = variants.get("mixed") or variants.get("plain")
-    head_name = "mixed" if "mixed" in variants else "plain"
-    # roofline of the dominant kernel (the suspension SpMV of the headline variant)
-    kname, s = ("spmv32", 4) if head_name == "mixed" else ("spmv64", 8)
+    head_name = "mixed_learned" if "mixed_learned" in variants else names[-1]
+    head = variants[head_name]
+    # roofline of the dominant HBM kernel: the suspension SpMV of the headline variant
+    kname, s = ("spmv32", 4) if "spmv32" in head["kernels"] else ("spmv64", 8)
     k = head["kernels"].get(kname, {"ms_avg": 0.0, "launches": 0})
     algo = spmv_bytes(head["nnz"], N, s)
     achieved = algo / (k["ms_avg"] * 1e-3) / 1e9 if k["ms_avg"] > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": f"{kname} (CSR SpMV)",
-                "algorithmic_bytes": algo, "ms_avg": k["ms_avg"]}
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": f"{kname} (suspension CSR SpMV)",
+                "algorithmic_bytes": algo, "ms_avg": round(k["ms_avg"], 5), "launches": k["launches"]}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -187,13 +233,14 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32+fp64" if head_name == "mixed" else "f64",
+        "dtype": "f64" if head["precision"] == "fp64" else "f32+f64",
         "data": "synthetic (seeded Delaunay mesh and forcing)",
         "config": {"workload": f"PBSM3D C2: {args.points} points -> {F} triangles x {L} layers, 900 s step, tol 1e-8",
                    "faces": F, "layers": L, "unknowns": N, "headline_variant": head_name},
         "variants": variants,
-        "speedup_mixed_vs_plain": (round(variants["mixed"]["steps_per_s"] / variants["plain"]["steps_per_s"], 3)
-                                   if "mixed" in variants and "plain" in variants else None),
+        "speedup_vs_plain": (round(head["steps_per_s"] / variants["plain"]["steps_per_s"], 3)
+                             if "plain" in variants else None),
+        "warm_start_training": train_info,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

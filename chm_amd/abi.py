@@ -160,7 +160,7 @@ def default_config(**overrides) -> Config:
         precond=PC_MCILU0,
         warm_start=WS_ZERO,
         dep_precond=PC_MCILU0,
-        ortho_passes=2,
+        ortho_passes=1,
         block_size=256,
         krylov=KRYLOV_GMRES,
     )

@@ -198,7 +198,7 @@ void pbsm3d_config_default(pbsm3d_config* c) {
     c->precond = PBSM3D_PC_MCILU0;
     c->warm_start = PBSM3D_WS_ZERO;
     c->dep_precond = PBSM3D_PC_MCILU0;
-    c->ortho_passes = 2;
+    c->ortho_passes = 1;
     c->block_size = 256;
     c->krylov = PBSM3D_KRYLOV_GMRES;
 }

@@ -267,14 +267,24 @@ class Mlp:
         assert v.size == 2 * N_FEATURES + 2 * N_TARGETS
         self._ck(lib().pbsm3d_mlp_set_normalization(self._h, v.ctypes.data_as(C.POINTER(C.c_float))))
 
+    @staticmethod
+    def _f32(*ts):
+        import torch
+
+        for t in ts:
+            if t.dtype != torch.float32 or not t.is_contiguous() or not t.is_cuda:
+                raise TypeError("network arrays must be contiguous float32 device tensors")
+
     def train(self, X, Y, epochs: int = 4, batch: int = 4096, lr: float = 1e-3, seed: int = 0) -> float:
         """X [n x 16], Y [n x 11] float32 device tensors."""
+        self._f32(X, Y)
         loss = C.c_double(0)
         self._ck(lib().pbsm3d_mlp_train(self._h, C.c_void_p(X.data_ptr()), C.c_void_p(Y.data_ptr()), X.shape[0],
                                         epochs, batch, lr, seed, C.byref(loss)))
         return loss.value
 
     def predict(self, X, Y):
+        self._f32(X, Y)
         self._ck(lib().pbsm3d_mlp_predict(self._h, C.c_void_p(X.data_ptr()), C.c_void_p(Y.data_ptr()), X.shape[0]))
 
     def gradients(self, X: np.ndarray, Y: np.ndarray):

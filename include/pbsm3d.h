@@ -104,8 +104,9 @@ typedef struct pbsm3d_config {
     int32_t precond;              /* PBSM3D_PC_MCILU0                              */
     int32_t warm_start;           /* PBSM3D_WS_ZERO                                */
     int32_t dep_precond;          /* PBSM3D_PC_MCILU0 (deposition system)          */
-    int32_t ortho_passes;         /* 2: classical Gram-Schmidt with a second pass when the
-                                     norm drops below 1/sqrt(2) (DGKS); 1: one pass  */
+    int32_t ortho_passes;         /* 1 (default): one classical Gram-Schmidt pass;
+                                     2: a second pass when the norm drops below
+                                     1/sqrt(2) (DGKS re-orthogonalisation)          */
     int32_t block_size;           /* reserved (256)                                */
     int32_t krylov;               /* PBSM3D_KRYLOV_GMRES (reference) or _BICGSTAB  */
 } pbsm3d_config;

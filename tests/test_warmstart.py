@@ -67,7 +67,7 @@ def test_network_gradients_match_reference():
     # forward agrees with the reference
     import torch
 
-    Xd = torch.tensor(X, device="cuda:0")
+    Xd = torch.tensor(X, dtype=torch.float32, device="cuda:0")
     P = torch.empty((n, 11), dtype=torch.float32, device="cuda:0")
     net.predict(Xd, P)
     ref = mlp_oracle.predict(p, X, norm, bf16=True)
