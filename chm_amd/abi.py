@@ -125,10 +125,13 @@ class Stats(C.Structure):
         ("ms_post", C.c_double),
         ("ms_dep_solve", C.c_double),
         ("ms_total", C.c_double),
+        ("susp_resid0", C.c_double),
+        ("susp_cold_fallback", C.c_int32),
+        ("reserved2", C.c_int32),
     ]
 
     def as_dict(self) -> dict:
-        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+        return {name: getattr(self, name) for name, _ in self._fields_ if not name.startswith("reserved")}
 
 
 OUT_FIELDS = ("Qsalt", "Qsusp", "Qsubl", "Qsubl_mass", "sum_subl", "drift_mass", "sum_drift", "pbsm_more_than_avail")

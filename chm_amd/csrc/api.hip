@@ -168,6 +168,58 @@ float ms_between(hipEvent_t a, hipEvent_t b) {
     return ms;
 }
 
+// Suspension solve with the multicolour D-ILU(0) preconditioner, carried out in
+// the factorisation's position order: the factor kernel also lays the operator
+// out by position (diag / lat / vert), so the SpMV, the preconditioner sweeps
+// and the Krylov vectors all read contiguous memory, and no column indices are
+// read.  b and x0 are permuted in and x is permuted back, once per solve.
+SolveInfo solve_in_position_order(pbsm3d_ctx* c, const pbsm3d_config& cfg, int64_t F, int L) {
+    hipStream_t s = c->stream;
+    const int64_t N = F * L;
+    const bool bicg = cfg.krylov == PBSM3D_KRYLOV_BICGSTAB;
+    {
+        ProfScope ps(&c->prof, s, PBSM3D_K_FACTOR);
+        dilu_factor<double>(s, c->co, F, L, c->susp.rp.p, c->susp.val.p, c->slots.p, c->dl_s);
+        if (cfg.precision == PBSM3D_MIXED) dilu_narrow(s, N, c->dl_s, c->dl32);
+    }
+    to_pos<double, double>(s, c->co, F, L, c->susp.rhs.p, c->bp.p);
+    to_pos<double, double>(s, c->co, F, L, c->susp.x.p, c->xp.p);
+    if (c->spmv_variant[0] < 0)
+        c->spmv_variant[0] = tune_pos_spmv<double>(s, c->co, F, L, c->dl_s, c->xp.p, c->r_s.p, c->spmv_tune_ms[0]);
+    if (cfg.precision == PBSM3D_MIXED && c->spmv_variant[1] < 0)
+        c->spmv_variant[1] = tune_pos_spmv<float>(s, c->co, F, L, c->dl32, c->r32.p, c->d32.p, c->spmv_tune_ms[1]);
+    CsrOp<double> A{N, nullptr, nullptr, nullptr, &c->prof, F, L};
+    A.co = &c->co;
+    A.pos = &c->dl_s;
+    A.variant = c->spmv_variant[0];
+    SolveInfo si;
+    if (cfg.precision == PBSM3D_FP64) {
+        PcFn<double> M = [c, F, L](hipStream_t st, const double* r, double* z) {
+            dilu_apply<double>(st, c->co, F, L, c->dl_s, r, z, true);
+        };
+        if (bicg)
+            si = bicgstab_solve(s, c->bw64, c->ws64, A, M, c->bp.p, c->xp.p, cfg.max_iters, cfg.tol);
+        else
+            si = gmres_solve(s, c->ws64, A, M, c->bp.p, c->xp.p, c->r_s.p, cfg.restart, cfg.max_iters, cfg.tol);
+    } else {
+        CsrOp<float> A32{N, nullptr, nullptr, nullptr, &c->prof, F, L};
+        A32.co = &c->co;
+        A32.pos = &c->dl32;
+        A32.variant = c->spmv_variant[1];
+        PcFn<float> M = [c, F, L](hipStream_t st, const float* r, float* z) {
+            dilu_apply<float>(st, c->co, F, L, c->dl32, r, z, true);
+        };
+        if (bicg)
+            si = bicgstab_mixed(s, c->bw32, c->ws32, A, A32, M, c->bp.p, c->xp.p, c->r_s.p, c->r32.p, c->d32.p,
+                                cfg.max_iters, cfg.tol, 1e-5);
+        else
+            si = mixed_solve(s, c->ws32, A, A32, M, c->bp.p, c->xp.p, c->r_s.p, c->r32.p, c->d32.p, cfg.restart,
+                             cfg.max_iters, cfg.tol, 1e-4);
+    }
+    from_pos<double, double>(s, c->co, F, L, c->xp.p, c->susp.x.p);
+    return si;
+}
+
 }  // namespace
 
 extern "C" {
@@ -335,10 +387,11 @@ int pbsm3d_create(pbsm3d_ctx** out, const pbsm3d_mesh* m, const pbsm3d_config* c
         if (cfg->precond == PBSM3D_PC_MCILU0 || cfg->dep_precond == PBSM3D_PC_MCILU0)
             build_color_order(c->co, F, m->neigh, s);
         if (cfg->precond == PBSM3D_PC_MCILU0) {
-            if (cfg->precision == PBSM3D_MIXED)
-                c->dl32.alloc(N);
-            else
-                c->dl_s.alloc(N);
+            // FP64 factors hold the operator in position order (residuals); FP32 copy for MIXED
+            c->dl_s.alloc(N);
+            if (cfg->precision == PBSM3D_MIXED) c->dl32.alloc(N);
+            c->bp.alloc(N);
+            c->xp.alloc(N);
         }
         if (cfg->dep_precond == PBSM3D_PC_MCILU0) c->dl_d.alloc(F);
         c->invdiag_d.alloc(F);
@@ -346,7 +399,7 @@ int pbsm3d_create(pbsm3d_ctx** out, const pbsm3d_mesh* m, const pbsm3d_config* c
         c->r_d.alloc(F);
         const bool bicg = cfg->krylov == PBSM3D_KRYLOV_BICGSTAB;
         if (cfg->precision == PBSM3D_MIXED) {
-            c->val32.alloc(c->susp.nnz);
+            if (cfg->precond != PBSM3D_PC_MCILU0) c->val32.alloc(c->susp.nnz);
             c->r32.alloc(N);
             c->d32.alloc(N);
             c->ws32.ensure(N, bicg ? 1 : cfg->restart);  // BiCGStab uses it for residuals only
@@ -477,18 +530,12 @@ int pbsm3d_step(pbsm3d_ctx* c, const pbsm3d_forcing* fo, int32_t dt_s, pbsm3d_ou
         }
         if (stats.suspension_present) {
             SolveInfo si;
-            CsrOp<double> A{N, c->susp.rp.p, c->susp.col.p, c->susp.val.p, &c->prof};
-            if (cfg.precision == PBSM3D_FP64) {
+            CsrOp<double> A{N, c->susp.rp.p, c->susp.col.p, c->susp.val.p, &c->prof, F, L, c->neigh.p, c->slots.p};
+            if (cfg.precond == PBSM3D_PC_MCILU0) {
+                si = solve_in_position_order(c, cfg, F, L);
+            } else if (cfg.precision == PBSM3D_FP64) {
                 PcFn<double> M;
-                if (cfg.precond == PBSM3D_PC_MCILU0) {
-                    {
-                        ProfScope ps(&c->prof, s, PBSM3D_K_PRECOND);
-                        dilu_factor<double>(s, c->co, F, L, c->susp.rp.p, c->susp.val.p, c->slots.p, c->dl_s);
-                    }
-                    M = [c, F, L](hipStream_t st2, const double* r, double* z) {
-                        dilu_apply<double>(st2, c->co, F, L, c->dl_s, r, z);
-                    };
-                } else if (cfg.precond == PBSM3D_PC_COLUMN)
+                if (cfg.precond == PBSM3D_PC_COLUMN)
                     M = [c, F, L](hipStream_t st2, const double* r, double* z) {
                         pc_column<double>(st2, F, L, c->colf.lo.p, c->colf.cp.p, c->colf.im.p, r, z);
                     };
@@ -505,17 +552,9 @@ int pbsm3d_step(pbsm3d_ctx* c, const pbsm3d_forcing* fo, int32_t dt_s, pbsm3d_ou
                                      cfg.max_iters, cfg.tol);
             } else {
                 to_f32(s, c->susp.nnz, c->susp.val.p, c->val32.p);
-                CsrOp<float> A32{N, c->susp.rp.p, c->susp.col.p, c->val32.p, &c->prof};
+                CsrOp<float> A32{N, c->susp.rp.p, c->susp.col.p, c->val32.p, &c->prof, F, L, c->neigh.p, c->slots.p};
                 PcFn<float> M;
-                if (cfg.precond == PBSM3D_PC_MCILU0) {
-                    {
-                        ProfScope ps(&c->prof, s, PBSM3D_K_PRECOND);
-                        dilu_factor<float>(s, c->co, F, L, c->susp.rp.p, c->val32.p, c->slots.p, c->dl32);
-                    }
-                    M = [c, F, L](hipStream_t st2, const float* r, float* z) {
-                        dilu_apply<float>(st2, c->co, F, L, c->dl32, r, z);
-                    };
-                } else if (cfg.precond == PBSM3D_PC_COLUMN)
+                if (cfg.precond == PBSM3D_PC_COLUMN)
                     M = [c, F, L](hipStream_t st2, const float* r, float* z) {
                         pc_column<float>(st2, F, L, c->colf.lo32.p, c->colf.cp32.p, c->colf.im32.p, r, z);
                     };
@@ -534,6 +573,8 @@ int pbsm3d_step(pbsm3d_ctx* c, const pbsm3d_forcing* fo, int32_t dt_s, pbsm3d_ou
             }
             stats.susp_iters = si.iters;
             stats.susp_resid = si.resid;
+            stats.susp_resid0 = si.resid0;
+            stats.susp_cold_fallback = si.cold_fallback ? 1 : 0;
             stats.susp_refinements = si.refinements;
             if (!si.converged) throw Error(PBSM3D_ENOCONV, "suspension solver failed to converge");
         }
@@ -624,6 +665,21 @@ int pbsm3d_profile_read(pbsm3d_ctx* c, int kernel, int64_t* launches, double* ms
         c->prof.count[kernel] = 0;
         c->prof.ms[kernel] = 0;
     }
+    return PBSM3D_OK;
+}
+
+int pbsm3d_spmv_tuning(const pbsm3d_ctx* c, int which, int32_t* chosen, float* ms, int32_t n) {
+    if (!c || (which != 0 && which != 1) || n < 0) return PBSM3D_EINVAL;
+    if (chosen) *chosen = c->spmv_variant[which];
+    if (ms)
+        for (int32_t v = 0; v < n; ++v) ms[v] = v < kPosSpmvVariants ? c->spmv_tune_ms[which][v] : 0.0f;
+    return PBSM3D_OK;
+}
+
+int pbsm3d_spmv_select(pbsm3d_ctx* c, int which, int variant) {
+    if (!c || (which != 0 && which != 1) || variant < -1 || variant >= kPosSpmvVariants) return PBSM3D_EINVAL;
+    if (variant >= 1 && variant <= 3 && c->L > 16) return PBSM3D_EINVAL;  // column shapes need L <= 16
+    c->spmv_variant[which] = variant;
     return PBSM3D_OK;
 }
 

@@ -31,6 +31,10 @@ struct pbsm3d_ctx {
     pb::DiluFactors<double> dl_s, dl_d;
     pb::DiluFactors<float> dl32;
     pb::DevBuf<double> invdiag_s, invdiag_d, r_s, r_d;
+    pb::DevBuf<double> bp, xp;  // suspension b and x in the D-ILU position order
+    // position-order SpMV launch shape per precision (index 0: FP64, 1: FP32), -1 until tuned
+    int spmv_variant[2] = {-1, -1};
+    float spmv_tune_ms[2][pb::kPosSpmvVariants] = {};
     pb::DevBuf<float> val32, r32, d32;
     pb::GmresWork<double> ws64, wd64;
     pb::GmresWork<float> ws32;

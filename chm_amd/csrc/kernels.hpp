@@ -5,6 +5,27 @@
 
 namespace pb {
 
+// wavefront-wide sum (64 lanes)
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// block-wide sum in a fixed order; result valid in thread 0; lds holds blockDim.x / 64 doubles
+__device__ __forceinline__ double block_sum(double v, double* lds) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) lds[wid] = v;
+    __syncthreads();
+    double s = 0;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) s += lds[k];
+    __syncthreads();
+    return s;
+}
+
 struct AssembleArgs {
     int64_t F;
     int L;
@@ -73,6 +94,14 @@ template <typename T>
 void residual(hipStream_t s, int64_t n, const int32_t* rp, const int32_t* col, const T* val, const T* x,
               const T* b, T* r, double* partial, int nblk);
 
+// Structure-aware SpMV of the extruded prism-column operator in its CSR value
+// layout: one thread per face walks the face's L rows; columns come from the
+// neighbour list and the slot table instead of col[].  With b: r = b - A x and
+// per-block partial sums of r.r (partial[block]).
+template <typename T>
+void spmv_col(hipStream_t s, int64_t F, int L, const int32_t* rp, const T* val, const int32_t* neigh,
+              const uint16_t* slots, const T* x, const T* b, T* y, double* partial);
+inline int64_t spmv_col_blocks(int64_t F) { return (F + kBlock - 1) / kBlock; }
 // partial[block*(nv+1) + c] = sum over the block's rows of V_c . w (c < nv), w . w (c == nv)
 template <typename T>
 void mdot(hipStream_t s, int64_t n, const T* V, int nv, const T* w, double* partial, int nblk);
@@ -119,11 +148,13 @@ struct DiluFactors {
     DevBuf<T> lat;   // [3N] coupling to neighbour j (row's own lateral entries)
     DevBuf<T> vert;  // [2N] coupling below (0) / above (1)
     DevBuf<T> dinv;  // [N]  1 / d~
+    DevBuf<T> diag;  // [N]  a_rr (with lat and vert: the operator itself in position order)
     DevBuf<T> y;     // [N]  intermediate of the apply
     void alloc(int64_t N) {
         lat.alloc(3 * N);
         vert.alloc(2 * N);
         dinv.alloc(N);
+        diag.alloc(N);
         y.alloc(N);
     }
 };
@@ -131,8 +162,34 @@ void build_color_order(ColorOrder& co, int64_t F, const int32_t* neigh_fm, hipSt
 template <typename T>
 void dilu_factor(hipStream_t s, const ColorOrder& co, int64_t F, int L, const int32_t* rp, const T* val,
                  const uint16_t* slots, DiluFactors<T>& d);
+// z = M^-1 r.  pos: r and z in position order (q = z*F + k), else in the reference order.
 template <typename T>
-void dilu_apply(hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFactors<T>& d, const T* r, T* z);
+void dilu_apply(hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFactors<T>& d, const T* r, T* z,
+                bool pos = false);
+// FP32 copy of FP64 factors (the FP32 preconditioner and operator of the mixed solve)
+void dilu_narrow(hipStream_t s, int64_t N, const DiluFactors<double>& a, DiluFactors<float>& b);
+// y = b - A x with per-block partial sums of y.y, everything in position order, A taken
+// from the factor arrays diag/lat/vert
+template <typename T>
+void pos_spmv(hipStream_t s, const ColorOrder& co, int64_t F, int L, const DiluFactors<T>& d, const T* x, const T* b,
+              T* y, double* partial);
+// y = A x in position order with launch shape v: 0 / 4 one thread per row (blocks of 256 / 128),
+// 1 / 2 / 3 one thread per face column (blocks of 256 / 128 / 64; L <= 16)
+constexpr int kPosSpmvVariants = PBSM3D_SPMV_VARIANTS;
+template <typename T>
+void pos_spmv_variant(hipStream_t s, int v, const ColorOrder& co, int64_t F, int L, const DiluFactors<T>& d,
+                      const T* x, T* y);
+// times every launch shape on (x, y) with HIP events on the stream; returns the fastest
+// (ms_out[v], if given, receives the average time of shape v, ms)
+template <typename T>
+int tune_pos_spmv(hipStream_t s, const ColorOrder& co, int64_t F, int L, const DiluFactors<T>& d, const T* x, T* y,
+                  float* ms_out);
+inline int64_t pos_spmv_blocks(int64_t F, int L) { return ((F + kBlock - 1) / kBlock) * L; }
+// out[z*F + k] = in[z*F + face(k)]  /  out[z*F + face(k)] = in[z*F + k]
+template <typename Ti, typename To>
+void to_pos(hipStream_t s, const ColorOrder& co, int64_t F, int L, const Ti* in, To* out);
+template <typename Ti, typename To>
+void from_pos(hipStream_t s, const ColorOrder& co, int64_t F, int L, const Ti* in, To* out);
 
 // BiCGStab vector updates (FP64 arithmetic on T storage)
 template <typename T>

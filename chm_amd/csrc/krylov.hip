@@ -34,7 +34,9 @@ void GmresWork<T>::ensure(int64_t n_, int m_) {
     V.alloc((size_t)(m + 1) * n);
     z.alloc(n);
     u.alloc(n);
-    const size_t spmv_blocks = (size_t)((n + kBlock - 1) / kBlock) + 1;
+    // residual partials: one per block of kBlock rows, or ceil(F / kBlock) * L <= n / kBlock + L
+    // for the launches that run layer by layer (L <= 64)
+    const size_t spmv_blocks = (size_t)((n + kBlock - 1) / kBlock) + 64;
     size_t pbytes = (size_t)kReduceBlocks * (m + 2);
     if (pbytes < spmv_blocks) pbytes = spmv_blocks;
     partial.alloc(pbytes);
@@ -68,7 +70,7 @@ int gmres_cycle(hipStream_t s, GmresWork<T>& w, const CsrOp<T>& A, const PcFn<T>
         }
         {
             ProfScope ps(A.prof, s, sizeof(T) == 8 ? PBSM3D_K_SPMV64 : PBSM3D_K_SPMV32);
-            spmv<T>(s, n, A.rp, A.col, A.val, w.z.p, vj1);
+            op_spmv<T>(s, A, w.z.p, vj1);
         }
         for (int i = 0; i <= j; ++i) h[i] = 0.0;
         double ss_before = 0, ss_after = 0;
@@ -135,13 +137,23 @@ double dev_norm2(hipStream_t s, GmresWork<double>& w, int64_t n, const double* v
 }
 
 namespace {
+// A start whose residual exceeds ||b|| (that of x0 = 0) is dropped: x is reset
+// to zero and the caller recomputes the residual.  Records ||r0|| / ||b||.
+bool drop_worse_start(hipStream_t s, SolveInfo& info, double beta, double nb, double* x, int64_t n) {
+    info.resid0 = beta / nb;
+    if (!(info.resid0 > 1.0)) return false;
+    PB_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, s));
+    info.cold_fallback = true;
+    return true;
+}
+
 template <typename T>
 double residual_norm(hipStream_t s, GmresWork<T>& w, const CsrOp<double>& A, const double* x, const double* b,
                      double* r) {
-    const int nblk = (int)((A.n + kBlock - 1) / kBlock);
+    int nblk = 0;
     {
         ProfScope ps(A.prof, s, PBSM3D_K_RESIDUAL);
-        residual<double>(s, A.n, A.rp, A.col, A.val, x, b, r, w.partial.p, nblk);
+        nblk = op_residual(s, A, x, b, r, w.partial.p);
     }
     reduce_cols(s, w.partial.p, nblk, 1, 1, w.dsc.p + off_ss(w.m) + 1);
     PB_HIP(hipMemcpyAsync(w.hsc.p, w.dsc.p + off_ss(w.m) + 1, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -171,6 +183,7 @@ SolveInfo gmres_solve(hipStream_t s, GmresWork<double>& w, const CsrOp<double>& 
         return info;
     }
     double beta = residual_norm(s, w, A, x, b, r);
+    if (drop_worse_start(s, info, beta, nb, x, A.n)) beta = residual_norm(s, w, A, x, b, r);
     while (true) {
         info.resid = beta / nb;
         if (info.resid <= tol) {
@@ -200,6 +213,7 @@ SolveInfo mixed_solve(hipStream_t s, GmresWork<float>& w, const CsrOp<double>& A
         return info;
     }
     double beta = residual_norm(s, w, A64, x, b, r);
+    if (drop_worse_start(s, info, beta, nb, x, A64.n)) beta = residual_norm(s, w, A64, x, b, r);
     while (true) {
         info.resid = beta / nb;
         if (info.resid <= tol) {
@@ -256,7 +270,7 @@ int bicg_run(hipStream_t s, BicgWork<T>& w, const CsrOp<T>& A, const PcFn<T>& M,
     } else {
         {
             ProfScope ps(A.prof, s, spmv_id);
-            spmv<T>(s, n, A.rp, A.col, A.val, x, w.v.p);
+            op_spmv<T>(s, A, x, w.v.p);
         }
         axpy_dots<T>(s, n, 1.0, w.v.p, b, w.r.p, nullptr, w.partial.p, nblk);
     }
@@ -285,7 +299,7 @@ int bicg_run(hipStream_t s, BicgWork<T>& w, const CsrOp<T>& A, const PcFn<T>& M,
         }
         {
             ProfScope ps(A.prof, s, spmv_id);
-            spmv<T>(s, n, A.rp, A.col, A.val, w.ph.p, w.v.p);
+            op_spmv<T>(s, A, w.ph.p, w.v.p);
         }
         {
             ProfScope ps(A.prof, s, PBSM3D_K_ORTHO);
@@ -307,7 +321,7 @@ int bicg_run(hipStream_t s, BicgWork<T>& w, const CsrOp<T>& A, const PcFn<T>& M,
         }
         {
             ProfScope ps(A.prof, s, spmv_id);
-            spmv<T>(s, n, A.rp, A.col, A.val, w.sh.p, w.t.p);
+            op_spmv<T>(s, A, w.sh.p, w.t.p);
         }
         {
             ProfScope ps(A.prof, s, PBSM3D_K_ORTHO);
@@ -350,6 +364,7 @@ SolveInfo bicgstab_solve(hipStream_t s, BicgWork<double>& w, GmresWork<double>& 
         return info;
     }
     double beta = residual_norm(s, dummy, A, x, b, w.r.p);
+    if (drop_worse_start(s, info, beta, nb, x, A.n)) beta = residual_norm(s, dummy, A, x, b, w.r.p);
     while (true) {
         info.resid = beta / nb;
         if (info.resid <= tol) {
@@ -377,6 +392,7 @@ SolveInfo bicgstab_mixed(hipStream_t s, BicgWork<float>& w, GmresWork<float>& gw
         return info;
     }
     double beta = residual_norm(s, gw, A64, x, b, r);
+    if (drop_worse_start(s, info, beta, nb, x, A64.n)) beta = residual_norm(s, gw, A64, x, b, r);
     while (true) {
         info.resid = beta / nb;
         if (info.resid <= tol) {

@@ -21,7 +21,43 @@ struct CsrOp {
     const int32_t* col = nullptr;
     const T* val = nullptr;
     Profiler* prof = nullptr;  // optional launch timing
+    // optional extruded-prism structure: when set, SpMV walks face columns (spmv_col)
+    int64_t F = 0;
+    int L = 0;
+    const int32_t* neigh = nullptr;
+    const uint16_t* slots = nullptr;
+    // optional operator held in the D-ILU position order (diag/lat/vert of the
+    // factorisation); when set, every vector of the solve is in position order
+    const ColorOrder* co = nullptr;
+    const DiluFactors<T>* pos = nullptr;
+    int variant = 0;  // launch shape of the position-order SpMV (tune_pos_spmv)
 };
+
+template <typename T>
+inline void op_spmv(hipStream_t s, const CsrOp<T>& A, const T* x, T* y) {
+    if (A.pos)
+        pos_spmv_variant<T>(s, A.variant, *A.co, A.F, A.L, *A.pos, x, y);
+    else if (A.neigh)
+        spmv_col<T>(s, A.F, A.L, A.rp, A.val, A.neigh, A.slots, x, nullptr, y, nullptr);
+    else
+        spmv<T>(s, A.n, A.rp, A.col, A.val, x, y);
+}
+
+// r = b - A x with per-block partial sums of r.r; returns the number of partials
+inline int op_residual(hipStream_t s, const CsrOp<double>& A, const double* x, const double* b, double* r,
+                       double* partial) {
+    if (A.pos) {
+        pos_spmv<double>(s, *A.co, A.F, A.L, *A.pos, x, b, r, partial);
+        return (int)pos_spmv_blocks(A.F, A.L);
+    }
+    if (A.neigh) {
+        spmv_col<double>(s, A.F, A.L, A.rp, A.val, A.neigh, A.slots, x, b, r, partial);
+        return (int)spmv_col_blocks(A.F);
+    }
+    const int nblk = (int)((A.n + kBlock - 1) / kBlock);
+    residual<double>(s, A.n, A.rp, A.col, A.val, x, b, r, partial, nblk);
+    return nblk;
+}
 
 // z = M^-1 r on the stream
 template <typename T>
@@ -32,7 +68,9 @@ struct SolveInfo {
     int cycles = 0;
     int refinements = 0;
     double resid = 0;  // explicit ||b - A x|| / ||b||
+    double resid0 = 1;  // the same for the initial guess
     bool converged = false;
+    bool cold_fallback = false;  // the initial guess was worse than x0 = 0 and was dropped
 };
 
 // Device workspace of one GMRES instance of storage type T.

@@ -14,26 +14,6 @@ namespace {
 constexpr int kRowsPerBlock = kBlock;  // one row per thread
 constexpr int kMaxRowLen = 6;          // self + 3 lateral + below + above
 
-template <typename T>
-__device__ __forceinline__ T wave_sum(T v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
-// block-wide sum; result valid in thread 0
-__device__ __forceinline__ double block_sum(double v, double* lds) {
-    v = wave_sum(v);
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0) lds[wid] = v;
-    __syncthreads();
-    double s = 0;
-    if (threadIdx.x == 0)
-        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) s += lds[k];
-    __syncthreads();
-    return s;
-}
-
 // ---------------------------------------------------------------------------
 // SpMV: a block of 256 rows copies its contiguous slice of val/col into LDS
 // with coalesced loads, then each thread forms its row's dot product.
@@ -77,6 +57,62 @@ __global__ void __launch_bounds__(kBlock) k_spmv_lds(int64_t n, const int32_t* _
     if (RESID) {
         const double s = block_sum(rr, s_red);
         if (threadIdx.x == 0) partial[blockIdx.x] = s;
+    }
+}
+
+// Column-walking SpMV of the extruded operator: thread = face, loop over the
+// face's rows z*F + i bottom to top.  Row layout in the CSR values:
+// [below (z>0)] [lateral block, sorted] [above (z<L-1)]; slot nibbles give the
+// position of the face itself and of neighbour j inside the lateral block.
+template <typename T, bool RESID>
+__global__ void __launch_bounds__(kBlock) k_spmv_col(int64_t F, int L, const int32_t* __restrict__ rp,
+                                                     const T* __restrict__ val, const int32_t* __restrict__ neigh,
+                                                     const uint16_t* __restrict__ slots, const T* __restrict__ x,
+                                                     const T* __restrict__ b, T* __restrict__ y,
+                                                     double* __restrict__ partial) {
+    __shared__ double s_red[kBlock / 64];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double rr = 0.0;
+    if (i < F) {
+        const uint32_t sl = slots[i];
+        int32_t nb[3];
+        int pos[3];
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            nb[f] = neigh[f * F + i];
+            pos[f] = (sl >> (4 * (f + 1))) & 0xF;
+        }
+        const int ps = sl & 0xF;
+        const int nlat = 1 + (nb[0] >= 0) + (nb[1] >= 0) + (nb[2] >= 0);
+        T xb = 0;       // x of the cell below
+        T xs = x[i];    // x of this cell
+        for (int z = 0; z < L; ++z) {
+            const int64_t r = (int64_t)z * F + i;
+            const int64_t base = (int64_t)rp[r] + (z > 0 ? 1 : 0);
+            T acc = val[base + ps] * xs;
+            if (z > 0) acc = fma(val[base - 1], xb, acc);
+#pragma unroll
+            for (int f = 0; f < 3; ++f)
+                if (nb[f] >= 0) acc = fma(val[base + pos[f]], x[(int64_t)z * F + nb[f]], acc);
+            T xa = 0;
+            if (z < L - 1) {
+                xa = x[r + F];
+                acc = fma(val[base + nlat], xa, acc);
+            }
+            if (RESID) {
+                const T res = b[r] - acc;
+                y[r] = res;
+                rr = fma((double)res, (double)res, rr);
+            } else {
+                y[r] = acc;
+            }
+            xb = xs;
+            xs = xa;
+        }
+    }
+    if (RESID) {
+        const double sum = block_sum(rr, s_red);
+        if (threadIdx.x == 0) partial[blockIdx.x] = sum;
     }
 }
 
@@ -415,6 +451,16 @@ void residual(hipStream_t s, int64_t n, const int32_t* rp, const int32_t* col, c
     hipLaunchKernelGGL((k_spmv_lds<T, true>), dim3(g), dim3(kBlock), 0, s, n, rp, col, val, x, b, r, partial);
 }
 
+template <typename T>
+void spmv_col(hipStream_t s, int64_t F, int L, const int32_t* rp, const T* val, const int32_t* neigh,
+              const uint16_t* slots, const T* x, const T* b, T* y, double* partial) {
+    const dim3 g((unsigned)spmv_col_blocks(F)), blk(kBlock);
+    if (b)
+        hipLaunchKernelGGL((k_spmv_col<T, true>), g, blk, 0, s, F, L, rp, val, neigh, slots, x, b, y, partial);
+    else
+        hipLaunchKernelGGL((k_spmv_col<T, false>), g, blk, 0, s, F, L, rp, val, neigh, slots, x, b, y, partial);
+}
+
 // 4-wide loads need n % 4 == 0 and 32-byte aligned vector starts
 template <typename T>
 static bool vec4_ok(int64_t n, const void* a, const void* b) {
@@ -525,6 +571,8 @@ void copy_f64(hipStream_t s, int64_t n, const double* a, double* b) {
     template void pc_column<T>(hipStream_t, int64_t, int, const T*, const T*, const T*, const T*, T*);             \
     template void pc_jacobi<T>(hipStream_t, int64_t, const double*, const T*, T*);                                \
     template void scale_host<T>(hipStream_t, int64_t, const T*, double, T*);                                       \
+    template void spmv_col<T>(hipStream_t, int64_t, int, const int32_t*, const T*, const int32_t*, const uint16_t*, \
+                              const T*, const T*, T*, double*);                                                    \
     template void bicg_p<T>(hipStream_t, int64_t, const T*, const T*, double, double, T*);                         \
     template void axpy_dots<T>(hipStream_t, int64_t, double, const T*, const T*, T*, const T*, double*, int);      \
     template void x_update<T>(hipStream_t, int64_t, double, const T*, double, const T*, T*);

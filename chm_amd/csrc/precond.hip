@@ -34,7 +34,8 @@ __global__ void __launch_bounds__(kBlock) k_dilu_factor(int64_t F, int L, int64_
                                                         const uint8_t* __restrict__ lowmask,
                                                         const int32_t* __restrict__ rp, const T* __restrict__ val,
                                                         const uint16_t* __restrict__ slots, T* __restrict__ lat,
-                                                        T* __restrict__ vert, T* __restrict__ dinv) {
+                                                        T* __restrict__ vert, T* __restrict__ dinv,
+                                                        T* __restrict__ diag) {
     const int64_t k = k0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= k1) return;
     const int64_t N = F * L;
@@ -58,6 +59,7 @@ __global__ void __launch_bounds__(kBlock) k_dilu_factor(int64_t F, int L, int64_
         const int64_t q = (int64_t)z * F + k;
         const int64_t base = (int64_t)rp[r] + (z > 0 ? 1 : 0);
         T d = val[base + nib(si, 0)];
+        diag[q] = d;
 #pragma unroll
         for (int f = 0; f < 3; ++f) {
             const T a_in = np_[f] >= 0 ? val[base + nib(si, f + 1)] : T(0);
@@ -83,59 +85,243 @@ __global__ void __launch_bounds__(kBlock) k_dilu_factor(int64_t F, int L, int64_
     }
 }
 
-template <typename T>
+// Forward sweep of one colour: y = D~^-1 (r - L y).  POS: r is in position
+// order (q = z*F + k) rather than in the reference order (z*F + face).
+// Neighbours of a face carry other colours, so the y values this launch reads
+// (ynb, at neighbour positions) and writes (y, own positions) are disjoint
+// elements of one array; the two restrict pointers let the loads run ahead
+// of the stores.  LM > 0: the column (L <= LM) is fully unrolled -- first all
+// terms that do not depend on the vertical recurrence, then the recurrence.
+template <typename T, bool POS, int LM>
 __global__ void __launch_bounds__(kBlock) k_dilu_fwd(int64_t F, int L, int64_t k0, int64_t k1,
                                                      const int32_t* __restrict__ faces, const int32_t* __restrict__ nbp,
                                                      const uint8_t* __restrict__ lowmask, const T* __restrict__ lat,
                                                      const T* __restrict__ vert, const T* __restrict__ dinv,
-                                                     const T* __restrict__ rin, T* __restrict__ y) {
+                                                     const T* __restrict__ rin, const T* __restrict__ ynb,
+                                                     T* __restrict__ y) {
     const int64_t k = k0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= k1) return;
     const int64_t N = F * L;
-    const int64_t i = faces[k];
+    const int64_t i = POS ? k : (int64_t)faces[k];
     const uint32_t lm = lowmask[k];
     int32_t np_[3];
 #pragma unroll
-    for (int f = 0; f < 3; ++f) np_[f] = nbp[f * F + k];
-    T prev = 0;
-    for (int z = 0; z < L; ++z) {
+    for (int f = 0; f < 3; ++f) np_[f] = ((lm >> f) & 1u) ? nbp[f * F + k] : -1;
+    auto independent = [&](int z) {
         const int64_t q = (int64_t)z * F + k;
         T acc = rin[(int64_t)z * F + i];
 #pragma unroll
         for (int f = 0; f < 3; ++f)
-            if ((lm >> f) & 1u) acc -= lat[f * N + q] * y[(int64_t)z * F + np_[f]];
-        if (z > 0) acc -= vert[q] * prev;
-        prev = acc * dinv[q];
-        y[q] = prev;
+            if (np_[f] >= 0) acc -= lat[f * N + q] * ynb[(int64_t)z * F + np_[f]];
+        return acc;
+    };
+    if constexpr (LM > 0) {
+        T sv[LM], vv[LM], dv[LM];
+#pragma unroll
+        for (int z = 0; z < LM; ++z)
+            if (z < L) {
+                const int64_t q = (int64_t)z * F + k;
+                sv[z] = independent(z);
+                vv[z] = z > 0 ? vert[q] : T(0);
+                dv[z] = dinv[q];
+            }
+        T prev = 0;
+#pragma unroll
+        for (int z = 0; z < LM; ++z)
+            if (z < L) {
+                prev = (sv[z] - vv[z] * prev) * dv[z];
+                y[(int64_t)z * F + k] = prev;
+            }
+    } else {
+        T prev = 0;
+        for (int z = 0; z < L; ++z) {
+            const int64_t q = (int64_t)z * F + k;
+            T acc = independent(z);
+            if (z > 0) acc -= vert[q] * prev;
+            prev = acc * dinv[q];
+            y[q] = prev;
+        }
     }
 }
 
-template <typename T>
+// Backward sweep of one colour: z = y - D~^-1 U z.  Reference order: y is
+// updated in place (higher-colour neighbours are read from it through ynb) and
+// z is scattered to z*F + face.  POS: z is in position order, the neighbours
+// are read from z itself (znb) and y is only read.  LM as in the forward sweep.
+template <typename T, bool POS, int LM>
 __global__ void __launch_bounds__(kBlock) k_dilu_bwd(int64_t F, int L, int64_t k0, int64_t k1,
                                                      const int32_t* __restrict__ faces, const int32_t* __restrict__ nbp,
                                                      const uint8_t* __restrict__ lowmask, const T* __restrict__ lat,
                                                      const T* __restrict__ vert, const T* __restrict__ dinv,
-                                                     T* __restrict__ y, T* __restrict__ zout) {
+                                                     T* __restrict__ y, const T* __restrict__ done,
+                                                     T* __restrict__ zout) {
     const int64_t k = k0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= k1) return;
     const int64_t N = F * L;
-    const int64_t i = faces[k];
+    const int64_t i = POS ? k : (int64_t)faces[k];
     const uint32_t lm = lowmask[k];
     int32_t np_[3];
 #pragma unroll
-    for (int f = 0; f < 3; ++f) np_[f] = nbp[f * F + k];
-    T next = 0;
-    for (int z = L - 1; z >= 0; --z) {
+    for (int f = 0; f < 3; ++f) {
+        const int32_t n = nbp[f * F + k];
+        np_[f] = (n >= 0 && !((lm >> f) & 1u)) ? n : -1;
+    }
+    auto independent = [&](int z) {
         const int64_t q = (int64_t)z * F + k;
         T acc = 0;
 #pragma unroll
         for (int f = 0; f < 3; ++f)
-            if (np_[f] >= 0 && !((lm >> f) & 1u)) acc += lat[f * N + q] * y[(int64_t)z * F + np_[f]];
-        if (z < L - 1) acc += vert[N + q] * next;
-        next = y[q] - dinv[q] * acc;
-        y[q] = next;
-        zout[(int64_t)z * F + i] = next;
+            if (np_[f] >= 0) acc += lat[f * N + q] * done[(int64_t)z * F + np_[f]];
+        return acc;
+    };
+    if constexpr (LM > 0) {
+        T av[LM], vv[LM], dv[LM], yv[LM];
+#pragma unroll
+        for (int z = 0; z < LM; ++z)
+            if (z < L) {
+                const int64_t q = (int64_t)z * F + k;
+                av[z] = independent(z);
+                vv[z] = z < L - 1 ? vert[N + q] : T(0);
+                dv[z] = dinv[q];
+                yv[z] = y[q];
+            }
+        T next = 0;
+#pragma unroll
+        for (int z = LM - 1; z >= 0; --z)
+            if (z < L) {
+                next = yv[z] - dv[z] * (av[z] + vv[z] * next);
+                if (!POS) y[(int64_t)z * F + k] = next;
+                zout[(int64_t)z * F + i] = next;
+            }
+    } else {
+        T next = 0;
+        for (int z = L - 1; z >= 0; --z) {
+            const int64_t q = (int64_t)z * F + k;
+            T acc = independent(z);
+            if (z < L - 1) acc += vert[N + q] * next;
+            next = y[q] - dinv[q] * acc;
+            if (!POS) y[q] = next;
+            zout[(int64_t)z * F + i] = next;
+        }
     }
+}
+
+// y = A x, or r = b - A x with per-block partial sums of r.r, for the operator
+// held in position order by the factorisation: diag [N], lat [3N] (neighbour
+// j, zero where there is none), vert [2N] (below, above; zero at the ends of
+// the column).  One thread per row; blockIdx.y is the layer.
+template <typename T, bool RESID>
+__global__ void __launch_bounds__(kBlock) k_pos_spmv(int64_t F, int L, const int32_t* __restrict__ nbp,
+                                                     const T* __restrict__ diag, const T* __restrict__ lat,
+                                                     const T* __restrict__ vert, const T* __restrict__ x,
+                                                     const T* __restrict__ b, T* __restrict__ y,
+                                                     double* __restrict__ partial) {
+    __shared__ double s_red[kBlock / 64];
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int z = blockIdx.y;
+    const int64_t N = F * L;
+    double rr = 0.0;
+    if (k < F) {
+        const int64_t q = (int64_t)z * F + k;
+        const int64_t lay = (int64_t)z * F;
+        T acc = diag[q] * x[q];
+        if (z > 0) acc = fma(vert[q], x[q - F], acc);
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            const int32_t n = nbp[f * F + k];
+            if (n >= 0) acc = fma(lat[f * N + q], x[lay + n], acc);
+        }
+        if (z < L - 1) acc = fma(vert[N + q], x[q + F], acc);
+        if (RESID) {
+            const T res = b[q] - acc;
+            y[q] = res;
+            rr = (double)res * (double)res;
+        } else {
+            y[q] = acc;
+        }
+    }
+    if (RESID) {
+        const double sum = block_sum(rr, s_red);
+        if (threadIdx.x == 0) partial[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = sum;
+    }
+}
+
+// y = A x in position order, one thread per face column (L <= LM, fully
+// unrolled): the neighbour positions are read once per column and the
+// vertical neighbours come from registers.  B is the block size.
+template <typename T, int LM, int B>
+__global__ void __launch_bounds__(B) k_pos_spmv_col(int64_t F, int L, const int32_t* __restrict__ nbp,
+                                                    const T* __restrict__ diag, const T* __restrict__ lat,
+                                                    const T* __restrict__ vert, const T* __restrict__ x,
+                                                    T* __restrict__ y) {
+    const int64_t k = (int64_t)blockIdx.x * B + threadIdx.x;
+    if (k >= F) return;
+    const int64_t N = F * L;
+    int32_t np_[3];
+#pragma unroll
+    for (int f = 0; f < 3; ++f) np_[f] = nbp[f * F + k];
+    T xv[LM];
+#pragma unroll
+    for (int z = 0; z < LM; ++z)
+        if (z < L) xv[z] = x[(int64_t)z * F + k];
+#pragma unroll
+    for (int z = 0; z < LM; ++z)
+        if (z < L) {
+            const int64_t q = (int64_t)z * F + k;
+            T acc = diag[q] * xv[z];
+            if (z > 0) acc = fma(vert[q], xv[z > 0 ? z - 1 : 0], acc);
+#pragma unroll
+            for (int f = 0; f < 3; ++f)
+                if (np_[f] >= 0) acc = fma(lat[f * N + q], x[(int64_t)z * F + np_[f]], acc);
+            if (z < L - 1) acc = fma(vert[N + q], xv[z + 1 < LM ? z + 1 : z], acc);
+            y[q] = acc;
+        }
+}
+
+// y = A x in position order, one thread per row, block size B (grid: F/B x L)
+template <typename T, int B>
+__global__ void __launch_bounds__(B) k_pos_spmv_row(int64_t F, int L, const int32_t* __restrict__ nbp,
+                                                    const T* __restrict__ diag, const T* __restrict__ lat,
+                                                    const T* __restrict__ vert, const T* __restrict__ x,
+                                                    T* __restrict__ y) {
+    const int64_t k = (int64_t)blockIdx.x * B + threadIdx.x;
+    if (k >= F) return;
+    const int z = blockIdx.y;
+    const int64_t N = F * L;
+    const int64_t q = (int64_t)z * F + k;
+    const int64_t lay = (int64_t)z * F;
+    T acc = diag[q] * x[q];
+    if (z > 0) acc = fma(vert[q], x[q - F], acc);
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+        const int32_t n = nbp[f * F + k];
+        if (n >= 0) acc = fma(lat[f * N + q], x[lay + n], acc);
+    }
+    if (z < L - 1) acc = fma(vert[N + q], x[q + F], acc);
+    y[q] = acc;
+}
+
+// reference order (z*F + face) <-> position order (z*F + k), with a type change
+template <typename Ti, typename To>
+__global__ void __launch_bounds__(kBlock) k_to_pos(int64_t F, const int32_t* __restrict__ faces,
+                                                   const Ti* __restrict__ in, To* __restrict__ out) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= F) return;
+    const int64_t lay = (int64_t)blockIdx.y * F;
+    out[lay + k] = (To)in[lay + faces[k]];
+}
+template <typename Ti, typename To>
+__global__ void __launch_bounds__(kBlock) k_from_pos(int64_t F, const int32_t* __restrict__ faces,
+                                                     const Ti* __restrict__ in, To* __restrict__ out) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= F) return;
+    const int64_t lay = (int64_t)blockIdx.y * F;
+    out[lay + faces[k]] = (To)in[lay + k];
+}
+
+__global__ void __launch_bounds__(kBlock) k_narrow(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
 }
 
 }  // namespace
@@ -197,24 +383,150 @@ void dilu_factor(hipStream_t s, const ColorOrder& co, int64_t F, int L, const in
         const int64_t k0 = co.off[c], k1 = co.off[c + 1];
         if (k1 == k0) continue;
         hipLaunchKernelGGL((k_dilu_factor<T>), dim3(grid_for(k1 - k0)), dim3(kBlock), 0, s, F, L, k0, k1, co.faces.p,
-                           co.nbp.p, co.rev.p, co.lowmask.p, rp, val, slots, d.lat.p, d.vert.p, d.dinv.p);
+                           co.nbp.p, co.rev.p, co.lowmask.p, rp, val, slots, d.lat.p, d.vert.p, d.dinv.p, d.diag.p);
     }
 }
 
-template <typename T>
-void dilu_apply(hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFactors<T>& d, const T* r, T* z) {
+namespace {
+template <typename T, bool POS, int LM>
+void dilu_apply_t(hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFactors<T>& d, const T* r, T* z) {
     for (int c = 0; c < co.ncolor; ++c) {
         const int64_t k0 = co.off[c], k1 = co.off[c + 1];
         if (k1 == k0) continue;
-        hipLaunchKernelGGL((k_dilu_fwd<T>), dim3(grid_for(k1 - k0)), dim3(kBlock), 0, s, F, L, k0, k1, co.faces.p,
-                           co.nbp.p, co.lowmask.p, d.lat.p, d.vert.p, d.dinv.p, r, d.y.p);
+        hipLaunchKernelGGL((k_dilu_fwd<T, POS, LM>), dim3(grid_for(k1 - k0)), dim3(kBlock), 0, s, F, L, k0, k1,
+                           co.faces.p, co.nbp.p, co.lowmask.p, d.lat.p, d.vert.p, d.dinv.p, r, d.y.p, d.y.p);
     }
+    // reference order: higher-colour results are read back from y (updated in place)
+    const T* done = POS ? z : d.y.p;
     for (int c = co.ncolor - 1; c >= 0; --c) {
         const int64_t k0 = co.off[c], k1 = co.off[c + 1];
         if (k1 == k0) continue;
-        hipLaunchKernelGGL((k_dilu_bwd<T>), dim3(grid_for(k1 - k0)), dim3(kBlock), 0, s, F, L, k0, k1, co.faces.p,
-                           co.nbp.p, co.lowmask.p, d.lat.p, d.vert.p, d.dinv.p, d.y.p, z);
+        hipLaunchKernelGGL((k_dilu_bwd<T, POS, LM>), dim3(grid_for(k1 - k0)), dim3(kBlock), 0, s, F, L, k0, k1,
+                           co.faces.p, co.nbp.p, co.lowmask.p, d.lat.p, d.vert.p, d.dinv.p, d.y.p, done, z);
     }
+}
+template <typename T, bool POS>
+void dilu_apply_l(hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFactors<T>& d, const T* r, T* z) {
+    if (L <= 1)
+        dilu_apply_t<T, POS, 1>(s, co, F, L, d, r, z);
+    else if (L <= 5)
+        dilu_apply_t<T, POS, 5>(s, co, F, L, d, r, z);
+    else if (L <= 10)
+        dilu_apply_t<T, POS, 10>(s, co, F, L, d, r, z);
+    else if (L <= 16)
+        dilu_apply_t<T, POS, 16>(s, co, F, L, d, r, z);
+    else
+        dilu_apply_t<T, POS, 0>(s, co, F, L, d, r, z);
+}
+}  // namespace
+
+template <typename T>
+void dilu_apply(hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFactors<T>& d, const T* r, T* z, bool pos) {
+    if (pos)
+        dilu_apply_l<T, true>(s, co, F, L, d, r, z);
+    else
+        dilu_apply_l<T, false>(s, co, F, L, d, r, z);
+}
+
+void dilu_narrow(hipStream_t s, int64_t N, const DiluFactors<double>& a, DiluFactors<float>& b) {
+    hipLaunchKernelGGL(k_narrow, dim3(grid_for(3 * N)), dim3(kBlock), 0, s, 3 * N, a.lat.p, b.lat.p);
+    hipLaunchKernelGGL(k_narrow, dim3(grid_for(2 * N)), dim3(kBlock), 0, s, 2 * N, a.vert.p, b.vert.p);
+    hipLaunchKernelGGL(k_narrow, dim3(grid_for(N)), dim3(kBlock), 0, s, N, a.dinv.p, b.dinv.p);
+    hipLaunchKernelGGL(k_narrow, dim3(grid_for(N)), dim3(kBlock), 0, s, N, a.diag.p, b.diag.p);
+}
+
+namespace {
+// the launch shapes the tuner chooses from (variant index -> kernel, block)
+template <typename T, int LM>
+void pos_spmv_col_launch(hipStream_t s, int v, int64_t F, int L, const ColorOrder& co, const DiluFactors<T>& d,
+                         const T* x, T* y) {
+    switch (v) {
+        case 1:
+            hipLaunchKernelGGL((k_pos_spmv_col<T, LM, 256>), dim3((unsigned)((F + 255) / 256)), dim3(256), 0, s, F, L,
+                               co.nbp.p, d.diag.p, d.lat.p, d.vert.p, x, y);
+            break;
+        case 2:
+            hipLaunchKernelGGL((k_pos_spmv_col<T, LM, 128>), dim3((unsigned)((F + 127) / 128)), dim3(128), 0, s, F, L,
+                               co.nbp.p, d.diag.p, d.lat.p, d.vert.p, x, y);
+            break;
+        default:
+            hipLaunchKernelGGL((k_pos_spmv_col<T, LM, 64>), dim3((unsigned)((F + 63) / 64)), dim3(64), 0, s, F, L,
+                               co.nbp.p, d.diag.p, d.lat.p, d.vert.p, x, y);
+            break;
+    }
+}
+}  // namespace
+
+template <typename T>
+void pos_spmv_variant(hipStream_t s, int v, const ColorOrder& co, int64_t F, int L, const DiluFactors<T>& d,
+                      const T* x, T* y) {
+    if (v >= 1 && v <= 3 && L <= 16) {
+        if (L <= 5)
+            pos_spmv_col_launch<T, 5>(s, v, F, L, co, d, x, y);
+        else if (L <= 10)
+            pos_spmv_col_launch<T, 10>(s, v, F, L, co, d, x, y);
+        else
+            pos_spmv_col_launch<T, 16>(s, v, F, L, co, d, x, y);
+        return;
+    }
+    if (v == 4) {
+        const dim3 g((unsigned)((F + 127) / 128), (unsigned)L);
+        hipLaunchKernelGGL((k_pos_spmv_row<T, 128>), g, dim3(128), 0, s, F, L, co.nbp.p, d.diag.p, d.lat.p,
+                           d.vert.p, x, y);
+        return;
+    }
+    const dim3 g((unsigned)((F + 255) / 256), (unsigned)L);
+    hipLaunchKernelGGL((k_pos_spmv_row<T, 256>), g, dim3(256), 0, s, F, L, co.nbp.p, d.diag.p, d.lat.p, d.vert.p,
+                       x, y);
+}
+
+template <typename T>
+int tune_pos_spmv(hipStream_t s, const ColorOrder& co, int64_t F, int L, const DiluFactors<T>& d, const T* x, T* y,
+                  float* ms_out) {
+    hipEvent_t e0, e1;
+    PB_HIP(hipEventCreate(&e0));
+    PB_HIP(hipEventCreate(&e1));
+    int best = 0;
+    float best_ms = 1e30f;
+    for (int v = 0; v < kPosSpmvVariants; ++v) {
+        if (v >= 1 && v <= 3 && L > 16) continue;
+        for (int w = 0; w < 2; ++w) pos_spmv_variant<T>(s, v, co, F, L, d, x, y);
+        PB_HIP(hipEventRecord(e0, s));
+        for (int w = 0; w < 5; ++w) pos_spmv_variant<T>(s, v, co, F, L, d, x, y);
+        PB_HIP(hipEventRecord(e1, s));
+        PB_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        PB_HIP(hipEventElapsedTime(&ms, e0, e1));
+        ms /= 5;
+        if (ms_out) ms_out[v] = ms;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = v;
+        }
+    }
+    PB_HIP(hipGetLastError());
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return best;
+}
+
+template <typename T>
+void pos_spmv(hipStream_t s, const ColorOrder& co, int64_t F, int L, const DiluFactors<T>& d, const T* x, const T* b,
+              T* y, double* partial) {
+    const dim3 g((unsigned)((F + kBlock - 1) / kBlock), (unsigned)L);
+    hipLaunchKernelGGL((k_pos_spmv<T, true>), g, dim3(kBlock), 0, s, F, L, co.nbp.p, d.diag.p, d.lat.p, d.vert.p, x,
+                       b, y, partial);
+}
+
+template <typename Ti, typename To>
+void to_pos(hipStream_t s, const ColorOrder& co, int64_t F, int L, const Ti* in, To* out) {
+    const dim3 g((unsigned)((F + kBlock - 1) / kBlock), (unsigned)L);
+    hipLaunchKernelGGL((k_to_pos<Ti, To>), g, dim3(kBlock), 0, s, F, co.faces.p, in, out);
+}
+template <typename Ti, typename To>
+void from_pos(hipStream_t s, const ColorOrder& co, int64_t F, int L, const Ti* in, To* out) {
+    const dim3 g((unsigned)((F + kBlock - 1) / kBlock), (unsigned)L);
+    hipLaunchKernelGGL((k_from_pos<Ti, To>), g, dim3(kBlock), 0, s, F, co.faces.p, in, out);
 }
 
 template void dilu_factor<double>(hipStream_t, const ColorOrder&, int64_t, int, const int32_t*, const double*,
@@ -222,8 +534,22 @@ template void dilu_factor<double>(hipStream_t, const ColorOrder&, int64_t, int, 
 template void dilu_factor<float>(hipStream_t, const ColorOrder&, int64_t, int, const int32_t*, const float*,
                                  const uint16_t*, DiluFactors<float>&);
 template void dilu_apply<double>(hipStream_t, const ColorOrder&, int64_t, int, DiluFactors<double>&, const double*,
-                                 double*);
+                                 double*, bool);
 template void dilu_apply<float>(hipStream_t, const ColorOrder&, int64_t, int, DiluFactors<float>&, const float*,
-                                float*);
+                                float*, bool);
+template void pos_spmv<double>(hipStream_t, const ColorOrder&, int64_t, int, const DiluFactors<double>&,
+                               const double*, const double*, double*, double*);
+template void pos_spmv_variant<double>(hipStream_t, int, const ColorOrder&, int64_t, int, const DiluFactors<double>&,
+                                       const double*, double*);
+template void pos_spmv_variant<float>(hipStream_t, int, const ColorOrder&, int64_t, int, const DiluFactors<float>&,
+                                      const float*, float*);
+template int tune_pos_spmv<double>(hipStream_t, const ColorOrder&, int64_t, int, const DiluFactors<double>&,
+                                   const double*, double*, float*);
+template int tune_pos_spmv<float>(hipStream_t, const ColorOrder&, int64_t, int, const DiluFactors<float>&,
+                                  const float*, float*, float*);
+template void pos_spmv<float>(hipStream_t, const ColorOrder&, int64_t, int, const DiluFactors<float>&, const float*,
+                              const float*, float*, double*);
+template void to_pos<double, double>(hipStream_t, const ColorOrder&, int64_t, int, const double*, double*);
+template void from_pos<double, double>(hipStream_t, const ColorOrder&, int64_t, int, const double*, double*);
 
 }  // namespace pb

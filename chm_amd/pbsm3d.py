@@ -29,6 +29,8 @@ EXPORTED = (
     "pbsm3d_last_error",
     "pbsm3d_profile_enable",
     "pbsm3d_profile_read",
+    "pbsm3d_spmv_tuning",
+    "pbsm3d_spmv_select",
     "pbsm3d_mlp_create",
     "pbsm3d_mlp_destroy",
     "pbsm3d_mlp_param_count",
@@ -45,7 +47,12 @@ EXPORTED = (
 
 N_FEATURES, N_TARGETS = 16, 11
 
-KERNEL_CLASSES = ("assemble", "spmv64", "spmv32", "precond", "ortho", "residual", "deposition", "warmstart")
+# launch shapes of the position-order SpMV: 0 / 4 one thread per row (blocks of 256 / 128),
+# 1 / 2 / 3 one thread per face column (blocks of 256 / 128 / 64)
+SPMV_VARIANTS = 5
+
+KERNEL_CLASSES = ("assemble", "spmv64", "spmv32", "precond", "ortho", "residual", "deposition", "warmstart",
+                  "factor")
 
 _lib = None
 
@@ -89,6 +96,8 @@ def lib():
         L.pbsm3d_last_error.restype = C.c_char_p
         L.pbsm3d_profile_enable.argtypes = [vp, C.c_int]
         L.pbsm3d_profile_read.argtypes = [vp, C.c_int, C.POINTER(C.c_int64), dp, C.c_int]
+        L.pbsm3d_spmv_tuning.argtypes = [vp, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_float), C.c_int32]
+        L.pbsm3d_spmv_select.argtypes = [vp, C.c_int, C.c_int]
         fp = C.POINTER(C.c_float)
         L.pbsm3d_mlp_create.argtypes = [C.POINTER(vp), C.c_int, C.c_uint64]
         L.pbsm3d_mlp_destroy.argtypes = [vp]
@@ -204,6 +213,18 @@ class Context:
             _check(lib().pbsm3d_profile_read(self._h, k, C.byref(n), C.byref(ms), 1 if reset else 0), self._h)
             out[name] = (int(n.value), float(ms.value))
         return out
+
+    def spmv_tuning(self, which: int = 0) -> dict:
+        """Launch shape the SpMV tuner chose for the FP64 (which=0) or FP32 (1) operator and
+        the average time of every shape tried (ms); chosen = -1 before the first solve."""
+        n = SPMV_VARIANTS
+        chosen, ms = C.c_int32(-1), (C.c_float * n)()
+        _check(lib().pbsm3d_spmv_tuning(self._h, which, C.byref(chosen), ms, n), self._h)
+        return {"chosen": int(chosen.value), "ms": [round(float(v), 5) for v in ms]}
+
+    def spmv_select(self, which: int, variant: int):
+        """Fix the SpMV launch shape of the FP64 (0) / FP32 (1) operator; -1 re-tunes."""
+        _check(lib().pbsm3d_spmv_select(self._h, which, variant), self._h)
 
     def training_pairs(self, X, Y):
         """Write features [F x 16] and solved targets [F x 11] of the last step into

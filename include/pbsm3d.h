@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define PBSM3D_ABI_VERSION 1
+#define PBSM3D_ABI_VERSION 2
 
 /* return codes */
 #define PBSM3D_OK 0
@@ -159,6 +159,9 @@ typedef struct pbsm3d_stats {
     double ms_post;
     double ms_dep_solve;
     double ms_total;
+    double susp_resid0;           /* ||b - A x0|| / ||b|| of the suspension start   */
+    int32_t susp_cold_fallback;   /* 1: the warm start was worse than x0 = 0 and    */
+    int32_t reserved2;            /*    the solve restarted from zero               */
 } pbsm3d_stats;
 
 typedef struct pbsm3d_ctx pbsm3d_ctx;
@@ -212,9 +215,24 @@ int pbsm3d_abi_version(void);
 #define PBSM3D_K_RESIDUAL 5    /* FP64 residual b - A x (fused norm)           */
 #define PBSM3D_K_DEPOSITION 6  /* post-processing + deposition assembly + solve */
 #define PBSM3D_K_WARMSTART 7   /* learned warm start (network forward)         */
-#define PBSM3D_K_COUNT 8
+#define PBSM3D_K_FACTOR 8      /* suspension D-ILU factorisation (once a step) */
+#define PBSM3D_K_COUNT 9
 int pbsm3d_profile_enable(pbsm3d_ctx* ctx, int enable);
 int pbsm3d_profile_read(pbsm3d_ctx* ctx, int kernel, int64_t* launches, double* ms_total, int reset);
+
+/* Launch shape of the suspension SpMV (no counterpart in the reference).  With
+ * the multicolour D-ILU preconditioner the suspension solve runs in the
+ * preconditioner's position order and its SpMV has several launch shapes (one
+ * thread per row or per face column, several block sizes); the first solve of
+ * each precision times them all with HIP events and keeps the fastest.
+ * which: 0 = FP64 operator, 1 = FP32 operator (MIXED).  *chosen = -1 until
+ * tuned; ms[v] (n entries, may be NULL) = average time of shape v in ms (0 if
+ * not tried). */
+int pbsm3d_spmv_tuning(const pbsm3d_ctx* ctx, int which, int32_t* chosen, float* ms, int32_t n);
+/* Fixes the launch shape (0 .. PBSM3D_SPMV_VARIANTS-1) instead of the tuner's
+ * choice, or -1 to tune again at the next solve. */
+#define PBSM3D_SPMV_VARIANTS 5
+int pbsm3d_spmv_select(pbsm3d_ctx* ctx, int which, int variant);
 
 /* Learned warm start (an extension; the reference always starts from x0 = 0,
  * LinearAlgebra.cpp:199-206).  A per-face network 16 -> 128 -> 128 -> 128 -> 11

@@ -233,3 +233,29 @@ def test_options_parity(pb, oracle, mesh_small):
         np.testing.assert_allclose(gg["Qsalt"], og["Qsalt"], rtol=1e-12, atol=0, err_msg=str(kw))
         for k in ("conc", "Qsusp", "drift_mass"):
             assert rel_err(gg[k], og[k]) < 1e-7, (kw, k)
+
+
+@pytest.mark.parametrize("precision", [abi.FP64, abi.MIXED])
+def test_every_spmv_launch_shape(pb, oracle, mesh_c1, precision):
+    """Each launch shape of the position-order SpMV, forced in turn, gives the
+    oracle's fields; the tuner's choice is one of the shapes and every shape
+    was timed."""
+    _print_case("spmv_launch_shapes", precision=precision)
+    F = mesh_c1.n_faces
+    fo = meshgen.forcing_c1(F, seed=0)
+    g, ctx, orc = _pair(pb, oracle, mesh_c1, L=10, precision=precision, tol=1e-12)
+    og, _ = orc.step(fo)
+    which = 0 if precision == abi.FP64 else 1
+    gg, st = ctx.step(fo, want_fields=True)
+    tun = ctx.spmv_tuning(which)
+    assert 0 <= tun["chosen"] < pb.SPMV_VARIANTS
+    assert all(ms > 0 for ms in tun["ms"]), tun
+    for v in range(pb.SPMV_VARIANTS):
+        ctx.spmv_select(which, v)
+        ctx.set_state(np.zeros(F), np.zeros(F))
+        gg, st = ctx.step(fo, want_fields=True)
+        assert st["susp_resid"] <= 1e-12
+        for k in ("conc", "Qsusp", "qdep", "drift_mass"):
+            assert rel_err(gg[k], og[k]) < 1e-8, (v, k, rel_err(gg[k], og[k]))
+    with pytest.raises(pb.Pbsm3dError):
+        ctx.spmv_select(which, pb.SPMV_VARIANTS)

@@ -115,7 +115,9 @@ def test_warm_start_gain_c1():
     warm.set_warm_start(net)
     oc, sc = cold.step(held, want_fields=True)
     ow, sw = warm.step(held, want_fields=True)
-    print("[warm] iters cold", sc["susp_iters"], sc["dep_iters"], "warm", sw["susp_iters"], sw["dep_iters"], flush=True)
+    print("[warm] iters cold", sc["susp_iters"], sc["dep_iters"], "warm", sw["susp_iters"], sw["dep_iters"],
+          "warm r0/|b|", sw["susp_resid0"], "fallback", sw["susp_cold_fallback"], flush=True)
+    assert sc["susp_resid0"] == 1.0 and not sc["susp_cold_fallback"]
     assert sw["susp_resid"] <= 1e-8 and sw["dep_resid"] <= 1e-8
     assert sw["susp_iters"] < sc["susp_iters"]
     for k in ("Qsusp", "drift_mass"):
