@@ -159,6 +159,9 @@ def main():
     ap.add_argument("--cpu-faces", type=int, default=200000)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="run one streaming kernel (y = -x, float32) reading and writing this many MB first: "
+                         "the calibration dispatch for rocprofv3 FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -183,6 +186,16 @@ def main():
     dev_outs = {k: torch.zeros(F, dtype=torch.float64, device="cuda:0") for k in abi.OUT_FIELDS}
     krylov_mixed = abi.KRYLOV_BICGSTAB if args.krylov_mixed == "bicgstab" else abi.KRYLOV_GMRES
 
+    if args.copy_probe_mb:
+        n = args.copy_probe_mb * 1000000 // 4
+        xs = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        ys = torch.empty_like(xs)
+        torch.cuda.synchronize()
+        torch.neg(xs, out=ys)
+        torch.cuda.synchronize()
+        log(f"[bench] calibration kernel: read {4 * n} B, wrote {4 * n} B")
+        del xs, ys
+
     names = args.variants.split(",")
     net, train_info = None, None
     if any(n in ("mixed_learned", "learned") for n in names):
@@ -192,6 +205,7 @@ def main():
         log(f"[bench] trained: {train_info}")
 
     variants = {}
+    plain_outs = None
     for name in names:
         cfg, learned = variant_config(pb, abi, name, L, krylov_mixed)
         log(f"[bench] variant {name}: warmup {args.warmup}, steps {args.steps}")
@@ -217,6 +231,15 @@ def main():
                         for k, (n, ms) in prof.items() if n},
             "nnz": nnz,
         }
+        # per-field difference of this variant's last-step outputs from the plain FP64 variant's
+        # (both solved to ||b - A x|| <= 1e-8 ||b||): max|x - x_plain| / max|x_plain|
+        outs_now = {k: v.clone() for k, v in dev_outs.items()}
+        if name == "plain":
+            plain_outs = outs_now
+        elif plain_outs is not None:
+            variants[name]["field_rel_diff_vs_plain"] = {
+                k: float((outs_now[k] - plain_outs[k]).abs().max() / plain_outs[k].abs().max().clamp_min(1e-300))
+                for k in ("Qsalt", "Qsusp", "Qsubl", "drift_mass", "sum_drift")}
         log(f"[bench] {name}: {variants[name]['steps_per_s']:.2f} steps/s, iters {st['susp_iters']}/{st['dep_iters']}")
 
     N = F * L

@@ -188,6 +188,10 @@ SolveInfo solve_in_position_order(pbsm3d_ctx* c, const pbsm3d_config& cfg, int64
         c->spmv_variant[0] = tune_pos_spmv<double>(s, c->co, F, L, c->dl_s, c->xp.p, c->r_s.p, c->spmv_tune_ms[0]);
     if (cfg.precision == PBSM3D_MIXED && c->spmv_variant[1] < 0)
         c->spmv_variant[1] = tune_pos_spmv<float>(s, c->co, F, L, c->dl32, c->r32.p, c->d32.p, c->spmv_tune_ms[1]);
+    if (cfg.precision == PBSM3D_FP64 && c->dilu_xcd[0] < 0)
+        c->dilu_xcd[0] = tune_dilu_xcd<double>(s, c->co, F, L, c->dl_s, c->bp.p, c->r_s.p, c->dilu_tune_ms[0]);
+    if (cfg.precision == PBSM3D_MIXED && c->dilu_xcd[1] < 0)
+        c->dilu_xcd[1] = tune_dilu_xcd<float>(s, c->co, F, L, c->dl32, c->r32.p, c->d32.p, c->dilu_tune_ms[1]);
     CsrOp<double> A{N, nullptr, nullptr, nullptr, &c->prof, F, L};
     A.co = &c->co;
     A.pos = &c->dl_s;
@@ -195,7 +199,7 @@ SolveInfo solve_in_position_order(pbsm3d_ctx* c, const pbsm3d_config& cfg, int64
     SolveInfo si;
     if (cfg.precision == PBSM3D_FP64) {
         PcFn<double> M = [c, F, L](hipStream_t st, const double* r, double* z) {
-            dilu_apply<double>(st, c->co, F, L, c->dl_s, r, z, true);
+            dilu_apply<double>(st, c->co, F, L, c->dl_s, r, z, true, c->dilu_xcd[0] == 1);
         };
         if (bicg)
             si = bicgstab_solve(s, c->bw64, c->ws64, A, M, c->bp.p, c->xp.p, cfg.max_iters, cfg.tol);
@@ -207,7 +211,7 @@ SolveInfo solve_in_position_order(pbsm3d_ctx* c, const pbsm3d_config& cfg, int64
         A32.pos = &c->dl32;
         A32.variant = c->spmv_variant[1];
         PcFn<float> M = [c, F, L](hipStream_t st, const float* r, float* z) {
-            dilu_apply<float>(st, c->co, F, L, c->dl32, r, z, true);
+            dilu_apply<float>(st, c->co, F, L, c->dl32, r, z, true, c->dilu_xcd[1] == 1);
         };
         if (bicg)
             si = bicgstab_mixed(s, c->bw32, c->ws32, A, A32, M, c->bp.p, c->xp.p, c->r_s.p, c->r32.p, c->d32.p,
@@ -676,9 +680,38 @@ int pbsm3d_spmv_tuning(const pbsm3d_ctx* c, int which, int32_t* chosen, float* m
     return PBSM3D_OK;
 }
 
+int pbsm3d_face_colors(int64_t n_faces, const int32_t* neigh, int32_t* color) {
+    if (n_faces <= 0 || !neigh || !color) return PBSM3D_EINVAL;
+    for (int64_t k = 0; k < 3 * n_faces; ++k)
+        if (neigh[k] < -1 || neigh[k] >= n_faces) return PBSM3D_EINVAL;
+    try {
+        return face_colors(n_faces, neigh, color);
+    } catch (const std::exception& e) {
+        set_global_error(e.what());
+        return PBSM3D_EINVAL;
+    }
+}
+
+int pbsm3d_precond_tuning(const pbsm3d_ctx* c, int which, int32_t* xcd, float* ms) {
+    if (!c || (which != 0 && which != 1)) return PBSM3D_EINVAL;
+    if (xcd) *xcd = c->dilu_xcd[which];
+    if (ms) {
+        ms[0] = c->dilu_tune_ms[which][0];
+        ms[1] = c->dilu_tune_ms[which][1];
+    }
+    return PBSM3D_OK;
+}
+
+int pbsm3d_precond_select(pbsm3d_ctx* c, int which, int xcd) {
+    if (!c || (which != 0 && which != 1) || xcd < -1 || xcd > 1) return PBSM3D_EINVAL;
+    c->dilu_xcd[which] = xcd;
+    return PBSM3D_OK;
+}
+
 int pbsm3d_spmv_select(pbsm3d_ctx* c, int which, int variant) {
     if (!c || (which != 0 && which != 1) || variant < -1 || variant >= kPosSpmvVariants) return PBSM3D_EINVAL;
-    if (variant >= 1 && variant <= 3 && c->L > 16) return PBSM3D_EINVAL;  // column shapes need L <= 16
+    const int shape = variant % 5;
+    if (variant >= 0 && shape >= 1 && shape <= 3 && c->L > 16) return PBSM3D_EINVAL;  // column shapes need L <= 16
     c->spmv_variant[which] = variant;
     return PBSM3D_OK;
 }

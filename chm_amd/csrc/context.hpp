@@ -35,6 +35,9 @@ struct pbsm3d_ctx {
     // position-order SpMV launch shape per precision (index 0: FP64, 1: FP32), -1 until tuned
     int spmv_variant[2] = {-1, -1};
     float spmv_tune_ms[2][pb::kPosSpmvVariants] = {};
+    // D-ILU sweeps with the XCD workgroup mapping (1) or without (0), -1 until tuned
+    int dilu_xcd[2] = {-1, -1};
+    float dilu_tune_ms[2][2] = {};
     pb::DevBuf<float> val32, r32, d32;
     pb::GmresWork<double> ws64, wd64;
     pb::GmresWork<float> ws32;

@@ -26,6 +26,25 @@ __device__ __forceinline__ double block_sum(double v, double* lds) {
     return s;
 }
 
+// Workgroups of a launch are dispatched to the 8 XCDs round-robin (workgroup b
+// runs on XCD b % 8), each XCD with its own L2.  With XCD = true a launch of
+// launch_blocks(nb) workgroups covers nb logical blocks so that XCD j works on
+// the j-th contiguous eighth of them: neighbouring faces, whose gathers share
+// cache lines, then meet in one L2.  Logical blocks >= nb do nothing (the
+// kernels' own range checks drop them).
+constexpr int kXcd = 8;
+template <bool XCD>
+inline int64_t launch_blocks(int64_t nb) {
+    return XCD ? (nb + kXcd - 1) / kXcd * kXcd : nb;
+}
+template <bool XCD>
+__device__ __forceinline__ int64_t logical_block(int64_t nb) {
+    const int64_t b = blockIdx.x;
+    if (!XCD) return b;
+    const int64_t per = (nb + kXcd - 1) / kXcd;
+    return (b % kXcd) * per + b / kXcd;
+}
+
 struct AssembleArgs {
     int64_t F;
     int L;
@@ -159,13 +178,20 @@ struct DiluFactors {
     }
 };
 void build_color_order(ColorOrder& co, int64_t F, const int32_t* neigh_fm, hipStream_t s);
+// the colouring alone (host): color[F], returns the number of colours
+int face_colors(int64_t F, const int32_t* neigh_fm, int32_t* color);
 template <typename T>
 void dilu_factor(hipStream_t s, const ColorOrder& co, int64_t F, int L, const int32_t* rp, const T* val,
                  const uint16_t* slots, DiluFactors<T>& d);
 // z = M^-1 r.  pos: r and z in position order (q = z*F + k), else in the reference order.
 template <typename T>
 void dilu_apply(hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFactors<T>& d, const T* r, T* z,
-                bool pos = false);
+                bool pos = false, bool xcd = false);
+// times the position-order apply with and without the XCD workgroup mapping (HIP events on the
+// stream); true if the mapping is faster.  ms_out[2] receives both averages (ms), if given.
+template <typename T>
+bool tune_dilu_xcd(hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFactors<T>& d, const T* r, T* z,
+                   float* ms_out);
 // FP32 copy of FP64 factors (the FP32 preconditioner and operator of the mixed solve)
 void dilu_narrow(hipStream_t s, int64_t N, const DiluFactors<double>& a, DiluFactors<float>& b);
 // y = b - A x with per-block partial sums of y.y, everything in position order, A taken
@@ -173,8 +199,9 @@ void dilu_narrow(hipStream_t s, int64_t N, const DiluFactors<double>& a, DiluFac
 template <typename T>
 void pos_spmv(hipStream_t s, const ColorOrder& co, int64_t F, int L, const DiluFactors<T>& d, const T* x, const T* b,
               T* y, double* partial);
-// y = A x in position order with launch shape v: 0 / 4 one thread per row (blocks of 256 / 128),
-// 1 / 2 / 3 one thread per face column (blocks of 256 / 128 / 64; L <= 16)
+// y = A x in position order with launch shape v: v % 5 = 0 / 4 one thread per row (blocks of
+// 256 / 128), 1 / 2 / 3 one thread per face column (blocks of 256 / 128 / 64; L <= 16); v >= 5
+// with the XCD workgroup mapping
 constexpr int kPosSpmvVariants = PBSM3D_SPMV_VARIANTS;
 template <typename T>
 void pos_spmv_variant(hipStream_t s, int v, const ColorOrder& co, int64_t F, int L, const DiluFactors<T>& d,

@@ -92,14 +92,19 @@ __global__ void __launch_bounds__(kBlock) k_dilu_factor(int64_t F, int L, int64_
 // elements of one array; the two restrict pointers let the loads run ahead
 // of the stores.  LM > 0: the column (L <= LM) is fully unrolled -- first all
 // terms that do not depend on the vertical recurrence, then the recurrence.
-template <typename T, bool POS, int LM>
+// TOP: this is the highest colour, whose faces have no higher-colour
+// neighbours, so its backward sweep (z = y - D~^-1 U z, U here only the
+// coupling above) runs in the same thread right after: z goes to zout (and, in
+// the reference order, also back into y, where lower colours read it).
+// XCD: workgroups are remapped so each XCD works on one contiguous range.
+template <typename T, bool POS, int LM, bool TOP, bool XCD>
 __global__ void __launch_bounds__(kBlock) k_dilu_fwd(int64_t F, int L, int64_t k0, int64_t k1,
                                                      const int32_t* __restrict__ faces, const int32_t* __restrict__ nbp,
                                                      const uint8_t* __restrict__ lowmask, const T* __restrict__ lat,
                                                      const T* __restrict__ vert, const T* __restrict__ dinv,
                                                      const T* __restrict__ rin, const T* __restrict__ ynb,
-                                                     T* __restrict__ y) {
-    const int64_t k = k0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+                                                     T* __restrict__ y, T* __restrict__ zout) {
+    const int64_t k = k0 + logical_block<XCD>((k1 - k0 + kBlock - 1) / kBlock) * kBlock + threadIdx.x;
     if (k >= k1) return;
     const int64_t N = F * L;
     const int64_t i = POS ? k : (int64_t)faces[k];
@@ -116,7 +121,7 @@ __global__ void __launch_bounds__(kBlock) k_dilu_fwd(int64_t F, int L, int64_t k
         return acc;
     };
     if constexpr (LM > 0) {
-        T sv[LM], vv[LM], dv[LM];
+        T sv[LM], vv[LM], dv[LM], va[LM];
 #pragma unroll
         for (int z = 0; z < LM; ++z)
             if (z < L) {
@@ -124,14 +129,28 @@ __global__ void __launch_bounds__(kBlock) k_dilu_fwd(int64_t F, int L, int64_t k
                 sv[z] = independent(z);
                 vv[z] = z > 0 ? vert[q] : T(0);
                 dv[z] = dinv[q];
+                if (TOP) va[z] = z < L - 1 ? vert[N + q] : T(0);
             }
         T prev = 0;
 #pragma unroll
         for (int z = 0; z < LM; ++z)
             if (z < L) {
                 prev = (sv[z] - vv[z] * prev) * dv[z];
-                y[(int64_t)z * F + k] = prev;
+                if (TOP)
+                    sv[z] = prev;
+                else
+                    y[(int64_t)z * F + k] = prev;
             }
+        if (TOP) {
+            T next = 0;
+#pragma unroll
+            for (int z = LM - 1; z >= 0; --z)
+                if (z < L) {
+                    next = sv[z] - dv[z] * (T(0) + va[z] * next);
+                    if (!POS) y[(int64_t)z * F + k] = next;
+                    zout[(int64_t)z * F + i] = next;
+                }
+        }
     } else {
         T prev = 0;
         for (int z = 0; z < L; ++z) {
@@ -141,21 +160,33 @@ __global__ void __launch_bounds__(kBlock) k_dilu_fwd(int64_t F, int L, int64_t k
             prev = acc * dinv[q];
             y[q] = prev;
         }
+        if (TOP) {
+            T next = 0;
+            for (int z = L - 1; z >= 0; --z) {
+                const int64_t q = (int64_t)z * F + k;
+                T acc = 0;
+                if (z < L - 1) acc += vert[N + q] * next;
+                next = y[q] - dinv[q] * acc;
+                if (!POS) y[q] = next;
+                zout[(int64_t)z * F + i] = next;
+            }
+        }
     }
 }
 
 // Backward sweep of one colour: z = y - D~^-1 U z.  Reference order: y is
-// updated in place (higher-colour neighbours are read from it through ynb) and
-// z is scattered to z*F + face.  POS: z is in position order, the neighbours
-// are read from z itself (znb) and y is only read.  LM as in the forward sweep.
-template <typename T, bool POS, int LM>
+// updated in place (higher-colour neighbours are read from it through done)
+// and z is scattered to z*F + face.  POS: z is in position order, the
+// neighbours are read from z itself (done) and y is only read.  LM and XCD as
+// in the forward sweep.
+template <typename T, bool POS, int LM, bool XCD>
 __global__ void __launch_bounds__(kBlock) k_dilu_bwd(int64_t F, int L, int64_t k0, int64_t k1,
                                                      const int32_t* __restrict__ faces, const int32_t* __restrict__ nbp,
                                                      const uint8_t* __restrict__ lowmask, const T* __restrict__ lat,
                                                      const T* __restrict__ vert, const T* __restrict__ dinv,
                                                      T* __restrict__ y, const T* __restrict__ done,
                                                      T* __restrict__ zout) {
-    const int64_t k = k0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t k = k0 + logical_block<XCD>((k1 - k0 + kBlock - 1) / kBlock) * kBlock + threadIdx.x;
     if (k >= k1) return;
     const int64_t N = F * L;
     const int64_t i = POS ? k : (int64_t)faces[k];
@@ -249,12 +280,12 @@ __global__ void __launch_bounds__(kBlock) k_pos_spmv(int64_t F, int L, const int
 // y = A x in position order, one thread per face column (L <= LM, fully
 // unrolled): the neighbour positions are read once per column and the
 // vertical neighbours come from registers.  B is the block size.
-template <typename T, int LM, int B>
+template <typename T, int LM, int B, bool XCD>
 __global__ void __launch_bounds__(B) k_pos_spmv_col(int64_t F, int L, const int32_t* __restrict__ nbp,
                                                     const T* __restrict__ diag, const T* __restrict__ lat,
                                                     const T* __restrict__ vert, const T* __restrict__ x,
                                                     T* __restrict__ y) {
-    const int64_t k = (int64_t)blockIdx.x * B + threadIdx.x;
+    const int64_t k = logical_block<XCD>((F + B - 1) / B) * B + threadIdx.x;
     if (k >= F) return;
     const int64_t N = F * L;
     int32_t np_[3];
@@ -279,12 +310,12 @@ __global__ void __launch_bounds__(B) k_pos_spmv_col(int64_t F, int L, const int3
 }
 
 // y = A x in position order, one thread per row, block size B (grid: F/B x L)
-template <typename T, int B>
+template <typename T, int B, bool XCD>
 __global__ void __launch_bounds__(B) k_pos_spmv_row(int64_t F, int L, const int32_t* __restrict__ nbp,
                                                     const T* __restrict__ diag, const T* __restrict__ lat,
                                                     const T* __restrict__ vert, const T* __restrict__ x,
                                                     T* __restrict__ y) {
-    const int64_t k = (int64_t)blockIdx.x * B + threadIdx.x;
+    const int64_t k = logical_block<XCD>((F + B - 1) / B) * B + threadIdx.x;
     if (k >= F) return;
     const int z = blockIdx.y;
     const int64_t N = F * L;
@@ -326,9 +357,21 @@ __global__ void __launch_bounds__(kBlock) k_narrow(int64_t n, const double* __re
 
 }  // namespace
 
-void build_color_order(ColorOrder& co, int64_t F, const int32_t* nb, hipStream_t s) {
+// Colouring of the face graph (every face has at most 3 neighbours).  Greedy in
+// index order gives at most 4 colours; faces left with colour 3 are then
+// repaired in index order (Brooks: graphs of maximum degree 3 other than K4 are
+// 3-colourable).  repair(v): take a colour below 3 that no neighbour holds, else
+// for the colour pairs (ca, cb) = (0, 1), (0, 2), (1, 2) in turn, with a / b the
+// neighbours of colours ca / cb, collect the connected set of faces reachable
+// from a through colours {ca, cb} (at most kKempeCap faces); if b is not in it,
+// swap ca and cb on the set (a Kempe swap) and give v colour ca.  A face that
+// repair cannot fix trades colours with a neighbour (v takes the neighbour's
+// colour c, for c = 0, 1, 2 in turn) when repair then fixes that neighbour.  A
+// fourth colour costs two nearly empty launches per preconditioner apply.
+constexpr int64_t kKempeCap = 4096;
+
+static std::vector<int32_t> color_faces(int64_t F, const int32_t* nb) {
     std::vector<int32_t> color(F, -1);
-    int nc = 0;
     for (int64_t i = 0; i < F; ++i) {
         unsigned used = 0;
         for (int f = 0; f < 3; ++f) {
@@ -338,8 +381,77 @@ void build_color_order(ColorOrder& co, int64_t F, const int32_t* nb, hipStream_t
         int c = 0;
         while (used & (1u << c)) ++c;
         color[i] = c;
-        nc = std::max(nc, c + 1);
     }
+    std::vector<int64_t> stamp(F, -1), chain;
+    int64_t visit = 0;
+    auto repair = [&](int64_t v) -> bool {
+        unsigned used = 0;
+        for (int f = 0; f < 3; ++f)
+            if (nb[3 * v + f] >= 0) used |= 1u << color[nb[3 * v + f]];
+        for (int c = 0; c < 3; ++c)
+            if (!(used & (1u << c))) {
+                color[v] = c;
+                return true;
+            }
+        int32_t by_color[3] = {-1, -1, -1};
+        for (int f = 0; f < 3; ++f) by_color[color[nb[3 * v + f]]] = nb[3 * v + f];
+        static const int pairs[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+        for (int pi = 0; pi < 3; ++pi) {
+            const int32_t ca = pairs[pi][0], cb = pairs[pi][1];
+            const int32_t a = by_color[ca], b = by_color[cb];
+            ++visit;
+            chain.clear();
+            chain.push_back(a);
+            stamp[a] = visit;
+            bool hit_b = false;
+            for (size_t h = 0; h < chain.size() && !hit_b && (int64_t)chain.size() <= kKempeCap; ++h) {
+                const int32_t u = (int32_t)chain[h];
+                for (int f = 0; f < 3; ++f) {
+                    const int32_t w = nb[3 * (int64_t)u + f];
+                    if (w < 0 || stamp[w] == visit) continue;
+                    if (color[w] != ca && color[w] != cb) continue;
+                    if (w == b) hit_b = true;
+                    stamp[w] = visit;
+                    chain.push_back(w);
+                }
+            }
+            if (hit_b || (int64_t)chain.size() > kKempeCap) continue;
+            for (int64_t u : chain) color[u] = color[u] == ca ? cb : ca;
+            color[v] = ca;
+            return true;
+        }
+        return false;
+    };
+    for (int64_t v = 0; v < F; ++v)
+        if (color[v] == 3 && !repair(v)) {
+            int32_t by_color[3] = {-1, -1, -1};
+            for (int f = 0; f < 3; ++f) by_color[color[nb[3 * v + f]]] = nb[3 * v + f];
+            for (int c = 0; c < 3; ++c) {
+                const int32_t a = by_color[c];
+                color[v] = c;
+                color[a] = 3;
+                if (repair(a)) break;
+                color[a] = c;
+                color[v] = 3;
+            }
+        }
+    return color;
+}
+
+int face_colors(int64_t F, const int32_t* nb, int32_t* color) {
+    const std::vector<int32_t> c = color_faces(F, nb);
+    int nc = 0;
+    for (int64_t i = 0; i < F; ++i) {
+        color[i] = c[i];
+        nc = std::max(nc, c[i] + 1);
+    }
+    return nc;
+}
+
+void build_color_order(ColorOrder& co, int64_t F, const int32_t* nb, hipStream_t s) {
+    std::vector<int32_t> color = color_faces(F, nb);
+    int nc = 0;
+    for (int64_t i = 0; i < F; ++i) nc = std::max(nc, color[i] + 1);
     co.ncolor = nc;
     co.off.assign(nc + 1, 0);
     for (int64_t i = 0; i < F; ++i) co.off[color[i] + 1]++;
@@ -388,44 +500,79 @@ void dilu_factor(hipStream_t s, const ColorOrder& co, int64_t F, int L, const in
 }
 
 namespace {
-template <typename T, bool POS, int LM>
+template <typename T, bool POS, int LM, bool XCD>
 void dilu_apply_t(hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFactors<T>& d, const T* r, T* z) {
-    for (int c = 0; c < co.ncolor; ++c) {
+    const int top = co.ncolor - 1;
+    for (int c = 0; c <= top; ++c) {
         const int64_t k0 = co.off[c], k1 = co.off[c + 1];
         if (k1 == k0) continue;
-        hipLaunchKernelGGL((k_dilu_fwd<T, POS, LM>), dim3(grid_for(k1 - k0)), dim3(kBlock), 0, s, F, L, k0, k1,
-                           co.faces.p, co.nbp.p, co.lowmask.p, d.lat.p, d.vert.p, d.dinv.p, r, d.y.p, d.y.p);
+        const dim3 g((unsigned)launch_blocks<XCD>((k1 - k0 + kBlock - 1) / kBlock));
+        if (c == top)
+            hipLaunchKernelGGL((k_dilu_fwd<T, POS, LM, true, XCD>), g, dim3(kBlock), 0, s, F, L, k0, k1, co.faces.p,
+                               co.nbp.p, co.lowmask.p, d.lat.p, d.vert.p, d.dinv.p, r, d.y.p, d.y.p, z);
+        else
+            hipLaunchKernelGGL((k_dilu_fwd<T, POS, LM, false, XCD>), g, dim3(kBlock), 0, s, F, L, k0, k1, co.faces.p,
+                               co.nbp.p, co.lowmask.p, d.lat.p, d.vert.p, d.dinv.p, r, d.y.p, d.y.p, z);
     }
     // reference order: higher-colour results are read back from y (updated in place)
     const T* done = POS ? z : d.y.p;
-    for (int c = co.ncolor - 1; c >= 0; --c) {
+    for (int c = top - 1; c >= 0; --c) {
         const int64_t k0 = co.off[c], k1 = co.off[c + 1];
         if (k1 == k0) continue;
-        hipLaunchKernelGGL((k_dilu_bwd<T, POS, LM>), dim3(grid_for(k1 - k0)), dim3(kBlock), 0, s, F, L, k0, k1,
-                           co.faces.p, co.nbp.p, co.lowmask.p, d.lat.p, d.vert.p, d.dinv.p, d.y.p, done, z);
+        const dim3 g((unsigned)launch_blocks<XCD>((k1 - k0 + kBlock - 1) / kBlock));
+        hipLaunchKernelGGL((k_dilu_bwd<T, POS, LM, XCD>), g, dim3(kBlock), 0, s, F, L, k0, k1, co.faces.p, co.nbp.p,
+                           co.lowmask.p, d.lat.p, d.vert.p, d.dinv.p, d.y.p, done, z);
     }
 }
-template <typename T, bool POS>
+template <typename T, bool POS, bool XCD>
 void dilu_apply_l(hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFactors<T>& d, const T* r, T* z) {
     if (L <= 1)
-        dilu_apply_t<T, POS, 1>(s, co, F, L, d, r, z);
+        dilu_apply_t<T, POS, 1, XCD>(s, co, F, L, d, r, z);
     else if (L <= 5)
-        dilu_apply_t<T, POS, 5>(s, co, F, L, d, r, z);
+        dilu_apply_t<T, POS, 5, XCD>(s, co, F, L, d, r, z);
     else if (L <= 10)
-        dilu_apply_t<T, POS, 10>(s, co, F, L, d, r, z);
+        dilu_apply_t<T, POS, 10, XCD>(s, co, F, L, d, r, z);
     else if (L <= 16)
-        dilu_apply_t<T, POS, 16>(s, co, F, L, d, r, z);
+        dilu_apply_t<T, POS, 16, XCD>(s, co, F, L, d, r, z);
     else
-        dilu_apply_t<T, POS, 0>(s, co, F, L, d, r, z);
+        dilu_apply_t<T, POS, 0, XCD>(s, co, F, L, d, r, z);
 }
 }  // namespace
 
 template <typename T>
-void dilu_apply(hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFactors<T>& d, const T* r, T* z, bool pos) {
-    if (pos)
-        dilu_apply_l<T, true>(s, co, F, L, d, r, z);
+void dilu_apply(hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFactors<T>& d, const T* r, T* z, bool pos,
+                bool xcd) {
+    if (pos && xcd)
+        dilu_apply_l<T, true, true>(s, co, F, L, d, r, z);
+    else if (pos)
+        dilu_apply_l<T, true, false>(s, co, F, L, d, r, z);
+    else if (xcd)
+        dilu_apply_l<T, false, true>(s, co, F, L, d, r, z);
     else
-        dilu_apply_l<T, false>(s, co, F, L, d, r, z);
+        dilu_apply_l<T, false, false>(s, co, F, L, d, r, z);
+}
+
+template <typename T>
+bool tune_dilu_xcd(hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFactors<T>& d, const T* r, T* z,
+                   float* ms_out) {
+    hipEvent_t e0, e1;
+    PB_HIP(hipEventCreate(&e0));
+    PB_HIP(hipEventCreate(&e1));
+    float ms[2] = {0, 0};
+    for (int v = 0; v < 2; ++v) {
+        dilu_apply<T>(s, co, F, L, d, r, z, true, v == 1);
+        PB_HIP(hipEventRecord(e0, s));
+        for (int w = 0; w < 3; ++w) dilu_apply<T>(s, co, F, L, d, r, z, true, v == 1);
+        PB_HIP(hipEventRecord(e1, s));
+        PB_HIP(hipEventSynchronize(e1));
+        PB_HIP(hipEventElapsedTime(&ms[v], e0, e1));
+        ms[v] /= 3;
+        if (ms_out) ms_out[v] = ms[v];
+    }
+    PB_HIP(hipGetLastError());
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return ms[1] < ms[0];
 }
 
 void dilu_narrow(hipStream_t s, int64_t N, const DiluFactors<double>& a, DiluFactors<float>& b) {
@@ -436,48 +583,60 @@ void dilu_narrow(hipStream_t s, int64_t N, const DiluFactors<double>& a, DiluFac
 }
 
 namespace {
-// the launch shapes the tuner chooses from (variant index -> kernel, block)
-template <typename T, int LM>
-void pos_spmv_col_launch(hipStream_t s, int v, int64_t F, int L, const ColorOrder& co, const DiluFactors<T>& d,
-                         const T* x, T* y) {
-    switch (v) {
-        case 1:
-            hipLaunchKernelGGL((k_pos_spmv_col<T, LM, 256>), dim3((unsigned)((F + 255) / 256)), dim3(256), 0, s, F, L,
-                               co.nbp.p, d.diag.p, d.lat.p, d.vert.p, x, y);
-            break;
-        case 2:
-            hipLaunchKernelGGL((k_pos_spmv_col<T, LM, 128>), dim3((unsigned)((F + 127) / 128)), dim3(128), 0, s, F, L,
-                               co.nbp.p, d.diag.p, d.lat.p, d.vert.p, x, y);
-            break;
-        default:
-            hipLaunchKernelGGL((k_pos_spmv_col<T, LM, 64>), dim3((unsigned)((F + 63) / 64)), dim3(64), 0, s, F, L,
-                               co.nbp.p, d.diag.p, d.lat.p, d.vert.p, x, y);
-            break;
+// the launch shapes the tuner chooses from: shape v % 5 (0 / 4 one thread per
+// row with blocks of 256 / 128; 1 / 2 / 3 one thread per face column with
+// blocks of 256 / 128 / 64), with the XCD workgroup mapping when v >= 5
+template <typename T, int B, bool XCD>
+void row_launch(hipStream_t s, int64_t F, int L, const ColorOrder& co, const DiluFactors<T>& d, const T* x, T* y) {
+    const dim3 g((unsigned)launch_blocks<XCD>((F + B - 1) / B), (unsigned)L);
+    hipLaunchKernelGGL((k_pos_spmv_row<T, B, XCD>), g, dim3(B), 0, s, F, L, co.nbp.p, d.diag.p, d.lat.p, d.vert.p,
+                       x, y);
+}
+template <typename T, int LM, int B, bool XCD>
+void col_launch(hipStream_t s, int64_t F, int L, const ColorOrder& co, const DiluFactors<T>& d, const T* x, T* y) {
+    const dim3 g((unsigned)launch_blocks<XCD>((F + B - 1) / B));
+    hipLaunchKernelGGL((k_pos_spmv_col<T, LM, B, XCD>), g, dim3(B), 0, s, F, L, co.nbp.p, d.diag.p, d.lat.p,
+                       d.vert.p, x, y);
+}
+template <typename T, int LM, bool XCD>
+void col_shapes(hipStream_t s, int shape, int64_t F, int L, const ColorOrder& co, const DiluFactors<T>& d,
+                const T* x, T* y) {
+    if (shape == 1)
+        col_launch<T, LM, 256, XCD>(s, F, L, co, d, x, y);
+    else if (shape == 2)
+        col_launch<T, LM, 128, XCD>(s, F, L, co, d, x, y);
+    else
+        col_launch<T, LM, 64, XCD>(s, F, L, co, d, x, y);
+}
+template <typename T, bool XCD>
+void shapes(hipStream_t s, int shape, int64_t F, int L, const ColorOrder& co, const DiluFactors<T>& d, const T* x,
+            T* y) {
+    if (shape >= 1 && shape <= 3 && L <= 16) {
+        if (L <= 5)
+            col_shapes<T, 5, XCD>(s, shape, F, L, co, d, x, y);
+        else if (L <= 10)
+            col_shapes<T, 10, XCD>(s, shape, F, L, co, d, x, y);
+        else
+            col_shapes<T, 16, XCD>(s, shape, F, L, co, d, x, y);
+    } else if (shape == 4) {
+        row_launch<T, 128, XCD>(s, F, L, co, d, x, y);
+    } else {
+        row_launch<T, 256, XCD>(s, F, L, co, d, x, y);
     }
+}
+bool shape_allowed(int v, int L) {
+    const int shape = v % 5;
+    return !(shape >= 1 && shape <= 3 && L > 16);
 }
 }  // namespace
 
 template <typename T>
 void pos_spmv_variant(hipStream_t s, int v, const ColorOrder& co, int64_t F, int L, const DiluFactors<T>& d,
                       const T* x, T* y) {
-    if (v >= 1 && v <= 3 && L <= 16) {
-        if (L <= 5)
-            pos_spmv_col_launch<T, 5>(s, v, F, L, co, d, x, y);
-        else if (L <= 10)
-            pos_spmv_col_launch<T, 10>(s, v, F, L, co, d, x, y);
-        else
-            pos_spmv_col_launch<T, 16>(s, v, F, L, co, d, x, y);
-        return;
-    }
-    if (v == 4) {
-        const dim3 g((unsigned)((F + 127) / 128), (unsigned)L);
-        hipLaunchKernelGGL((k_pos_spmv_row<T, 128>), g, dim3(128), 0, s, F, L, co.nbp.p, d.diag.p, d.lat.p,
-                           d.vert.p, x, y);
-        return;
-    }
-    const dim3 g((unsigned)((F + 255) / 256), (unsigned)L);
-    hipLaunchKernelGGL((k_pos_spmv_row<T, 256>), g, dim3(256), 0, s, F, L, co.nbp.p, d.diag.p, d.lat.p, d.vert.p,
-                       x, y);
+    if (v >= 5)
+        shapes<T, true>(s, v - 5, F, L, co, d, x, y);
+    else
+        shapes<T, false>(s, v, F, L, co, d, x, y);
 }
 
 template <typename T>
@@ -489,7 +648,7 @@ int tune_pos_spmv(hipStream_t s, const ColorOrder& co, int64_t F, int L, const D
     int best = 0;
     float best_ms = 1e30f;
     for (int v = 0; v < kPosSpmvVariants; ++v) {
-        if (v >= 1 && v <= 3 && L > 16) continue;
+        if (!shape_allowed(v, L)) continue;
         for (int w = 0; w < 2; ++w) pos_spmv_variant<T>(s, v, co, F, L, d, x, y);
         PB_HIP(hipEventRecord(e0, s));
         for (int w = 0; w < 5; ++w) pos_spmv_variant<T>(s, v, co, F, L, d, x, y);
@@ -534,9 +693,13 @@ template void dilu_factor<double>(hipStream_t, const ColorOrder&, int64_t, int, 
 template void dilu_factor<float>(hipStream_t, const ColorOrder&, int64_t, int, const int32_t*, const float*,
                                  const uint16_t*, DiluFactors<float>&);
 template void dilu_apply<double>(hipStream_t, const ColorOrder&, int64_t, int, DiluFactors<double>&, const double*,
-                                 double*, bool);
+                                 double*, bool, bool);
 template void dilu_apply<float>(hipStream_t, const ColorOrder&, int64_t, int, DiluFactors<float>&, const float*,
-                                float*, bool);
+                                float*, bool, bool);
+template bool tune_dilu_xcd<double>(hipStream_t, const ColorOrder&, int64_t, int, DiluFactors<double>&, const double*,
+                                    double*, float*);
+template bool tune_dilu_xcd<float>(hipStream_t, const ColorOrder&, int64_t, int, DiluFactors<float>&, const float*,
+                                   float*, float*);
 template void pos_spmv<double>(hipStream_t, const ColorOrder&, int64_t, int, const DiluFactors<double>&,
                                const double*, const double*, double*, double*);
 template void pos_spmv_variant<double>(hipStream_t, int, const ColorOrder&, int64_t, int, const DiluFactors<double>&,

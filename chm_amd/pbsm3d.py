@@ -31,6 +31,9 @@ EXPORTED = (
     "pbsm3d_profile_read",
     "pbsm3d_spmv_tuning",
     "pbsm3d_spmv_select",
+    "pbsm3d_face_colors",
+    "pbsm3d_precond_tuning",
+    "pbsm3d_precond_select",
     "pbsm3d_mlp_create",
     "pbsm3d_mlp_destroy",
     "pbsm3d_mlp_param_count",
@@ -47,9 +50,10 @@ EXPORTED = (
 
 N_FEATURES, N_TARGETS = 16, 11
 
-# launch shapes of the position-order SpMV: 0 / 4 one thread per row (blocks of 256 / 128),
-# 1 / 2 / 3 one thread per face column (blocks of 256 / 128 / 64)
-SPMV_VARIANTS = 5
+# launch shapes of the position-order SpMV: v % 5 = 0 / 4 one thread per row (blocks of
+# 256 / 128), 1 / 2 / 3 one thread per face column (blocks of 256 / 128 / 64); v >= 5 with the
+# XCD workgroup mapping
+SPMV_VARIANTS = 10
 
 KERNEL_CLASSES = ("assemble", "spmv64", "spmv32", "precond", "ortho", "residual", "deposition", "warmstart",
                   "factor")
@@ -98,6 +102,9 @@ def lib():
         L.pbsm3d_profile_read.argtypes = [vp, C.c_int, C.POINTER(C.c_int64), dp, C.c_int]
         L.pbsm3d_spmv_tuning.argtypes = [vp, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_float), C.c_int32]
         L.pbsm3d_spmv_select.argtypes = [vp, C.c_int, C.c_int]
+        L.pbsm3d_face_colors.argtypes = [C.c_int64, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+        L.pbsm3d_precond_tuning.argtypes = [vp, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_float)]
+        L.pbsm3d_precond_select.argtypes = [vp, C.c_int, C.c_int]
         fp = C.POINTER(C.c_float)
         L.pbsm3d_mlp_create.argtypes = [C.POINTER(vp), C.c_int, C.c_uint64]
         L.pbsm3d_mlp_destroy.argtypes = [vp]
@@ -126,6 +133,17 @@ def _check(rc: int, h=None):
 def face_geometry(mesh) -> abi.Geometry:
     """Per-face geometry of a :class:`chm_amd.meshgen.Mesh` through the library's face API."""
     return abi.geometry_args(lib().pbsm3d_face_geometry, mesh)
+
+
+def face_colors(neigh) -> tuple:
+    """Face colours of the multicolour D-ILU preconditioner (host computation) and their count."""
+    nb = np.ascontiguousarray(neigh, dtype=np.int32).reshape(-1)
+    F = nb.size // 3
+    col = np.zeros(F, dtype=np.int32)
+    nc = lib().pbsm3d_face_colors(F, nb.ctypes.data_as(C.POINTER(C.c_int32)), col.ctypes.data_as(C.POINTER(C.c_int32)))
+    if nc < 0:
+        raise Pbsm3dError(nc, "pbsm3d_face_colors failed")
+    return col, nc
 
 
 def default_config(**overrides) -> abi.Config:
@@ -221,6 +239,17 @@ class Context:
         chosen, ms = C.c_int32(-1), (C.c_float * n)()
         _check(lib().pbsm3d_spmv_tuning(self._h, which, C.byref(chosen), ms, n), self._h)
         return {"chosen": int(chosen.value), "ms": [round(float(v), 5) for v in ms]}
+
+    def precond_tuning(self, which: int = 0) -> dict:
+        """Whether the D-ILU sweeps use the XCD workgroup mapping (FP64 which=0 / FP32 1),
+        and the tuner's average apply times without / with it (ms)."""
+        xcd, ms = C.c_int32(-1), (C.c_float * 2)()
+        _check(lib().pbsm3d_precond_tuning(self._h, which, C.byref(xcd), ms), self._h)
+        return {"xcd": int(xcd.value), "ms": [round(float(v), 5) for v in ms]}
+
+    def precond_select(self, which: int, xcd: int):
+        """Fix the D-ILU workgroup mapping (0 / 1) of the FP64 (0) / FP32 (1) solve; -1 re-tunes."""
+        _check(lib().pbsm3d_precond_select(self._h, which, xcd), self._h)
 
     def spmv_select(self, which: int, variant: int):
         """Fix the SpMV launch shape of the FP64 (0) / FP32 (1) operator; -1 re-tunes."""

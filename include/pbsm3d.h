@@ -223,7 +223,8 @@ int pbsm3d_profile_read(pbsm3d_ctx* ctx, int kernel, int64_t* launches, double* 
 /* Launch shape of the suspension SpMV (no counterpart in the reference).  With
  * the multicolour D-ILU preconditioner the suspension solve runs in the
  * preconditioner's position order and its SpMV has several launch shapes (one
- * thread per row or per face column, several block sizes); the first solve of
+ * thread per row or per face column, several block sizes, with or without a
+ * workgroup order that keeps neighbouring faces on one XCD); the first solve of
  * each precision times them all with HIP events and keeps the fastest.
  * which: 0 = FP64 operator, 1 = FP32 operator (MIXED).  *chosen = -1 until
  * tuned; ms[v] (n entries, may be NULL) = average time of shape v in ms (0 if
@@ -231,8 +232,20 @@ int pbsm3d_profile_read(pbsm3d_ctx* ctx, int kernel, int64_t* launches, double* 
 int pbsm3d_spmv_tuning(const pbsm3d_ctx* ctx, int which, int32_t* chosen, float* ms, int32_t n);
 /* Fixes the launch shape (0 .. PBSM3D_SPMV_VARIANTS-1) instead of the tuner's
  * choice, or -1 to tune again at the next solve. */
-#define PBSM3D_SPMV_VARIANTS 5
+#define PBSM3D_SPMV_VARIANTS 10
 int pbsm3d_spmv_select(pbsm3d_ctx* ctx, int which, int variant);
+/* The same for the multicolour D-ILU sweeps: *xcd = 1 if the workgroup order
+ * that keeps neighbouring faces on one XCD was faster at the first solve, 0 if
+ * not, -1 until tuned; ms[2] = average apply time without / with it (ms).
+ * pbsm3d_precond_select fixes it (0 / 1) or re-tunes (-1). */
+int pbsm3d_precond_tuning(const pbsm3d_ctx* ctx, int which, int32_t* xcd, float* ms);
+int pbsm3d_precond_select(pbsm3d_ctx* ctx, int which, int xcd);
+
+/* Face colours of the multicolour D-ILU preconditioner (host computation, no
+ * GPU needed; no counterpart in the reference): neighbouring faces never share
+ * a colour.  neigh as in pbsm3d_mesh; color[F] out.  Returns the number of
+ * colours (3 on triangulations unless a Kempe swap is blocked), < 0 on error. */
+int pbsm3d_face_colors(int64_t n_faces, const int32_t* neigh, int32_t* color);
 
 /* Learned warm start (an extension; the reference always starts from x0 = 0,
  * LinearAlgebra.cpp:199-206).  A per-face network 16 -> 128 -> 128 -> 128 -> 11

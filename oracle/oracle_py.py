@@ -55,6 +55,7 @@ def lib():
                                        C.c_int64, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
         L.oracle_dense_solve.argtypes = [C.c_int64, ip, ip, dp, dp, dp]
         L.oracle_precond_apply.argtypes = [C.c_int64, ip, ip, dp, C.c_int, C.c_int64, C.c_int, dp, dp]
+        L.oracle_face_colors.argtypes = [C.c_int64, ip, ip]
         L.oracle_config_default.argtypes = [C.POINTER(abi.Config)]
         L.oracle_config_default.restype = None
         L.oracle_max_threads.restype = C.c_int
@@ -170,6 +171,16 @@ def solve_csr(rp, col, val, b, precond=abi.PC_ILUT, restart=30, maxit=1000, tol=
     rc = lib().oracle_solve_csr(n, keep.iptr(rp), keep.iptr(col), keep.dptr(val), keep.dptr(b), keep.dptr(x),
                                 precond, restart, maxit, tol, F, L, C.byref(it), C.byref(res))
     return x, it.value, res.value, rc
+
+
+def face_colors(neigh: np.ndarray):
+    """Face colours of the multicolour D-ILU and their count (neigh: [F x 3] or flat)."""
+    nb = np.ascontiguousarray(neigh, dtype=np.int32).reshape(-1)
+    F = nb.size // 3
+    col = np.zeros(F, dtype=np.int32)
+    keep = abi.Keep()
+    nc = lib().oracle_face_colors(F, keep.iptr(nb), keep.iptr(col))
+    return col, nc
 
 
 def dense_solve(rp, col, val, b):

@@ -36,6 +36,7 @@
 #include "../include/pbsm3d.h"
 
 #include <algorithm>
+#include <set>
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -386,10 +387,89 @@ struct PcIlut : Precond {
     }
 };
 
+// Face colours of the multicolour D-ILU (rule in the comment of PcDilu), from
+// the lateral couplings of the first F rows of Am.
+std::vector<int> dilu_colors(const Csr& Am, int64_t F) {
+    std::vector<int> color(F, -1);
+    for (int64_t i = 0; i < F; ++i) {
+        unsigned used = 0;
+        for (int32_t k = Am.rp[i]; k < Am.rp[i + 1]; ++k) {
+            const int64_t c = Am.col[k];
+            if (c < F && c != i && color[c] >= 0) used |= 1u << color[c];
+        }
+        int c = 0;
+        while (used & (1u << c)) ++c;
+        color[i] = c;
+    }
+    auto nbrs = [&](int64_t i) {
+        std::vector<int64_t> v;
+        for (int32_t k = Am.rp[i]; k < Am.rp[i + 1]; ++k)
+            if (Am.col[k] < F && Am.col[k] != i) v.push_back(Am.col[k]);
+        return v;
+    };
+    auto neighbour_of_colour = [&](int64_t v, int c) {
+        for (int64_t u : nbrs(v))
+            if (color[u] == c) return u;
+        return (int64_t)-1;
+    };
+    auto repair = [&](int64_t v) {
+        std::set<int> present;
+        for (int64_t u : nbrs(v)) present.insert(color[u]);
+        for (int c = 0; c < 3; ++c)
+            if (!present.count(c)) {
+                color[v] = c;
+                return true;
+            }
+        const int pairs[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+        for (const auto& pr : pairs) {
+            const int ca = pr[0], cb = pr[1];
+            const int64_t a = neighbour_of_colour(v, ca), b = neighbour_of_colour(v, cb);
+            // component of {ca, cb} faces containing a (depth first, capped)
+            std::set<int64_t> comp{a};
+            std::vector<int64_t> todo{a};
+            bool too_big = false;
+            while (!todo.empty() && !too_big) {
+                const int64_t u = todo.back();
+                todo.pop_back();
+                for (int64_t w : nbrs(u))
+                    if ((color[w] == ca || color[w] == cb) && !comp.count(w)) {
+                        comp.insert(w);
+                        todo.push_back(w);
+                        if ((int64_t)comp.size() > 4096) too_big = true;
+                    }
+            }
+            if (too_big || comp.count(b)) continue;
+            for (int64_t u : comp) color[u] = color[u] == ca ? cb : ca;
+            color[v] = ca;
+            return true;
+        }
+        return false;
+    };
+    for (int64_t v = 0; v < F; ++v) {
+        if (color[v] != 3 || repair(v)) continue;
+        for (int c = 0; c < 3; ++c) {  // trade colours with the neighbour of colour c
+            const int64_t a = neighbour_of_colour(v, c);
+            color[v] = c;
+            color[a] = 3;
+            if (repair(a)) break;
+            color[a] = c;
+            color[v] = 3;
+        }
+    }
+    return color;
+}
+
 // D-ILU(0) in the multicolour order (face colour, layer, face): faces coloured
 // greedily in index order with the smallest colour no lower-indexed neighbour
-// holds; d~_r = a_rr - sum over earlier k of a_rk a_kr / d~_k (Pommerell's
-// diagonal ILU); M = (D~ + L) D~^-1 (D~ + U) in that order.
+// holds, then every face left with colour 3 is repaired in index order: a free
+// colour below 3 if one exists, else the first Kempe swap that frees one (for
+// colour pairs (0,1), (0,2), (1,2): the component of the {ca, cb}-coloured faces
+// reachable from v's ca-neighbour, if it does not contain v's cb-neighbour and
+// has at most 4096 faces, has ca and cb exchanged and v takes ca), else v trades
+// colours with its neighbour of colour c (c = 0, 1, 2 in turn) when that
+// neighbour can then be repaired the same way;
+// d~_r = a_rr - sum over earlier k of a_rk a_kr / d~_k (Pommerell's diagonal
+// ILU); M = (D~ + L) D~^-1 (D~ + U) in that order.
 struct PcDilu : Precond {
     int64_t F;
     int L;
@@ -400,17 +480,7 @@ struct PcDilu : Precond {
     void compute(const Csr& Am) override {
         A = &Am;
         const int64_t N = Am.n;
-        std::vector<int> color(F, -1);
-        for (int64_t i = 0; i < F; ++i) {
-            unsigned used = 0;
-            for (int32_t k = Am.rp[i]; k < Am.rp[i + 1]; ++k) {
-                const int64_t c = Am.col[k];
-                if (c < F && c != i && color[c] >= 0) used |= 1u << color[c];
-            }
-            int c = 0;
-            while (used & (1u << c)) ++c;
-            color[i] = c;
-        }
+        const std::vector<int> color = dilu_colors(Am, F);
         key.resize(N);
         for (int64_t r = 0; r < N; ++r) key[r] = ((int64_t)color[r % F] * L + r / F) * F + r % F;
         ord.resize(N);
@@ -672,6 +742,19 @@ int oracle_csr_pattern(int64_t F, int L, const int32_t* neigh, int32_t* rp, int3
     std::memcpy(rp, A.rp.data(), (A.n + 1) * sizeof(int32_t));
     std::memcpy(col, A.col.data(), A.col.size() * sizeof(int32_t));
     return 0;
+}
+
+// face colours of the multicolour D-ILU; returns the number of colours
+int oracle_face_colors(int64_t F, const int32_t* neigh, int32_t* color) {
+    Csr A;
+    build_pattern(F, 1, neigh, A);
+    const std::vector<int> c = dilu_colors(A, F);
+    int nc = 0;
+    for (int64_t i = 0; i < F; ++i) {
+        color[i] = c[i];
+        nc = std::max(nc, c[i] + 1);
+    }
+    return nc;
 }
 
 void oracle_config_default(pbsm3d_config* c) {

@@ -237,9 +237,9 @@ def test_options_parity(pb, oracle, mesh_small):
 
 @pytest.mark.parametrize("precision", [abi.FP64, abi.MIXED])
 def test_every_spmv_launch_shape(pb, oracle, mesh_c1, precision):
-    """Each launch shape of the position-order SpMV, forced in turn, gives the
-    oracle's fields; the tuner's choice is one of the shapes and every shape
-    was timed."""
+    """Each launch shape of the position-order SpMV and both workgroup mappings of
+    the D-ILU sweeps, forced in turn, give the oracle's fields; the tuners'
+    choices are among them and every option was timed."""
     _print_case("spmv_launch_shapes", precision=precision)
     F = mesh_c1.n_faces
     fo = meshgen.forcing_c1(F, seed=0)
@@ -250,8 +250,11 @@ def test_every_spmv_launch_shape(pb, oracle, mesh_c1, precision):
     tun = ctx.spmv_tuning(which)
     assert 0 <= tun["chosen"] < pb.SPMV_VARIANTS
     assert all(ms > 0 for ms in tun["ms"]), tun
+    ptun = ctx.precond_tuning(which)
+    assert ptun["xcd"] in (0, 1) and all(ms > 0 for ms in ptun["ms"]), ptun
     for v in range(pb.SPMV_VARIANTS):
         ctx.spmv_select(which, v)
+        ctx.precond_select(which, v % 2)  # both D-ILU workgroup mappings
         ctx.set_state(np.zeros(F), np.zeros(F))
         gg, st = ctx.step(fo, want_fields=True)
         assert st["susp_resid"] <= 1e-12
@@ -259,3 +262,36 @@ def test_every_spmv_launch_shape(pb, oracle, mesh_c1, precision):
             assert rel_err(gg[k], og[k]) < 1e-8, (v, k, rel_err(gg[k], og[k]))
     with pytest.raises(pb.Pbsm3dError):
         ctx.spmv_select(which, pb.SPMV_VARIANTS)
+
+
+def test_c3_mixed_precision_96_steps(pb, oracle, mesh_c1):
+    """The mixed-precision workload on the C1 mesh: 96 x 900 s steps with wind speed
+    and direction advected by a seeded smooth random field (correlation length
+    2 km), FP32 SpMV / preconditioner with FP64 refinement at tol 1e-8, state
+    (sum_drift, sum_subl) carried across steps.  Per-field error against the FP64
+    oracle solved to 1e-13 is printed for the FP64 and the mixed GPU paths and
+    bounded by the forward-error band of a 1e-8 residual."""
+    _print_case("c3_mixed_96_steps", F=mesh_c1.n_faces, L=5)
+    F = mesh_c1.n_faces
+    g = pb.face_geometry(mesh_c1)
+    orc = oracle.Oracle(g, oracle.oracle_config(nLayer=5, tol=1e-13))
+    gpu = {p: pb.Context(g, pb.default_config(nLayer=5, precision=p, krylov=abi.KRYLOV_BICGSTAB if p == abi.MIXED
+                                              else abi.KRYLOV_GMRES, tol=1e-8))
+           for p in (abi.FP64, abi.MIXED)}
+    fields = ("Qsalt", "Qsusp", "Qsubl", "qdep", "drift_mass", "sum_drift", "sum_subl")
+    worst = {p: {k: 0.0 for k in fields} for p in gpu}
+    for step in range(96):
+        fo = meshgen.forcing_advected(mesh_c1, step, seed=3)
+        og, _ = orc.step(fo)
+        for p, ctx in gpu.items():
+            gg, st = ctx.step(fo, want_fields=True)
+            assert st["susp_resid"] <= 1e-8 and st["dep_resid"] <= 1e-8
+            for k in fields:
+                if (og[k] != -9999).any():
+                    worst[p][k] = max(worst[p][k], rel_err(gg[k], og[k]))
+    for p, name in ((abi.FP64, "fp64"), (abi.MIXED, "mixed")):
+        print(f"[c3] {name} max over 96 steps of max|x - x_oracle| / max|x_oracle|: "
+              + ", ".join(f"{k} {v:.2e}" for k, v in worst[p].items()), flush=True)
+        for k, bound in (("Qsalt", 1e-12), ("Qsusp", 1e-5), ("Qsubl", 1e-5), ("qdep", 1e-3), ("drift_mass", 1e-3),
+                         ("sum_drift", 1e-3), ("sum_subl", 1e-5)):
+            assert worst[p][k] < bound, (name, k, worst[p][k])

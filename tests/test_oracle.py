@@ -311,14 +311,9 @@ def test_column_preconditioner_is_exact_on_columns(oracle, mesh_small):
     assert np.linalg.norm(A @ z - r) / np.linalg.norm(r) < 1e-13
 
 
-def _colour_key(neigh, F, L):
-    colr = -np.ones(F, int)
-    for i in range(F):
-        used = {colr[n] for n in neigh[i] if 0 <= n < i}
-        c = 0
-        while c in used:
-            c += 1
-        colr[i] = c
+def _colour_key(oracle, neigh, F, L):
+    """Row keys of the multicolour order (colour, layer, face) from the oracle's face colours."""
+    colr = oracle.face_colors(neigh)[0].astype(np.int64)
     r = np.arange(F * L)
     return (colr[r % F] * L + r // F) * F + r % F, colr
 
@@ -331,8 +326,8 @@ def test_dilu_exact_on_triangular_parts(oracle, mesh_small):
     fo = meshgen.forcing_c1(F, seed=0)
     o, _, _, _ = _run(oracle, mesh_small, fo, nLayer=L)
     rp, col, val, _ = o.system(0)
-    key, colr = _colour_key(mesh_small.neigh, F, L)
-    assert colr.max() <= 3 and all((colr[mesh_small.neigh[i][mesh_small.neigh[i] >= 0]] != colr[i]).all()
+    key, colr = _colour_key(oracle, mesh_small.neigh, F, L)
+    assert colr.max() <= 2 and all((colr[mesh_small.neigh[i][mesh_small.neigh[i] >= 0]] != colr[i]).all()
                                    for i in range(F))
     rows = np.repeat(np.arange(F * L), np.diff(rp))
     import scipy.sparse as sp
@@ -400,3 +395,36 @@ def test_multi_step_accumulates_and_caps(oracle, mesh_small):
     assert (outs["pbsm_more_than_avail"] == 1).any()
     sd, ss = o.get_state()
     np.testing.assert_array_equal(sd, outs["sum_drift"])
+
+
+# ---------------------------------------------------------------------------
+# multicolour D-ILU colouring
+# ---------------------------------------------------------------------------
+def _greedy_colors(nb):
+    col = np.full(len(nb), -1)
+    for i in range(len(nb)):
+        used = {col[n] for n in nb[i] if n >= 0 and col[n] >= 0}
+        c = 0
+        while c in used:
+            c += 1
+        col[i] = c
+    return col
+
+
+@pytest.mark.parametrize("points", [2500, 20000])
+def test_face_colouring_valid_three_colours_and_matches_library(oracle, points):
+    """The library's host colouring and the oracle's restatement agree face by face;
+    no two neighbours share a colour; the Kempe repair removes the fourth colour
+    that plain greedy colouring leaves."""
+    from chm_amd import pbsm3d as pb
+
+    mesh = meshgen.synthetic_mesh(points, seed=0)
+    nb = mesh.neigh
+    col, nc = pb.face_colors(nb)
+    ocol, onc = oracle.face_colors(nb)
+    np.testing.assert_array_equal(col, ocol)
+    assert nc == onc == 3
+    for f in range(3):
+        has = nb[:, f] >= 0
+        assert (col[has] != col[nb[has, f]]).all()
+    assert _greedy_colors(nb).max() == 3  # greedy alone needs a fourth colour on these meshes
