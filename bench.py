@@ -89,7 +89,8 @@ def run_variant(pb, g, dev_forcing, dev_outs, cfg, net, steps, warmup, torch):
     prof = ctx.profile_read(reset=True)
     ctx.profile(False)
     nnz = pb.lib().pbsm3d_system_nnz(ctx._h, 0)
-    tuning = {"fp64": ctx.spmv_tuning(0), "fp32": ctx.spmv_tuning(1)}
+    tuning = {"spmv_fp64": ctx.spmv_tuning(0), "spmv_fp32": ctx.spmv_tuning(1),
+              "dilu_fp64": ctx.precond_tuning(0), "dilu_fp32": ctx.precond_tuning(1)}
     ctx.close()
     return (t1 - t0), stats, prof, nnz, tuning
 
@@ -143,6 +144,38 @@ def train_network(pb, abi, meshgen, warmstart, g, F, draws, samples, epochs, tor
     return net, info
 
 
+def warm_start_gain(pb, abi, meshgen, g, F, L, krylov_mixed, net, seeds, torch):
+    """One step per held-out forcing draw (seeds disjoint from the training draws and the
+    timed forcing) with the headline solver, cold start vs learned start: suspension
+    iterations, initial residual ||b - A x0|| / ||b|| and step time.  Each context first
+    steps once on the timed forcing so its launch-shape tuning is not counted."""
+    warm0 = meshgen.forcing_c2(F, seed=0)
+    out = {"seeds": seeds}
+    for label, learned in (("cold", False), ("learned", True)):
+        cfg = pb.default_config(nLayer=L, precision=abi.MIXED, krylov=krylov_mixed,
+                                warm_start=abi.WS_LEARNED if learned else abi.WS_ZERO)
+        ctx = pb.Context(g, cfg)
+        if learned:
+            ctx.set_warm_start(net)
+        ctx.step(warm0, want_fields=False)
+        its, r0, ms = [], [], []
+        for sd in seeds:
+            fo = meshgen.forcing_c2(F, seed=sd)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            _, st = ctx.step(fo, want_fields=False)
+            torch.cuda.synchronize()
+            ms.append(round(1000.0 * (time.perf_counter() - t0), 2))
+            its.append(st["susp_iters"])
+            r0.append(round(st["susp_resid0"], 4))
+        ctx.close()
+        out[label] = {"susp_iters": its, "resid0": r0, "ms_per_step": ms}
+    out["iters_ratio_learned_to_cold"] = round(sum(out["learned"]["susp_iters"]) / max(1, sum(out["cold"]["susp_iters"])), 4)
+    out["time_ratio_learned_to_cold"] = round(sum(out["learned"]["ms_per_step"]) / max(1e-9, sum(out["cold"]["ms_per_step"])), 4)
+    out["note"] = "step times include host-side forcing upload (host arrays); iterations are the comparison"
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -153,9 +186,11 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--variants", default="plain,mixed,mixed_learned")
     ap.add_argument("--krylov-mixed", default="bicgstab", choices=["gmres", "bicgstab"])
-    ap.add_argument("--train-draws", type=int, default=4)
-    ap.add_argument("--train-samples", type=int, default=2000000)
-    ap.add_argument("--train-epochs", type=int, default=3)
+    ap.add_argument("--train-draws", type=int, default=8)
+    ap.add_argument("--train-samples", type=int, default=4000000)
+    ap.add_argument("--train-epochs", type=int, default=8)
+    ap.add_argument("--gain-seeds", default="100,101,102,103",
+                    help="held-out forcing draws on which the learned start is compared with the cold start")
     ap.add_argument("--cpu-faces", type=int, default=200000)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -225,7 +260,7 @@ def main():
             "refinements": st["susp_refinements"],
             "susp_resid0": st["susp_resid0"],
             "cold_fallback": bool(st["susp_cold_fallback"]),
-            "spmv_tuning": tuning,
+            "launch_tuning": tuning,
             "ms_phase": {k: round(st[k], 3) for k in ("ms_assembly", "ms_susp_solve", "ms_post", "ms_dep_solve")},
             "kernels": {k: {"launches": n, "ms_avg": (ms / n if n else 0.0), "ms_total": ms}
                         for k, (n, ms) in prof.items() if n},
@@ -241,6 +276,13 @@ def main():
                 k: float((outs_now[k] - plain_outs[k]).abs().max() / plain_outs[k].abs().max().clamp_min(1e-300))
                 for k in ("Qsalt", "Qsusp", "Qsubl", "drift_mass", "sum_drift")}
         log(f"[bench] {name}: {variants[name]['steps_per_s']:.2f} steps/s, iters {st['susp_iters']}/{st['dep_iters']}")
+
+    gain = None
+    if net is not None and "mixed_learned" in names and args.gain_seeds:
+        log("[bench] learned vs cold start on held-out forcing draws")
+        gain = warm_start_gain(pb, abi, meshgen, g, F, L, krylov_mixed, net,
+                               [int(x) for x in args.gain_seeds.split(",")], torch)
+        log(f"[bench] gain: {gain}")
 
     N = F * L
     head_name = "mixed_learned" if "mixed_learned" in variants else names[-1]
@@ -290,6 +332,7 @@ def main():
         "speedup_vs_plain": (round(head["steps_per_s"] / variants["plain"]["steps_per_s"], 3)
                              if "plain" in variants else None),
         "warm_start_training": train_info,
+        "warm_start_gain": gain,
         "roofline": roofline,
         "kernel_rooflines": rooflines,
         "cpu_baseline": cpu,

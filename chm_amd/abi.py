@@ -75,6 +75,7 @@ class Config(C.Structure):
         ("ortho_passes", C.c_int32),
         ("block_size", C.c_int32),
         ("krylov", C.c_int32),
+        ("inner_rtol", C.c_double),
     ]
 
 
@@ -105,6 +106,17 @@ class Out(C.Structure):
         ("conc", _dp),
         ("qdep", _dp),
         ("on_device", C.c_int32),
+    ]
+
+
+class FetchConfig(C.Structure):
+    """pbsm3d_fetch_config of include/pbsm3d_fetch.h."""
+
+    _fields_ = [
+        ("steps", C.c_int32),
+        ("incl_veg", C.c_int32),
+        ("max_distance", C.c_double),
+        ("I", C.c_double),
     ]
 
 
@@ -166,6 +178,7 @@ def default_config(**overrides) -> Config:
         ortho_passes=1,
         block_size=256,
         krylov=KRYLOV_GMRES,
+        inner_rtol=0.0,
     )
     for k, v in overrides.items():
         if not hasattr(c, k):

@@ -133,6 +133,7 @@ void validate(const pbsm3d_mesh* m, const pbsm3d_config* c) {
     if (c->krylov != PBSM3D_KRYLOV_GMRES && c->krylov != PBSM3D_KRYLOV_BICGSTAB)
         throw Error(PBSM3D_EINVAL, "unknown Krylov method");
     if (c->max_iters < 1) throw Error(PBSM3D_EINVAL, "max_iters must be >= 1");
+    if (!(c->inner_rtol >= 0 && c->inner_rtol < 1)) throw Error(PBSM3D_EINVAL, "inner_rtol must be in [0, 1)");
     if (!(c->tol > 0)) throw Error(PBSM3D_EINVAL, "tol must be > 0");
     if (c->precision != PBSM3D_FP64 && c->precision != PBSM3D_MIXED) throw Error(PBSM3D_EINVAL, "unknown precision");
     if (c->precond != PBSM3D_PC_NONE && c->precond != PBSM3D_PC_JACOBI && c->precond != PBSM3D_PC_COLUMN &&
@@ -215,10 +216,10 @@ SolveInfo solve_in_position_order(pbsm3d_ctx* c, const pbsm3d_config& cfg, int64
         };
         if (bicg)
             si = bicgstab_mixed(s, c->bw32, c->ws32, A, A32, M, c->bp.p, c->xp.p, c->r_s.p, c->r32.p, c->d32.p,
-                                cfg.max_iters, cfg.tol, 1e-5);
+                                cfg.max_iters, cfg.tol, cfg.inner_rtol > 0 ? cfg.inner_rtol : 1e-4);
         else
             si = mixed_solve(s, c->ws32, A, A32, M, c->bp.p, c->xp.p, c->r_s.p, c->r32.p, c->d32.p, cfg.restart,
-                             cfg.max_iters, cfg.tol, 1e-4);
+                             cfg.max_iters, cfg.tol, cfg.inner_rtol > 0 ? cfg.inner_rtol : 1e-4);
     }
     from_pos<double, double>(s, c->co, F, L, c->xp.p, c->susp.x.p);
     return si;
@@ -570,10 +571,10 @@ int pbsm3d_step(pbsm3d_ctx* c, const pbsm3d_forcing* fo, int32_t dt_s, pbsm3d_ou
                     };
                 if (cfg.krylov == PBSM3D_KRYLOV_BICGSTAB)
                     si = bicgstab_mixed(s, c->bw32, c->ws32, A, A32, M, c->susp.rhs.p, c->susp.x.p, c->r_s.p, c->r32.p,
-                                        c->d32.p, cfg.max_iters, cfg.tol, 1e-5);
+                                        c->d32.p, cfg.max_iters, cfg.tol, cfg.inner_rtol > 0 ? cfg.inner_rtol : 1e-4);
                 else
                     si = mixed_solve(s, c->ws32, A, A32, M, c->susp.rhs.p, c->susp.x.p, c->r_s.p, c->r32.p, c->d32.p,
-                                     cfg.restart, cfg.max_iters, cfg.tol, 1e-4);
+                                     cfg.restart, cfg.max_iters, cfg.tol, cfg.inner_rtol > 0 ? cfg.inner_rtol : 1e-4);
             }
             stats.susp_iters = si.iters;
             stats.susp_resid = si.resid;

@@ -206,6 +206,12 @@ constexpr int kPosSpmvVariants = PBSM3D_SPMV_VARIANTS;
 template <typename T>
 void pos_spmv_variant(hipStream_t s, int v, const ColorOrder& co, int64_t F, int L, const DiluFactors<T>& d,
                       const T* x, T* y);
+// the same SpMV with per-block partial sums partial[2b] = w.y, partial[2b+1] = y.y (FP64);
+// returns the number of blocks b (at most n / 64 + 1024 for n = F * L, L >= 2)
+template <typename T>
+int64_t pos_spmv_dots(hipStream_t s, int v, const ColorOrder& co, int64_t F, int L, const DiluFactors<T>& d,
+                      const T* x, T* y, const T* w, double* partial);
+inline int64_t pos_spmv_dots_max_blocks(int64_t n) { return n / 64 + 1024; }
 // times every launch shape on (x, y) with HIP events on the stream; returns the fastest
 // (ms_out[v], if given, receives the average time of shape v, ms)
 template <typename T>
@@ -219,14 +225,29 @@ template <typename Ti, typename To>
 void from_pos(hipStream_t s, const ColorOrder& co, int64_t F, int L, const Ti* in, To* out);
 
 // BiCGStab vector updates (FP64 arithmetic on T storage)
-template <typename T>
-void bicg_p(hipStream_t s, int64_t n, const T* r, const T* v, double beta, double omega, T* p);  // p = r + b(p - w v)
 // out = y - a x; partial[2*blk] = rh.out (if rh), partial[2*blk+1] = out.out
 template <typename T>
 void axpy_dots(hipStream_t s, int64_t n, double a, const T* x, const T* y, T* out, const T* rh, double* partial,
                int nblk);
+
+// BiCGStab with its scalars kept on the device (slots of sc[]), so an iteration
+// needs one host synchronisation (the convergence test) instead of three.
+enum BicgSc { SC_RHO_PREV = 0, SC_RHO, SC_ALPHA, SC_OMEGA, SC_RHV, SC_SS, SC_ST, SC_TT, SC_RR, SC_FLAG, SC_COUNT };
+// p = r + beta (p - omega v) with beta from the scalars of the previous step
 template <typename T>
-void x_update(hipStream_t s, int64_t n, double alpha, const T* ph, double omega, const T* sh, T* x);  // x += a ph + w sh
+void bicg_p_dev(hipStream_t s, int64_t n, const T* r, const T* v, const double* sc, T* p);
+// s = r - alpha v, partial[b] = s.s (nblk blocks)
+template <typename T>
+void bicg_s_dev(hipStream_t s, int64_t n, const T* v, const T* r, const double* sc, T* s_out, double* partial,
+                int nblk);
+// x += alpha ph + omega sh; r = s - omega t; partial[2b] = rh.r, partial[2b+1] = r.r
+template <typename T>
+void bicg_xr_dev(hipStream_t s, int64_t n, const T* ph, const T* sh, const T* sv, const T* tv, const T* rh,
+                 const double* sc, T* x, T* r, double* partial, int nblk);
+// one block sums ncols (<= 2) partial columns in a fixed order and updates the scalars:
+// stage 0 start (rho, rr), 1 alpha from rh.v, 2 s.s, 3 omega from (s.t, t.t), 4 new (rho, rr);
+// sc[SC_FLAG] != 0 marks a breakdown (1: rh.v = 0, 2: omega = 0, 3: non-finite residual)
+void bicg_stage(hipStream_t s, const double* partial, int64_t nblk, int stride, int ncols, int stage, double* sc);
 
 // conversions / vector updates
 template <typename T>

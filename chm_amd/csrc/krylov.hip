@@ -5,6 +5,7 @@
 //   v_{j+1} -= V h (one pass, fused norm) ; v_{j+1} /= ||v_{j+1}||
 // and only the (j+2) Hessenberg entries travel to the host, where the Givens
 // rotations and the least-squares solve run in FP64.
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -244,27 +245,39 @@ void BicgWork<T>::ensure(int64_t n_) {
     if (n == n_ && r.p) return;
     n = n_;
     for (DevBuf<T>* b : {&r, &rh, &p, &v, &s, &t, &ph, &sh}) b->alloc(n);
-    partial.alloc(4 * (size_t)kReduceBlocks);
-    dsc.alloc(8);
-    hsc.alloc(8);
+    // two partial sums per block of the fused SpMV + dot launches, or of the vector kernels
+    partial.alloc(std::max<size_t>(4 * (size_t)kReduceBlocks, 2 * (size_t)pos_spmv_dots_max_blocks(n)));
+    dsc.alloc(SC_COUNT);
+    hsc.alloc(SC_COUNT);
 }
 
 namespace {
-// one BiCGStab run on A x = b from the x given; stops when the implicit
-// residual drops to stop_abs or after maxit iterations; returns iterations
+// y = A x with the partial sums (w.y, y.y) of each block (two per block); returns the block count
+template <typename T>
+int64_t spmv_dots(hipStream_t s, const CsrOp<T>& A, const T* x, T* y, const T* w, double* partial, int nblk) {
+    if (A.pos) return pos_spmv_dots<T>(s, A.variant, *A.co, A.F, A.L, *A.pos, x, y, w, partial);
+    op_spmv<T>(s, A, x, y);
+    mdot<T>(s, A.n, w, 1, y, partial, nblk);
+    return nblk;
+}
+
+// One BiCGStab run on A x = b from the x given; stops when ||r|| <= stop_abs,
+// on a breakdown (the caller restarts from the true residual) or after maxit
+// iterations; returns the iterations.  alpha, omega, rho live on the device:
+// the only host synchronisation per iteration is the convergence test.
 template <typename T>
 int bicg_run(hipStream_t s, BicgWork<T>& w, const CsrOp<T>& A, const PcFn<T>& M, const T* b, T* x, bool x_zero,
              double stop_abs, int maxit, double* res_out) {
     const int64_t n = A.n;
     const int nblk = reduce_blocks(n);
     const int spmv_id = sizeof(T) == 8 ? PBSM3D_K_SPMV64 : PBSM3D_K_SPMV32;
-    double* d = w.dsc.p;
+    double* sc = w.dsc.p;
     double* h = w.hsc.p;
-    auto fetch = [&](int cnt) {
-        PB_HIP(hipMemcpyAsync(h, d, sizeof(double) * cnt, hipMemcpyDeviceToHost, s));
+    auto fetch = [&]() {
+        PB_HIP(hipMemcpyAsync(h, sc, sizeof(double) * SC_COUNT, hipMemcpyDeviceToHost, s));
         PB_HIP(hipStreamSynchronize(s));
     };
-    // r = b - A x, rh = r
+    // r = b - A x, rh = r, (rh.r, r.r)
     if (x_zero) {
         PB_HIP(hipMemcpyAsync(w.r.p, b, n * sizeof(T), hipMemcpyDeviceToDevice, s));
     } else {
@@ -275,45 +288,35 @@ int bicg_run(hipStream_t s, BicgWork<T>& w, const CsrOp<T>& A, const PcFn<T>& M,
         axpy_dots<T>(s, n, 1.0, w.v.p, b, w.r.p, nullptr, w.partial.p, nblk);
     }
     PB_HIP(hipMemcpyAsync(w.rh.p, w.r.p, n * sizeof(T), hipMemcpyDeviceToDevice, s));
-    mdot<T>(s, n, w.rh.p, 1, w.r.p, w.partial.p, nblk);  // (rh.r, r.r)
-    reduce_cols(s, w.partial.p, nblk, 2, 2, d);
-    fetch(2);
-    double rho_new = h[0], rr = std::sqrt(h[1]);
+    mdot<T>(s, n, w.rh.p, 1, w.r.p, w.partial.p, nblk);
+    bicg_stage(s, w.partial.p, nblk, 2, 2, 0, sc);
+    fetch();
+    double rr = std::sqrt(h[SC_RR]);
     *res_out = rr;
     if (rr <= stop_abs) return 0;
-    double rho = 1, alpha = 1, omega = 1;
     int it = 0;
     while (it < maxit) {
         ++it;
         if (it == 1) {
             PB_HIP(hipMemcpyAsync(w.p.p, w.r.p, n * sizeof(T), hipMemcpyDeviceToDevice, s));
         } else {
-            const double beta = (rho_new / rho) * (alpha / omega);
             ProfScope ps(A.prof, s, PBSM3D_K_ORTHO);
-            bicg_p<T>(s, n, w.r.p, w.v.p, beta, omega, w.p.p);
+            bicg_p_dev<T>(s, n, w.r.p, w.v.p, sc, w.p.p);
         }
-        rho = rho_new;
         {
             ProfScope ps(A.prof, s, PBSM3D_K_PRECOND);
             M(s, w.p.p, w.ph.p);
         }
+        int64_t nb = 0;
         {
             ProfScope ps(A.prof, s, spmv_id);
-            op_spmv<T>(s, A, w.ph.p, w.v.p);
+            nb = spmv_dots<T>(s, A, w.ph.p, w.v.p, w.rh.p, w.partial.p, nblk);  // v = A ph; rh.v, v.v
         }
         {
             ProfScope ps(A.prof, s, PBSM3D_K_ORTHO);
-            mdot<T>(s, n, w.rh.p, 1, w.v.p, w.partial.p, nblk);  // (rh.v, v.v)
-            reduce_cols(s, w.partial.p, nblk, 2, 1, d);
-        }
-        fetch(1);
-        const double rhv = h[0];
-        if (rhv == 0.0 || !std::isfinite(rhv)) break;  // breakdown: caller restarts
-        alpha = rho / rhv;
-        {
-            ProfScope ps(A.prof, s, PBSM3D_K_ORTHO);
-            axpy_dots<T>(s, n, alpha, w.v.p, w.r.p, w.s.p, nullptr, w.partial.p, nblk);  // s = r - a v
-            reduce_cols(s, w.partial.p, nblk, 2, 2, d + 2);                                // d[3] = s.s
+            bicg_stage(s, w.partial.p, nb, 2, 2, 1, sc);  // alpha
+            bicg_s_dev<T>(s, n, w.v.p, w.r.p, sc, w.s.p, w.partial.p, nblk);
+            bicg_stage(s, w.partial.p, nblk, 1, 1, 2, sc);
         }
         {
             ProfScope ps(A.prof, s, PBSM3D_K_PRECOND);
@@ -321,32 +324,18 @@ int bicg_run(hipStream_t s, BicgWork<T>& w, const CsrOp<T>& A, const PcFn<T>& M,
         }
         {
             ProfScope ps(A.prof, s, spmv_id);
-            op_spmv<T>(s, A, w.sh.p, w.t.p);
+            nb = spmv_dots<T>(s, A, w.sh.p, w.t.p, w.s.p, w.partial.p, nblk);  // t = A sh; s.t, t.t
         }
         {
             ProfScope ps(A.prof, s, PBSM3D_K_ORTHO);
-            mdot<T>(s, n, w.s.p, 1, w.t.p, w.partial.p, nblk);  // (s.t, t.t)
-            reduce_cols(s, w.partial.p, nblk, 2, 2, d);
+            bicg_stage(s, w.partial.p, nb, 2, 2, 3, sc);  // omega
+            bicg_xr_dev<T>(s, n, w.ph.p, w.sh.p, w.s.p, w.t.p, w.rh.p, sc, x, w.r.p, w.partial.p, nblk);
+            bicg_stage(s, w.partial.p, nblk, 2, 2, 4, sc);  // rho, rr
         }
-        fetch(4);
-        const double st = h[0], tt = h[1], ss = std::sqrt(h[3]);
-        if (ss <= stop_abs || tt == 0.0) {
-            x_update<T>(s, n, alpha, w.ph.p, 0.0, w.sh.p, x);
-            *res_out = ss;
-            return it;
-        }
-        omega = st / tt;
-        {
-            ProfScope ps(A.prof, s, PBSM3D_K_ORTHO);
-            x_update<T>(s, n, alpha, w.ph.p, omega, w.sh.p, x);
-            axpy_dots<T>(s, n, omega, w.t.p, w.s.p, w.r.p, w.rh.p, w.partial.p, nblk);  // r = s - w t
-            reduce_cols(s, w.partial.p, nblk, 2, 2, d);
-        }
-        fetch(2);
-        rho_new = h[0];
-        rr = std::sqrt(h[1]);
+        fetch();
+        rr = std::sqrt(h[SC_RR]);
         *res_out = rr;
-        if (rr <= stop_abs || omega == 0.0 || !std::isfinite(rr)) return it;
+        if (h[SC_FLAG] != 0.0 || rr <= stop_abs || !std::isfinite(rr)) return it;
     }
     return it;
 }

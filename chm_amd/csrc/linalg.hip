@@ -365,13 +365,6 @@ __global__ void __launch_bounds__(kBlock) k_pc_jacobi(int64_t n, const double* _
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_bicg_p(int64_t n, const T* __restrict__ r, const T* __restrict__ v,
-                                                   double beta, double omega, T* __restrict__ p) {
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) p[k] = (T)fma(beta, fma(-omega, (double)v[k], (double)p[k]), (double)r[k]);
-}
-
-template <typename T>
 __global__ void __launch_bounds__(kBlock) k_axpy_dots(int64_t n, double a, const T* __restrict__ x,
                                                       const T* __restrict__ y, T* __restrict__ out,
                                                       const T* __restrict__ rh, double* __restrict__ partial) {
@@ -393,11 +386,116 @@ __global__ void __launch_bounds__(kBlock) k_axpy_dots(int64_t n, double a, const
     }
 }
 
+// ---- BiCGStab with device-resident scalars (slots pb::BicgSc) ----
+// p = r + beta (p - omega v), beta = (rho / rho_prev) (alpha / omega) of the previous step
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_x_update(int64_t n, double alpha, const T* __restrict__ ph, double omega,
-                                                     const T* __restrict__ sh, T* __restrict__ x) {
+__global__ void __launch_bounds__(kBlock) k_bicg_p_dev(int64_t n, const T* __restrict__ r, const T* __restrict__ v,
+                                                       const double* __restrict__ sc, T* __restrict__ p) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) x[k] = (T)fma(omega, (double)sh[k], fma(alpha, (double)ph[k], (double)x[k]));
+    if (k >= n) return;
+    const double omega = sc[SC_OMEGA];
+    const double beta = (sc[SC_RHO] / sc[SC_RHO_PREV]) * (sc[SC_ALPHA] / omega);
+    p[k] = (T)fma(beta, fma(-omega, (double)v[k], (double)p[k]), (double)r[k]);
+}
+
+// s = r - alpha v, partial[b] = s.s
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_bicg_s_dev(int64_t n, const T* __restrict__ v, const T* __restrict__ r,
+                                                       const double* __restrict__ sc, T* __restrict__ s_out,
+                                                       double* __restrict__ partial) {
+    __shared__ double s_red[kBlock / 64];
+    const double alpha = sc[SC_ALPHA];
+    double ss = 0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+        const T o = (T)fma(-alpha, (double)v[k], (double)r[k]);
+        s_out[k] = o;
+        ss = fma((double)o, (double)o, ss);
+    }
+    const double t = block_sum(ss, s_red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// x += alpha ph + omega sh;  r = s - omega t;  partial[2b] = rh.r, partial[2b+1] = r.r
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_bicg_xr_dev(int64_t n, const T* __restrict__ ph, const T* __restrict__ sh,
+                                                        const T* __restrict__ sv, const T* __restrict__ tv,
+                                                        const T* __restrict__ rh, const double* __restrict__ sc,
+                                                        T* __restrict__ x, T* __restrict__ r,
+                                                        double* __restrict__ partial) {
+    __shared__ double s_red[kBlock / 64];
+    const double alpha = sc[SC_ALPHA], omega = sc[SC_OMEGA];
+    double d1 = 0, d2 = 0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+        x[k] = (T)fma(omega, (double)sh[k], fma(alpha, (double)ph[k], (double)x[k]));
+        const T o = (T)fma(-omega, (double)tv[k], (double)sv[k]);
+        r[k] = o;
+        d1 = fma((double)rh[k], (double)o, d1);
+        d2 = fma((double)o, (double)o, d2);
+    }
+    const double a = block_sum(d1, s_red);
+    const double b = block_sum(d2, s_red);
+    if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x] = a;
+        partial[2 * blockIdx.x + 1] = b;
+    }
+}
+
+// Sums ncols (<= 2) columns of partial (stride, fixed order) in one block and
+// updates the scalars of the given BiCGStab stage (see bicg_stage()).
+constexpr int kStageThreads = 1024;
+__global__ void __launch_bounds__(kStageThreads) k_bicg_stage(const double* __restrict__ partial, int64_t nblk,
+                                                              int stride, int ncols, int stage,
+                                                              double* __restrict__ sc) {
+    __shared__ double s_red[kStageThreads / 64];
+    double sum[2] = {0, 0};
+    for (int c = 0; c < ncols; ++c) {
+        double a = 0;
+        for (int64_t b = threadIdx.x; b < nblk; b += blockDim.x) a += partial[b * stride + c];
+        sum[c] = block_sum(a, s_red);
+    }
+    if (threadIdx.x != 0) return;
+    const double a = sum[0], b = sum[1];
+    switch (stage) {
+        case 0:  // rho = rh.r, rr = r.r of the starting residual
+            sc[SC_RHO] = a;
+            sc[SC_RR] = b;
+            sc[SC_RHO_PREV] = 1;
+            sc[SC_ALPHA] = 1;
+            sc[SC_OMEGA] = 1;
+            sc[SC_FLAG] = 0;
+            break;
+        case 1:  // a = rh.v -> alpha
+            sc[SC_RHV] = a;
+            if (a != 0.0 && isfinite(a)) {
+                sc[SC_ALPHA] = sc[SC_RHO] / a;
+            } else {
+                sc[SC_ALPHA] = 0;
+                sc[SC_FLAG] = 1;  // breakdown: the caller restarts
+            }
+            break;
+        case 2:  // a = s.s
+            sc[SC_SS] = a;
+            break;
+        case 3:  // a = s.t, b = t.t -> omega
+            sc[SC_ST] = a;
+            sc[SC_TT] = b;
+            if (b > 0.0 && isfinite(a) && isfinite(b)) {
+                sc[SC_OMEGA] = a / b;
+            } else {
+                sc[SC_OMEGA] = 0;
+                sc[SC_FLAG] = 2;
+            }
+            if (sc[SC_OMEGA] == 0.0 && sc[SC_FLAG] == 0) sc[SC_FLAG] = 2;
+            break;
+        default:  // a = rh.r, b = r.r of the new residual
+            sc[SC_RHO_PREV] = sc[SC_RHO];
+            sc[SC_RHO] = a;
+            sc[SC_RR] = b;
+            if (!isfinite(b)) sc[SC_FLAG] = 3;
+            break;
+    }
 }
 
 template <typename T>
@@ -523,17 +621,26 @@ void pc_jacobi(hipStream_t s, int64_t n, const double* invdiag, const T* r, T* z
 }
 
 template <typename T>
-void bicg_p(hipStream_t s, int64_t n, const T* r, const T* v, double beta, double omega, T* p) {
-    hipLaunchKernelGGL((k_bicg_p<T>), dim3(grid_for(n)), dim3(kBlock), 0, s, n, r, v, beta, omega, p);
-}
-template <typename T>
 void axpy_dots(hipStream_t s, int64_t n, double a, const T* x, const T* y, T* out, const T* rh, double* partial,
                int nblk) {
     hipLaunchKernelGGL((k_axpy_dots<T>), dim3(nblk), dim3(kBlock), 0, s, n, a, x, y, out, rh, partial);
 }
 template <typename T>
-void x_update(hipStream_t s, int64_t n, double alpha, const T* ph, double omega, const T* sh, T* x) {
-    hipLaunchKernelGGL((k_x_update<T>), dim3(grid_for(n)), dim3(kBlock), 0, s, n, alpha, ph, omega, sh, x);
+void bicg_p_dev(hipStream_t s, int64_t n, const T* r, const T* v, const double* sc, T* p) {
+    hipLaunchKernelGGL((k_bicg_p_dev<T>), dim3(grid_for(n)), dim3(kBlock), 0, s, n, r, v, sc, p);
+}
+template <typename T>
+void bicg_s_dev(hipStream_t s, int64_t n, const T* v, const T* r, const double* sc, T* s_out, double* partial,
+                int nblk) {
+    hipLaunchKernelGGL((k_bicg_s_dev<T>), dim3(nblk), dim3(kBlock), 0, s, n, v, r, sc, s_out, partial);
+}
+template <typename T>
+void bicg_xr_dev(hipStream_t s, int64_t n, const T* ph, const T* sh, const T* sv, const T* tv, const T* rh,
+                 const double* sc, T* x, T* r, double* partial, int nblk) {
+    hipLaunchKernelGGL((k_bicg_xr_dev<T>), dim3(nblk), dim3(kBlock), 0, s, n, ph, sh, sv, tv, rh, sc, x, r, partial);
+}
+void bicg_stage(hipStream_t s, const double* partial, int64_t nblk, int stride, int ncols, int stage, double* sc) {
+    hipLaunchKernelGGL(k_bicg_stage, dim3(1), dim3(kStageThreads), 0, s, partial, nblk, stride, ncols, stage, sc);
 }
 
 template <typename T>
@@ -573,9 +680,11 @@ void copy_f64(hipStream_t s, int64_t n, const double* a, double* b) {
     template void scale_host<T>(hipStream_t, int64_t, const T*, double, T*);                                       \
     template void spmv_col<T>(hipStream_t, int64_t, int, const int32_t*, const T*, const int32_t*, const uint16_t*, \
                               const T*, const T*, T*, double*);                                                    \
-    template void bicg_p<T>(hipStream_t, int64_t, const T*, const T*, double, double, T*);                         \
     template void axpy_dots<T>(hipStream_t, int64_t, double, const T*, const T*, T*, const T*, double*, int);      \
-    template void x_update<T>(hipStream_t, int64_t, double, const T*, double, const T*, T*);
+    template void bicg_p_dev<T>(hipStream_t, int64_t, const T*, const T*, const double*, T*);                     \
+    template void bicg_s_dev<T>(hipStream_t, int64_t, const T*, const T*, const double*, T*, double*, int);       \
+    template void bicg_xr_dev<T>(hipStream_t, int64_t, const T*, const T*, const T*, const T*, const T*,          \
+                                 const double*, T*, T*, double*, int);
 PB_INST(double)
 PB_INST(float)
 #undef PB_INST

@@ -280,56 +280,89 @@ __global__ void __launch_bounds__(kBlock) k_pos_spmv(int64_t F, int L, const int
 // y = A x in position order, one thread per face column (L <= LM, fully
 // unrolled): the neighbour positions are read once per column and the
 // vertical neighbours come from registers.  B is the block size.
-template <typename T, int LM, int B, bool XCD>
+// DOTS: the kernel also forms per-block partial sums of w.y and y.y
+// (FP64) into partial[2 * block], the block being the physical workgroup
+// index (x + y * gridDim.x), so a Krylov step needs no separate dot pass.
+template <int B, bool DOTS>
+__device__ __forceinline__ void block_dots(double wy, double yy, double* __restrict__ partial) {
+    if constexpr (DOTS) {
+        __shared__ double s_red[B / 64];
+        const double a = block_sum(wy, s_red);
+        const double b = block_sum(yy, s_red);
+        if (threadIdx.x == 0) {
+            const int64_t blk = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
+            partial[2 * blk] = a;
+            partial[2 * blk + 1] = b;
+        }
+    }
+}
+
+template <typename T, int LM, int B, bool XCD, bool DOTS>
 __global__ void __launch_bounds__(B) k_pos_spmv_col(int64_t F, int L, const int32_t* __restrict__ nbp,
                                                     const T* __restrict__ diag, const T* __restrict__ lat,
                                                     const T* __restrict__ vert, const T* __restrict__ x,
-                                                    T* __restrict__ y) {
+                                                    T* __restrict__ y, const T* __restrict__ w,
+                                                    double* __restrict__ partial) {
     const int64_t k = logical_block<XCD>((F + B - 1) / B) * B + threadIdx.x;
-    if (k >= F) return;
-    const int64_t N = F * L;
-    int32_t np_[3];
+    double wy = 0, yy = 0;
+    if (k < F) {
+        const int64_t N = F * L;
+        int32_t np_[3];
 #pragma unroll
-    for (int f = 0; f < 3; ++f) np_[f] = nbp[f * F + k];
-    T xv[LM];
+        for (int f = 0; f < 3; ++f) np_[f] = nbp[f * F + k];
+        T xv[LM];
 #pragma unroll
-    for (int z = 0; z < LM; ++z)
-        if (z < L) xv[z] = x[(int64_t)z * F + k];
+        for (int z = 0; z < LM; ++z)
+            if (z < L) xv[z] = x[(int64_t)z * F + k];
 #pragma unroll
-    for (int z = 0; z < LM; ++z)
-        if (z < L) {
-            const int64_t q = (int64_t)z * F + k;
-            T acc = diag[q] * xv[z];
-            if (z > 0) acc = fma(vert[q], xv[z > 0 ? z - 1 : 0], acc);
+        for (int z = 0; z < LM; ++z)
+            if (z < L) {
+                const int64_t q = (int64_t)z * F + k;
+                T acc = diag[q] * xv[z];
+                if (z > 0) acc = fma(vert[q], xv[z > 0 ? z - 1 : 0], acc);
 #pragma unroll
-            for (int f = 0; f < 3; ++f)
-                if (np_[f] >= 0) acc = fma(lat[f * N + q], x[(int64_t)z * F + np_[f]], acc);
-            if (z < L - 1) acc = fma(vert[N + q], xv[z + 1 < LM ? z + 1 : z], acc);
-            y[q] = acc;
-        }
+                for (int f = 0; f < 3; ++f)
+                    if (np_[f] >= 0) acc = fma(lat[f * N + q], x[(int64_t)z * F + np_[f]], acc);
+                if (z < L - 1) acc = fma(vert[N + q], xv[z + 1 < LM ? z + 1 : z], acc);
+                y[q] = acc;
+                if (DOTS) {
+                    wy = fma((double)w[q], (double)acc, wy);
+                    yy = fma((double)acc, (double)acc, yy);
+                }
+            }
+    }
+    block_dots<B, DOTS>(wy, yy, partial);
 }
 
 // y = A x in position order, one thread per row, block size B (grid: F/B x L)
-template <typename T, int B, bool XCD>
+template <typename T, int B, bool XCD, bool DOTS>
 __global__ void __launch_bounds__(B) k_pos_spmv_row(int64_t F, int L, const int32_t* __restrict__ nbp,
                                                     const T* __restrict__ diag, const T* __restrict__ lat,
                                                     const T* __restrict__ vert, const T* __restrict__ x,
-                                                    T* __restrict__ y) {
+                                                    T* __restrict__ y, const T* __restrict__ w,
+                                                    double* __restrict__ partial) {
     const int64_t k = logical_block<XCD>((F + B - 1) / B) * B + threadIdx.x;
-    if (k >= F) return;
-    const int z = blockIdx.y;
-    const int64_t N = F * L;
-    const int64_t q = (int64_t)z * F + k;
-    const int64_t lay = (int64_t)z * F;
-    T acc = diag[q] * x[q];
-    if (z > 0) acc = fma(vert[q], x[q - F], acc);
+    double wy = 0, yy = 0;
+    if (k < F) {
+        const int z = blockIdx.y;
+        const int64_t N = F * L;
+        const int64_t q = (int64_t)z * F + k;
+        const int64_t lay = (int64_t)z * F;
+        T acc = diag[q] * x[q];
+        if (z > 0) acc = fma(vert[q], x[q - F], acc);
 #pragma unroll
-    for (int f = 0; f < 3; ++f) {
-        const int32_t n = nbp[f * F + k];
-        if (n >= 0) acc = fma(lat[f * N + q], x[lay + n], acc);
+        for (int f = 0; f < 3; ++f) {
+            const int32_t n = nbp[f * F + k];
+            if (n >= 0) acc = fma(lat[f * N + q], x[lay + n], acc);
+        }
+        if (z < L - 1) acc = fma(vert[N + q], x[q + F], acc);
+        y[q] = acc;
+        if (DOTS) {
+            wy = (double)w[q] * (double)acc;
+            yy = (double)acc * (double)acc;
+        }
     }
-    if (z < L - 1) acc = fma(vert[N + q], x[q + F], acc);
-    y[q] = acc;
+    block_dots<B, DOTS>(wy, yy, partial);
 }
 
 // reference order (z*F + face) <-> position order (z*F + k), with a type change
@@ -586,43 +619,39 @@ namespace {
 // the launch shapes the tuner chooses from: shape v % 5 (0 / 4 one thread per
 // row with blocks of 256 / 128; 1 / 2 / 3 one thread per face column with
 // blocks of 256 / 128 / 64), with the XCD workgroup mapping when v >= 5
-template <typename T, int B, bool XCD>
-void row_launch(hipStream_t s, int64_t F, int L, const ColorOrder& co, const DiluFactors<T>& d, const T* x, T* y) {
+template <typename T, int B, bool XCD, bool DOTS>
+int64_t row_launch(hipStream_t s, int64_t F, int L, const ColorOrder& co, const DiluFactors<T>& d, const T* x, T* y,
+                   const T* w, double* partial) {
     const dim3 g((unsigned)launch_blocks<XCD>((F + B - 1) / B), (unsigned)L);
-    hipLaunchKernelGGL((k_pos_spmv_row<T, B, XCD>), g, dim3(B), 0, s, F, L, co.nbp.p, d.diag.p, d.lat.p, d.vert.p,
-                       x, y);
+    hipLaunchKernelGGL((k_pos_spmv_row<T, B, XCD, DOTS>), g, dim3(B), 0, s, F, L, co.nbp.p, d.diag.p, d.lat.p,
+                       d.vert.p, x, y, w, partial);
+    return (int64_t)g.x * g.y;
 }
-template <typename T, int LM, int B, bool XCD>
-void col_launch(hipStream_t s, int64_t F, int L, const ColorOrder& co, const DiluFactors<T>& d, const T* x, T* y) {
+template <typename T, int LM, int B, bool XCD, bool DOTS>
+int64_t col_launch(hipStream_t s, int64_t F, int L, const ColorOrder& co, const DiluFactors<T>& d, const T* x, T* y,
+                   const T* w, double* partial) {
     const dim3 g((unsigned)launch_blocks<XCD>((F + B - 1) / B));
-    hipLaunchKernelGGL((k_pos_spmv_col<T, LM, B, XCD>), g, dim3(B), 0, s, F, L, co.nbp.p, d.diag.p, d.lat.p,
-                       d.vert.p, x, y);
+    hipLaunchKernelGGL((k_pos_spmv_col<T, LM, B, XCD, DOTS>), g, dim3(B), 0, s, F, L, co.nbp.p, d.diag.p, d.lat.p,
+                       d.vert.p, x, y, w, partial);
+    return (int64_t)g.x;
 }
-template <typename T, int LM, bool XCD>
-void col_shapes(hipStream_t s, int shape, int64_t F, int L, const ColorOrder& co, const DiluFactors<T>& d,
-                const T* x, T* y) {
-    if (shape == 1)
-        col_launch<T, LM, 256, XCD>(s, F, L, co, d, x, y);
-    else if (shape == 2)
-        col_launch<T, LM, 128, XCD>(s, F, L, co, d, x, y);
-    else
-        col_launch<T, LM, 64, XCD>(s, F, L, co, d, x, y);
+template <typename T, int LM, bool XCD, bool DOTS>
+int64_t col_shapes(hipStream_t s, int shape, int64_t F, int L, const ColorOrder& co, const DiluFactors<T>& d,
+                   const T* x, T* y, const T* w, double* partial) {
+    if (shape == 1) return col_launch<T, LM, 256, XCD, DOTS>(s, F, L, co, d, x, y, w, partial);
+    if (shape == 2) return col_launch<T, LM, 128, XCD, DOTS>(s, F, L, co, d, x, y, w, partial);
+    return col_launch<T, LM, 64, XCD, DOTS>(s, F, L, co, d, x, y, w, partial);
 }
-template <typename T, bool XCD>
-void shapes(hipStream_t s, int shape, int64_t F, int L, const ColorOrder& co, const DiluFactors<T>& d, const T* x,
-            T* y) {
+template <typename T, bool XCD, bool DOTS>
+int64_t shapes(hipStream_t s, int shape, int64_t F, int L, const ColorOrder& co, const DiluFactors<T>& d, const T* x,
+               T* y, const T* w, double* partial) {
     if (shape >= 1 && shape <= 3 && L <= 16) {
-        if (L <= 5)
-            col_shapes<T, 5, XCD>(s, shape, F, L, co, d, x, y);
-        else if (L <= 10)
-            col_shapes<T, 10, XCD>(s, shape, F, L, co, d, x, y);
-        else
-            col_shapes<T, 16, XCD>(s, shape, F, L, co, d, x, y);
-    } else if (shape == 4) {
-        row_launch<T, 128, XCD>(s, F, L, co, d, x, y);
-    } else {
-        row_launch<T, 256, XCD>(s, F, L, co, d, x, y);
+        if (L <= 5) return col_shapes<T, 5, XCD, DOTS>(s, shape, F, L, co, d, x, y, w, partial);
+        if (L <= 10) return col_shapes<T, 10, XCD, DOTS>(s, shape, F, L, co, d, x, y, w, partial);
+        return col_shapes<T, 16, XCD, DOTS>(s, shape, F, L, co, d, x, y, w, partial);
     }
+    if (shape == 4) return row_launch<T, 128, XCD, DOTS>(s, F, L, co, d, x, y, w, partial);
+    return row_launch<T, 256, XCD, DOTS>(s, F, L, co, d, x, y, w, partial);
 }
 bool shape_allowed(int v, int L) {
     const int shape = v % 5;
@@ -634,9 +663,16 @@ template <typename T>
 void pos_spmv_variant(hipStream_t s, int v, const ColorOrder& co, int64_t F, int L, const DiluFactors<T>& d,
                       const T* x, T* y) {
     if (v >= 5)
-        shapes<T, true>(s, v - 5, F, L, co, d, x, y);
+        shapes<T, true, false>(s, v - 5, F, L, co, d, x, y, nullptr, nullptr);
     else
-        shapes<T, false>(s, v, F, L, co, d, x, y);
+        shapes<T, false, false>(s, v, F, L, co, d, x, y, nullptr, nullptr);
+}
+
+template <typename T>
+int64_t pos_spmv_dots(hipStream_t s, int v, const ColorOrder& co, int64_t F, int L, const DiluFactors<T>& d,
+                      const T* x, T* y, const T* w, double* partial) {
+    if (v >= 5) return shapes<T, true, true>(s, v - 5, F, L, co, d, x, y, w, partial);
+    return shapes<T, false, true>(s, v, F, L, co, d, x, y, w, partial);
 }
 
 template <typename T>
@@ -706,6 +742,10 @@ template void pos_spmv_variant<double>(hipStream_t, int, const ColorOrder&, int6
                                        const double*, double*);
 template void pos_spmv_variant<float>(hipStream_t, int, const ColorOrder&, int64_t, int, const DiluFactors<float>&,
                                       const float*, float*);
+template int64_t pos_spmv_dots<double>(hipStream_t, int, const ColorOrder&, int64_t, int, const DiluFactors<double>&,
+                                       const double*, double*, const double*, double*);
+template int64_t pos_spmv_dots<float>(hipStream_t, int, const ColorOrder&, int64_t, int, const DiluFactors<float>&,
+                                      const float*, float*, const float*, double*);
 template int tune_pos_spmv<double>(hipStream_t, const ColorOrder&, int64_t, int, const DiluFactors<double>&,
                                    const double*, double*, float*);
 template int tune_pos_spmv<float>(hipStream_t, const ColorOrder&, int64_t, int, const DiluFactors<float>&,

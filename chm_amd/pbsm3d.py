@@ -34,6 +34,12 @@ EXPORTED = (
     "pbsm3d_face_colors",
     "pbsm3d_precond_tuning",
     "pbsm3d_precond_select",
+    "pbsm3d_fetch_config_default",
+    "pbsm3d_fetch_create",
+    "pbsm3d_fetch_run",
+    "pbsm3d_fetch_nearest",
+    "pbsm3d_fetch_destroy",
+    "pbsm3d_fetch_last_error",
     "pbsm3d_mlp_create",
     "pbsm3d_mlp_destroy",
     "pbsm3d_mlp_param_count",
@@ -105,6 +111,16 @@ def lib():
         L.pbsm3d_face_colors.argtypes = [C.c_int64, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
         L.pbsm3d_precond_tuning.argtypes = [vp, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_float)]
         L.pbsm3d_precond_select.argtypes = [vp, C.c_int, C.c_int]
+        L.pbsm3d_fetch_config_default.argtypes = [C.POINTER(abi.FetchConfig)]
+        L.pbsm3d_fetch_config_default.restype = None
+        L.pbsm3d_fetch_create.argtypes = [C.POINTER(vp), C.c_int, C.c_int64, dp, dp, dp, dp,
+                                          C.POINTER(abi.FetchConfig)]
+        L.pbsm3d_fetch_run.argtypes = [vp, vp, vp, C.c_int]
+        L.pbsm3d_fetch_nearest.argtypes = [vp, C.c_int64, dp, dp, C.POINTER(C.c_int32)]
+        L.pbsm3d_fetch_destroy.argtypes = [vp]
+        L.pbsm3d_fetch_destroy.restype = None
+        L.pbsm3d_fetch_last_error.argtypes = [vp]
+        L.pbsm3d_fetch_last_error.restype = C.c_char_p
         fp = C.POINTER(C.c_float)
         L.pbsm3d_mlp_create.argtypes = [C.POINTER(vp), C.c_int, C.c_uint64]
         L.pbsm3d_mlp_destroy.argtypes = [vp]
@@ -347,3 +363,69 @@ class Mlp:
         self._ck(lib().pbsm3d_mlp_gradients(self._h, X.ctypes.data_as(fp), Y.ctypes.data_as(fp), X.shape[0],
                                             g.ctypes.data_as(fp), C.byref(loss)))
         return loss.value, g
+
+
+def fetch_config(**overrides) -> abi.FetchConfig:
+    c = abi.FetchConfig()
+    lib().pbsm3d_fetch_config_default(C.byref(c))
+    for k, v in overrides.items():
+        if not hasattr(c, k):
+            raise KeyError(k)
+        setattr(c, k, v)
+    return c
+
+
+class Fetch:
+    """Upwind fetch on the GPU (the fetchr module's computation) for fixed face centroids.
+
+    canopy_height: per-face canopy heights, or None when the mesh carries no vegetation."""
+
+    def __init__(self, center_x, center_y, center_z, canopy_height=None, cfg: abi.FetchConfig | None = None,
+                 device: int = 0):
+        L = lib()
+        self._keep = abi.Keep()
+        self.n_faces = int(len(center_x))
+        cfg = cfg if cfg is not None else fetch_config()
+        h = C.c_void_p()
+        k = self._keep
+        rc = L.pbsm3d_fetch_create(C.byref(h), device, self.n_faces, k.dptr(center_x), k.dptr(center_y),
+                                   k.dptr(center_z), k.dptr(canopy_height) if canopy_height is not None else None,
+                                   C.byref(cfg))
+        if rc != 0:
+            raise Pbsm3dError(rc, L.pbsm3d_last_error(None).decode())
+        self._h = h
+
+    def run(self, vw_dir):
+        """fetch per face for wind directions vw_dir: numpy (host) or a float64 CUDA tensor (device)."""
+        L = lib()
+        if hasattr(vw_dir, "data_ptr"):
+            import torch
+
+            out = torch.empty_like(vw_dir)
+            rc = L.pbsm3d_fetch_run(self._h, C.c_void_p(vw_dir.data_ptr()), C.c_void_p(out.data_ptr()), 1)
+        else:
+            vw = np.ascontiguousarray(vw_dir, dtype=np.float64)
+            out = np.zeros(self.n_faces)
+            rc = L.pbsm3d_fetch_run(self._h, vw.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p), 0)
+        if rc != 0:
+            raise Pbsm3dError(rc, L.pbsm3d_fetch_last_error(self._h).decode())
+        return out
+
+    def nearest(self, qx, qy):
+        L = lib()
+        qx = np.ascontiguousarray(qx, dtype=np.float64)
+        qy = np.ascontiguousarray(qy, dtype=np.float64)
+        idx = np.zeros(len(qx), dtype=np.int32)
+        rc = L.pbsm3d_fetch_nearest(self._h, len(qx), qx.ctypes.data_as(C.POINTER(C.c_double)),
+                                    qy.ctypes.data_as(C.POINTER(C.c_double)), idx.ctypes.data_as(C.POINTER(C.c_int32)))
+        if rc != 0:
+            raise Pbsm3dError(rc, L.pbsm3d_fetch_last_error(self._h).decode())
+        return idx
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pbsm3d_fetch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()

@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define PBSM3D_ABI_VERSION 2
+#define PBSM3D_ABI_VERSION 3
 
 /* return codes */
 #define PBSM3D_OK 0
@@ -109,6 +109,8 @@ typedef struct pbsm3d_config {
                                      1/sqrt(2) (DGKS re-orthogonalisation)          */
     int32_t block_size;           /* reserved (256)                                */
     int32_t krylov;               /* PBSM3D_KRYLOV_GMRES (reference) or _BICGSTAB  */
+    double inner_rtol;            /* MIXED: relative residual each FP32 inner solve
+                                     aims for; 0 = default 1e-4                    */
 } pbsm3d_config;
 
 /* Krylov method of the suspension solve (the deposition solve is GMRES(restart)) */

@@ -56,6 +56,10 @@ def lib():
         L.oracle_dense_solve.argtypes = [C.c_int64, ip, ip, dp, dp, dp]
         L.oracle_precond_apply.argtypes = [C.c_int64, ip, ip, dp, C.c_int, C.c_int64, C.c_int, dp, dp]
         L.oracle_face_colors.argtypes = [C.c_int64, ip, ip]
+        L.oracle_fetch_nearest.argtypes = [C.c_int64, dp, dp, C.c_int64, dp, dp, ip]
+        L.oracle_fetch_nearest.restype = None
+        L.oracle_fetch.argtypes = [C.c_int64, dp, dp, dp, dp, C.POINTER(abi.FetchConfig), dp, dp]
+        L.oracle_fetch.restype = None
         L.oracle_config_default.argtypes = [C.POINTER(abi.Config)]
         L.oracle_config_default.restype = None
         L.oracle_max_threads.restype = C.c_int
@@ -171,6 +175,26 @@ def solve_csr(rp, col, val, b, precond=abi.PC_ILUT, restart=30, maxit=1000, tol=
     rc = lib().oracle_solve_csr(n, keep.iptr(rp), keep.iptr(col), keep.dptr(val), keep.dptr(b), keep.dptr(x),
                                 precond, restart, maxit, tol, F, L, C.byref(it), C.byref(res))
     return x, it.value, res.value, rc
+
+
+def fetch_nearest(cx, cy, qx, qy):
+    """Index of the centroid nearest to each query point (brute force, ties to the lower index)."""
+    keep = abi.Keep()
+    qx = np.ascontiguousarray(qx, dtype=np.float64)
+    idx = np.zeros(len(qx), dtype=np.int32)
+    lib().oracle_fetch_nearest(len(cx), keep.dptr(cx), keep.dptr(cy), len(qx), keep.dptr(qx), keep.dptr(qy),
+                               keep.iptr(idx))
+    return idx
+
+
+def fetch(cx, cy, cz, canopy, cfg, vw_dir):
+    """fetchr's fetch per face (canopy None: no vegetation)."""
+    keep = abi.Keep()
+    out = np.zeros(len(cx))
+    lib().oracle_fetch(len(cx), keep.dptr(cx), keep.dptr(cy), keep.dptr(cz),
+                       keep.dptr(canopy) if canopy is not None else None, C.byref(cfg), keep.dptr(vw_dir),
+                       keep.dptr(out))
+    return out
 
 
 def face_colors(neigh: np.ndarray):

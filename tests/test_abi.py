@@ -1,4 +1,4 @@
-"""The C-ABI library loads and exports what include/pbsm3d.h declares (no GPU needed)."""
+"""The C-ABI library loads and exports what include/*.h declares (no GPU needed)."""
 import ctypes as C
 import os
 import re
@@ -12,8 +12,12 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def _declared():
-    txt = open(os.path.join(ROOT, "include", "pbsm3d.h")).read()
-    return sorted(set(re.findall(r"\b(pbsm3d_[a-z_0-9]+)\s*\(", txt)))
+    names = set()
+    for h in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if h.endswith(".h"):
+            txt = open(os.path.join(ROOT, "include", h)).read()
+            names |= set(re.findall(r"\b(pbsm3d_[a-z_0-9]+)\s*\(", txt))
+    return sorted(names)
 
 
 def test_library_exports_every_declared_symbol():
@@ -24,7 +28,7 @@ def test_library_exports_every_declared_symbol():
     assert len(names) >= 10
     for n in names:
         assert hasattr(L, n), n
-    assert L.pbsm3d_abi_version() == 2
+    assert L.pbsm3d_abi_version() == 3
     assert set(pbsm3d.EXPORTED) <= set(names)
 
 
