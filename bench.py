@@ -194,6 +194,7 @@ def main():
     ap.add_argument("--cpu-faces", type=int, default=200000)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fetch", action="store_true", help="skip timing the fetch (fetchr) kernel")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="run one streaming kernel (y = -x, float32) reading and writing this many MB first: "
                          "the calibration dispatch for rocprofv3 FETCH_SIZE / WRITE_SIZE")
@@ -204,6 +205,7 @@ def main():
               f"DESIGN.md and no multi-GPU figure is produced.", file=sys.stderr)
         sys.exit(2)
 
+    import numpy as np
     import torch
 
     from chm_amd import abi, meshgen, warmstart
@@ -277,6 +279,23 @@ def main():
                 for k in ("Qsalt", "Qsusp", "Qsubl", "drift_mass", "sum_drift")}
         log(f"[bench] {name}: {variants[name]['steps_per_s']:.2f} steps/s, iters {st['susp_iters']}/{st['dep_iters']}")
 
+    # upstream producer of the fetch input (fetchr): the GPU upwind search on the same mesh
+    fetch_info = None
+    if not args.no_fetch:
+        cxyz = mesh.vertex[mesh.elem].mean(axis=1)
+        fctx = pb.Fetch(np.ascontiguousarray(cxyz[:, 0]), np.ascontiguousarray(cxyz[:, 1]),
+                        np.ascontiguousarray(cxyz[:, 2]))
+        fetch_out = fctx.run(dev_forcing["vw_dir"])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(5):
+            fetch_out = fctx.run(dev_forcing["vw_dir"])
+        torch.cuda.synchronize()
+        fetch_info = {"ms_per_call": round(1000.0 * (time.perf_counter() - t0) / 5, 3), "faces": F,
+                      "queries_per_face": 10, "mean_fetch_m": round(float(fetch_out.mean()), 2)}
+        fctx.close()
+        log(f"[bench] fetch: {fetch_info}")
+
     gain = None
     if net is not None and "mixed_learned" in names and args.gain_seeds:
         log("[bench] learned vs cold start on held-out forcing draws")
@@ -333,6 +352,7 @@ def main():
                              if "plain" in variants else None),
         "warm_start_training": train_info,
         "warm_start_gain": gain,
+        "fetch": fetch_info,
         "roofline": roofline,
         "kernel_rooflines": rooflines,
         "cpu_baseline": cpu,

@@ -120,6 +120,16 @@ class FetchConfig(C.Structure):
     ]
 
 
+class WindConfig(C.Structure):
+    """pbsm3d_wind_config of include/pbsm3d_wind.h."""
+
+    _fields_ = [
+        ("ignore_canopy", C.c_int32),
+        ("smooth", C.c_int32),
+        ("z0_snow", C.c_double),
+    ]
+
+
 class Stats(C.Structure):
     _fields_ = [
         ("suspension_present", C.c_int32),

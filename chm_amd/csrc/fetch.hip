@@ -65,6 +65,7 @@ __device__ __forceinline__ void span(double best, double gap, double q, double o
 }
 
 __device__ int32_t nearest_face(const CentroidGrid& g, double qx, double qy) {
+    if (!isfinite(qx) || !isfinite(qy)) return -1;
     const double fx = floor((qx - g.xmin) / g.h), fy = floor((qy - g.ymin) / g.h);
     const int64_t lim = (int64_t)1 << 40;
     const int64_t ix = (int64_t)fmin(fmax(fx, -(double)lim), (double)lim);
@@ -129,6 +130,10 @@ __global__ void __launch_bounds__(kBlock) k_fetch(CentroidGrid g, FetchArgs a) {
         a.fetch[i] = 0;
         return;
     }
+    if (!isfinite(wind_dir)) {  // no upwind point to search from (the -9999 default is finite)
+        a.fetch[i] = out;
+        return;
+    }
     // point_from_bearing_UTM (coordinates.cpp:60-71)
     const double DegreesToRadians = M_PI / 180.0;
     const double bearing = wind_dir * DegreesToRadians;
@@ -184,6 +189,8 @@ struct pbsm3d_fetch_ctx {
 };
 
 namespace {
+std::string g_fetch_err;  // last creation error (no context to hold it)
+
 template <typename Fn>
 int fetch_guard(pbsm3d_fetch_ctx* c, Fn&& fn) {
     try {
@@ -191,9 +198,11 @@ int fetch_guard(pbsm3d_fetch_ctx* c, Fn&& fn) {
         return PBSM3D_OK;
     } catch (const Error& e) {
         if (c) c->err = e.what();
+        g_fetch_err = e.what();
         return e.code;
     } catch (const std::exception& e) {
         if (c) c->err = e.what();
+        g_fetch_err = e.what();
         return PBSM3D_EINVAL;
     }
 }
@@ -211,8 +220,14 @@ void pbsm3d_fetch_config_default(pbsm3d_fetch_config* c) {
 
 int pbsm3d_fetch_create(pbsm3d_fetch_ctx** out, int device, int64_t F, const double* x, const double* y,
                         const double* z, const double* canopy, const pbsm3d_fetch_config* cfg) {
-    if (!out || F <= 0 || F > INT32_MAX || !x || !y || !z || !cfg) return PBSM3D_EINVAL;
-    if (cfg->steps < 1 || !(cfg->max_distance > 0)) return PBSM3D_EINVAL;
+    if (!out || F <= 0 || F > INT32_MAX || !x || !y || !z || !cfg) {
+        g_fetch_err = "invalid argument";
+        return PBSM3D_EINVAL;
+    }
+    if (cfg->steps < 1 || !(cfg->max_distance > 0)) {
+        g_fetch_err = "steps must be >= 1 and max_distance > 0";
+        return PBSM3D_EINVAL;
+    }
     *out = nullptr;
     auto* c = new pbsm3d_fetch_ctx();
     const int rc = fetch_guard(c, [&] {
@@ -320,6 +335,6 @@ int pbsm3d_fetch_nearest(pbsm3d_fetch_ctx* c, int64_t n, const double* qx, const
 
 void pbsm3d_fetch_destroy(pbsm3d_fetch_ctx* c) { delete c; }
 
-const char* pbsm3d_fetch_last_error(const pbsm3d_fetch_ctx* c) { return c ? c->err.c_str() : ""; }
+const char* pbsm3d_fetch_last_error(const pbsm3d_fetch_ctx* c) { return c ? c->err.c_str() : g_fetch_err.c_str(); }
 
 }  // extern "C"

@@ -48,8 +48,8 @@ struct GemmArgs {
     int relu;
     uint16_t* Cb;  // bf16 copy of the (activated) output, may be null
     int ldcb;
-    int atomic;  // split-K: atomically add into C
-    int ksplit;  // K range per blockIdx.z
+    float* slab;  // split-K: blockIdx.z writes its raw partial product to slab[z][M][N]
+    int ksplit;   // K range per blockIdx.z
 };
 
 __global__ void __launch_bounds__(256) k_gemm_bf16(GemmArgs g) {
@@ -122,8 +122,8 @@ __global__ void __launch_bounds__(256) k_gemm_bf16(GemmArgs g) {
                 const int col = n0 + wn + tn * 16 + (lane & 15);
                 if (row >= g.M || col >= g.N) continue;
                 float v = acc[tm][tn][j];
-                if (g.atomic) {
-                    atomicAdd(&g.C[(size_t)row * g.ldc + col], v);
+                if (g.slab) {
+                    g.slab[((size_t)blockIdx.z * g.M + row) * g.N + col] = v;
                     continue;
                 }
                 if (g.bias) v += g.bias[col];
@@ -134,13 +134,35 @@ __global__ void __launch_bounds__(256) k_gemm_bf16(GemmArgs g) {
             }
 }
 
-void gemm(hipStream_t s, GemmArgs g, int splits = 1) {
+constexpr int kMaxSplits = 64;  // split-K factor of the weight-gradient GEMMs (K = batch rows)
+
+// C[r][c] = sum over z = 0 .. splits-1, in that order, of slab[z][r][c] (deterministic split-K)
+__global__ void k_sum_slabs(int M, int N, int splits, const float* __restrict__ slab, float* __restrict__ C,
+                            int ldc) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)M * N) return;
+    float acc = 0.f;
+    for (int z = 0; z < splits; ++z) acc += slab[(size_t)z * M * N + t];
+    C[(t / N) * ldc + t % N] = acc;
+}
+
+// Split-K GEMMs (splits > 1) need ws with room for splits * M * N floats; the
+// partial products are summed in a fixed order, so results do not depend on
+// scheduling (bias, masks and bf16 copies are not supported with splits).
+void gemm(hipStream_t s, GemmArgs g, int splits = 1, float* ws = nullptr) {
     if (splits < 1) splits = 1;
     g.ksplit = ((g.K + splits - 1) / splits + TK - 1) / TK * TK;
     splits = (g.K + g.ksplit - 1) / g.ksplit;
-    g.atomic = splits > 1 ? 1 : g.atomic;
+    g.slab = splits > 1 ? ws : nullptr;
+    if (splits > 1 && (!ws || g.bias || g.mask || g.relu || g.Cb || !g.C))
+        throw Error(PBSM3D_EINVAL, "split-K GEMM needs a workspace and a plain FP32 output");
     dim3 grid((g.N + TN - 1) / TN, (g.M + TM - 1) / TM, splits);
     hipLaunchKernelGGL(k_gemm_bf16, grid, dim3(256), 0, s, g);
+    if (splits > 1) {
+        const int64_t mn = (int64_t)g.M * g.N;
+        hipLaunchKernelGGL(k_sum_slabs, dim3((unsigned)((mn + 255) / 256)), dim3(256), 0, s, g.M, g.N, splits, ws,
+                           g.C, g.ldc);
+    }
 }
 
 // x (rows x 16, f32) -> normalised bf16 rows; optional row gather idx
@@ -318,6 +340,7 @@ void Mlp::ensure(int64_t n) {
     dYb.alloc(n * MlpDims::OUTP);
     idx.alloc(n);
     partial.alloc(g1(n * MlpDims::OUTP) + 1);
+    if (!splitws.p) splitws.alloc((size_t)kMaxSplits * MlpDims::H * MlpDims::H);
 }
 
 const float* Mlp::xmu() const { return norm.p; }
@@ -358,12 +381,11 @@ double Mlp::backward_prepared(int64_t n, float* dY_f32) {
     hipStream_t s = stream;
     const unsigned nb = g1(n * MlpDims::OUTP);
     hipLaunchKernelGGL(k_mse_grad, dim3(nb), dim3(256), 0, s, n, Yp.p, Tn.p, dYb.p, dY_f32, partial.p);
-    const int splits = (int)std::min<int64_t>(64, std::max<int64_t>(1, n / 2048));
+    const int splits = (int)std::min<int64_t>(kMaxSplits, std::max<int64_t>(1, n / 2048));
     const uint16_t* dcur = dYb.p;  // gradient w.r.t. this layer's pre-activation (bf16)
     for (int l = 3; l >= 0; --l) {
         const int di = MlpDims::din(l), dn = MlpDims::dout(l);
         const uint16_t* ain = l == 0 ? Xb.p : Ab[l - 1].p;
-        PB_HIP(hipMemsetAsync(gW[l].p, 0, gW[l].bytes(), s));
         GemmArgs gw{};  // gW = dZ^T A_in  (M=dout, N=din, K=n)
         gw.M = dn;
         gw.N = di;
@@ -375,8 +397,7 @@ double Mlp::backward_prepared(int64_t n, float* dY_f32) {
         gw.ldb = di;
         gw.C = gW[l].p;
         gw.ldc = di;
-        gw.atomic = 1;
-        gemm(s, gw, splits);
+        gemm(s, gw, splits, splitws.p);
         hipLaunchKernelGGL(k_colsum, dim3(dn), dim3(256), 0, s, n, dn, dcur, gb[l].p);
         if (l > 0) {  // dZ_{l-1} = (dZ W) * [Z_{l-1} > 0]   (M=n, N=din, K=dout)
             GemmArgs gd{};

@@ -48,6 +48,7 @@ class Mlp {
     int64_t cap = 0;
     DevBuf<uint16_t> Xb, Ab[3], dZb[3], dYb;
     DevBuf<float> Z[3], Yp, Tn, partial;
+    DevBuf<float> splitws;  // split-K partial products of the weight gradients
     DevBuf<int64_t> idx;
     int64_t t_adam = 0;
 };

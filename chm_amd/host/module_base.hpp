@@ -68,6 +68,7 @@ class module_base {
 
     const std::vector<std::string>& depends_list() const { return depends_; }
     const std::vector<std::string>& provides_list() const { return provides_; }
+    const std::vector<std::string>& optional_list() const { return optional_; }
 
     const std::string ID;
     const parallel parallel_type;
@@ -76,12 +77,14 @@ class module_base {
   protected:
     void depends(const std::string& v) { depends_.push_back(v); }
     void provides(const std::string& v) { provides_.push_back(v); }
+    // an input used when some module provides it (module_base.hpp:319-341)
+    void optional(const std::string& v) { optional_.push_back(v); }
     // module_base::is_nan (module_base.hpp:471-479): -9999 or NaN
     static bool is_nan(double v) { return v == -9999.0 || v != v; }
     config_file cfg;
 
   private:
-    std::vector<std::string> depends_, provides_;
+    std::vector<std::string> depends_, provides_, optional_;
 };
 
 }  // namespace chm

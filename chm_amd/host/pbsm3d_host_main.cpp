@@ -13,12 +13,15 @@
 #include <cmath>
 #include <cstdio>
 #include <fstream>
+#include <memory>
 #include <iostream>
 #include <sstream>
 #include <string>
 #include <vector>
 
 #include "PBSM3D.hpp"
+#include "fetchr.hpp"
+#include "scale_wind_vert.hpp"
 
 using namespace chm;
 
@@ -82,9 +85,32 @@ static int run(int argc, char** argv) {
     if (!in) throw std::runtime_error("short forcing file");
     for (int k = 0; k < 8; ++k) m->variable(kForcing[k]).assign(buf.begin() + k * F, buf.begin() + (k + 1) * F);
 
+    // with use_fetchr=1 / use_scale_wind_vert=1 those modules provide "fetch" and
+    // "U_2m_above_srf" each step, run before PBSM3D as CHM's core orders them by
+    // their depends / provides
+    std::unique_ptr<fetchr> fetch_mod;
+    std::unique_ptr<scale_wind_vert> wind_mod;
+    if (cfg.get("use_scale_wind_vert", false)) {
+        wind_mod = std::make_unique<scale_wind_vert>(cfg);
+        wind_mod->init(m);
+    }
+    if (cfg.get("use_fetchr", false)) {
+        fetch_mod = std::make_unique<fetchr>(cfg);
+        fetch_mod->init(m);
+    }
     PBSM3D mod{cfg};
     mod.init(m);
-    for (int s = 0; s < nsteps; ++s) mod.run(m);
+    for (int s = 0; s < nsteps; ++s) {
+        if (wind_mod) wind_mod->run(m);
+        if (fetch_mod) fetch_mod->run(m);
+        mod.run(m);
+    }
+    auto dump = [&](const char* var, const std::string& path) {
+        std::ofstream fo(path, std::ios::binary);
+        fo.write(reinterpret_cast<const char*>(m->variable(var).data()), F * sizeof(double));
+    };
+    if (fetch_mod) dump("fetch", out_path + ".fetch");
+    if (wind_mod) dump("U_2m_above_srf", out_path + ".u2");
     const pbsm3d_stats st = mod.last_stats();
     std::ofstream out(out_path, std::ios::binary);
     for (int k = 0; k < 8; ++k)

@@ -40,6 +40,11 @@ EXPORTED = (
     "pbsm3d_fetch_nearest",
     "pbsm3d_fetch_destroy",
     "pbsm3d_fetch_last_error",
+    "pbsm3d_wind_config_default",
+    "pbsm3d_wind_create",
+    "pbsm3d_wind_run",
+    "pbsm3d_wind_destroy",
+    "pbsm3d_wind_last_error",
     "pbsm3d_mlp_create",
     "pbsm3d_mlp_destroy",
     "pbsm3d_mlp_param_count",
@@ -121,6 +126,15 @@ def lib():
         L.pbsm3d_fetch_destroy.restype = None
         L.pbsm3d_fetch_last_error.argtypes = [vp]
         L.pbsm3d_fetch_last_error.restype = C.c_char_p
+        L.pbsm3d_wind_config_default.argtypes = [C.POINTER(abi.WindConfig)]
+        L.pbsm3d_wind_config_default.restype = None
+        L.pbsm3d_wind_create.argtypes = [C.POINTER(vp), C.c_int, C.c_int64, C.POINTER(C.c_int32), dp, dp, dp, dp,
+                                         C.POINTER(abi.WindConfig)]
+        L.pbsm3d_wind_run.argtypes = [vp, vp, vp, vp, C.c_int]
+        L.pbsm3d_wind_destroy.argtypes = [vp]
+        L.pbsm3d_wind_destroy.restype = None
+        L.pbsm3d_wind_last_error.argtypes = [vp]
+        L.pbsm3d_wind_last_error.restype = C.c_char_p
         fp = C.POINTER(C.c_float)
         L.pbsm3d_mlp_create.argtypes = [C.POINTER(vp), C.c_int, C.c_uint64]
         L.pbsm3d_mlp_destroy.argtypes = [vp]
@@ -392,7 +406,7 @@ class Fetch:
                                    k.dptr(center_z), k.dptr(canopy_height) if canopy_height is not None else None,
                                    C.byref(cfg))
         if rc != 0:
-            raise Pbsm3dError(rc, L.pbsm3d_last_error(None).decode())
+            raise Pbsm3dError(rc, L.pbsm3d_fetch_last_error(None).decode())
         self._h = h
 
     def run(self, vw_dir):
@@ -425,6 +439,64 @@ class Fetch:
     def close(self):
         if getattr(self, "_h", None):
             lib().pbsm3d_fetch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+def wind_config(**overrides) -> abi.WindConfig:
+    c = abi.WindConfig()
+    lib().pbsm3d_wind_config_default(C.byref(c))
+    for k, v in overrides.items():
+        if not hasattr(c, k):
+            raise KeyError(k)
+        setattr(c, k, v)
+    return c
+
+
+class Wind:
+    """Vertical wind scaling on the GPU (the scale_wind_vert module's computation):
+    U_2m_above_srf from U_R and snowdepthavg."""
+
+    def __init__(self, neigh, center_x, center_y, canopy_height=None, lai=None, cfg: abi.WindConfig | None = None,
+                 device: int = 0):
+        L = lib()
+        self._keep = abi.Keep()
+        k = self._keep
+        self.n_faces = int(len(center_x))
+        cfg = cfg if cfg is not None else wind_config()
+        h = C.c_void_p()
+        rc = L.pbsm3d_wind_create(C.byref(h), device, self.n_faces, k.iptr(np.asarray(neigh).reshape(-1)),
+                                  k.dptr(center_x), k.dptr(center_y), k.dptr(canopy_height), k.dptr(lai),
+                                  C.byref(cfg))
+        if rc != 0:
+            raise Pbsm3dError(rc, L.pbsm3d_wind_last_error(None).decode())
+        self._h = h
+
+    def run(self, U_R, snowdepthavg=None):
+        """numpy arrays (host) or float64 CUDA tensors (device); snowdepthavg None: absent."""
+        L = lib()
+        if hasattr(U_R, "data_ptr"):
+            import torch
+
+            out = torch.empty_like(U_R)
+            sd = C.c_void_p(snowdepthavg.data_ptr()) if snowdepthavg is not None else None
+            rc = L.pbsm3d_wind_run(self._h, C.c_void_p(U_R.data_ptr()), sd, C.c_void_p(out.data_ptr()), 1)
+        else:
+            u = np.ascontiguousarray(U_R, dtype=np.float64)
+            sdv = np.ascontiguousarray(snowdepthavg, dtype=np.float64) if snowdepthavg is not None else None
+            out = np.zeros(self.n_faces)
+            rc = L.pbsm3d_wind_run(self._h, u.ctypes.data_as(C.c_void_p),
+                                   sdv.ctypes.data_as(C.c_void_p) if sdv is not None else None,
+                                   out.ctypes.data_as(C.c_void_p), 0)
+        if rc != 0:
+            raise Pbsm3dError(rc, L.pbsm3d_wind_last_error(self._h).decode())
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pbsm3d_wind_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -49,10 +49,13 @@ int pbsm3d_fetch_create(pbsm3d_fetch_ctx** out, int device, int64_t n_faces, con
 int pbsm3d_fetch_run(pbsm3d_fetch_ctx* ctx, const double* vw_dir, double* fetch, int on_device);
 
 /* The nearest-face query alone (exposed for testing): idx[k] = face whose
- * centroid is nearest to (qx[k], qy[k]), ties to the lower index.  Host arrays. */
+ * centroid is nearest to (qx[k], qy[k]), ties to the lower index, -1 for a
+ * non-finite query point.  Host arrays.  (fetch for a non-finite vw_dir is
+ * max_distance.) */
 int pbsm3d_fetch_nearest(pbsm3d_fetch_ctx* ctx, int64_t n, const double* qx, const double* qy, int32_t* idx);
 
 void pbsm3d_fetch_destroy(pbsm3d_fetch_ctx* ctx);
+/* ctx NULL: the last pbsm3d_fetch_create error */
 const char* pbsm3d_fetch_last_error(const pbsm3d_fetch_ctx* ctx);
 
 #ifdef __cplusplus

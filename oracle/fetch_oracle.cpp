@@ -69,6 +69,7 @@ void oracle_fetch(int64_t F, const double* cx, const double* cy, const double* c
                 continue;
             }
         }
+        if (!std::isfinite(wind_dir)) continue;  // no upwind point (undefined in the reference): max_distance
         for (int j = 1; j <= steps; ++j) {
             const double distance = j * size_of_step;
             // point_from_bearing_UTM(center, wind_dir, distance)

@@ -60,6 +60,10 @@ def lib():
         L.oracle_fetch_nearest.restype = None
         L.oracle_fetch.argtypes = [C.c_int64, dp, dp, dp, dp, C.POINTER(abi.FetchConfig), dp, dp]
         L.oracle_fetch.restype = None
+        L.oracle_expint_e1.argtypes = [C.c_double]
+        L.oracle_expint_e1.restype = C.c_double
+        L.oracle_scale_wind.argtypes = [C.c_int64, ip, dp, dp, dp, dp, C.POINTER(abi.WindConfig), dp, dp, dp]
+        L.oracle_scale_wind.restype = None
         L.oracle_config_default.argtypes = [C.POINTER(abi.Config)]
         L.oracle_config_default.restype = None
         L.oracle_max_threads.restype = C.c_int
@@ -194,6 +198,20 @@ def fetch(cx, cy, cz, canopy, cfg, vw_dir):
     lib().oracle_fetch(len(cx), keep.dptr(cx), keep.dptr(cy), keep.dptr(cz),
                        keep.dptr(canopy) if canopy is not None else None, C.byref(cfg), keep.dptr(vw_dir),
                        keep.dptr(out))
+    return out
+
+
+def expint_e1(x: float) -> float:
+    return lib().oracle_expint_e1(float(x))
+
+
+def scale_wind(neigh, cx, cy, canopy, lai, cfg, U_R, snowdepthavg=None):
+    """scale_wind_vert's U_2m_above_srf per face (domain mode when cfg.smooth)."""
+    keep = abi.Keep()
+    out = np.zeros(len(cx))
+    lib().oracle_scale_wind(len(cx), keep.iptr(np.asarray(neigh).reshape(-1)), keep.dptr(cx), keep.dptr(cy),
+                            keep.dptr(canopy), keep.dptr(lai), C.byref(cfg), keep.dptr(U_R), keep.dptr(snowdepthavg),
+                            keep.dptr(out))
     return out
 
 

@@ -76,3 +76,55 @@ def test_module_run_matches_abi(tmp_path):
         ref, _ = ctx.step(fo)
     for k, name in enumerate(abi.OUT_FIELDS):
         np.testing.assert_array_equal(got[k], ref[name], err_msg=name)
+
+
+@pytest.mark.gpu
+def test_fetchr_module_feeds_pbsm3d(tmp_path, oracle):
+    """The C++ fetchr module (reference keys steps / max_distance / I / incl_veg)
+    provides `fetch` each step before PBSM3D runs, as the reference's core
+    orders them; its output equals the oracle's fetch for the same directions."""
+    print("[case] host fetchr module", flush=True)
+    m = meshgen.synthetic_mesh(600, seed=9)
+    F = m.n_faces
+    p = tmp_path / "m.json"
+    meshgen.save_mesh_json(m, str(p))
+    fo = meshgen.forcing_c2(F, seed=9)
+    fbin = tmp_path / "f.bin"
+    np.concatenate([fo[k] for k in abi.FORCING_FIELDS]).astype("<f8").tofile(fbin)
+    obin = tmp_path / "o.bin"
+    r = subprocess.run([_bin(), "run", str(p), str(fbin), "2", str(obin), "nLayer=6", "use_fetchr=1", "steps=8",
+                        "max_distance=800", "I=0.03"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = np.fromfile(str(obin) + ".fetch", dtype="<f8")
+    c = m.vertex[m.elem].mean(axis=1)
+    cfg = abi.FetchConfig()
+    cfg.steps, cfg.incl_veg, cfg.max_distance, cfg.I = 8, 1, 800.0, 0.03
+    want = oracle.fetch(np.ascontiguousarray(c[:, 0]), np.ascontiguousarray(c[:, 1]), np.ascontiguousarray(c[:, 2]),
+                        None, cfg, fo["vw_dir"])
+    np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.gpu
+def test_scale_wind_vert_module_feeds_pbsm3d(tmp_path, oracle):
+    """The C++ scale_wind_vert module (domain mode) provides U_2m_above_srf from U_R
+    and snowdepthavg before PBSM3D runs; it equals the oracle's value."""
+    print("[case] host scale_wind_vert module", flush=True)
+    m = meshgen.synthetic_mesh(600, seed=9)
+    F = m.n_faces
+    p = tmp_path / "m.json"
+    meshgen.save_mesh_json(m, str(p))
+    fo = meshgen.forcing_c2(F, seed=9)
+    fo["snowdepthavg"] = np.random.default_rng(2).uniform(0.1, 2.0, F)
+    fbin = tmp_path / "f.bin"
+    np.concatenate([fo[k] for k in abi.FORCING_FIELDS]).astype("<f8").tofile(fbin)
+    obin = tmp_path / "o.bin"
+    r = subprocess.run([_bin(), "run", str(p), str(fbin), "1", str(obin), "nLayer=6", "use_scale_wind_vert=1"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = np.fromfile(str(obin) + ".u2", dtype="<f8")
+    c = m.vertex[m.elem].mean(axis=1)
+    cfg = abi.WindConfig()
+    cfg.ignore_canopy, cfg.smooth, cfg.z0_snow = 0, 1, 0.01
+    want = oracle.scale_wind(m.neigh, np.ascontiguousarray(c[:, 0]), np.ascontiguousarray(c[:, 1]), None, None, cfg,
+                             fo["U_R"], fo["snowdepthavg"])
+    np.testing.assert_allclose(got, want, rtol=1e-12)

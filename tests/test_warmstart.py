@@ -91,6 +91,16 @@ def test_training_reduces_loss_on_a_known_function():
     l0, _ = net.gradients(X[:4096].cpu().numpy(), Y[:4096].cpu().numpy())
     loss = net.train(X, Y, epochs=8, batch=2048, lr=2e-3, seed=2)
     assert loss < 0.1 * l0, (l0, loss)
+    # training is deterministic (split-K partial products summed in a fixed order): a second
+    # network from the same seed on the same data ends with bitwise identical parameters
+    net2 = pb.Mlp(seed=1)
+    net2.set_normalization(np.zeros(16), np.ones(16), np.zeros(11), np.ones(11))
+    loss2 = net2.train(X, Y, epochs=8, batch=8192, lr=2e-3, seed=2)
+    net3 = pb.Mlp(seed=1)
+    net3.set_normalization(np.zeros(16), np.ones(16), np.zeros(11), np.ones(11))
+    loss3 = net3.train(X, Y, epochs=8, batch=8192, lr=2e-3, seed=2)
+    assert loss2 == loss3
+    np.testing.assert_array_equal(net2.get_params(), net3.get_params())
 
 
 @pytest.mark.gpu
