@@ -42,6 +42,24 @@ def spmv_bytes(nnz: int, n: int, f: int, s: int) -> int:
     return nnz * s + 2 * n * s + 12 * f
 
 
+def spmv_dots_bytes(nnz: int, n: int, f: int, s: int) -> int:
+    """The SpMV of the BiCGStab iteration, which also reads the vector w of its fused
+    dot products (w.y, y.y)."""
+    return spmv_bytes(nnz, n, f, s) + n * s
+
+
+def measured_traffic(F: int, L: int) -> dict:
+    """HBM bytes per launch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE passes for this
+    workload (profiles/hbm_traffic_c2.json), or {} for other workloads."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_c2.json")
+    try:
+        with open(path) as fh:
+            t = json.load(fh)
+    except OSError:
+        return {}
+    return t if (t.get("faces"), t.get("layers")) == (F, L) else {}
+
+
 def dilu_apply_bytes(nnz: int, n: int, f: int, s: int) -> int:
     """Algorithmic bytes of one multicolour D-ILU(0) apply in position order:
     forward sweep reads r, the strictly lower couplings, 1/d and writes y;
@@ -311,18 +329,24 @@ def main():
     s_bytes = 4 if "spmv32" in head["kernels"] else 8
     kspmv = "spmv32" if s_bytes == 4 else "spmv64"
     nnz = head["nnz"]
-    algos = {kspmv: (spmv_bytes(nnz, N, F, s_bytes), "suspension SpMV, position order"),
+    bicg = head["krylov"] == "bicgstab"
+    algos = {kspmv: ((spmv_dots_bytes if bicg else spmv_bytes)(nnz, N, F, s_bytes),
+                     "suspension SpMV, position order" + (", fused BiCGStab dots" if bicg else "")),
              "precond": (dilu_apply_bytes(nnz, N, F, s_bytes), "multicolour D-ILU(0) apply, position order")}
+    traffic = measured_traffic(F, L)
     rooflines = {}
     for kname, (algo, what) in algos.items():
         k = head["kernels"].get(kname)
         if not k or k["ms_avg"] <= 0:
             continue
         achieved = algo / (k["ms_avg"] * 1e-3) / 1e9
+        tr = traffic.get(kname, {}).get("bytes_per_launch") if s_bytes == 4 else None
         rooflines[kname] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": f"{kname} ({what})",
+                            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": tr, "kernel": f"{kname} ({what})",
                             "algorithmic_bytes": algo, "ms_avg": round(k["ms_avg"], 5), "launches": k["launches"],
                             "ms_total": round(k["ms_total"], 3)}
+        if tr:
+            rooflines[kname]["traffic_source"] = traffic["source"]
     dom = max(rooflines, key=lambda n: rooflines[n]["ms_total"]) if rooflines else None
     roofline = rooflines.get(dom)
 
