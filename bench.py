@@ -42,6 +42,18 @@ def spmv_bytes(nnz: int, n: int, f: int, s: int) -> int:
     return nnz * s + 2 * n * s + 12 * f
 
 
+def split_apply_bytes(nnz: int, n: int, f: int, f0: int, L: int, s: int) -> int:
+    """Algorithmic bytes of one application of the split (Eisenstat) D-ILU operator with
+    the BiCGStab dots fused in: every off-diagonal coupling once (upper ones in the
+    backward sweep, lower ones in the forward sweep); per row of colours >= 1 the
+    backward sweep reads w, 1/d~ and writes t, the forward sweep reads w, 1/d~, D - d~,
+    t and the dot vector and writes v and the result (10 vectors); colour 0 (f0 faces)
+    does both sweeps in one pass with t in registers (6 vectors); the neighbour
+    positions and colour mask once per sweep (13 bytes per face and sweep)."""
+    n0 = f0 * L
+    return (nnz - n) * s + (10 * (n - n0) + 6 * n0) * s + 26 * f - 13 * f0
+
+
 def spmv_dots_bytes(nnz: int, n: int, f: int, s: int) -> int:
     """The SpMV of the BiCGStab iteration, which also reads the vector w of its fused
     dot products (w.y, y.y)."""
@@ -78,6 +90,12 @@ def variant_config(pb, abi, name: str, L: int, krylov_mixed: int):
         return pb.default_config(nLayer=L, precision=abi.FP64), False
     if name == "plain_bicgstab":
         return pb.default_config(nLayer=L, precision=abi.FP64, krylov=abi.KRYLOV_BICGSTAB), False
+    if name == "plain_split":
+        return pb.default_config(nLayer=L, precision=abi.FP64, krylov=abi.KRYLOV_BICGSTAB_SPLIT), False
+    if name == "mixed_bicgstab":
+        return pb.default_config(nLayer=L, precision=abi.MIXED, krylov=abi.KRYLOV_BICGSTAB), False
+    if name == "mixed_split":
+        return pb.default_config(nLayer=L, precision=abi.MIXED, krylov=abi.KRYLOV_BICGSTAB_SPLIT), False
     if name == "mixed":
         return pb.default_config(nLayer=L, precision=abi.MIXED, krylov=krylov_mixed), False
     if name == "mixed_learned":
@@ -203,7 +221,7 @@ def main():
     ap.add_argument("--layers", type=int, default=10)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--variants", default="plain,mixed,mixed_learned")
-    ap.add_argument("--krylov-mixed", default="bicgstab", choices=["gmres", "bicgstab"])
+    ap.add_argument("--krylov-mixed", default="split", choices=["gmres", "bicgstab", "split"])
     ap.add_argument("--train-draws", type=int, default=8)
     ap.add_argument("--train-samples", type=int, default=4000000)
     ap.add_argument("--train-epochs", type=int, default=8)
@@ -239,7 +257,8 @@ def main():
     log(f"[bench] mesh F={F} L={L} N={F * L} built in {time.perf_counter() - t:.1f}s")
     dev_forcing = {k: torch.tensor(v, device="cuda:0") for k, v in forcing.items()}
     dev_outs = {k: torch.zeros(F, dtype=torch.float64, device="cuda:0") for k in abi.OUT_FIELDS}
-    krylov_mixed = abi.KRYLOV_BICGSTAB if args.krylov_mixed == "bicgstab" else abi.KRYLOV_GMRES
+    krylov_mixed = {"bicgstab": abi.KRYLOV_BICGSTAB, "split": abi.KRYLOV_BICGSTAB_SPLIT,
+                    "gmres": abi.KRYLOV_GMRES}[args.krylov_mixed]
 
     if args.copy_probe_mb:
         n = args.copy_probe_mb * 1000000 // 4
@@ -270,7 +289,8 @@ def main():
         variants[name] = {
             "steps_per_s": args.steps / secs,
             "ms_per_step": 1000.0 * secs / args.steps,
-            "krylov": "bicgstab" if cfg.krylov == abi.KRYLOV_BICGSTAB else f"gmres({cfg.restart})",
+            "krylov": {abi.KRYLOV_BICGSTAB: "bicgstab", abi.KRYLOV_BICGSTAB_SPLIT: "bicgstab_split"}.get(
+                cfg.krylov, f"gmres({cfg.restart})"),
             "precision": "fp64" if cfg.precision == abi.FP64 else "fp32-inner/fp64-refinement",
             "warm_start": bool(learned),
             "susp_iters": st["susp_iters"],
@@ -330,17 +350,26 @@ def main():
     kspmv = "spmv32" if s_bytes == 4 else "spmv64"
     nnz = head["nnz"]
     bicg = head["krylov"] == "bicgstab"
-    algos = {kspmv: ((spmv_dots_bytes if bicg else spmv_bytes)(nnz, N, F, s_bytes),
-                     "suspension SpMV, position order" + (", fused BiCGStab dots" if bicg else "")),
-             "precond": (dilu_apply_bytes(nnz, N, F, s_bytes), "multicolour D-ILU(0) apply, position order")}
+    if head["krylov"] == "bicgstab_split":
+        colors, _ = pb.face_colors(mesh.neigh)
+        f0 = int((colors == 0).sum())
+        algos = {"precond": (split_apply_bytes(nnz, N, F, f0, L, s_bytes),
+                             "split D-ILU operator (Eisenstat): backward + forward sweep, fused BiCGStab dots",
+                             "split_apply")}
+    else:
+        algos = {kspmv: ((spmv_dots_bytes if bicg else spmv_bytes)(nnz, N, F, s_bytes),
+                         "suspension SpMV, position order" + (", fused BiCGStab dots" if bicg else ""),
+                         "spmv_dots" if bicg else "spmv"),
+                 "precond": (dilu_apply_bytes(nnz, N, F, s_bytes), "multicolour D-ILU(0) apply, position order",
+                             "dilu_apply")}
     traffic = measured_traffic(F, L)
     rooflines = {}
-    for kname, (algo, what) in algos.items():
+    for kname, (algo, what, kid) in algos.items():
         k = head["kernels"].get(kname)
         if not k or k["ms_avg"] <= 0:
             continue
         achieved = algo / (k["ms_avg"] * 1e-3) / 1e9
-        tr = traffic.get(kname, {}).get("bytes_per_launch") if s_bytes == 4 else None
+        tr = traffic.get(f"{kid}_fp{8 * s_bytes}", {}).get("bytes_per_launch")
         rooflines[kname] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": tr, "kernel": f"{kname} ({what})",
                             "algorithmic_bytes": algo, "ms_avg": round(k["ms_avg"], 5), "launches": k["launches"],

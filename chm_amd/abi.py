@@ -22,7 +22,7 @@ MIXED = 1
 
 PC_NONE, PC_JACOBI, PC_COLUMN, PC_ILU0, PC_ILUT, PC_MCILU0 = range(6)
 WS_ZERO, WS_LEARNED = 0, 1
-KRYLOV_GMRES, KRYLOV_BICGSTAB = 0, 1
+KRYLOV_GMRES, KRYLOV_BICGSTAB, KRYLOV_BICGSTAB_SPLIT = 0, 1, 2
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)

@@ -130,8 +130,11 @@ void validate(const pbsm3d_mesh* m, const pbsm3d_config* c) {
     if (c->use_exp_fetch && c->use_tanh_fetch) throw Error(PBSM3D_EINVAL, "Cannot specify both exp_fetch and tanh_fetch");
     if (c->restart < 1 || c->restart > 31) throw Error(PBSM3D_EINVAL, "restart must be in [1, 31]");
     if (c->ortho_passes != 1 && c->ortho_passes != 2) throw Error(PBSM3D_EINVAL, "ortho_passes must be 1 or 2");
-    if (c->krylov != PBSM3D_KRYLOV_GMRES && c->krylov != PBSM3D_KRYLOV_BICGSTAB)
+    if (c->krylov != PBSM3D_KRYLOV_GMRES && c->krylov != PBSM3D_KRYLOV_BICGSTAB &&
+        c->krylov != PBSM3D_KRYLOV_BICGSTAB_SPLIT)
         throw Error(PBSM3D_EINVAL, "unknown Krylov method");
+    if (c->krylov == PBSM3D_KRYLOV_BICGSTAB_SPLIT && (c->precond != PBSM3D_PC_MCILU0 || c->nLayer > 16))
+        throw Error(PBSM3D_EINVAL, "the split BiCGStab needs the multicolour D-ILU preconditioner and nLayer <= 16");
     if (c->max_iters < 1) throw Error(PBSM3D_EINVAL, "max_iters must be >= 1");
     if (!(c->inner_rtol >= 0 && c->inner_rtol < 1)) throw Error(PBSM3D_EINVAL, "inner_rtol must be in [0, 1)");
     if (!(c->tol > 0)) throw Error(PBSM3D_EINVAL, "tol must be > 0");
@@ -198,7 +201,26 @@ SolveInfo solve_in_position_order(pbsm3d_ctx* c, const pbsm3d_config& cfg, int64
     A.pos = &c->dl_s;
     A.variant = c->spmv_variant[0];
     SolveInfo si;
-    if (cfg.precision == PBSM3D_FP64) {
+    if (cfg.krylov == PBSM3D_KRYLOV_BICGSTAB_SPLIT) {
+        if (cfg.precision == PBSM3D_FP64) {
+            CsrOp<double> Ah{N, nullptr, nullptr, nullptr, &c->prof, F, L};
+            Ah.co = &c->co;
+            Ah.split = &c->dl_s;
+            Ah.xcd = c->dilu_xcd[0] == 1;
+            Ah.prof_class = PBSM3D_K_PRECOND;
+            si = bicgstab_split<double>(s, c->bw64, c->ws64, A, Ah, c->bp.p, c->xp.p, c->r_s.p, c->split64[0].p,
+                                        c->split64[1].p, c->split64[2].p, cfg.max_iters, cfg.tol, cfg.inner_rtol);
+        } else {
+            CsrOp<float> Ah{N, nullptr, nullptr, nullptr, &c->prof, F, L};
+            Ah.co = &c->co;
+            Ah.split = &c->dl32;
+            Ah.xcd = c->dilu_xcd[1] == 1;
+            Ah.prof_class = PBSM3D_K_PRECOND;
+            si = bicgstab_split<float>(s, c->bw32, c->ws32, A, Ah, c->bp.p, c->xp.p, c->r_s.p, c->split32[0].p,
+                                       c->split32[1].p, c->split32[2].p, cfg.max_iters, cfg.tol,
+                                       cfg.inner_rtol > 0 ? cfg.inner_rtol : 1e-3);
+        }
+    } else if (cfg.precision == PBSM3D_FP64) {
         PcFn<double> M = [c, F, L](hipStream_t st, const double* r, double* z) {
             dilu_apply<double>(st, c->co, F, L, c->dl_s, r, z, true, c->dilu_xcd[0] == 1);
         };
@@ -402,7 +424,15 @@ int pbsm3d_create(pbsm3d_ctx** out, const pbsm3d_mesh* m, const pbsm3d_config* c
         c->invdiag_d.alloc(F);
         c->r_s.alloc(N);
         c->r_d.alloc(F);
-        const bool bicg = cfg->krylov == PBSM3D_KRYLOV_BICGSTAB;
+        const bool bicg = cfg->krylov == PBSM3D_KRYLOV_BICGSTAB || cfg->krylov == PBSM3D_KRYLOV_BICGSTAB_SPLIT;
+        if (cfg->krylov == PBSM3D_KRYLOV_BICGSTAB_SPLIT) {
+            for (int k = 0; k < 3; ++k) {
+                if (cfg->precision == PBSM3D_MIXED)
+                    c->split32[k].alloc(N);
+                else
+                    c->split64[k].alloc(N);
+            }
+        }
         if (cfg->precision == PBSM3D_MIXED) {
             if (cfg->precond != PBSM3D_PC_MCILU0) c->val32.alloc(c->susp.nnz);
             c->r32.alloc(N);

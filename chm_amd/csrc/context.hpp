@@ -32,6 +32,8 @@ struct pbsm3d_ctx {
     pb::DiluFactors<float> dl32;
     pb::DevBuf<double> invdiag_s, invdiag_d, r_s, r_d;
     pb::DevBuf<double> bp, xp;  // suspension b and x in the D-ILU position order
+    pb::DevBuf<double> split64[3];  // work vectors of the split BiCGStab (FP64 / MIXED)
+    pb::DevBuf<float> split32[3];
     // position-order SpMV launch shape per precision (index 0: FP64, 1: FP32), -1 until tuned
     int spmv_variant[2] = {-1, -1};
     float spmv_tune_ms[2][pb::kPosSpmvVariants] = {};

@@ -168,13 +168,17 @@ struct DiluFactors {
     DevBuf<T> vert;  // [2N] coupling below (0) / above (1)
     DevBuf<T> dinv;  // [N]  1 / d~
     DevBuf<T> diag;  // [N]  a_rr (with lat and vert: the operator itself in position order)
+    DevBuf<T> E;     // [N]  a_rr - d~ (Eisenstat split operator)
     DevBuf<T> y;     // [N]  intermediate of the apply
+    DevBuf<T> t;     // [N]  intermediate of the split operator
     void alloc(int64_t N) {
         lat.alloc(3 * N);
         vert.alloc(2 * N);
         dinv.alloc(N);
         diag.alloc(N);
+        E.alloc(N);
         y.alloc(N);
+        t.alloc(N);
     }
 };
 void build_color_order(ColorOrder& co, int64_t F, const int32_t* neigh_fm, hipStream_t s);
@@ -192,6 +196,18 @@ void dilu_apply(hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFacto
 template <typename T>
 bool tune_dilu_xcd(hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFactors<T>& d, const T* r, T* z,
                    float* ms_out);
+// Split (Eisenstat) form of the D-ILU preconditioned operator, position order (nLayer <= 16):
+// which = 0: out = Ahat in = D~ (D~ + L)^-1 A (D~ + U)^-1 in (one backward + one forward sweep);
+// 1: out = D~ (D~ + L)^-1 in (left transform of a right-hand side);
+// 2: out = (D~ + U)^-1 in (right transform back to the unknowns).  in and out must differ.
+template <typename T>
+void split_op(int which, hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFactors<T>& d, const T* in, T* out,
+              bool xcd);
+// split_op(0) with per-block partial sums partial[2b] = dw.out, partial[2b+1] = out.out; returns
+// the number of blocks b (at most F / kBlock + 8 per colour)
+template <typename T>
+int64_t split_apply_dots(hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFactors<T>& d, const T* in, T* out,
+                         const T* dw, double* partial, bool xcd);
 // FP32 copy of FP64 factors (the FP32 preconditioner and operator of the mixed solve)
 void dilu_narrow(hipStream_t s, int64_t N, const DiluFactors<double>& a, DiluFactors<float>& b);
 // y = b - A x with per-block partial sums of y.y, everything in position order, A taken

@@ -255,6 +255,7 @@ namespace {
 // y = A x with the partial sums (w.y, y.y) of each block (two per block); returns the block count
 template <typename T>
 int64_t spmv_dots(hipStream_t s, const CsrOp<T>& A, const T* x, T* y, const T* w, double* partial, int nblk) {
+    if (A.split) return split_apply_dots<T>(s, *A.co, A.F, A.L, *A.split, x, y, w, partial, A.xcd);
     if (A.pos) return pos_spmv_dots<T>(s, A.variant, *A.co, A.F, A.L, *A.pos, x, y, w, partial);
     op_spmv<T>(s, A, x, y);
     mdot<T>(s, A.n, w, 1, y, partial, nblk);
@@ -265,12 +266,17 @@ int64_t spmv_dots(hipStream_t s, const CsrOp<T>& A, const T* x, T* y, const T* w
 // on a breakdown (the caller restarts from the true residual) or after maxit
 // iterations; returns the iterations.  alpha, omega, rho live on the device:
 // the only host synchronisation per iteration is the convergence test.
+// An empty M means no preconditioner (the operator already carries it, as the
+// split operator does).  stop_rel: stop_abs is relative to the starting ||r||.
 template <typename T>
 int bicg_run(hipStream_t s, BicgWork<T>& w, const CsrOp<T>& A, const PcFn<T>& M, const T* b, T* x, bool x_zero,
-             double stop_abs, int maxit, double* res_out) {
+             double stop_abs, int maxit, double* res_out, bool stop_rel = false) {
     const int64_t n = A.n;
     const int nblk = reduce_blocks(n);
-    const int spmv_id = sizeof(T) == 8 ? PBSM3D_K_SPMV64 : PBSM3D_K_SPMV32;
+    const int spmv_id = A.prof_class >= 0 ? A.prof_class : (sizeof(T) == 8 ? PBSM3D_K_SPMV64 : PBSM3D_K_SPMV32);
+    const bool pc = static_cast<bool>(M);
+    T* ph = pc ? w.ph.p : w.p.p;  // the preconditioned search directions
+    T* sh = pc ? w.sh.p : w.s.p;
     double* sc = w.dsc.p;
     double* h = w.hsc.p;
     auto fetch = [&]() {
@@ -293,6 +299,7 @@ int bicg_run(hipStream_t s, BicgWork<T>& w, const CsrOp<T>& A, const PcFn<T>& M,
     fetch();
     double rr = std::sqrt(h[SC_RR]);
     *res_out = rr;
+    if (stop_rel) stop_abs *= rr;
     if (rr <= stop_abs) return 0;
     int it = 0;
     while (it < maxit) {
@@ -303,14 +310,14 @@ int bicg_run(hipStream_t s, BicgWork<T>& w, const CsrOp<T>& A, const PcFn<T>& M,
             ProfScope ps(A.prof, s, PBSM3D_K_ORTHO);
             bicg_p_dev<T>(s, n, w.r.p, w.v.p, sc, w.p.p);
         }
-        {
+        if (pc) {
             ProfScope ps(A.prof, s, PBSM3D_K_PRECOND);
             M(s, w.p.p, w.ph.p);
         }
         int64_t nb = 0;
         {
             ProfScope ps(A.prof, s, spmv_id);
-            nb = spmv_dots<T>(s, A, w.ph.p, w.v.p, w.rh.p, w.partial.p, nblk);  // v = A ph; rh.v, v.v
+            nb = spmv_dots<T>(s, A, ph, w.v.p, w.rh.p, w.partial.p, nblk);  // v = A ph; rh.v, v.v
         }
         {
             ProfScope ps(A.prof, s, PBSM3D_K_ORTHO);
@@ -318,18 +325,18 @@ int bicg_run(hipStream_t s, BicgWork<T>& w, const CsrOp<T>& A, const PcFn<T>& M,
             bicg_s_dev<T>(s, n, w.v.p, w.r.p, sc, w.s.p, w.partial.p, nblk);
             bicg_stage(s, w.partial.p, nblk, 1, 1, 2, sc);
         }
-        {
+        if (pc) {
             ProfScope ps(A.prof, s, PBSM3D_K_PRECOND);
             M(s, w.s.p, w.sh.p);
         }
         {
             ProfScope ps(A.prof, s, spmv_id);
-            nb = spmv_dots<T>(s, A, w.sh.p, w.t.p, w.s.p, w.partial.p, nblk);  // t = A sh; s.t, t.t
+            nb = spmv_dots<T>(s, A, sh, w.t.p, w.s.p, w.partial.p, nblk);  // t = A sh; s.t, t.t
         }
         {
             ProfScope ps(A.prof, s, PBSM3D_K_ORTHO);
             bicg_stage(s, w.partial.p, nb, 2, 2, 3, sc);  // omega
-            bicg_xr_dev<T>(s, n, w.ph.p, w.sh.p, w.s.p, w.t.p, w.rh.p, sc, x, w.r.p, w.partial.p, nblk);
+            bicg_xr_dev<T>(s, n, ph, sh, w.s.p, w.t.p, w.rh.p, sc, x, w.r.p, w.partial.p, nblk);
             bicg_stage(s, w.partial.p, nblk, 2, 2, 4, sc);  // rho, rr
         }
         fetch();
@@ -368,6 +375,65 @@ SolveInfo bicgstab_solve(hipStream_t s, BicgWork<double>& w, GmresWork<double>& 
     }
     return info;
 }
+
+namespace {
+void scale_into(hipStream_t s, int64_t n, const double* r, double a, double* out) { scale_host<double>(s, n, r, a, out); }
+void scale_into(hipStream_t s, int64_t n, const double* r, double a, float* out) { scale_f64_to_f32_host(s, n, r, a, out); }
+void add_into(hipStream_t s, int64_t n, double a, const double* d, double* x) { axpy_f64(s, n, a, d, x); }
+void add_into(hipStream_t s, int64_t n, double a, const float* d, double* x) { axpy_f32_into_f64(s, n, a, d, x); }
+}  // namespace
+
+template <typename T>
+SolveInfo bicgstab_split(hipStream_t s, BicgWork<T>& w, GmresWork<T>& gw, const CsrOp<double>& A64,
+                         const CsrOp<T>& Ahat, const double* b, double* x, double* r, T* work1, T* work2, T* work3,
+                         int maxit, double tol, double inner_rtol) {
+    SolveInfo info;
+    const int64_t n = A64.n;
+    w.ensure(n);
+    const double nb = norm_of(s, gw, n, b);
+    if (nb == 0.0) {
+        PB_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, s));
+        info.converged = true;
+        return info;
+    }
+    double beta = residual_norm(s, gw, A64, x, b, r);
+    if (drop_worse_start(s, info, beta, nb, x, n)) beta = residual_norm(s, gw, A64, x, b, r);
+    const int pclass = Ahat.prof_class >= 0 ? Ahat.prof_class : PBSM3D_K_PRECOND;
+    while (true) {
+        info.resid = beta / nb;
+        if (info.resid <= tol) {
+            info.converged = true;
+            break;
+        }
+        if (info.iters >= maxit) break;
+        scale_into(s, n, r, 1.0 / beta, work1);  // r / ||r||
+        {
+            ProfScope ps(Ahat.prof, s, pclass);
+            split_op<T>(1, s, *Ahat.co, Ahat.F, Ahat.L, *Ahat.split, work1, work2, Ahat.xcd);  // rhat
+        }
+        PB_HIP(hipMemsetAsync(work3, 0, n * sizeof(T), s));
+        double stop = 0.5 * tol * nb / beta;
+        if (stop < inner_rtol) stop = inner_rtol;
+        double est = 0;
+        info.iters += bicg_run<T>(s, w, Ahat, PcFn<T>(), work2, work3, true, stop, maxit - info.iters, &est, true);
+        {
+            ProfScope ps(Ahat.prof, s, pclass);
+            split_op<T>(2, s, *Ahat.co, Ahat.F, Ahat.L, *Ahat.split, work3, work1, Ahat.xcd);  // d = (D~+U)^-1 dhat
+        }
+        add_into(s, n, beta, work1, x);
+        info.cycles += 1;
+        info.refinements += 1;
+        beta = residual_norm(s, gw, A64, x, b, r);
+    }
+    return info;
+}
+
+template SolveInfo bicgstab_split<double>(hipStream_t, BicgWork<double>&, GmresWork<double>&, const CsrOp<double>&,
+                                          const CsrOp<double>&, const double*, double*, double*, double*, double*,
+                                          double*, int, double, double);
+template SolveInfo bicgstab_split<float>(hipStream_t, BicgWork<float>&, GmresWork<float>&, const CsrOp<double>&,
+                                         const CsrOp<float>&, const double*, double*, double*, float*, float*, float*,
+                                         int, double, double);
 
 SolveInfo bicgstab_mixed(hipStream_t s, BicgWork<float>& w, GmresWork<float>& gw, const CsrOp<double>& A64,
                          const CsrOp<float>& A32, const PcFn<float>& M32, const double* b, double* x, double* r,

@@ -31,11 +31,17 @@ struct CsrOp {
     const ColorOrder* co = nullptr;
     const DiluFactors<T>* pos = nullptr;
     int variant = 0;  // launch shape of the position-order SpMV (tune_pos_spmv)
+    // optional split (Eisenstat) D-ILU operator Ahat = D~ (D~ + L)^-1 A (D~ + U)^-1 (split_op)
+    DiluFactors<T>* split = nullptr;
+    bool xcd = false;      // XCD workgroup mapping of the split operator's sweeps
+    int prof_class = -1;   // profile class of its applications (-1: the SpMV class)
 };
 
 template <typename T>
 inline void op_spmv(hipStream_t s, const CsrOp<T>& A, const T* x, T* y) {
-    if (A.pos)
+    if (A.split)
+        split_op<T>(0, s, *A.co, A.F, A.L, *A.split, x, y, A.xcd);
+    else if (A.pos)
         pos_spmv_variant<T>(s, A.variant, *A.co, A.F, A.L, *A.pos, x, y);
     else if (A.neigh)
         spmv_col<T>(s, A.F, A.L, A.rp, A.val, A.neigh, A.slots, x, nullptr, y, nullptr);
@@ -119,6 +125,16 @@ struct BicgWork {
 
 SolveInfo bicgstab_solve(hipStream_t s, BicgWork<double>& w, GmresWork<double>& gw, const CsrOp<double>& A,
                          const PcFn<double>& M, const double* b, double* x, int maxit, double tol);
+// BiCGStab on the split-preconditioned system (Eisenstat's trick) inside FP64
+// iterative refinement: each cycle takes the FP64 residual r = b - A64 x,
+// transforms r / ||r|| with Ahat's left factor, runs unpreconditioned BiCGStab on
+// Ahat (storage T) from zero to a relative inner_rtol, maps the correction back
+// with the right factor and adds it to x in FP64.  Ahat must have .split set;
+// work1, work2 are n-vectors of T.
+template <typename T>
+SolveInfo bicgstab_split(hipStream_t s, BicgWork<T>& w, GmresWork<T>& gw, const CsrOp<double>& A64,
+                         const CsrOp<T>& Ahat, const double* b, double* x, double* r, T* work1, T* work2, T* work3,
+                         int maxit, double tol, double inner_rtol);
 // FP64 refinement around FP32 BiCGStab runs (FP64 residuals, FP64 dot accumulation)
 SolveInfo bicgstab_mixed(hipStream_t s, BicgWork<float>& w, GmresWork<float>& gw, const CsrOp<double>& A64,
                          const CsrOp<float>& A32, const PcFn<float>& M32, const double* b, double* x, double* r,

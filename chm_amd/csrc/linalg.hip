@@ -482,7 +482,13 @@ __global__ void __launch_bounds__(kStageThreads) k_bicg_stage(const double* __re
             sc[SC_ST] = a;
             sc[SC_TT] = b;
             if (b > 0.0 && isfinite(a) && isfinite(b)) {
-                sc[SC_OMEGA] = a / b;
+                // minimal-residual omega, enlarged when the angle between t and s is wide:
+                // |cos(t, s)| < 0.7 -> omega *= 0.7 / |cos| (Sleijpen & van der Vorst 1995),
+                // which keeps omega away from 0 where plain BiCGStab stagnates
+                double omega = a / b;
+                const double cs = a / sqrt(b * sc[SC_SS]);
+                if (isfinite(cs) && fabs(cs) < 0.7 && fabs(cs) > 0.0) omega *= 0.7 / fabs(cs);
+                sc[SC_OMEGA] = omega;
             } else {
                 sc[SC_OMEGA] = 0;
                 sc[SC_FLAG] = 2;

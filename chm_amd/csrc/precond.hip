@@ -35,7 +35,7 @@ __global__ void __launch_bounds__(kBlock) k_dilu_factor(int64_t F, int L, int64_
                                                         const int32_t* __restrict__ rp, const T* __restrict__ val,
                                                         const uint16_t* __restrict__ slots, T* __restrict__ lat,
                                                         T* __restrict__ vert, T* __restrict__ dinv,
-                                                        T* __restrict__ diag) {
+                                                        T* __restrict__ diag, T* __restrict__ E) {
     const int64_t k = k0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= k1) return;
     const int64_t N = F * L;
@@ -81,6 +81,7 @@ __global__ void __launch_bounds__(kBlock) k_dilu_factor(int64_t F, int L, int64_
         }
         const T di = d != T(0) ? T(1) / d : T(1);
         dinv[q] = di;
+        E[q] = diag[q] - (d != T(0) ? d : T(1));  // D - D~ (Eisenstat split operator)
         dprev = di;
     }
 }
@@ -236,6 +237,175 @@ __global__ void __launch_bounds__(kBlock) k_dilu_bwd(int64_t F, int L, int64_t k
         }
     }
 }
+
+// ---------------------------------------------------------------------------
+// Split (Eisenstat) form of the D-ILU preconditioned operator, position order.
+// With A = (D~ + L) + (D~ + U) - (2 D~ - D) and M = (D~ + L) D~^-1 (D~ + U):
+//   Ahat = D~ (D~ + L)^-1 A (D~ + U)^-1,   Ahat w = w + E t - L v,
+// where E = D - D~, t = (D~ + U)^-1 w (backward sweep) and v solves
+// (D~ + L) v = w - (2 D~ - D) t, i.e. v = D~^-1 (w + E t - L v) - t (forward
+// sweep).  One backward and one forward sweep take the place of
+// "precondition, then multiply by A".  Ahat x^ = b^, b^ = D~ (D~ + L)^-1 b,
+// x = (D~ + U)^-1 x^ is the same system (Eisenstat 1981).
+// ---------------------------------------------------------------------------
+
+// t = D~^-1 (w - U t) for one colour; tnb reads t at the (higher-colour)
+// neighbour positions, t is written at the colour's own positions
+template <typename T, int LM, bool XCD>
+__global__ void __launch_bounds__(kBlock) k_eis_bwd(int64_t F, int L, int64_t k0, int64_t k1,
+                                                    const int32_t* __restrict__ nbp,
+                                                    const uint8_t* __restrict__ lowmask, const T* __restrict__ lat,
+                                                    const T* __restrict__ vert, const T* __restrict__ dinv,
+                                                    const T* __restrict__ w, const T* __restrict__ tnb,
+                                                    T* __restrict__ t) {
+    const int64_t k = k0 + logical_block<XCD>((k1 - k0 + kBlock - 1) / kBlock) * kBlock + threadIdx.x;
+    if (k >= k1) return;
+    const int64_t N = F * L;
+    const uint32_t lm = lowmask[k];
+    int32_t np_[3];
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+        const int32_t n = nbp[f * F + k];
+        np_[f] = (n >= 0 && !((lm >> f) & 1u)) ? n : -1;
+    }
+    T av[LM], va[LM], dv[LM], wv[LM];
+#pragma unroll
+    for (int z = 0; z < LM; ++z)
+        if (z < L) {
+            const int64_t q = (int64_t)z * F + k;
+            T acc = 0;
+#pragma unroll
+            for (int f = 0; f < 3; ++f)
+                if (np_[f] >= 0) acc += lat[f * N + q] * tnb[(int64_t)z * F + np_[f]];
+            av[z] = acc;
+            va[z] = z < L - 1 ? vert[N + q] : T(0);
+            dv[z] = dinv[q];
+            wv[z] = w[q];
+        }
+    T next = 0;
+#pragma unroll
+    for (int z = LM - 1; z >= 0; --z)
+        if (z < L) {
+            next = dv[z] * (wv[z] - (av[z] + va[z] * next));
+            t[(int64_t)z * F + k] = next;
+        }
+}
+
+// Forward sweep of one colour: acc = L v (lower neighbours from vnb, the cell
+// below from the column), o = w + E t - acc, v = D~^-1 o - t (WITH_T), or
+// o = w - acc, v = D~^-1 o (without t: the left transform b^ = D~ (D~ + L)^-1 b).
+// FUSE: colour 0 (no lower neighbours), whose backward sweep runs first in the
+// same thread and whose t stays in registers.
+template <typename T, int LM, bool WITH_T, bool FUSE, bool DOTS>
+__device__ __forceinline__ void eis_fwd_column(int64_t F, int L, int64_t k, const int32_t* __restrict__ nbp,
+                                               const uint8_t* __restrict__ lowmask, const T* __restrict__ lat,
+                                               const T* __restrict__ vert, const T* __restrict__ dinv,
+                                               const T* __restrict__ E, const T* __restrict__ w,
+                                               const T* __restrict__ t, const T* __restrict__ tnb,
+                                               const T* __restrict__ vnb, T* __restrict__ v, T* __restrict__ o,
+                                               const T* __restrict__ dw, double& d1, double& d2) {
+    const int64_t N = F * L;
+    const uint32_t lm = lowmask[k];
+    int32_t lo[3], up[3];
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+        const int32_t n = nbp[f * F + k];
+        const bool low = (lm >> f) & 1u;
+        lo[f] = (n >= 0 && low) ? n : -1;
+        up[f] = (n >= 0 && !low) ? n : -1;
+    }
+    T wv[LM], dv[LM], vb[LM], lv[LM], tv[LM], ev[LM];
+#pragma unroll
+    for (int z = 0; z < LM; ++z)
+        if (z < L) {
+            const int64_t q = (int64_t)z * F + k;
+            wv[z] = w[q];
+            dv[z] = dinv[q];
+            vb[z] = z > 0 ? vert[q] : T(0);
+            T acc = 0;
+            if (!FUSE) {
+#pragma unroll
+                for (int f = 0; f < 3; ++f)
+                    if (lo[f] >= 0) acc += lat[f * N + q] * vnb[(int64_t)z * F + lo[f]];
+            }
+            lv[z] = acc;
+            if (WITH_T) {
+                ev[z] = E[q];
+                if (!FUSE) tv[z] = t[q];
+            }
+        }
+    if constexpr (FUSE) {
+        // backward sweep of this column first: t = D~^-1 (w - U t), all neighbours upper
+        T av[LM], va[LM];
+#pragma unroll
+        for (int z = 0; z < LM; ++z)
+            if (z < L) {
+                const int64_t q = (int64_t)z * F + k;
+                T acc = 0;
+#pragma unroll
+                for (int f = 0; f < 3; ++f)
+                    if (up[f] >= 0) acc += lat[f * N + q] * tnb[(int64_t)z * F + up[f]];
+                av[z] = acc;
+                va[z] = z < L - 1 ? vert[N + q] : T(0);
+            }
+        T next = 0;
+#pragma unroll
+        for (int z = LM - 1; z >= 0; --z)
+            if (z < L) {
+                next = dv[z] * (wv[z] - (av[z] + va[z] * next));
+                tv[z] = next;
+            }
+    }
+    T prev = 0;
+#pragma unroll
+    for (int z = 0; z < LM; ++z)
+        if (z < L) {
+            const int64_t q = (int64_t)z * F + k;
+            const T acc = lv[z] + vb[z] * prev;
+            T oz, vz;
+            if (WITH_T) {
+                oz = wv[z] + ev[z] * tv[z] - acc;
+                vz = dv[z] * oz - tv[z];
+            } else {
+                oz = wv[z] - acc;
+                vz = dv[z] * oz;
+            }
+            v[q] = vz;
+            o[q] = oz;
+            prev = vz;
+            if (DOTS) {
+                d1 = fma((double)dw[q], (double)oz, d1);
+                d2 = fma((double)oz, (double)oz, d2);
+            }
+        }
+}
+
+// DOTS: per-block partial sums (dw . o, o . o) into partial[2 * blockIdx.x] (FP64).
+template <typename T, int LM, bool XCD, bool WITH_T, bool FUSE, bool DOTS>
+__global__ void __launch_bounds__(kBlock) k_eis_fwd(int64_t F, int L, int64_t k0, int64_t k1,
+                                                    const int32_t* __restrict__ nbp,
+                                                    const uint8_t* __restrict__ lowmask, const T* __restrict__ lat,
+                                                    const T* __restrict__ vert, const T* __restrict__ dinv,
+                                                    const T* __restrict__ E, const T* __restrict__ w,
+                                                    const T* __restrict__ t, const T* __restrict__ tnb,
+                                                    const T* __restrict__ vnb, T* __restrict__ v,
+                                                    T* __restrict__ o, const T* __restrict__ dw,
+                                                    double* __restrict__ partial) {
+    const int64_t k = k0 + logical_block<XCD>((k1 - k0 + kBlock - 1) / kBlock) * kBlock + threadIdx.x;
+    double d1 = 0, d2 = 0;
+    if (k < k1) eis_fwd_column<T, LM, WITH_T, FUSE, DOTS>(F, L, k, nbp, lowmask, lat, vert, dinv, E, w, t, tnb, vnb,
+                                                           v, o, dw, d1, d2);
+    if constexpr (DOTS) {
+        __shared__ double s_red[kBlock / 64];
+        const double a = block_sum(d1, s_red);
+        const double b = block_sum(d2, s_red);
+        if (threadIdx.x == 0) {
+            partial[2 * blockIdx.x] = a;
+            partial[2 * blockIdx.x + 1] = b;
+        }
+    }
+}
+
 
 // y = A x, or r = b - A x with per-block partial sums of r.r, for the operator
 // held in position order by the factorisation: diag [N], lat [3N] (neighbour
@@ -528,7 +698,7 @@ void dilu_factor(hipStream_t s, const ColorOrder& co, int64_t F, int L, const in
         const int64_t k0 = co.off[c], k1 = co.off[c + 1];
         if (k1 == k0) continue;
         hipLaunchKernelGGL((k_dilu_factor<T>), dim3(grid_for(k1 - k0)), dim3(kBlock), 0, s, F, L, k0, k1, co.faces.p,
-                           co.nbp.p, co.rev.p, co.lowmask.p, rp, val, slots, d.lat.p, d.vert.p, d.dinv.p, d.diag.p);
+                           co.nbp.p, co.rev.p, co.lowmask.p, rp, val, slots, d.lat.p, d.vert.p, d.dinv.p, d.diag.p, d.E.p);
     }
 }
 
@@ -608,11 +778,116 @@ bool tune_dilu_xcd(hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFa
     return ms[1] < ms[0];
 }
 
+namespace {
+// one forward launch of colour c; with dw the block partials of (dw . o, o . o) go to
+// partial[2 * (nb + block)] and nb advances by the launch's block count
+template <typename T, int LM, bool XCD, bool WITH_T, bool FUSE>
+void launch_fwd(hipStream_t s, dim3 g, int64_t F, int L, const ColorOrder& co, int c, DiluFactors<T>& d, const T* w,
+                T* o, const T* dw, double* partial, int64_t& nb) {
+    if (dw) {
+        hipLaunchKernelGGL((k_eis_fwd<T, LM, XCD, WITH_T, FUSE, true>), g, dim3(kBlock), 0, s, F, L, co.off[c],
+                           co.off[c + 1], co.nbp.p, co.lowmask.p, d.lat.p, d.vert.p, d.dinv.p, d.E.p, w, d.t.p,
+                           d.t.p, d.y.p, d.y.p, o, dw, partial + 2 * nb);
+        nb += g.x;
+    } else {
+        hipLaunchKernelGGL((k_eis_fwd<T, LM, XCD, WITH_T, FUSE, false>), g, dim3(kBlock), 0, s, F, L, co.off[c],
+                           co.off[c + 1], co.nbp.p, co.lowmask.p, d.lat.p, d.vert.p, d.dinv.p, d.E.p, w, d.t.p,
+                           d.t.p, d.y.p, d.y.p, o, dw, partial);
+    }
+}
+
+template <typename T, int LM, bool XCD>
+int64_t split_apply_t(hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFactors<T>& d, const T* w, T* o,
+                      const T* dw, double* partial) {
+    int64_t nb = 0;
+    const int top = co.ncolor - 1;
+    auto blocks = [&](int c) { return dim3((unsigned)launch_blocks<XCD>((co.off[c + 1] - co.off[c] + kBlock - 1) / kBlock)); };
+    // backward sweep, colours top .. 1: t
+    for (int c = top; c >= 1; --c) {
+        if (co.off[c + 1] == co.off[c]) continue;
+        hipLaunchKernelGGL((k_eis_bwd<T, LM, XCD>), blocks(c), dim3(kBlock), 0, s, F, L, co.off[c], co.off[c + 1],
+                           co.nbp.p, co.lowmask.p, d.lat.p, d.vert.p, d.dinv.p, w, d.t.p, d.t.p);
+    }
+    // colour 0: backward and forward sweeps in one pass
+    if (co.off[1] > co.off[0])
+        launch_fwd<T, LM, XCD, true, true>(s, blocks(0), F, L, co, 0, d, w, o, dw, partial, nb);
+    // forward sweep, colours 1 .. top
+    for (int c = 1; c <= top; ++c) {
+        if (co.off[c + 1] == co.off[c]) continue;
+        launch_fwd<T, LM, XCD, true, false>(s, blocks(c), F, L, co, c, d, w, o, dw, partial, nb);
+    }
+    return nb;
+}
+template <typename T, int LM, bool XCD>
+void split_lsolve_t(hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFactors<T>& d, const T* b, T* bhat) {
+    for (int c = 0; c < co.ncolor; ++c) {
+        if (co.off[c + 1] == co.off[c]) continue;
+        const dim3 g((unsigned)launch_blocks<XCD>((co.off[c + 1] - co.off[c] + kBlock - 1) / kBlock));
+        hipLaunchKernelGGL((k_eis_fwd<T, LM, XCD, false, false, false>), g, dim3(kBlock), 0, s, F, L, co.off[c],
+                           co.off[c + 1], co.nbp.p, co.lowmask.p, d.lat.p, d.vert.p, d.dinv.p, d.E.p, b, d.t.p,
+                           d.t.p, d.y.p, d.y.p, bhat, (const T*)nullptr, (double*)nullptr);
+    }
+}
+template <typename T, int LM, bool XCD>
+void split_rsolve_t(hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFactors<T>& d, const T* xhat, T* x) {
+    for (int c = co.ncolor - 1; c >= 0; --c) {
+        if (co.off[c + 1] == co.off[c]) continue;
+        const dim3 g((unsigned)launch_blocks<XCD>((co.off[c + 1] - co.off[c] + kBlock - 1) / kBlock));
+        hipLaunchKernelGGL((k_eis_bwd<T, LM, XCD>), g, dim3(kBlock), 0, s, F, L, co.off[c], co.off[c + 1],
+                           co.nbp.p, co.lowmask.p, d.lat.p, d.vert.p, d.dinv.p, xhat, x, x);
+    }
+}
+// which: 0 apply, 1 left transform, 2 right transform
+template <typename T, bool XCD>
+int64_t split_dispatch(int which, hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFactors<T>& d,
+                       const T* in, T* out, const T* dw, double* partial) {
+    int64_t nb = 0;
+#define PB_SPLIT(LMV)                                                           \
+    {                                                                           \
+        if (which == 0)                                                         \
+            nb = split_apply_t<T, LMV, XCD>(s, co, F, L, d, in, out, dw, partial); \
+        else if (which == 1)                                                    \
+            split_lsolve_t<T, LMV, XCD>(s, co, F, L, d, in, out);               \
+        else                                                                    \
+            split_rsolve_t<T, LMV, XCD>(s, co, F, L, d, in, out);               \
+    }
+    if (L <= 1)
+        PB_SPLIT(1)
+    else if (L <= 5)
+        PB_SPLIT(5)
+    else if (L <= 10)
+        PB_SPLIT(10)
+    else if (L <= 16)
+        PB_SPLIT(16)
+    else
+        throw Error(PBSM3D_EINVAL, "the split D-ILU operator needs nLayer <= 16");
+#undef PB_SPLIT
+    return nb;
+}
+}  // namespace
+
+template <typename T>
+void split_op(int which, hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFactors<T>& d, const T* in, T* out,
+              bool xcd) {
+    if (xcd)
+        split_dispatch<T, true>(which, s, co, F, L, d, in, out, nullptr, nullptr);
+    else
+        split_dispatch<T, false>(which, s, co, F, L, d, in, out, nullptr, nullptr);
+}
+
+template <typename T>
+int64_t split_apply_dots(hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFactors<T>& d, const T* in, T* out,
+                         const T* dw, double* partial, bool xcd) {
+    if (xcd) return split_dispatch<T, true>(0, s, co, F, L, d, in, out, dw, partial);
+    return split_dispatch<T, false>(0, s, co, F, L, d, in, out, dw, partial);
+}
+
 void dilu_narrow(hipStream_t s, int64_t N, const DiluFactors<double>& a, DiluFactors<float>& b) {
     hipLaunchKernelGGL(k_narrow, dim3(grid_for(3 * N)), dim3(kBlock), 0, s, 3 * N, a.lat.p, b.lat.p);
     hipLaunchKernelGGL(k_narrow, dim3(grid_for(2 * N)), dim3(kBlock), 0, s, 2 * N, a.vert.p, b.vert.p);
     hipLaunchKernelGGL(k_narrow, dim3(grid_for(N)), dim3(kBlock), 0, s, N, a.dinv.p, b.dinv.p);
     hipLaunchKernelGGL(k_narrow, dim3(grid_for(N)), dim3(kBlock), 0, s, N, a.diag.p, b.diag.p);
+    hipLaunchKernelGGL(k_narrow, dim3(grid_for(N)), dim3(kBlock), 0, s, N, a.E.p, b.E.p);
 }
 
 namespace {
@@ -732,6 +1007,14 @@ template void dilu_apply<double>(hipStream_t, const ColorOrder&, int64_t, int, D
                                  double*, bool, bool);
 template void dilu_apply<float>(hipStream_t, const ColorOrder&, int64_t, int, DiluFactors<float>&, const float*,
                                 float*, bool, bool);
+template void split_op<double>(int, hipStream_t, const ColorOrder&, int64_t, int, DiluFactors<double>&, const double*,
+                               double*, bool);
+template void split_op<float>(int, hipStream_t, const ColorOrder&, int64_t, int, DiluFactors<float>&, const float*,
+                              float*, bool);
+template int64_t split_apply_dots<double>(hipStream_t, const ColorOrder&, int64_t, int, DiluFactors<double>&,
+                                          const double*, double*, const double*, double*, bool);
+template int64_t split_apply_dots<float>(hipStream_t, const ColorOrder&, int64_t, int, DiluFactors<float>&,
+                                         const float*, float*, const float*, double*, bool);
 template bool tune_dilu_xcd<double>(hipStream_t, const ColorOrder&, int64_t, int, DiluFactors<double>&, const double*,
                                     double*, float*);
 template bool tune_dilu_xcd<float>(hipStream_t, const ColorOrder&, int64_t, int, DiluFactors<float>&, const float*,

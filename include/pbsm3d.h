@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define PBSM3D_ABI_VERSION 3
+#define PBSM3D_ABI_VERSION 4
 
 /* return codes */
 #define PBSM3D_OK 0
@@ -110,12 +110,17 @@ typedef struct pbsm3d_config {
     int32_t block_size;           /* reserved (256)                                */
     int32_t krylov;               /* PBSM3D_KRYLOV_GMRES (reference) or _BICGSTAB  */
     double inner_rtol;            /* MIXED: relative residual each FP32 inner solve
-                                     aims for; 0 = default 1e-4                    */
+                                     aims for; 0 = default (1e-3 for the split
+                                     BiCGStab, 1e-4 otherwise)                     */
 } pbsm3d_config;
 
 /* Krylov method of the suspension solve (the deposition solve is GMRES(restart)) */
 #define PBSM3D_KRYLOV_GMRES 0     /* restarted GMRES(restart), as Belos "GMRES"     */
 #define PBSM3D_KRYLOV_BICGSTAB 1  /* BiCGStab: short recurrences, 2 SpMV/iteration  */
+#define PBSM3D_KRYLOV_BICGSTAB_SPLIT 2  /* BiCGStab on the split D-ILU system (Eisenstat's
+                                          trick: each operator application is one backward
+                                          and one forward sweep, no SpMV) inside FP64
+                                          refinement; precond MCILU0, nLayer <= 16 */
 
 typedef struct pbsm3d_forcing {
     const double* U_R;            /* [F] m/s at Z_U_R = 50 m                       */

@@ -28,7 +28,7 @@ def test_library_exports_every_declared_symbol():
     assert len(names) >= 10
     for n in names:
         assert hasattr(L, n), n
-    assert L.pbsm3d_abi_version() == 3
+    assert L.pbsm3d_abi_version() == 4
     assert set(pbsm3d.EXPORTED) <= set(names)
 
 

@@ -91,6 +91,7 @@ def test_dilu_matches_oracle_dilu(pb, oracle, mesh_c1):
 @pytest.mark.parametrize("precision,precond,krylov", [
     (abi.FP64, abi.PC_MCILU0, abi.KRYLOV_GMRES), (abi.MIXED, abi.PC_MCILU0, abi.KRYLOV_GMRES),
     (abi.FP64, abi.PC_MCILU0, abi.KRYLOV_BICGSTAB), (abi.MIXED, abi.PC_MCILU0, abi.KRYLOV_BICGSTAB),
+    (abi.FP64, abi.PC_MCILU0, abi.KRYLOV_BICGSTAB_SPLIT), (abi.MIXED, abi.PC_MCILU0, abi.KRYLOV_BICGSTAB_SPLIT),
     (abi.FP64, abi.PC_COLUMN, abi.KRYLOV_GMRES), (abi.FP64, abi.PC_JACOBI, abi.KRYLOV_GMRES),
     (abi.MIXED, abi.PC_COLUMN, abi.KRYLOV_GMRES)])
 def test_fields_c1(pb, oracle, mesh_c1, precision, precond, krylov):
@@ -267,7 +268,7 @@ def test_every_spmv_launch_shape(pb, oracle, mesh_c1, precision):
 def test_c3_mixed_precision_96_steps(pb, oracle, mesh_c1):
     """The mixed-precision workload on the C1 mesh: 96 x 900 s steps with wind speed
     and direction advected by a seeded smooth random field (correlation length
-    2 km), FP32 SpMV / preconditioner with FP64 refinement at tol 1e-8, state
+    2 km), FP32 split D-ILU BiCGStab with FP64 refinement at tol 1e-8, state
     (sum_drift, sum_subl) carried across steps.  Per-field error against the FP64
     oracle solved to 1e-13 is printed for the FP64 and the mixed GPU paths and
     bounded by the forward-error band of a 1e-8 residual."""
@@ -275,8 +276,8 @@ def test_c3_mixed_precision_96_steps(pb, oracle, mesh_c1):
     F = mesh_c1.n_faces
     g = pb.face_geometry(mesh_c1)
     orc = oracle.Oracle(g, oracle.oracle_config(nLayer=5, tol=1e-13))
-    gpu = {p: pb.Context(g, pb.default_config(nLayer=5, precision=p, krylov=abi.KRYLOV_BICGSTAB if p == abi.MIXED
-                                              else abi.KRYLOV_GMRES, tol=1e-8))
+    gpu = {p: pb.Context(g, pb.default_config(nLayer=5, precision=p, krylov=abi.KRYLOV_BICGSTAB_SPLIT
+                                              if p == abi.MIXED else abi.KRYLOV_GMRES, tol=1e-8))
            for p in (abi.FP64, abi.MIXED)}
     fields = ("Qsalt", "Qsusp", "Qsubl", "qdep", "drift_mass", "sum_drift", "sum_subl")
     worst = {p: {k: 0.0 for k in fields} for p in gpu}
