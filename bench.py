@@ -105,8 +105,13 @@ def variant_config(pb, abi, name: str, L: int, krylov_mixed: int):
     raise SystemExit(f"unknown variant {name}")
 
 
-def run_variant(pb, g, dev_forcing, dev_outs, cfg, net, steps, warmup, torch):
+def run_variant(pb, g, dev_forcing, dev_outs, cfg, net, steps, warmup, torch, spmv_shape=-1, dilu_xcd=-1):
     ctx = pb.Context(g, cfg)
+    for which in (0, 1):  # pinned launch shapes (profiling runs), else the tuner decides
+        if spmv_shape >= 0:
+            ctx.spmv_select(which, spmv_shape)
+        if dilu_xcd >= 0:
+            ctx.precond_select(which, dilu_xcd)
     if net is not None:
         ctx.set_warm_start(net)
     for _ in range(warmup):
@@ -231,6 +236,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fetch", action="store_true", help="skip timing the fetch (fetchr) kernel")
+    ap.add_argument("--spmv-shape", type=int, default=-1, help="pin the SpMV launch shape (default: tuned)")
+    ap.add_argument("--dilu-xcd", type=int, default=-1, choices=[-1, 0, 1],
+                    help="pin the D-ILU sweeps' XCD workgroup mapping (default: tuned)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="run one streaming kernel (y = -x, float32) reading and writing this many MB first: "
                          "the calibration dispatch for rocprofv3 FETCH_SIZE / WRITE_SIZE")
@@ -284,7 +292,7 @@ def main():
         cfg, learned = variant_config(pb, abi, name, L, krylov_mixed)
         log(f"[bench] variant {name}: warmup {args.warmup}, steps {args.steps}")
         secs, stats, prof, nnz, tuning = run_variant(pb, g, dev_forcing, dev_outs, cfg, net if learned else None, args.steps,
-                                             args.warmup, torch)
+                                                     args.warmup, torch, args.spmv_shape, args.dilu_xcd)
         st = stats[-1]
         variants[name] = {
             "steps_per_s": args.steps / secs,

@@ -425,6 +425,10 @@ int pbsm3d_create(pbsm3d_ctx** out, const pbsm3d_mesh* m, const pbsm3d_config* c
         c->r_s.alloc(N);
         c->r_d.alloc(F);
         const bool bicg = cfg->krylov == PBSM3D_KRYLOV_BICGSTAB || cfg->krylov == PBSM3D_KRYLOV_BICGSTAB_SPLIT;
+        if (cfg->krylov == PBSM3D_KRYLOV_BICGSTAB_SPLIT && cfg->dep_precond == PBSM3D_PC_MCILU0) {
+            for (int k = 0; k < 5; ++k) c->dsplit[k].alloc(F);
+            c->bwd64.ensure(F);
+        }
         if (cfg->krylov == PBSM3D_KRYLOV_BICGSTAB_SPLIT) {
             for (int k = 0; k < 3; ++k) {
                 if (cfg->precision == PBSM3D_MIXED)
@@ -635,6 +639,24 @@ int pbsm3d_step(pbsm3d_ctx* c, const pbsm3d_forcing* fo, int32_t dt_s, pbsm3d_ou
         if (stats.deposition_present) {
             CsrOp<double> D{F, c->dep.rp.p, c->dep.col.p, c->dep.val.p};
             PcFn<double> M;
+            SolveInfo si;
+            if (cfg.krylov == PBSM3D_KRYLOV_BICGSTAB_SPLIT && cfg.dep_precond == PBSM3D_PC_MCILU0) {
+                // the 2-D system the same way as the suspension: FP64 split D-ILU BiCGStab in
+                // the preconditioner's position order (one layer)
+                dilu_factor<double>(s, c->co, F, 1, c->dep.rp.p, c->dep.val.p, c->slots.p, c->dl_d);
+                to_pos<double, double>(s, c->co, F, 1, c->dep.rhs.p, c->dsplit[0].p);
+                to_pos<double, double>(s, c->co, F, 1, c->dep.x.p, c->dsplit[1].p);
+                CsrOp<double> Dp{F, nullptr, nullptr, nullptr, nullptr, F, 1};
+                Dp.co = &c->co;
+                Dp.pos = &c->dl_d;
+                CsrOp<double> Dh = Dp;
+                Dh.pos = nullptr;
+                Dh.split = &c->dl_d;
+                si = bicgstab_split<double>(s, c->bwd64, c->wd64, Dp, Dh, c->dsplit[0].p, c->dsplit[1].p, c->r_d.p,
+                                            c->dsplit[2].p, c->dsplit[3].p, c->dsplit[4].p, cfg.max_iters, cfg.tol,
+                                            0.0);
+                from_pos<double, double>(s, c->co, F, 1, c->dsplit[1].p, c->dep.x.p);
+            } else {
             if (cfg.dep_precond == PBSM3D_PC_MCILU0) {
                 dilu_factor<double>(s, c->co, F, 1, c->dep.rp.p, c->dep.val.p, c->slots.p, c->dl_d);
                 M = [c, F](hipStream_t st2, const double* r, double* z) {
@@ -646,8 +668,9 @@ int pbsm3d_step(pbsm3d_ctx* c, const pbsm3d_forcing* fo, int32_t dt_s, pbsm3d_ou
                 M = [F](hipStream_t st2, const double* r, double* z) {
                     PB_HIP(hipMemcpyAsync(z, r, F * sizeof(double), hipMemcpyDeviceToDevice, st2));
                 };
-            SolveInfo si = gmres_solve(s, c->wd64, D, M, c->dep.rhs.p, c->dep.x.p, c->r_d.p, cfg.restart,
-                                       cfg.max_iters, cfg.tol);
+            si = gmres_solve(s, c->wd64, D, M, c->dep.rhs.p, c->dep.x.p, c->r_d.p, cfg.restart, cfg.max_iters,
+                             cfg.tol);
+            }
             stats.dep_iters = si.iters;
             stats.dep_resid = si.resid;
             if (!si.converged) throw Error(PBSM3D_ENOCONV, "deposition solver failed to converge");

@@ -34,6 +34,8 @@ struct pbsm3d_ctx {
     pb::DevBuf<double> bp, xp;  // suspension b and x in the D-ILU position order
     pb::DevBuf<double> split64[3];  // work vectors of the split BiCGStab (FP64 / MIXED)
     pb::DevBuf<float> split32[3];
+    pb::DevBuf<double> dsplit[5];  // deposition b, x (position order) and split BiCGStab work vectors
+    pb::BicgWork<double> bwd64;    // deposition BiCGStab workspace
     // position-order SpMV launch shape per precision (index 0: FP64, 1: FP32), -1 until tuned
     int spmv_variant[2] = {-1, -1};
     float spmv_tune_ms[2][pb::kPosSpmvVariants] = {};

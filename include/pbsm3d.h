@@ -120,7 +120,9 @@ typedef struct pbsm3d_config {
 #define PBSM3D_KRYLOV_BICGSTAB_SPLIT 2  /* BiCGStab on the split D-ILU system (Eisenstat's
                                           trick: each operator application is one backward
                                           and one forward sweep, no SpMV) inside FP64
-                                          refinement; precond MCILU0, nLayer <= 16 */
+                                          refinement; precond MCILU0, nLayer <= 16.  With
+                                          dep_precond MCILU0 the deposition system is solved
+                                          the same way (FP64) */
 
 typedef struct pbsm3d_forcing {
     const double* U_R;            /* [F] m/s at Z_U_R = 50 m                       */
