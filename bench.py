@@ -237,6 +237,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fetch", action="store_true", help="skip timing the fetch (fetchr) kernel")
     ap.add_argument("--spmv-shape", type=int, default=-1, help="pin the SpMV launch shape (default: tuned)")
+    ap.add_argument("--ensemble", type=int, default=8, help="parameter-ensemble members to time (0: skip)")
+    ap.add_argument("--ensemble-steps", type=int, default=3)
     ap.add_argument("--dilu-xcd", type=int, default=-1, choices=[-1, 0, 1],
                     help="pin the D-ILU sweeps' XCD workgroup mapping (default: tuned)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
@@ -342,6 +344,30 @@ def main():
         fctx.close()
         log(f"[bench] fetch: {fetch_info}")
 
+    # parameter ensemble: uneven members run concurrently (one stream each) vs one at a time
+    ens = None
+    if args.ensemble > 0:
+        from chm_amd import ensemble
+
+        log(f"[bench] ensemble: {args.ensemble} members x {args.ensemble_steps} steps")
+        members = ensemble.draw_members(args.ensemble, seed=0)
+        built = ensemble.build(members, precision=abi.MIXED, krylov=krylov_mixed, warm_net=net)
+        ensemble.run(built, 1)  # first steps include the launch-shape tuning
+        seq = ensemble.run(built, args.ensemble_steps, workers=1)
+        par = ensemble.run(built, args.ensemble_steps, workers=len(built))
+        ens = {"members": len(built), "steps": args.ensemble_steps,
+               "faces": [r["faces"] for r in par["per_member"]], "layers": [r["layers"] for r in par["per_member"]],
+               "member_steps_per_s_concurrent": round(par["member_steps_per_s"], 3),
+               "member_steps_per_s_one_at_a_time": round(seq["member_steps_per_s"], 3),
+               "cell_steps_per_s_concurrent": round(par["cell_steps_per_s"], 1),
+               "speedup_concurrent": round(par["member_steps_per_s"] / seq["member_steps_per_s"], 3),
+               "max_resid": max(r["last_resid"] for r in par["per_member"]),
+               "note": "mixed precision, learned start on 10-layer members; members on separate HIP streams "
+                       "driven by host threads (not graph-captured)"}
+        for _, _, ctx in built:
+            ctx.close()
+        log(f"[bench] ensemble: {ens}")
+
     gain = None
     if net is not None and "mixed_learned" in names and args.gain_seeds:
         log("[bench] learned vs cold start on held-out forcing draws")
@@ -414,6 +440,7 @@ def main():
         "warm_start_training": train_info,
         "warm_start_gain": gain,
         "fetch": fetch_info,
+        "ensemble": ens,
         "roofline": roofline,
         "kernel_rooflines": rooflines,
         "cpu_baseline": cpu,

@@ -53,7 +53,8 @@ def member_forcing(mesh: meshgen.Mesh, m: Member, step: int) -> dict:
     return fo
 
 
-def build(members: list[Member], precision=abi.MIXED, krylov=abi.KRYLOV_BICGSTAB, warm_net=None, device: int = 0):
+def build(members: list[Member], precision=abi.MIXED, krylov=abi.KRYLOV_BICGSTAB_SPLIT, warm_net=None,
+          device: int = 0):
     """Meshes, geometry and contexts of all members (set-up, not timed)."""
     built = []
     for m in members:

@@ -75,7 +75,9 @@ def test_create_rejects_bad_config_without_gpu():
     m = meshgen.synthetic_mesh(30, seed=1)
     g = pbsm3d.face_geometry(m)
     for bad in (dict(nLayer=1), dict(settling_velocity=-1.0), dict(use_exp_fetch=1, use_tanh_fetch=1),
-                dict(restart=0), dict(precond=abi.PC_ILUT)):
+                dict(restart=0), dict(precond=abi.PC_ILUT), dict(krylov=7), dict(inner_rtol=1.5),
+                dict(nLayer=17, krylov=abi.KRYLOV_BICGSTAB_SPLIT),
+                dict(precond=abi.PC_JACOBI, krylov=abi.KRYLOV_BICGSTAB_SPLIT)):
         with pytest.raises(pbsm3d.Pbsm3dError) as e:
             pbsm3d.Context(g, pbsm3d.default_config(**bad))
         assert e.value.code == abi.PBSM3D_EINVAL

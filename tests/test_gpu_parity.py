@@ -296,3 +296,22 @@ def test_c3_mixed_precision_96_steps(pb, oracle, mesh_c1):
         for k, bound in (("Qsalt", 1e-12), ("Qsusp", 1e-5), ("Qsubl", 1e-5), ("qdep", 1e-3), ("drift_mass", 1e-3),
                          ("sum_drift", 1e-3), ("sum_subl", 1e-5)):
             assert worst[p][k] < bound, (name, k, worst[p][k])
+
+
+@pytest.mark.parametrize("L,precision,krylov", [
+    (2, abi.FP64, abi.KRYLOV_BICGSTAB_SPLIT), (16, abi.MIXED, abi.KRYLOV_BICGSTAB_SPLIT),
+    (16, abi.FP64, abi.KRYLOV_BICGSTAB), (20, abi.FP64, abi.KRYLOV_BICGSTAB), (20, abi.MIXED, abi.KRYLOV_GMRES),
+    (64, abi.FP64, abi.KRYLOV_GMRES)])
+def test_layer_counts(pb, oracle, mesh_small, L, precision, krylov):
+    """The column kernels at the bounds of their unrolled forms (nLayer 2, 16) and the
+    looped forms above 16 (D-ILU sweeps, row-per-thread SpMV), against the oracle."""
+    _print_case("layers", L=L, precision=precision, krylov=krylov)
+    F = mesh_small.n_faces
+    fo = meshgen.forcing_c2(F, seed=6)
+    g, ctx, orc = _pair(pb, oracle, mesh_small, L=L, precision=precision, krylov=krylov, tol=1e-12)
+    og, _ = orc.step(fo)
+    gg, st = ctx.step(fo, want_fields=True)
+    assert st["susp_resid"] <= 1e-12 and st["dep_resid"] <= 1e-12
+    for k in ("conc", "Qsusp", "qdep", "drift_mass"):
+        assert rel_err(gg[k], og[k]) < 1e-8, (L, k, rel_err(gg[k], og[k]))
+
