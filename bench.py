@@ -105,6 +105,29 @@ def variant_config(pb, abi, name: str, L: int, krylov_mixed: int):
     raise SystemExit(f"unknown variant {name}")
 
 
+def host_io_rate(pb, g, cfg, net, forcing, steps, spmv_shape=-1, dilu_xcd=-1):
+    """steps/s of the same step with host forcing and host outputs (the PCIe copies of
+    the forcing in and the eight output fields out included)."""
+    import numpy as np
+
+    ctx = pb.Context(g, cfg)
+    for which in (0, 1):
+        if spmv_shape >= 0:
+            ctx.spmv_select(which, spmv_shape)
+        if dilu_xcd >= 0:
+            ctx.precond_select(which, dilu_xcd)
+    if net is not None:
+        ctx.set_warm_start(net)
+    outs = {k: np.zeros(g.n_faces) for k in pb.abi.OUT_FIELDS}
+    ctx.step(forcing, outs=outs)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.step(forcing, outs=outs)
+    dt = time.perf_counter() - t0
+    ctx.close()
+    return steps / dt
+
+
 def run_variant(pb, g, dev_forcing, dev_outs, cfg, net, steps, warmup, torch, spmv_shape=-1, dilu_xcd=-1):
     ctx = pb.Context(g, cfg)
     for which in (0, 1):  # pinned launch shapes (profiling runs), else the tuner decides
@@ -239,6 +262,7 @@ def main():
     ap.add_argument("--spmv-shape", type=int, default=-1, help="pin the SpMV launch shape (default: tuned)")
     ap.add_argument("--ensemble", type=int, default=8, help="parameter-ensemble members to time (0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=3)
+    ap.add_argument("--host-io-steps", type=int, default=3, help="steps timed with host forcing / outputs (0: skip)")
     ap.add_argument("--dilu-xcd", type=int, default=-1, choices=[-1, 0, 1],
                     help="pin the D-ILU sweeps' XCD workgroup mapping (default: tuned)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
@@ -368,6 +392,17 @@ def main():
             ctx.close()
         log(f"[bench] ensemble: {ens}")
 
+    pcie = None
+    if args.host_io_steps > 0 and names:
+        hn = "mixed_learned" if "mixed_learned" in names else names[-1]
+        hcfg, hlearned = variant_config(pb, abi, hn, L, krylov_mixed)
+        pcie = {"variant": hn, "steps_per_s": round(host_io_rate(pb, g, hcfg, net if hlearned else None, forcing,
+                                                                 args.host_io_steps, args.spmv_shape,
+                                                                 args.dilu_xcd), 4),
+                "note": "host (numpy) forcing and outputs: H2D of 8 forcing fields and D2H of 8 output fields "
+                        "per step included"}
+        log(f"[bench] host-array (PCIe-inclusive) rate: {pcie}")
+
     gain = None
     if net is not None and "mixed_learned" in names and args.gain_seeds:
         log("[bench] learned vs cold start on held-out forcing draws")
@@ -441,6 +476,7 @@ def main():
         "warm_start_gain": gain,
         "fetch": fetch_info,
         "ensemble": ens,
+        "pcie_inclusive": pcie,
         "roofline": roofline,
         "kernel_rooflines": rooflines,
         "cpu_baseline": cpu,
