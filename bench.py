@@ -105,17 +105,24 @@ def variant_config(pb, abi, name: str, L: int, krylov_mixed: int):
     raise SystemExit(f"unknown variant {name}")
 
 
-def host_io_rate(pb, g, cfg, net, forcing, steps, spmv_shape=-1, dilu_xcd=-1):
+def _pin(pb, ctx, cfg, spmv_shape, dilu_mode):
+    """Pinned launch shapes (profiling runs), else the tuner decides.  The non-temporal
+    sweep modes (bit 1) exist for the split operator only; other solvers keep bit 0."""
+    for which in (0, 1):
+        if spmv_shape >= 0:
+            ctx.spmv_select(which, spmv_shape)
+        if dilu_mode >= 0:
+            split = cfg.krylov == pb.abi.KRYLOV_BICGSTAB_SPLIT
+            ctx.precond_select(which, dilu_mode if split else dilu_mode & 1)
+
+
+def host_io_rate(pb, g, cfg, net, forcing, steps, spmv_shape=-1, dilu_mode=-1):
     """steps/s of the same step with host forcing and host outputs (the PCIe copies of
     the forcing in and the eight output fields out included)."""
     import numpy as np
 
     ctx = pb.Context(g, cfg)
-    for which in (0, 1):
-        if spmv_shape >= 0:
-            ctx.spmv_select(which, spmv_shape)
-        if dilu_xcd >= 0:
-            ctx.precond_select(which, dilu_xcd)
+    _pin(pb, ctx, cfg, spmv_shape, dilu_mode)
     if net is not None:
         ctx.set_warm_start(net)
     outs = {k: np.zeros(g.n_faces) for k in pb.abi.OUT_FIELDS}
@@ -128,19 +135,17 @@ def host_io_rate(pb, g, cfg, net, forcing, steps, spmv_shape=-1, dilu_xcd=-1):
     return steps / dt
 
 
-def run_variant(pb, g, dev_forcing, dev_outs, cfg, net, steps, warmup, torch, spmv_shape=-1, dilu_xcd=-1):
+def run_variant(pb, g, dev_forcing, dev_outs, cfg, net, steps, warmup, torch, spmv_shape=-1, dilu_mode=-1):
     ctx = pb.Context(g, cfg)
-    for which in (0, 1):  # pinned launch shapes (profiling runs), else the tuner decides
-        if spmv_shape >= 0:
-            ctx.spmv_select(which, spmv_shape)
-        if dilu_xcd >= 0:
-            ctx.precond_select(which, dilu_xcd)
+    _pin(pb, ctx, cfg, spmv_shape, dilu_mode)
     if net is not None:
         ctx.set_warm_start(net)
     for _ in range(warmup):
         ctx.step(dev_forcing, outs=dev_outs, on_device=True)
     torch.cuda.synchronize()
-    ctx.profile(True)
+    # the timed steps bracket only the classes whose rooflines are reported (each bracket
+    # adds device idle time); two further steps bracket every class for the breakdown
+    ctx.profile(True, classes=timed_classes(pb, cfg))
     ctx.profile_read(reset=True)
     stats = []
     torch.cuda.synchronize()
@@ -151,12 +156,27 @@ def run_variant(pb, g, dev_forcing, dev_outs, cfg, net, steps, warmup, torch, sp
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     prof = ctx.profile_read(reset=True)
+    ctx.profile(True)
+    for _ in range(BREAKDOWN_STEPS):
+        ctx.step(dev_forcing, outs=dev_outs, on_device=True)
+    torch.cuda.synchronize()
+    breakdown = ctx.profile_read(reset=True)
     ctx.profile(False)
     nnz = pb.lib().pbsm3d_system_nnz(ctx._h, 0)
     tuning = {"spmv_fp64": ctx.spmv_tuning(0), "spmv_fp32": ctx.spmv_tuning(1),
               "dilu_fp64": ctx.precond_tuning(0), "dilu_fp32": ctx.precond_tuning(1)}
     ctx.close()
-    return (t1 - t0), stats, prof, nnz, tuning
+    return (t1 - t0), stats, prof, breakdown, nnz, tuning
+
+
+BREAKDOWN_STEPS = 2
+
+
+def timed_classes(pb, cfg):
+    """Kernel classes bracketed during the timed steps: the ones the rooflines report."""
+    if cfg.krylov == pb.abi.KRYLOV_BICGSTAB_SPLIT:
+        return ("precond",)
+    return ("spmv64" if cfg.precision == pb.abi.FP64 else "spmv32", "precond")
 
 
 def cpu_baseline(g_full, forcing, n_layer, faces, threads):
@@ -263,8 +283,9 @@ def main():
     ap.add_argument("--ensemble", type=int, default=8, help="parameter-ensemble members to time (0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=3)
     ap.add_argument("--host-io-steps", type=int, default=3, help="steps timed with host forcing / outputs (0: skip)")
-    ap.add_argument("--dilu-xcd", type=int, default=-1, choices=[-1, 0, 1],
-                    help="pin the D-ILU sweeps' XCD workgroup mapping (default: tuned)")
+    ap.add_argument("--dilu-mode", type=int, default=-1, choices=[-1, 0, 1, 2, 3],
+                    help="pin the D-ILU sweep mode: bit 0 XCD workgroup mapping, bit 1 non-temporal factor "
+                         "loads (split operator only) (default: tuned)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="run one streaming kernel (y = -x, float32) reading and writing this many MB first: "
                          "the calibration dispatch for rocprofv3 FETCH_SIZE / WRITE_SIZE")
@@ -317,8 +338,8 @@ def main():
     for name in names:
         cfg, learned = variant_config(pb, abi, name, L, krylov_mixed)
         log(f"[bench] variant {name}: warmup {args.warmup}, steps {args.steps}")
-        secs, stats, prof, nnz, tuning = run_variant(pb, g, dev_forcing, dev_outs, cfg, net if learned else None, args.steps,
-                                                     args.warmup, torch, args.spmv_shape, args.dilu_xcd)
+        secs, stats, prof, breakdown, nnz, tuning = run_variant(pb, g, dev_forcing, dev_outs, cfg, net if learned else None, args.steps,
+                                                     args.warmup, torch, args.spmv_shape, args.dilu_mode)
         st = stats[-1]
         variants[name] = {
             "steps_per_s": args.steps / secs,
@@ -338,6 +359,8 @@ def main():
             "ms_phase": {k: round(st[k], 3) for k in ("ms_assembly", "ms_susp_solve", "ms_post", "ms_dep_solve")},
             "kernels": {k: {"launches": n, "ms_avg": (ms / n if n else 0.0), "ms_total": ms}
                         for k, (n, ms) in prof.items() if n},
+            f"kernel_breakdown_{BREAKDOWN_STEPS}_steps": {k: {"launches": n, "ms_total": round(ms, 3)}
+                                                          for k, (n, ms) in breakdown.items() if n},
             "nnz": nnz,
         }
         # per-field difference of this variant's last-step outputs from the plain FP64 variant's
@@ -398,7 +421,7 @@ def main():
         hcfg, hlearned = variant_config(pb, abi, hn, L, krylov_mixed)
         pcie = {"variant": hn, "steps_per_s": round(host_io_rate(pb, g, hcfg, net if hlearned else None, forcing,
                                                                  args.host_io_steps, args.spmv_shape,
-                                                                 args.dilu_xcd), 4),
+                                                                 args.dilu_mode), 4),
                 "note": "host (numpy) forcing and outputs: H2D of 8 forcing fields and D2H of 8 output fields "
                         "per step included"}
         log(f"[bench] host-array (PCIe-inclusive) rate: {pcie}")
@@ -415,7 +438,8 @@ def main():
     head = variants[head_name]
     # rooflines of the suspension solve's HBM-bound kernels in the headline variant; the
     # reported one is the kernel class with the largest device time
-    s_bytes = 4 if "spmv32" in head["kernels"] else 8
+    # element width of the solve's operator and preconditioner: FP32 in the mixed variants
+    s_bytes = 8 if head["precision"] == "fp64" else 4
     kspmv = "spmv32" if s_bytes == 4 else "spmv64"
     nnz = head["nnz"]
     bicg = head["krylov"] == "bicgstab"

@@ -192,10 +192,21 @@ SolveInfo solve_in_position_order(pbsm3d_ctx* c, const pbsm3d_config& cfg, int64
         c->spmv_variant[0] = tune_pos_spmv<double>(s, c->co, F, L, c->dl_s, c->xp.p, c->r_s.p, c->spmv_tune_ms[0]);
     if (cfg.precision == PBSM3D_MIXED && c->spmv_variant[1] < 0)
         c->spmv_variant[1] = tune_pos_spmv<float>(s, c->co, F, L, c->dl32, c->r32.p, c->d32.p, c->spmv_tune_ms[1]);
-    if (cfg.precision == PBSM3D_FP64 && c->dilu_xcd[0] < 0)
-        c->dilu_xcd[0] = tune_dilu_xcd<double>(s, c->co, F, L, c->dl_s, c->bp.p, c->r_s.p, c->dilu_tune_ms[0]);
-    if (cfg.precision == PBSM3D_MIXED && c->dilu_xcd[1] < 0)
-        c->dilu_xcd[1] = tune_dilu_xcd<float>(s, c->co, F, L, c->dl32, c->r32.p, c->d32.p, c->dilu_tune_ms[1]);
+    if (cfg.krylov == PBSM3D_KRYLOV_BICGSTAB_SPLIT) {
+        if (cfg.precision == PBSM3D_FP64 && c->dilu_mode[0] < 0)
+            c->dilu_mode[0] = tune_split_mode<double>(s, c->co, F, L, c->dl_s, c->bp.p, c->split64[0].p,
+                                                      c->dilu_tune_ms[0]);
+        if (cfg.precision == PBSM3D_MIXED && c->dilu_mode[1] < 0) {
+            to_pos<double, float>(s, c->co, F, L, c->susp.rhs.p, c->r32.p);
+            c->dilu_mode[1] = tune_split_mode<float>(s, c->co, F, L, c->dl32, c->r32.p, c->split32[0].p,
+                                                     c->dilu_tune_ms[1]);
+        }
+    } else {
+        if (cfg.precision == PBSM3D_FP64 && c->dilu_mode[0] < 0)
+            c->dilu_mode[0] = tune_dilu_xcd<double>(s, c->co, F, L, c->dl_s, c->bp.p, c->r_s.p, c->dilu_tune_ms[0]);
+        if (cfg.precision == PBSM3D_MIXED && c->dilu_mode[1] < 0)
+            c->dilu_mode[1] = tune_dilu_xcd<float>(s, c->co, F, L, c->dl32, c->r32.p, c->d32.p, c->dilu_tune_ms[1]);
+    }
     CsrOp<double> A{N, nullptr, nullptr, nullptr, &c->prof, F, L};
     A.co = &c->co;
     A.pos = &c->dl_s;
@@ -206,7 +217,7 @@ SolveInfo solve_in_position_order(pbsm3d_ctx* c, const pbsm3d_config& cfg, int64
             CsrOp<double> Ah{N, nullptr, nullptr, nullptr, &c->prof, F, L};
             Ah.co = &c->co;
             Ah.split = &c->dl_s;
-            Ah.xcd = c->dilu_xcd[0] == 1;
+            Ah.mode = c->dilu_mode[0];
             Ah.prof_class = PBSM3D_K_PRECOND;
             si = bicgstab_split<double>(s, c->bw64, c->ws64, A, Ah, c->bp.p, c->xp.p, c->r_s.p, c->split64[0].p,
                                         c->split64[1].p, c->split64[2].p, cfg.max_iters, cfg.tol, cfg.inner_rtol);
@@ -214,7 +225,7 @@ SolveInfo solve_in_position_order(pbsm3d_ctx* c, const pbsm3d_config& cfg, int64
             CsrOp<float> Ah{N, nullptr, nullptr, nullptr, &c->prof, F, L};
             Ah.co = &c->co;
             Ah.split = &c->dl32;
-            Ah.xcd = c->dilu_xcd[1] == 1;
+            Ah.mode = c->dilu_mode[1];
             Ah.prof_class = PBSM3D_K_PRECOND;
             si = bicgstab_split<float>(s, c->bw32, c->ws32, A, Ah, c->bp.p, c->xp.p, c->r_s.p, c->split32[0].p,
                                        c->split32[1].p, c->split32[2].p, cfg.max_iters, cfg.tol,
@@ -222,7 +233,7 @@ SolveInfo solve_in_position_order(pbsm3d_ctx* c, const pbsm3d_config& cfg, int64
         }
     } else if (cfg.precision == PBSM3D_FP64) {
         PcFn<double> M = [c, F, L](hipStream_t st, const double* r, double* z) {
-            dilu_apply<double>(st, c->co, F, L, c->dl_s, r, z, true, c->dilu_xcd[0] == 1);
+            dilu_apply<double>(st, c->co, F, L, c->dl_s, r, z, true, (c->dilu_mode[0] & 1) != 0);
         };
         if (bicg)
             si = bicgstab_solve(s, c->bw64, c->ws64, A, M, c->bp.p, c->xp.p, cfg.max_iters, cfg.tol);
@@ -234,7 +245,7 @@ SolveInfo solve_in_position_order(pbsm3d_ctx* c, const pbsm3d_config& cfg, int64
         A32.pos = &c->dl32;
         A32.variant = c->spmv_variant[1];
         PcFn<float> M = [c, F, L](hipStream_t st, const float* r, float* z) {
-            dilu_apply<float>(st, c->co, F, L, c->dl32, r, z, true, c->dilu_xcd[1] == 1);
+            dilu_apply<float>(st, c->co, F, L, c->dl32, r, z, true, (c->dilu_mode[1] & 1) != 0);
         };
         if (bicg)
             si = bicgstab_mixed(s, c->bw32, c->ws32, A, A32, M, c->bp.p, c->xp.p, c->r_s.p, c->r32.p, c->d32.p,
@@ -715,6 +726,12 @@ int pbsm3d_profile_enable(pbsm3d_ctx* c, int enable) {
     return PBSM3D_OK;
 }
 
+int pbsm3d_profile_classes(pbsm3d_ctx* c, uint32_t mask) {
+    if (!c || (mask >> PBSM3D_K_COUNT) != 0) return PBSM3D_EINVAL;
+    c->prof.mask = mask;
+    return PBSM3D_OK;
+}
+
 int pbsm3d_profile_read(pbsm3d_ctx* c, int kernel, int64_t* launches, double* ms_total, int reset) {
     if (!c || kernel < 0 || kernel >= PBSM3D_K_COUNT) return PBSM3D_EINVAL;
     if (launches) *launches = c->prof.count[kernel];
@@ -746,19 +763,18 @@ int pbsm3d_face_colors(int64_t n_faces, const int32_t* neigh, int32_t* color) {
     }
 }
 
-int pbsm3d_precond_tuning(const pbsm3d_ctx* c, int which, int32_t* xcd, float* ms) {
+int pbsm3d_precond_tuning(const pbsm3d_ctx* c, int which, int32_t* mode, float* ms) {
     if (!c || (which != 0 && which != 1)) return PBSM3D_EINVAL;
-    if (xcd) *xcd = c->dilu_xcd[which];
-    if (ms) {
-        ms[0] = c->dilu_tune_ms[which][0];
-        ms[1] = c->dilu_tune_ms[which][1];
-    }
+    if (mode) *mode = c->dilu_mode[which];
+    if (ms)
+        for (int m = 0; m < kSplitModes; ++m) ms[m] = c->dilu_tune_ms[which][m];
     return PBSM3D_OK;
 }
 
-int pbsm3d_precond_select(pbsm3d_ctx* c, int which, int xcd) {
-    if (!c || (which != 0 && which != 1) || xcd < -1 || xcd > 1) return PBSM3D_EINVAL;
-    c->dilu_xcd[which] = xcd;
+int pbsm3d_precond_select(pbsm3d_ctx* c, int which, int mode) {
+    if (!c || (which != 0 && which != 1) || mode < -1 || mode >= kSplitModes) return PBSM3D_EINVAL;
+    if (mode > 1 && c->cfg.krylov != PBSM3D_KRYLOV_BICGSTAB_SPLIT) return PBSM3D_EINVAL;  // NT: split sweeps only
+    c->dilu_mode[which] = mode;
     return PBSM3D_OK;
 }
 

@@ -40,8 +40,10 @@ struct pbsm3d_ctx {
     int spmv_variant[2] = {-1, -1};
     float spmv_tune_ms[2][pb::kPosSpmvVariants] = {};
     // D-ILU sweeps with the XCD workgroup mapping (1) or without (0), -1 until tuned
-    int dilu_xcd[2] = {-1, -1};
-    float dilu_tune_ms[2][2] = {};
+    // D-ILU sweep mode per precision: bit 0 the XCD workgroup mapping, bit 1 (split
+    // operator only) the non-temporal factor loads; -1 until tuned
+    int dilu_mode[2] = {-1, -1};
+    float dilu_tune_ms[2][pb::kSplitModes] = {};
     pb::DevBuf<float> val32, r32, d32;
     pb::GmresWork<double> ws64, wd64;
     pb::GmresWork<float> ws32;

@@ -200,14 +200,22 @@ bool tune_dilu_xcd(hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFa
 // which = 0: out = Ahat in = D~ (D~ + L)^-1 A (D~ + U)^-1 in (one backward + one forward sweep);
 // 1: out = D~ (D~ + L)^-1 in (left transform of a right-hand side);
 // 2: out = (D~ + U)^-1 in (right transform back to the unknowns).  in and out must differ.
+// mode: bit 0 the XCD workgroup mapping, bit 1 the factor arrays loaded with the non-temporal hint.
 template <typename T>
 void split_op(int which, hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFactors<T>& d, const T* in, T* out,
-              bool xcd);
+              int mode);
 // split_op(0) with per-block partial sums partial[2b] = dw.out, partial[2b+1] = out.out; returns
-// the number of blocks b (at most F / kBlock + 8 per colour)
+// the number of blocks b (at most F / kBlock + 8 per colour).  stop (may be NULL): the
+// launches return at entry when *stop != 0 (a BiCGStab run that has finished).
 template <typename T>
 int64_t split_apply_dots(hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFactors<T>& d, const T* in, T* out,
-                         const T* dw, double* partial, bool xcd);
+                         const T* dw, double* partial, int mode, const double* stop = nullptr);
+constexpr int kSplitModes = 4;
+// times split_op(0) in each mode (HIP events on the stream) and returns the fastest;
+// ms_out[kSplitModes] receives the averages (ms), if given.
+template <typename T>
+int tune_split_mode(hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFactors<T>& d, const T* w, T* o,
+                    float* ms_out);
 // FP32 copy of FP64 factors (the FP32 preconditioner and operator of the mixed solve)
 void dilu_narrow(hipStream_t s, int64_t N, const DiluFactors<double>& a, DiluFactors<float>& b);
 // y = b - A x with per-block partial sums of y.y, everything in position order, A taken
@@ -246,9 +254,12 @@ template <typename T>
 void axpy_dots(hipStream_t s, int64_t n, double a, const T* x, const T* y, T* out, const T* rh, double* partial,
                int nblk);
 
-// BiCGStab with its scalars kept on the device (slots of sc[]), so an iteration
-// needs one host synchronisation (the convergence test) instead of three.
-enum BicgSc { SC_RHO_PREV = 0, SC_RHO, SC_ALPHA, SC_OMEGA, SC_RHV, SC_SS, SC_ST, SC_TT, SC_RR, SC_FLAG, SC_COUNT };
+// BiCGStab with its scalars kept on the device (slots of sc[]).  The convergence test is
+// made on the device too: once sc[SC_STOP] is set every later BiCGStab kernel of the run
+// returns at entry, so the host can queue iteration k+1 before it reads the result of k.
+enum BicgSc {
+    SC_RHO_PREV = 0, SC_RHO, SC_ALPHA, SC_OMEGA, SC_RHV, SC_SS, SC_ST, SC_TT, SC_RR, SC_FLAG, SC_STOP, SC_COUNT
+};
 // p = r + beta (p - omega v) with beta from the scalars of the previous step
 template <typename T>
 void bicg_p_dev(hipStream_t s, int64_t n, const T* r, const T* v, const double* sc, T* p);
@@ -262,8 +273,11 @@ void bicg_xr_dev(hipStream_t s, int64_t n, const T* ph, const T* sh, const T* sv
                  const double* sc, T* x, T* r, double* partial, int nblk);
 // one block sums ncols (<= 2) partial columns in a fixed order and updates the scalars:
 // stage 0 start (rho, rr), 1 alpha from rh.v, 2 s.s, 3 omega from (s.t, t.t), 4 new (rho, rr);
-// sc[SC_FLAG] != 0 marks a breakdown (1: rh.v = 0, 2: omega = 0, 3: non-finite residual)
-void bicg_stage(hipStream_t s, const double* partial, int64_t nblk, int stride, int ncols, int stage, double* sc);
+// sc[SC_FLAG] != 0 marks a breakdown (1: rh.v = 0, 2: omega = 0, 3: non-finite residual).
+// Stage 0 clears sc[SC_STOP]; stage 4 sets it when a breakdown was flagged, sqrt(rr) <=
+// stop_abs or rr is not finite.  Stages 1-4 do nothing once it is set.
+void bicg_stage(hipStream_t s, const double* partial, int64_t nblk, int stride, int ncols, int stage, double* sc,
+                double stop_abs = 0.0);
 
 // conversions / vector updates
 template <typename T>

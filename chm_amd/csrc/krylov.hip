@@ -248,14 +248,18 @@ void BicgWork<T>::ensure(int64_t n_) {
     // two partial sums per block of the fused SpMV + dot launches, or of the vector kernels
     partial.alloc(std::max<size_t>(4 * (size_t)kReduceBlocks, 2 * (size_t)pos_spmv_dots_max_blocks(n)));
     dsc.alloc(SC_COUNT);
-    hsc.alloc(SC_COUNT);
+    hsc.alloc(2 * SC_COUNT);
+    for (hipEvent_t& e : ev)
+        if (!e) PB_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
 }
 
 namespace {
 // y = A x with the partial sums (w.y, y.y) of each block (two per block); returns the block count
+// (stop: the split operator's launches return at entry once *stop != 0)
 template <typename T>
-int64_t spmv_dots(hipStream_t s, const CsrOp<T>& A, const T* x, T* y, const T* w, double* partial, int nblk) {
-    if (A.split) return split_apply_dots<T>(s, *A.co, A.F, A.L, *A.split, x, y, w, partial, A.xcd);
+int64_t spmv_dots(hipStream_t s, const CsrOp<T>& A, const T* x, T* y, const T* w, double* partial, int nblk,
+                  const double* stop) {
+    if (A.split) return split_apply_dots<T>(s, *A.co, A.F, A.L, *A.split, x, y, w, partial, A.mode, stop);
     if (A.pos) return pos_spmv_dots<T>(s, A.variant, *A.co, A.F, A.L, *A.pos, x, y, w, partial);
     op_spmv<T>(s, A, x, y);
     mdot<T>(s, A.n, w, 1, y, partial, nblk);
@@ -264,8 +268,11 @@ int64_t spmv_dots(hipStream_t s, const CsrOp<T>& A, const T* x, T* y, const T* w
 
 // One BiCGStab run on A x = b from the x given; stops when ||r|| <= stop_abs,
 // on a breakdown (the caller restarts from the true residual) or after maxit
-// iterations; returns the iterations.  alpha, omega, rho live on the device:
-// the only host synchronisation per iteration is the convergence test.
+// iterations; returns the iterations.  alpha, omega, rho and the stop test live
+// on the device, and the host queues iteration k+1 before it waits for the
+// scalars of iteration k, so the device never idles on the host round trip;
+// after the stop every kernel of the queued iteration returns at entry (x and r
+// keep the values of the stopping iteration).
 // An empty M means no preconditioner (the operator already carries it, as the
 // split operator does).  stop_rel: stop_abs is relative to the starting ||r||.
 template <typename T>
@@ -301,9 +308,8 @@ int bicg_run(hipStream_t s, BicgWork<T>& w, const CsrOp<T>& A, const PcFn<T>& M,
     *res_out = rr;
     if (stop_rel) stop_abs *= rr;
     if (rr <= stop_abs) return 0;
-    int it = 0;
-    while (it < maxit) {
-        ++it;
+    const double* stop = sc + SC_STOP;
+    auto iterate = [&](int it) {
         if (it == 1) {
             PB_HIP(hipMemcpyAsync(w.p.p, w.r.p, n * sizeof(T), hipMemcpyDeviceToDevice, s));
         } else {
@@ -317,7 +323,7 @@ int bicg_run(hipStream_t s, BicgWork<T>& w, const CsrOp<T>& A, const PcFn<T>& M,
         int64_t nb = 0;
         {
             ProfScope ps(A.prof, s, spmv_id);
-            nb = spmv_dots<T>(s, A, ph, w.v.p, w.rh.p, w.partial.p, nblk);  // v = A ph; rh.v, v.v
+            nb = spmv_dots<T>(s, A, ph, w.v.p, w.rh.p, w.partial.p, nblk, stop);  // v = A ph; rh.v, v.v
         }
         {
             ProfScope ps(A.prof, s, PBSM3D_K_ORTHO);
@@ -331,20 +337,28 @@ int bicg_run(hipStream_t s, BicgWork<T>& w, const CsrOp<T>& A, const PcFn<T>& M,
         }
         {
             ProfScope ps(A.prof, s, spmv_id);
-            nb = spmv_dots<T>(s, A, sh, w.t.p, w.s.p, w.partial.p, nblk);  // t = A sh; s.t, t.t
+            nb = spmv_dots<T>(s, A, sh, w.t.p, w.s.p, w.partial.p, nblk, stop);  // t = A sh; s.t, t.t
         }
         {
             ProfScope ps(A.prof, s, PBSM3D_K_ORTHO);
             bicg_stage(s, w.partial.p, nb, 2, 2, 3, sc);  // omega
             bicg_xr_dev<T>(s, n, ph, sh, w.s.p, w.t.p, w.rh.p, sc, x, w.r.p, w.partial.p, nblk);
-            bicg_stage(s, w.partial.p, nblk, 2, 2, 4, sc);  // rho, rr
+            bicg_stage(s, w.partial.p, nblk, 2, 2, 4, sc, stop_abs);  // rho, rr, stop test
         }
-        fetch();
-        rr = std::sqrt(h[SC_RR]);
+        // the scalars of iteration it into host slot it % 2
+        PB_HIP(hipMemcpyAsync(h + SC_COUNT * (it & 1), sc, sizeof(double) * SC_COUNT, hipMemcpyDeviceToHost, s));
+        PB_HIP(hipEventRecord(w.ev[it & 1], s));
+    };
+    if (maxit <= 0) return 0;
+    iterate(1);
+    for (int it = 1;; ++it) {
+        if (it < maxit) iterate(it + 1);
+        PB_HIP(hipEventSynchronize(w.ev[it & 1]));
+        const double* hk = h + SC_COUNT * (it & 1);
+        rr = std::sqrt(hk[SC_RR]);
         *res_out = rr;
-        if (h[SC_FLAG] != 0.0 || rr <= stop_abs || !std::isfinite(rr)) return it;
+        if (hk[SC_STOP] != 0.0 || it >= maxit) return it;
     }
-    return it;
 }
 }  // namespace
 
@@ -409,7 +423,7 @@ SolveInfo bicgstab_split(hipStream_t s, BicgWork<T>& w, GmresWork<T>& gw, const 
         scale_into(s, n, r, 1.0 / beta, work1);  // r / ||r||
         {
             ProfScope ps(Ahat.prof, s, pclass);
-            split_op<T>(1, s, *Ahat.co, Ahat.F, Ahat.L, *Ahat.split, work1, work2, Ahat.xcd);  // rhat
+            split_op<T>(1, s, *Ahat.co, Ahat.F, Ahat.L, *Ahat.split, work1, work2, Ahat.mode);  // rhat
         }
         PB_HIP(hipMemsetAsync(work3, 0, n * sizeof(T), s));
         double stop = 0.5 * tol * nb / beta;
@@ -418,7 +432,7 @@ SolveInfo bicgstab_split(hipStream_t s, BicgWork<T>& w, GmresWork<T>& gw, const 
         info.iters += bicg_run<T>(s, w, Ahat, PcFn<T>(), work2, work3, true, stop, maxit - info.iters, &est, true);
         {
             ProfScope ps(Ahat.prof, s, pclass);
-            split_op<T>(2, s, *Ahat.co, Ahat.F, Ahat.L, *Ahat.split, work3, work1, Ahat.xcd);  // d = (D~+U)^-1 dhat
+            split_op<T>(2, s, *Ahat.co, Ahat.F, Ahat.L, *Ahat.split, work3, work1, Ahat.mode);  // d = (D~+U)^-1 dhat
         }
         add_into(s, n, beta, work1, x);
         info.cycles += 1;

@@ -33,14 +33,14 @@ struct CsrOp {
     int variant = 0;  // launch shape of the position-order SpMV (tune_pos_spmv)
     // optional split (Eisenstat) D-ILU operator Ahat = D~ (D~ + L)^-1 A (D~ + U)^-1 (split_op)
     DiluFactors<T>* split = nullptr;
-    bool xcd = false;      // XCD workgroup mapping of the split operator's sweeps
+    int mode = 0;          // split_op mode of the split operator's sweeps (XCD mapping, NT loads)
     int prof_class = -1;   // profile class of its applications (-1: the SpMV class)
 };
 
 template <typename T>
 inline void op_spmv(hipStream_t s, const CsrOp<T>& A, const T* x, T* y) {
     if (A.split)
-        split_op<T>(0, s, *A.co, A.F, A.L, *A.split, x, y, A.xcd);
+        split_op<T>(0, s, *A.co, A.F, A.L, *A.split, x, y, A.mode);
     else if (A.pos)
         pos_spmv_variant<T>(s, A.variant, *A.co, A.F, A.L, *A.pos, x, y);
     else if (A.neigh)
@@ -119,7 +119,15 @@ struct BicgWork {
     int64_t n = 0;
     DevBuf<T> r, rh, p, v, s, t, ph, sh;
     DevBuf<double> partial, dsc;
-    HostBuf<double> hsc;
+    HostBuf<double> hsc;  // two slots of the scalars: iterations k and k+1 in flight
+    hipEvent_t ev[2] = {};
+    BicgWork() = default;
+    BicgWork(const BicgWork&) = delete;
+    BicgWork& operator=(const BicgWork&) = delete;
+    ~BicgWork() {
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
     void ensure(int64_t n_);
 };
 

@@ -392,7 +392,7 @@ template <typename T>
 __global__ void __launch_bounds__(kBlock) k_bicg_p_dev(int64_t n, const T* __restrict__ r, const T* __restrict__ v,
                                                        const double* __restrict__ sc, T* __restrict__ p) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
+    if (k >= n || sc[SC_STOP] != 0.0) return;
     const double omega = sc[SC_OMEGA];
     const double beta = (sc[SC_RHO] / sc[SC_RHO_PREV]) * (sc[SC_ALPHA] / omega);
     p[k] = (T)fma(beta, fma(-omega, (double)v[k], (double)p[k]), (double)r[k]);
@@ -404,6 +404,7 @@ __global__ void __launch_bounds__(kBlock) k_bicg_s_dev(int64_t n, const T* __res
                                                        const double* __restrict__ sc, T* __restrict__ s_out,
                                                        double* __restrict__ partial) {
     __shared__ double s_red[kBlock / 64];
+    if (sc[SC_STOP] != 0.0) return;  // uniform over the grid
     const double alpha = sc[SC_ALPHA];
     double ss = 0;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -424,6 +425,7 @@ __global__ void __launch_bounds__(kBlock) k_bicg_xr_dev(int64_t n, const T* __re
                                                         T* __restrict__ x, T* __restrict__ r,
                                                         double* __restrict__ partial) {
     __shared__ double s_red[kBlock / 64];
+    if (sc[SC_STOP] != 0.0) return;  // uniform over the grid
     const double alpha = sc[SC_ALPHA], omega = sc[SC_OMEGA];
     double d1 = 0, d2 = 0;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -447,8 +449,9 @@ __global__ void __launch_bounds__(kBlock) k_bicg_xr_dev(int64_t n, const T* __re
 constexpr int kStageThreads = 1024;
 __global__ void __launch_bounds__(kStageThreads) k_bicg_stage(const double* __restrict__ partial, int64_t nblk,
                                                               int stride, int ncols, int stage,
-                                                              double* __restrict__ sc) {
+                                                              double* __restrict__ sc, double stop_abs) {
     __shared__ double s_red[kStageThreads / 64];
+    if (stage != 0 && sc[SC_STOP] != 0.0) return;  // uniform over the block
     double sum[2] = {0, 0};
     for (int c = 0; c < ncols; ++c) {
         double a = 0;
@@ -465,6 +468,7 @@ __global__ void __launch_bounds__(kStageThreads) k_bicg_stage(const double* __re
             sc[SC_ALPHA] = 1;
             sc[SC_OMEGA] = 1;
             sc[SC_FLAG] = 0;
+            sc[SC_STOP] = 0;
             break;
         case 1:  // a = rh.v -> alpha
             sc[SC_RHV] = a;
@@ -495,12 +499,15 @@ __global__ void __launch_bounds__(kStageThreads) k_bicg_stage(const double* __re
             }
             if (sc[SC_OMEGA] == 0.0 && sc[SC_FLAG] == 0) sc[SC_FLAG] = 2;
             break;
-        default:  // a = rh.r, b = r.r of the new residual
+        default: {  // a = rh.r, b = r.r of the new residual
             sc[SC_RHO_PREV] = sc[SC_RHO];
             sc[SC_RHO] = a;
             sc[SC_RR] = b;
             if (!isfinite(b)) sc[SC_FLAG] = 3;
+            const double rr = sqrt(b);
+            if (sc[SC_FLAG] != 0.0 || rr <= stop_abs || !isfinite(rr)) sc[SC_STOP] = 1;
             break;
+        }
     }
 }
 
@@ -645,8 +652,10 @@ void bicg_xr_dev(hipStream_t s, int64_t n, const T* ph, const T* sh, const T* sv
                  const double* sc, T* x, T* r, double* partial, int nblk) {
     hipLaunchKernelGGL((k_bicg_xr_dev<T>), dim3(nblk), dim3(kBlock), 0, s, n, ph, sh, sv, tv, rh, sc, x, r, partial);
 }
-void bicg_stage(hipStream_t s, const double* partial, int64_t nblk, int stride, int ncols, int stage, double* sc) {
-    hipLaunchKernelGGL(k_bicg_stage, dim3(1), dim3(kStageThreads), 0, s, partial, nblk, stride, ncols, stage, sc);
+void bicg_stage(hipStream_t s, const double* partial, int64_t nblk, int stride, int ncols, int stage, double* sc,
+                double stop_abs) {
+    hipLaunchKernelGGL(k_bicg_stage, dim3(1), dim3(kStageThreads), 0, s, partial, nblk, stride, ncols, stage, sc,
+                       stop_abs);
 }
 
 template <typename T>

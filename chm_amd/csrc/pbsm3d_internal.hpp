@@ -91,6 +91,7 @@ struct ColumnFactors {
 // HIP-event bracketing of kernel launches on one stream (off by default).
 struct Profiler {
     bool on = false;
+    uint32_t mask = ~0u;  // bit k: class k is bracketed (each pair of events costs a few us)
     std::vector<hipEvent_t> pool;
     std::vector<int> ids;
     size_t used = 0;
@@ -99,8 +100,8 @@ struct Profiler {
     ~Profiler() {
         for (auto e : pool) (void)hipEventDestroy(e);
     }
-    void begin(hipStream_t s, int id) {
-        if (!on) return;
+    bool begin(hipStream_t s, int id) {
+        if (!on || !((mask >> id) & 1u)) return false;
         while (pool.size() < used + 2) {
             hipEvent_t e;
             PB_HIP(hipEventCreate(&e));
@@ -108,6 +109,7 @@ struct Profiler {
         }
         PB_HIP(hipEventRecord(pool[used], s));
         ids.push_back(id);
+        return true;
     }
     void end(hipStream_t s) {
         if (!on) return;
@@ -132,11 +134,12 @@ struct Profiler {
 struct ProfScope {
     Profiler* p;
     hipStream_t s;
+    bool active = false;
     ProfScope(Profiler* p_, hipStream_t s_, int id) : p(p_), s(s_) {
-        if (p) p->begin(s, id);
+        if (p) active = p->begin(s, id);
     }
     ~ProfScope() noexcept(false) {
-        if (p) p->end(s);
+        if (active) p->end(s);
     }
 };
 

@@ -26,6 +26,16 @@ namespace {
 
 __device__ __forceinline__ int nib(uint32_t slot, int k) { return (slot >> (4 * k)) & 0xF; }
 
+// coefficient load; NT: with the non-temporal hint (read once per launch, so they need
+// not displace the neighbour values the gathers reuse from L2)
+template <bool NT, typename T>
+__device__ __forceinline__ T ldc(const T* p) {
+    if constexpr (NT)
+        return __builtin_nontemporal_load(p);
+    else
+        return *p;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_dilu_factor(int64_t F, int L, int64_t k0, int64_t k1,
                                                         const int32_t* __restrict__ faces,
@@ -251,13 +261,14 @@ __global__ void __launch_bounds__(kBlock) k_dilu_bwd(int64_t F, int L, int64_t k
 
 // t = D~^-1 (w - U t) for one colour; tnb reads t at the (higher-colour)
 // neighbour positions, t is written at the colour's own positions
-template <typename T, int LM, bool XCD>
+template <typename T, int LM, bool XCD, bool NT>
 __global__ void __launch_bounds__(kBlock) k_eis_bwd(int64_t F, int L, int64_t k0, int64_t k1,
                                                     const int32_t* __restrict__ nbp,
                                                     const uint8_t* __restrict__ lowmask, const T* __restrict__ lat,
                                                     const T* __restrict__ vert, const T* __restrict__ dinv,
                                                     const T* __restrict__ w, const T* __restrict__ tnb,
-                                                    T* __restrict__ t) {
+                                                    T* __restrict__ t, const double* __restrict__ stop) {
+    if (stop && *stop != 0.0) return;  // a finished BiCGStab run (uniform over the grid)
     const int64_t k = k0 + logical_block<XCD>((k1 - k0 + kBlock - 1) / kBlock) * kBlock + threadIdx.x;
     if (k >= k1) return;
     const int64_t N = F * L;
@@ -276,10 +287,10 @@ __global__ void __launch_bounds__(kBlock) k_eis_bwd(int64_t F, int L, int64_t k0
             T acc = 0;
 #pragma unroll
             for (int f = 0; f < 3; ++f)
-                if (np_[f] >= 0) acc += lat[f * N + q] * tnb[(int64_t)z * F + np_[f]];
+                if (np_[f] >= 0) acc += ldc<NT>(&lat[f * N + q]) * tnb[(int64_t)z * F + np_[f]];
             av[z] = acc;
-            va[z] = z < L - 1 ? vert[N + q] : T(0);
-            dv[z] = dinv[q];
+            va[z] = z < L - 1 ? ldc<NT>(&vert[N + q]) : T(0);
+            dv[z] = ldc<NT>(&dinv[q]);
             wv[z] = w[q];
         }
     T next = 0;
@@ -296,7 +307,7 @@ __global__ void __launch_bounds__(kBlock) k_eis_bwd(int64_t F, int L, int64_t k0
 // o = w - acc, v = D~^-1 o (without t: the left transform b^ = D~ (D~ + L)^-1 b).
 // FUSE: colour 0 (no lower neighbours), whose backward sweep runs first in the
 // same thread and whose t stays in registers.
-template <typename T, int LM, bool WITH_T, bool FUSE, bool DOTS>
+template <typename T, int LM, bool WITH_T, bool FUSE, bool DOTS, bool NT>
 __device__ __forceinline__ void eis_fwd_column(int64_t F, int L, int64_t k, const int32_t* __restrict__ nbp,
                                                const uint8_t* __restrict__ lowmask, const T* __restrict__ lat,
                                                const T* __restrict__ vert, const T* __restrict__ dinv,
@@ -320,17 +331,17 @@ __device__ __forceinline__ void eis_fwd_column(int64_t F, int L, int64_t k, cons
         if (z < L) {
             const int64_t q = (int64_t)z * F + k;
             wv[z] = w[q];
-            dv[z] = dinv[q];
-            vb[z] = z > 0 ? vert[q] : T(0);
+            dv[z] = ldc<NT>(&dinv[q]);
+            vb[z] = z > 0 ? ldc<NT>(&vert[q]) : T(0);
             T acc = 0;
             if (!FUSE) {
 #pragma unroll
                 for (int f = 0; f < 3; ++f)
-                    if (lo[f] >= 0) acc += lat[f * N + q] * vnb[(int64_t)z * F + lo[f]];
+                    if (lo[f] >= 0) acc += ldc<NT>(&lat[f * N + q]) * vnb[(int64_t)z * F + lo[f]];
             }
             lv[z] = acc;
             if (WITH_T) {
-                ev[z] = E[q];
+                ev[z] = ldc<NT>(&E[q]);
                 if (!FUSE) tv[z] = t[q];
             }
         }
@@ -344,9 +355,9 @@ __device__ __forceinline__ void eis_fwd_column(int64_t F, int L, int64_t k, cons
                 T acc = 0;
 #pragma unroll
                 for (int f = 0; f < 3; ++f)
-                    if (up[f] >= 0) acc += lat[f * N + q] * tnb[(int64_t)z * F + up[f]];
+                    if (up[f] >= 0) acc += ldc<NT>(&lat[f * N + q]) * tnb[(int64_t)z * F + up[f]];
                 av[z] = acc;
-                va[z] = z < L - 1 ? vert[N + q] : T(0);
+                va[z] = z < L - 1 ? ldc<NT>(&vert[N + q]) : T(0);
             }
         T next = 0;
 #pragma unroll
@@ -381,7 +392,7 @@ __device__ __forceinline__ void eis_fwd_column(int64_t F, int L, int64_t k, cons
 }
 
 // DOTS: per-block partial sums (dw . o, o . o) into partial[2 * blockIdx.x] (FP64).
-template <typename T, int LM, bool XCD, bool WITH_T, bool FUSE, bool DOTS>
+template <typename T, int LM, bool XCD, bool WITH_T, bool FUSE, bool DOTS, bool NT>
 __global__ void __launch_bounds__(kBlock) k_eis_fwd(int64_t F, int L, int64_t k0, int64_t k1,
                                                     const int32_t* __restrict__ nbp,
                                                     const uint8_t* __restrict__ lowmask, const T* __restrict__ lat,
@@ -390,10 +401,11 @@ __global__ void __launch_bounds__(kBlock) k_eis_fwd(int64_t F, int L, int64_t k0
                                                     const T* __restrict__ t, const T* __restrict__ tnb,
                                                     const T* __restrict__ vnb, T* __restrict__ v,
                                                     T* __restrict__ o, const T* __restrict__ dw,
-                                                    double* __restrict__ partial) {
+                                                    double* __restrict__ partial, const double* __restrict__ stop) {
+    if (stop && *stop != 0.0) return;  // a finished BiCGStab run (uniform over the grid)
     const int64_t k = k0 + logical_block<XCD>((k1 - k0 + kBlock - 1) / kBlock) * kBlock + threadIdx.x;
     double d1 = 0, d2 = 0;
-    if (k < k1) eis_fwd_column<T, LM, WITH_T, FUSE, DOTS>(F, L, k, nbp, lowmask, lat, vert, dinv, E, w, t, tnb, vnb,
+    if (k < k1) eis_fwd_column<T, LM, WITH_T, FUSE, DOTS, NT>(F, L, k, nbp, lowmask, lat, vert, dinv, E, w, t, tnb, vnb,
                                                            v, o, dw, d1, d2);
     if constexpr (DOTS) {
         __shared__ double s_red[kBlock / 64];
@@ -781,75 +793,75 @@ bool tune_dilu_xcd(hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFa
 namespace {
 // one forward launch of colour c; with dw the block partials of (dw . o, o . o) go to
 // partial[2 * (nb + block)] and nb advances by the launch's block count
-template <typename T, int LM, bool XCD, bool WITH_T, bool FUSE>
+template <typename T, int LM, bool XCD, bool NT, bool WITH_T, bool FUSE>
 void launch_fwd(hipStream_t s, dim3 g, int64_t F, int L, const ColorOrder& co, int c, DiluFactors<T>& d, const T* w,
-                T* o, const T* dw, double* partial, int64_t& nb) {
+                T* o, const T* dw, double* partial, int64_t& nb, const double* stop) {
     if (dw) {
-        hipLaunchKernelGGL((k_eis_fwd<T, LM, XCD, WITH_T, FUSE, true>), g, dim3(kBlock), 0, s, F, L, co.off[c],
+        hipLaunchKernelGGL((k_eis_fwd<T, LM, XCD, WITH_T, FUSE, true, NT>), g, dim3(kBlock), 0, s, F, L, co.off[c],
                            co.off[c + 1], co.nbp.p, co.lowmask.p, d.lat.p, d.vert.p, d.dinv.p, d.E.p, w, d.t.p,
-                           d.t.p, d.y.p, d.y.p, o, dw, partial + 2 * nb);
+                           d.t.p, d.y.p, d.y.p, o, dw, partial + 2 * nb, stop);
         nb += g.x;
     } else {
-        hipLaunchKernelGGL((k_eis_fwd<T, LM, XCD, WITH_T, FUSE, false>), g, dim3(kBlock), 0, s, F, L, co.off[c],
+        hipLaunchKernelGGL((k_eis_fwd<T, LM, XCD, WITH_T, FUSE, false, NT>), g, dim3(kBlock), 0, s, F, L, co.off[c],
                            co.off[c + 1], co.nbp.p, co.lowmask.p, d.lat.p, d.vert.p, d.dinv.p, d.E.p, w, d.t.p,
-                           d.t.p, d.y.p, d.y.p, o, dw, partial);
+                           d.t.p, d.y.p, d.y.p, o, dw, partial, stop);
     }
 }
 
-template <typename T, int LM, bool XCD>
+template <typename T, int LM, bool XCD, bool NT>
 int64_t split_apply_t(hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFactors<T>& d, const T* w, T* o,
-                      const T* dw, double* partial) {
+                      const T* dw, double* partial, const double* stop) {
     int64_t nb = 0;
     const int top = co.ncolor - 1;
     auto blocks = [&](int c) { return dim3((unsigned)launch_blocks<XCD>((co.off[c + 1] - co.off[c] + kBlock - 1) / kBlock)); };
     // backward sweep, colours top .. 1: t
     for (int c = top; c >= 1; --c) {
         if (co.off[c + 1] == co.off[c]) continue;
-        hipLaunchKernelGGL((k_eis_bwd<T, LM, XCD>), blocks(c), dim3(kBlock), 0, s, F, L, co.off[c], co.off[c + 1],
-                           co.nbp.p, co.lowmask.p, d.lat.p, d.vert.p, d.dinv.p, w, d.t.p, d.t.p);
+        hipLaunchKernelGGL((k_eis_bwd<T, LM, XCD, NT>), blocks(c), dim3(kBlock), 0, s, F, L, co.off[c], co.off[c + 1],
+                           co.nbp.p, co.lowmask.p, d.lat.p, d.vert.p, d.dinv.p, w, d.t.p, d.t.p, stop);
     }
     // colour 0: backward and forward sweeps in one pass
     if (co.off[1] > co.off[0])
-        launch_fwd<T, LM, XCD, true, true>(s, blocks(0), F, L, co, 0, d, w, o, dw, partial, nb);
+        launch_fwd<T, LM, XCD, NT, true, true>(s, blocks(0), F, L, co, 0, d, w, o, dw, partial, nb, stop);
     // forward sweep, colours 1 .. top
     for (int c = 1; c <= top; ++c) {
         if (co.off[c + 1] == co.off[c]) continue;
-        launch_fwd<T, LM, XCD, true, false>(s, blocks(c), F, L, co, c, d, w, o, dw, partial, nb);
+        launch_fwd<T, LM, XCD, NT, true, false>(s, blocks(c), F, L, co, c, d, w, o, dw, partial, nb, stop);
     }
     return nb;
 }
-template <typename T, int LM, bool XCD>
+template <typename T, int LM, bool XCD, bool NT>
 void split_lsolve_t(hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFactors<T>& d, const T* b, T* bhat) {
     for (int c = 0; c < co.ncolor; ++c) {
         if (co.off[c + 1] == co.off[c]) continue;
         const dim3 g((unsigned)launch_blocks<XCD>((co.off[c + 1] - co.off[c] + kBlock - 1) / kBlock));
-        hipLaunchKernelGGL((k_eis_fwd<T, LM, XCD, false, false, false>), g, dim3(kBlock), 0, s, F, L, co.off[c],
+        hipLaunchKernelGGL((k_eis_fwd<T, LM, XCD, false, false, false, NT>), g, dim3(kBlock), 0, s, F, L, co.off[c],
                            co.off[c + 1], co.nbp.p, co.lowmask.p, d.lat.p, d.vert.p, d.dinv.p, d.E.p, b, d.t.p,
-                           d.t.p, d.y.p, d.y.p, bhat, (const T*)nullptr, (double*)nullptr);
+                           d.t.p, d.y.p, d.y.p, bhat, (const T*)nullptr, (double*)nullptr, (const double*)nullptr);
     }
 }
-template <typename T, int LM, bool XCD>
+template <typename T, int LM, bool XCD, bool NT>
 void split_rsolve_t(hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFactors<T>& d, const T* xhat, T* x) {
     for (int c = co.ncolor - 1; c >= 0; --c) {
         if (co.off[c + 1] == co.off[c]) continue;
         const dim3 g((unsigned)launch_blocks<XCD>((co.off[c + 1] - co.off[c] + kBlock - 1) / kBlock));
-        hipLaunchKernelGGL((k_eis_bwd<T, LM, XCD>), g, dim3(kBlock), 0, s, F, L, co.off[c], co.off[c + 1],
-                           co.nbp.p, co.lowmask.p, d.lat.p, d.vert.p, d.dinv.p, xhat, x, x);
+        hipLaunchKernelGGL((k_eis_bwd<T, LM, XCD, NT>), g, dim3(kBlock), 0, s, F, L, co.off[c], co.off[c + 1],
+                           co.nbp.p, co.lowmask.p, d.lat.p, d.vert.p, d.dinv.p, xhat, x, x, (const double*)nullptr);
     }
 }
 // which: 0 apply, 1 left transform, 2 right transform
-template <typename T, bool XCD>
+template <typename T, bool XCD, bool NT>
 int64_t split_dispatch(int which, hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFactors<T>& d,
-                       const T* in, T* out, const T* dw, double* partial) {
+                       const T* in, T* out, const T* dw, double* partial, const double* stop) {
     int64_t nb = 0;
 #define PB_SPLIT(LMV)                                                           \
     {                                                                           \
         if (which == 0)                                                         \
-            nb = split_apply_t<T, LMV, XCD>(s, co, F, L, d, in, out, dw, partial); \
+            nb = split_apply_t<T, LMV, XCD, NT>(s, co, F, L, d, in, out, dw, partial, stop); \
         else if (which == 1)                                                    \
-            split_lsolve_t<T, LMV, XCD>(s, co, F, L, d, in, out);               \
+            split_lsolve_t<T, LMV, XCD, NT>(s, co, F, L, d, in, out);           \
         else                                                                    \
-            split_rsolve_t<T, LMV, XCD>(s, co, F, L, d, in, out);               \
+            split_rsolve_t<T, LMV, XCD, NT>(s, co, F, L, d, in, out);           \
     }
     if (L <= 1)
         PB_SPLIT(1)
@@ -866,20 +878,58 @@ int64_t split_dispatch(int which, hipStream_t s, const ColorOrder& co, int64_t F
 }
 }  // namespace
 
+namespace {
+template <typename T>
+int64_t split_mode_dispatch(int which, hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFactors<T>& d,
+                            const T* in, T* out, const T* dw, double* partial, int mode, const double* stop) {
+    switch (mode & 3) {
+        case 0: return split_dispatch<T, false, false>(which, s, co, F, L, d, in, out, dw, partial, stop);
+        case 1: return split_dispatch<T, true, false>(which, s, co, F, L, d, in, out, dw, partial, stop);
+        case 2: return split_dispatch<T, false, true>(which, s, co, F, L, d, in, out, dw, partial, stop);
+        default: return split_dispatch<T, true, true>(which, s, co, F, L, d, in, out, dw, partial, stop);
+    }
+}
+}  // namespace
+
 template <typename T>
 void split_op(int which, hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFactors<T>& d, const T* in, T* out,
-              bool xcd) {
-    if (xcd)
-        split_dispatch<T, true>(which, s, co, F, L, d, in, out, nullptr, nullptr);
-    else
-        split_dispatch<T, false>(which, s, co, F, L, d, in, out, nullptr, nullptr);
+              int mode) {
+    split_mode_dispatch<T>(which, s, co, F, L, d, in, out, nullptr, nullptr, mode, nullptr);
 }
 
 template <typename T>
 int64_t split_apply_dots(hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFactors<T>& d, const T* in, T* out,
-                         const T* dw, double* partial, bool xcd) {
-    if (xcd) return split_dispatch<T, true>(0, s, co, F, L, d, in, out, dw, partial);
-    return split_dispatch<T, false>(0, s, co, F, L, d, in, out, dw, partial);
+                         const T* dw, double* partial, int mode, const double* stop) {
+    return split_mode_dispatch<T>(0, s, co, F, L, d, in, out, dw, partial, mode, stop);
+}
+
+template <typename T>
+int tune_split_mode(hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFactors<T>& d, const T* w, T* o,
+                    float* ms_out) {
+    hipEvent_t e0, e1;
+    PB_HIP(hipEventCreate(&e0));
+    PB_HIP(hipEventCreate(&e1));
+    int best = 0;
+    float best_ms = 0;
+    for (int m = 0; m < kSplitModes; ++m) {
+        float ms = 0;
+        split_op<T>(0, s, co, F, L, d, w, o, m);
+        PB_HIP(hipEventRecord(e0, s));
+        for (int k = 0; k < 3; ++k) split_op<T>(0, s, co, F, L, d, w, o, m);
+        PB_HIP(hipEventRecord(e1, s));
+        PB_HIP(hipEventSynchronize(e1));
+        PB_HIP(hipEventElapsedTime(&ms, e0, e1));
+        ms /= 3;
+        if (ms_out) ms_out[m] = ms;
+        if (m == 0 || ms < best_ms) {
+            best = m;
+            best_ms = ms;
+        }
+    }
+    PB_HIP(hipGetLastError());
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return best;
 }
 
 void dilu_narrow(hipStream_t s, int64_t N, const DiluFactors<double>& a, DiluFactors<float>& b) {
@@ -1008,13 +1058,17 @@ template void dilu_apply<double>(hipStream_t, const ColorOrder&, int64_t, int, D
 template void dilu_apply<float>(hipStream_t, const ColorOrder&, int64_t, int, DiluFactors<float>&, const float*,
                                 float*, bool, bool);
 template void split_op<double>(int, hipStream_t, const ColorOrder&, int64_t, int, DiluFactors<double>&, const double*,
-                               double*, bool);
+                               double*, int);
 template void split_op<float>(int, hipStream_t, const ColorOrder&, int64_t, int, DiluFactors<float>&, const float*,
-                              float*, bool);
+                              float*, int);
 template int64_t split_apply_dots<double>(hipStream_t, const ColorOrder&, int64_t, int, DiluFactors<double>&,
-                                          const double*, double*, const double*, double*, bool);
+                                          const double*, double*, const double*, double*, int, const double*);
 template int64_t split_apply_dots<float>(hipStream_t, const ColorOrder&, int64_t, int, DiluFactors<float>&,
-                                         const float*, float*, const float*, double*, bool);
+                                         const float*, float*, const float*, double*, int, const double*);
+template int tune_split_mode<double>(hipStream_t, const ColorOrder&, int64_t, int, DiluFactors<double>&,
+                                     const double*, double*, float*);
+template int tune_split_mode<float>(hipStream_t, const ColorOrder&, int64_t, int, DiluFactors<float>&, const float*,
+                                    float*, float*);
 template bool tune_dilu_xcd<double>(hipStream_t, const ColorOrder&, int64_t, int, DiluFactors<double>&, const double*,
                                     double*, float*);
 template bool tune_dilu_xcd<float>(hipStream_t, const ColorOrder&, int64_t, int, DiluFactors<float>&, const float*,
@@ -1036,6 +1090,7 @@ template int tune_pos_spmv<float>(hipStream_t, const ColorOrder&, int64_t, int, 
 template void pos_spmv<float>(hipStream_t, const ColorOrder&, int64_t, int, const DiluFactors<float>&, const float*,
                               const float*, float*, double*);
 template void to_pos<double, double>(hipStream_t, const ColorOrder&, int64_t, int, const double*, double*);
+template void to_pos<double, float>(hipStream_t, const ColorOrder&, int64_t, int, const double*, float*);
 template void from_pos<double, double>(hipStream_t, const ColorOrder&, int64_t, int, const double*, double*);
 
 }  // namespace pb

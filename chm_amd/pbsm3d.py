@@ -29,6 +29,7 @@ EXPORTED = (
     "pbsm3d_last_error",
     "pbsm3d_profile_enable",
     "pbsm3d_profile_read",
+    "pbsm3d_profile_classes",
     "pbsm3d_spmv_tuning",
     "pbsm3d_spmv_select",
     "pbsm3d_face_colors",
@@ -65,6 +66,7 @@ N_FEATURES, N_TARGETS = 16, 11
 # 256 / 128), 1 / 2 / 3 one thread per face column (blocks of 256 / 128 / 64); v >= 5 with the
 # XCD workgroup mapping
 SPMV_VARIANTS = 10
+PRECOND_MODES = 4
 
 KERNEL_CLASSES = ("assemble", "spmv64", "spmv32", "precond", "ortho", "residual", "deposition", "warmstart",
                   "factor")
@@ -111,6 +113,7 @@ def lib():
         L.pbsm3d_last_error.restype = C.c_char_p
         L.pbsm3d_profile_enable.argtypes = [vp, C.c_int]
         L.pbsm3d_profile_read.argtypes = [vp, C.c_int, C.POINTER(C.c_int64), dp, C.c_int]
+        L.pbsm3d_profile_classes.argtypes = [vp, C.c_uint32]
         L.pbsm3d_spmv_tuning.argtypes = [vp, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_float), C.c_int32]
         L.pbsm3d_spmv_select.argtypes = [vp, C.c_int, C.c_int]
         L.pbsm3d_face_colors.argtypes = [C.c_int64, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
@@ -249,8 +252,12 @@ class Context:
                self._h)
         return rp, col, val, rhs
 
-    def profile(self, enable: bool = True):
-        """Turn HIP-event timing of the kernel classes on or off."""
+    def profile(self, enable: bool = True, classes=None):
+        """Turn HIP-event timing of the kernel classes on or off; classes: the names
+        (KERNEL_CLASSES) to bracket, None = all."""
+        mask = (1 << len(KERNEL_CLASSES)) - 1 if classes is None else sum(
+            1 << KERNEL_CLASSES.index(n) for n in classes)
+        _check(lib().pbsm3d_profile_classes(self._h, mask), self._h)
         _check(lib().pbsm3d_profile_enable(self._h, 1 if enable else 0), self._h)
 
     def profile_read(self, reset: bool = True) -> dict:
@@ -271,15 +278,17 @@ class Context:
         return {"chosen": int(chosen.value), "ms": [round(float(v), 5) for v in ms]}
 
     def precond_tuning(self, which: int = 0) -> dict:
-        """Whether the D-ILU sweeps use the XCD workgroup mapping (FP64 which=0 / FP32 1),
-        and the tuner's average apply times without / with it (ms)."""
-        xcd, ms = C.c_int32(-1), (C.c_float * 2)()
-        _check(lib().pbsm3d_precond_tuning(self._h, which, C.byref(xcd), ms), self._h)
-        return {"xcd": int(xcd.value), "ms": [round(float(v), 5) for v in ms]}
+        """The D-ILU sweep mode of the FP64 (which=0) / FP32 (1) solve -- bit 0 the XCD
+        workgroup mapping, bit 1 non-temporal factor loads (split operator only) -- and
+        the tuner's average apply time of each mode (ms, 0 = not tried)."""
+        mode, ms = C.c_int32(-1), (C.c_float * PRECOND_MODES)()
+        _check(lib().pbsm3d_precond_tuning(self._h, which, C.byref(mode), ms), self._h)
+        return {"mode": int(mode.value), "ms": [round(float(v), 5) for v in ms]}
 
-    def precond_select(self, which: int, xcd: int):
-        """Fix the D-ILU workgroup mapping (0 / 1) of the FP64 (0) / FP32 (1) solve; -1 re-tunes."""
-        _check(lib().pbsm3d_precond_select(self._h, which, xcd), self._h)
+    def precond_select(self, which: int, mode: int):
+        """Fix the D-ILU sweep mode of the FP64 (0) / FP32 (1) solve (0 .. 1, or 0 .. 3 with
+        the split operator); -1 re-tunes."""
+        _check(lib().pbsm3d_precond_select(self._h, which, mode), self._h)
 
     def spmv_select(self, which: int, variant: int):
         """Fix the SpMV launch shape of the FP64 (0) / FP32 (1) operator; -1 re-tunes."""

@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define PBSM3D_ABI_VERSION 4
+#define PBSM3D_ABI_VERSION 5
 
 /* return codes */
 #define PBSM3D_OK 0
@@ -228,6 +228,11 @@ int pbsm3d_abi_version(void);
 #define PBSM3D_K_COUNT 9
 int pbsm3d_profile_enable(pbsm3d_ctx* ctx, int enable);
 int pbsm3d_profile_read(pbsm3d_ctx* ctx, int kernel, int64_t* launches, double* ms_total, int reset);
+/* Restricts the bracketing to the classes whose bit (1 << PBSM3D_K_*) is set
+ * (default: all).  Each bracket adds two event records to the stream, a few
+ * microseconds of device idle time, so a timing run brackets only the classes
+ * it reports. */
+int pbsm3d_profile_classes(pbsm3d_ctx* ctx, uint32_t mask);
 
 /* Launch shape of the suspension SpMV (no counterpart in the reference).  With
  * the multicolour D-ILU preconditioner the suspension solve runs in the
@@ -243,12 +248,16 @@ int pbsm3d_spmv_tuning(const pbsm3d_ctx* ctx, int which, int32_t* chosen, float*
  * choice, or -1 to tune again at the next solve. */
 #define PBSM3D_SPMV_VARIANTS 10
 int pbsm3d_spmv_select(pbsm3d_ctx* ctx, int which, int variant);
-/* The same for the multicolour D-ILU sweeps: *xcd = 1 if the workgroup order
- * that keeps neighbouring faces on one XCD was faster at the first solve, 0 if
- * not, -1 until tuned; ms[2] = average apply time without / with it (ms).
- * pbsm3d_precond_select fixes it (0 / 1) or re-tunes (-1). */
-int pbsm3d_precond_tuning(const pbsm3d_ctx* ctx, int which, int32_t* xcd, float* ms);
-int pbsm3d_precond_select(pbsm3d_ctx* ctx, int which, int xcd);
+/* The same for the multicolour D-ILU sweeps.  *mode (-1 until tuned) is the
+ * sweep mode the first solve found fastest: bit 0 = the workgroup order that
+ * keeps neighbouring faces on one XCD, bit 1 = the factor arrays read with the
+ * non-temporal hint (split operator, PBSM3D_KRYLOV_BICGSTAB_SPLIT, only; the
+ * other solvers tune bit 0 alone).  ms[PBSM3D_PRECOND_MODES] = average apply
+ * time of each mode (ms; 0 if not tried).  pbsm3d_precond_select fixes the mode
+ * (0 .. 1, or 0 .. 3 with the split operator) or re-tunes (-1). */
+#define PBSM3D_PRECOND_MODES 4
+int pbsm3d_precond_tuning(const pbsm3d_ctx* ctx, int which, int32_t* mode, float* ms);
+int pbsm3d_precond_select(pbsm3d_ctx* ctx, int which, int mode);
 
 /* Face colours of the multicolour D-ILU preconditioner (host computation, no
  * GPU needed; no counterpart in the reference): neighbouring faces never share

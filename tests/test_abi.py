@@ -28,7 +28,7 @@ def test_library_exports_every_declared_symbol():
     assert len(names) >= 10
     for n in names:
         assert hasattr(L, n), n
-    assert L.pbsm3d_abi_version() == 4
+    assert L.pbsm3d_abi_version() == 5
     assert set(pbsm3d.EXPORTED) <= set(names)
 
 
@@ -81,3 +81,14 @@ def test_create_rejects_bad_config_without_gpu():
         with pytest.raises(pbsm3d.Pbsm3dError) as e:
             pbsm3d.Context(g, pbsm3d.default_config(**bad))
         assert e.value.code == abi.PBSM3D_EINVAL
+
+
+def test_context_calls_reject_null_context():
+    """Calls on a context validate the handle before touching it (no GPU needed)."""
+    from chm_amd import pbsm3d
+
+    L = pbsm3d.lib()
+    assert L.pbsm3d_profile_enable(None, 1) == abi.PBSM3D_EINVAL
+    assert L.pbsm3d_profile_classes(None, 1) == abi.PBSM3D_EINVAL
+    assert L.pbsm3d_precond_select(None, 0, 1) == abi.PBSM3D_EINVAL
+    assert L.pbsm3d_spmv_select(None, 0, 1) == abi.PBSM3D_EINVAL

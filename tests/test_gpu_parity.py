@@ -252,7 +252,8 @@ def test_every_spmv_launch_shape(pb, oracle, mesh_c1, precision):
     assert 0 <= tun["chosen"] < pb.SPMV_VARIANTS
     assert all(ms > 0 for ms in tun["ms"]), tun
     ptun = ctx.precond_tuning(which)
-    assert ptun["xcd"] in (0, 1) and all(ms > 0 for ms in ptun["ms"]), ptun
+    assert ptun["mode"] in (0, 1) and all(ms > 0 for ms in ptun["ms"][:2]), ptun
+    assert ptun["ms"][2:] == [0.0, 0.0]  # the non-temporal modes belong to the split operator
     for v in range(pb.SPMV_VARIANTS):
         ctx.spmv_select(which, v)
         ctx.precond_select(which, v % 2)  # both D-ILU workgroup mappings
@@ -263,6 +264,40 @@ def test_every_spmv_launch_shape(pb, oracle, mesh_c1, precision):
             assert rel_err(gg[k], og[k]) < 1e-8, (v, k, rel_err(gg[k], og[k]))
     with pytest.raises(pb.Pbsm3dError):
         ctx.spmv_select(which, pb.SPMV_VARIANTS)
+    with pytest.raises(pb.Pbsm3dError):
+        ctx.precond_select(which, 2)
+
+
+@pytest.mark.parametrize("precision", [abi.FP64, abi.MIXED])
+def test_every_split_sweep_mode(pb, oracle, mesh_c1, precision):
+    """The split operator's four sweep modes (XCD workgroup mapping, non-temporal factor
+    loads) change the launch, not the arithmetic of a column: each solves to the oracle."""
+    _print_case("every_split_sweep_mode", precision=precision)
+    F = mesh_c1.n_faces
+    fo = meshgen.forcing_c1(F, seed=0)
+    g, ctx, orc = _pair(pb, oracle, mesh_c1, L=10, precision=precision, tol=1e-12,
+                        krylov=abi.KRYLOV_BICGSTAB_SPLIT)
+    og, _ = orc.step(fo)
+    which = 0 if precision == abi.FP64 else 1
+    ctx.step(fo)
+    ptun = ctx.precond_tuning(which)
+    assert 0 <= ptun["mode"] < pb.PRECOND_MODES and all(ms > 0 for ms in ptun["ms"]), ptun
+    for m in range(pb.PRECOND_MODES):
+        ctx.precond_select(which, m)
+        ctx.set_state(np.zeros(F), np.zeros(F))
+        gg, st = ctx.step(fo, want_fields=True)
+        assert st["susp_resid"] <= 1e-12
+        for k in ("conc", "Qsusp", "qdep", "drift_mass"):
+            assert rel_err(gg[k], og[k]) < 1e-8, (m, k, rel_err(gg[k], og[k]))
+    with pytest.raises(pb.Pbsm3dError):
+        ctx.precond_select(which, pb.PRECOND_MODES)
+    # profiling restricted to one kernel class brackets that class only
+    ctx.profile(True, classes=("precond",))
+    ctx.step(fo)
+    prof = ctx.profile_read(reset=True)
+    ctx.profile(False)
+    assert prof["precond"][0] > 0 and prof["precond"][1] > 0, prof
+    assert all(n == 0 for k, (n, _) in prof.items() if k != "precond"), prof
 
 
 def test_c3_mixed_precision_96_steps(pb, oracle, mesh_c1):
