@@ -283,9 +283,9 @@ def main():
     ap.add_argument("--ensemble", type=int, default=8, help="parameter-ensemble members to time (0: skip)")
     ap.add_argument("--ensemble-steps", type=int, default=3)
     ap.add_argument("--host-io-steps", type=int, default=3, help="steps timed with host forcing / outputs (0: skip)")
-    ap.add_argument("--dilu-mode", type=int, default=-1, choices=[-1, 0, 1, 2, 3],
-                    help="pin the D-ILU sweep mode: bit 0 XCD workgroup mapping, bit 1 non-temporal factor "
-                         "loads (split operator only) (default: tuned)")
+    ap.add_argument("--dilu-mode", type=int, default=-1, choices=list(range(-1, 8)),
+                    help="pin the D-ILU sweep mode: bit 0 XCD workgroup mapping; split operator only: bit 1 "
+                         "non-temporal factor loads, bit 2 layer-parallel sweeps (default: tuned)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="run one streaming kernel (y = -x, float32) reading and writing this many MB first: "
                          "the calibration dispatch for rocprofv3 FETCH_SIZE / WRITE_SIZE")

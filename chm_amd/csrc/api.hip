@@ -773,7 +773,7 @@ int pbsm3d_precond_tuning(const pbsm3d_ctx* c, int which, int32_t* mode, float* 
 
 int pbsm3d_precond_select(pbsm3d_ctx* c, int which, int mode) {
     if (!c || (which != 0 && which != 1) || mode < -1 || mode >= kSplitModes) return PBSM3D_EINVAL;
-    if (mode > 1 && c->cfg.krylov != PBSM3D_KRYLOV_BICGSTAB_SPLIT) return PBSM3D_EINVAL;  // NT: split sweeps only
+    if (mode > 1 && c->cfg.krylov != PBSM3D_KRYLOV_BICGSTAB_SPLIT) return PBSM3D_EINVAL;  // split sweeps only
     c->dilu_mode[which] = mode;
     return PBSM3D_OK;
 }

@@ -200,7 +200,8 @@ bool tune_dilu_xcd(hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFa
 // which = 0: out = Ahat in = D~ (D~ + L)^-1 A (D~ + U)^-1 in (one backward + one forward sweep);
 // 1: out = D~ (D~ + L)^-1 in (left transform of a right-hand side);
 // 2: out = (D~ + U)^-1 in (right transform back to the unknowns).  in and out must differ.
-// mode: bit 0 the XCD workgroup mapping, bit 1 the factor arrays loaded with the non-temporal hint.
+// mode: bit 0 the XCD workgroup mapping, bit 1 the factor arrays loaded with the non-temporal hint,
+// bit 2 (which = 0) the layer-parallel sweeps (a workgroup per 64 face columns, a wave per layer).
 template <typename T>
 void split_op(int which, hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFactors<T>& d, const T* in, T* out,
               int mode);
@@ -210,7 +211,7 @@ void split_op(int which, hipStream_t s, const ColorOrder& co, int64_t F, int L, 
 template <typename T>
 int64_t split_apply_dots(hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFactors<T>& d, const T* in, T* out,
                          const T* dw, double* partial, int mode, const double* stop = nullptr);
-constexpr int kSplitModes = 4;
+constexpr int kSplitModes = 8;
 // times split_op(0) in each mode (HIP events on the stream) and returns the fastest;
 // ms_out[kSplitModes] receives the averages (ms), if given.
 template <typename T>

@@ -419,6 +419,177 @@ __global__ void __launch_bounds__(kBlock) k_eis_fwd(int64_t F, int L, int64_t k0
 }
 
 
+// ---------------------------------------------------------------------------
+// Layer-parallel form of the split operator's sweeps.  A workgroup is 64 face
+// columns x L layers, one wave per layer (blockDim = (64, L), L <= 16): every
+// row's loads and neighbour gathers run in parallel and each wave reads one
+// layer, 64 consecutive positions, so every access is coalesced.  Only the
+// vertical recurrence is sequential; it is affine in the neighbour value,
+//   backward  t_z = a_z - b_z t_{z+1},   a_z = d_z (w_z - U_lat t),  b_z = d_z u_z,
+//   forward   v_z = al_z - be_z v_{z-1}, al_z = d_z g_z - t_z,      be_z = d_z l_z,
+//             g_z = w_z + E_z t_z - L_lat v  (d = 1/d~, u / l the couplings above / below),
+// so wave 0 runs it from (a, b) or (al, be) staged in LDS, and every wave then
+// forms its own layer's result o_z = g_z - l_z v_{z-1}.  Short waves and many
+// of them: no tail of long-lived column threads, and few registers per lane.
+// ---------------------------------------------------------------------------
+constexpr int kLpCols = 64;
+
+template <typename T, bool XCD, bool NT>
+__global__ void __launch_bounds__(1024) k_eis_bwd_lp(int64_t F, int L, int64_t k0, int64_t k1,
+                                                     const int32_t* __restrict__ nbp,
+                                                     const uint8_t* __restrict__ lowmask,
+                                                     const T* __restrict__ lat, const T* __restrict__ vert,
+                                                     const T* __restrict__ dinv, const T* __restrict__ w,
+                                                     const T* __restrict__ tnb, T* __restrict__ t,
+                                                     const double* __restrict__ stop) {
+    if (stop && *stop != 0.0) return;  // a finished BiCGStab run (uniform over the grid)
+    extern __shared__ __align__(16) unsigned char lp_smem[];
+    T* sa = reinterpret_cast<T*>(lp_smem);
+    T* sb = sa + L * kLpCols;
+    const int lane = threadIdx.x, z = threadIdx.y;
+    const int64_t k = k0 + logical_block<XCD>((k1 - k0 + kLpCols - 1) / kLpCols) * kLpCols + lane;
+    const bool valid = k < k1;
+    const int64_t N = F * L;
+    if (valid) {
+        const uint32_t lm = lowmask[k];
+        const int64_t q = (int64_t)z * F + k;
+        T acc = 0;
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            const int32_t n = nbp[f * F + k];
+            if (n >= 0 && !((lm >> f) & 1u)) acc += ldc<NT>(&lat[f * N + q]) * tnb[(int64_t)z * F + n];
+        }
+        const T dv = ldc<NT>(&dinv[q]);
+        const T va = z < L - 1 ? ldc<NT>(&vert[N + q]) : T(0);
+        sa[z * kLpCols + lane] = dv * (w[q] - acc);
+        sb[z * kLpCols + lane] = dv * va;
+    }
+    __syncthreads();
+    if (z == 0 && valid) {
+        T next = 0;
+        for (int zz = L - 1; zz >= 0; --zz) {
+            next = sa[zz * kLpCols + lane] - sb[zz * kLpCols + lane] * next;
+            t[(int64_t)zz * F + k] = next;
+        }
+    }
+}
+
+// FUSE: colour 0 (no lower neighbours), both sweeps in one launch with t kept in
+// LDS.  DOTS: partial[2 * blockIdx.x] = (dw . o, o . o) over the workgroup (FP64).
+template <typename T, bool XCD, bool FUSE, bool DOTS, bool NT>
+__global__ void __launch_bounds__(1024) k_eis_fwd_lp(int64_t F, int L, int64_t k0, int64_t k1,
+                                                     const int32_t* __restrict__ nbp,
+                                                     const uint8_t* __restrict__ lowmask,
+                                                     const T* __restrict__ lat, const T* __restrict__ vert,
+                                                     const T* __restrict__ dinv, const T* __restrict__ E,
+                                                     const T* __restrict__ w, const T* __restrict__ t,
+                                                     const T* __restrict__ tnb, const T* __restrict__ vnb,
+                                                     T* __restrict__ v, T* __restrict__ o,
+                                                     const T* __restrict__ dw, double* __restrict__ partial,
+                                                     const double* __restrict__ stop) {
+    if (stop && *stop != 0.0) return;  // a finished BiCGStab run (uniform over the grid)
+    extern __shared__ __align__(16) unsigned char lp_smem[];
+    __shared__ double s_red[2 * 16];
+    const int LC = L * kLpCols;
+    T* sal = reinterpret_cast<T*>(lp_smem);
+    T* sbe = sal + LC;
+    T* sv = sbe + LC;
+    T* st = sv + LC;  // FUSE only: t, then (a, b) of the backward recurrence
+    T* sa = st + LC;
+    T* sb = sa + LC;
+    const int lane = threadIdx.x, z = threadIdx.y;
+    const int64_t k = k0 + logical_block<XCD>((k1 - k0 + kLpCols - 1) / kLpCols) * kLpCols + lane;
+    const bool valid = k < k1;
+    const int64_t N = F * L;
+    const int64_t q = (int64_t)z * F + k;
+    const int s = z * kLpCols + lane;
+    T wv = 0, dv = 0, lz = 0, ev = 0, g = 0;
+    if (valid) {
+        const uint32_t lm = lowmask[k];
+        wv = w[q];
+        dv = ldc<NT>(&dinv[q]);
+        lz = z > 0 ? ldc<NT>(&vert[q]) : T(0);
+        ev = ldc<NT>(&E[q]);
+        if constexpr (FUSE) {
+            T acc = 0;
+#pragma unroll
+            for (int f = 0; f < 3; ++f) {
+                const int32_t n = nbp[f * F + k];
+                if (n >= 0 && !((lm >> f) & 1u)) acc += ldc<NT>(&lat[f * N + q]) * tnb[(int64_t)z * F + n];
+            }
+            const T uz = z < L - 1 ? ldc<NT>(&vert[N + q]) : T(0);
+            sa[s] = dv * (wv - acc);
+            sb[s] = dv * uz;
+        } else {
+            T acc = 0;
+#pragma unroll
+            for (int f = 0; f < 3; ++f) {
+                const int32_t n = nbp[f * F + k];
+                if (n >= 0 && ((lm >> f) & 1u)) acc += ldc<NT>(&lat[f * N + q]) * vnb[(int64_t)z * F + n];
+            }
+            const T tv = t[q];
+            g = wv + ev * tv - acc;
+            sal[s] = dv * g - tv;
+            sbe[s] = dv * lz;
+        }
+    }
+    if constexpr (FUSE) {
+        __syncthreads();
+        if (z == 0 && valid) {
+            T next = 0;
+            for (int zz = L - 1; zz >= 0; --zz) {
+                next = sa[zz * kLpCols + lane] - sb[zz * kLpCols + lane] * next;
+                st[zz * kLpCols + lane] = next;
+            }
+        }
+        __syncthreads();
+        if (valid) {
+            const T tv = st[s];
+            g = wv + ev * tv;
+            sal[s] = dv * g - tv;
+            sbe[s] = dv * lz;
+        }
+    }
+    __syncthreads();
+    if (z == 0 && valid) {
+        T prev = 0;
+        for (int zz = 0; zz < L; ++zz) {
+            prev = sal[zz * kLpCols + lane] - sbe[zz * kLpCols + lane] * prev;
+            sv[zz * kLpCols + lane] = prev;
+        }
+    }
+    __syncthreads();
+    double d1 = 0, d2 = 0;
+    if (valid) {
+        const T vz = sv[s];
+        const T oz = g - lz * (z > 0 ? sv[s - kLpCols] : T(0));
+        v[q] = vz;
+        o[q] = oz;
+        if (DOTS) {
+            d1 = (double)dw[q] * (double)oz;
+            d2 = (double)oz * (double)oz;
+        }
+    }
+    if constexpr (DOTS) {
+        d1 = wave_sum(d1);
+        d2 = wave_sum(d2);
+        if (lane == 0) {
+            s_red[2 * z] = d1;
+            s_red[2 * z + 1] = d2;
+        }
+        __syncthreads();
+        if (z == 0 && lane == 0) {
+            double a = 0, b = 0;
+            for (int zz = 0; zz < L; ++zz) {
+                a += s_red[2 * zz];
+                b += s_red[2 * zz + 1];
+            }
+            partial[2 * blockIdx.x] = a;
+            partial[2 * blockIdx.x + 1] = b;
+        }
+    }
+}
+
 // y = A x, or r = b - A x with per-block partial sums of r.r, for the operator
 // held in position order by the factorisation: diag [N], lat [3N] (neighbour
 // j, zero where there is none), vert [2N] (below, above; zero at the ends of
@@ -879,9 +1050,59 @@ int64_t split_dispatch(int which, hipStream_t s, const ColorOrder& co, int64_t F
 }  // namespace
 
 namespace {
+// the split operator with the layer-parallel sweeps (L <= 16)
+template <typename T, bool XCD, bool NT>
+int64_t split_apply_lp(hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFactors<T>& d, const T* w, T* o,
+                       const T* dw, double* partial, const double* stop) {
+    if (L < 1 || L > 16) throw Error(PBSM3D_EINVAL, "the split D-ILU operator needs nLayer <= 16");
+    int64_t nb = 0;
+    const int top = co.ncolor - 1;
+    const dim3 blk(kLpCols, (unsigned)L);
+    const size_t lc = (size_t)L * kLpCols * sizeof(T);
+    auto blocks = [&](int c) {
+        return dim3((unsigned)launch_blocks<XCD>((co.off[c + 1] - co.off[c] + kLpCols - 1) / kLpCols));
+    };
+    for (int c = top; c >= 1; --c) {
+        if (co.off[c + 1] == co.off[c]) continue;
+        hipLaunchKernelGGL((k_eis_bwd_lp<T, XCD, NT>), blocks(c), blk, 2 * lc, s, F, L, co.off[c], co.off[c + 1],
+                           co.nbp.p, co.lowmask.p, d.lat.p, d.vert.p, d.dinv.p, w, d.t.p, d.t.p, stop);
+    }
+    for (int c = 0; c <= top; ++c) {
+        if (co.off[c + 1] == co.off[c]) continue;
+        const dim3 g = blocks(c);
+        double* part = dw ? partial + 2 * nb : nullptr;
+#define PB_LP_FWD(FUSE, DOTS)                                                                                      \
+    hipLaunchKernelGGL((k_eis_fwd_lp<T, XCD, FUSE, DOTS, NT>), g, blk, (FUSE ? 6 : 3) * lc, s, F, L, co.off[c],   \
+                       co.off[c + 1], co.nbp.p, co.lowmask.p, d.lat.p, d.vert.p, d.dinv.p, d.E.p, w, d.t.p, d.t.p, \
+                       d.y.p, d.y.p, o, dw, part, stop)
+        if (c == 0) {
+            if (dw)
+                PB_LP_FWD(true, true);
+            else
+                PB_LP_FWD(true, false);
+        } else {
+            if (dw)
+                PB_LP_FWD(false, true);
+            else
+                PB_LP_FWD(false, false);
+        }
+#undef PB_LP_FWD
+        if (dw) nb += g.x;
+    }
+    return nb;
+}
+
 template <typename T>
 int64_t split_mode_dispatch(int which, hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFactors<T>& d,
                             const T* in, T* out, const T* dw, double* partial, int mode, const double* stop) {
+    if (which == 0 && (mode & 4)) {  // layer-parallel sweeps (the transforms keep the column kernels)
+        switch (mode & 3) {
+            case 0: return split_apply_lp<T, false, false>(s, co, F, L, d, in, out, dw, partial, stop);
+            case 1: return split_apply_lp<T, true, false>(s, co, F, L, d, in, out, dw, partial, stop);
+            case 2: return split_apply_lp<T, false, true>(s, co, F, L, d, in, out, dw, partial, stop);
+            default: return split_apply_lp<T, true, true>(s, co, F, L, d, in, out, dw, partial, stop);
+        }
+    }
     switch (mode & 3) {
         case 0: return split_dispatch<T, false, false>(which, s, co, F, L, d, in, out, dw, partial, stop);
         case 1: return split_dispatch<T, true, false>(which, s, co, F, L, d, in, out, dw, partial, stop);

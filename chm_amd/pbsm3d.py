@@ -66,7 +66,7 @@ N_FEATURES, N_TARGETS = 16, 11
 # 256 / 128), 1 / 2 / 3 one thread per face column (blocks of 256 / 128 / 64); v >= 5 with the
 # XCD workgroup mapping
 SPMV_VARIANTS = 10
-PRECOND_MODES = 4
+PRECOND_MODES = 8
 
 KERNEL_CLASSES = ("assemble", "spmv64", "spmv32", "precond", "ortho", "residual", "deposition", "warmstart",
                   "factor")
@@ -279,14 +279,15 @@ class Context:
 
     def precond_tuning(self, which: int = 0) -> dict:
         """The D-ILU sweep mode of the FP64 (which=0) / FP32 (1) solve -- bit 0 the XCD
-        workgroup mapping, bit 1 non-temporal factor loads (split operator only) -- and
-        the tuner's average apply time of each mode (ms, 0 = not tried)."""
+        workgroup mapping; split operator only: bit 1 non-temporal factor loads, bit 2
+        layer-parallel sweeps -- and the tuner's average apply time of each mode (ms,
+        0 = not tried)."""
         mode, ms = C.c_int32(-1), (C.c_float * PRECOND_MODES)()
         _check(lib().pbsm3d_precond_tuning(self._h, which, C.byref(mode), ms), self._h)
         return {"mode": int(mode.value), "ms": [round(float(v), 5) for v in ms]}
 
     def precond_select(self, which: int, mode: int):
-        """Fix the D-ILU sweep mode of the FP64 (0) / FP32 (1) solve (0 .. 1, or 0 .. 3 with
+        """Fix the D-ILU sweep mode of the FP64 (0) / FP32 (1) solve (0 .. 1, or 0 .. 7 with
         the split operator); -1 re-tunes."""
         _check(lib().pbsm3d_precond_select(self._h, which, mode), self._h)
 

@@ -250,12 +250,15 @@ int pbsm3d_spmv_tuning(const pbsm3d_ctx* ctx, int which, int32_t* chosen, float*
 int pbsm3d_spmv_select(pbsm3d_ctx* ctx, int which, int variant);
 /* The same for the multicolour D-ILU sweeps.  *mode (-1 until tuned) is the
  * sweep mode the first solve found fastest: bit 0 = the workgroup order that
- * keeps neighbouring faces on one XCD, bit 1 = the factor arrays read with the
- * non-temporal hint (split operator, PBSM3D_KRYLOV_BICGSTAB_SPLIT, only; the
- * other solvers tune bit 0 alone).  ms[PBSM3D_PRECOND_MODES] = average apply
- * time of each mode (ms; 0 if not tried).  pbsm3d_precond_select fixes the mode
- * (0 .. 1, or 0 .. 3 with the split operator) or re-tunes (-1). */
-#define PBSM3D_PRECOND_MODES 4
+ * keeps neighbouring faces on one XCD; with the split operator
+ * (PBSM3D_KRYLOV_BICGSTAB_SPLIT) also bit 1 = the factor arrays read with the
+ * non-temporal hint and bit 2 = layer-parallel sweeps (a workgroup per 64 face
+ * columns, one wave per layer, the vertical recurrence from LDS) instead of one
+ * thread per column; the other solvers tune bit 0 alone.
+ * ms[PBSM3D_PRECOND_MODES] = average apply time of each mode (ms; 0 if not
+ * tried).  pbsm3d_precond_select fixes the mode (0 .. 1, or 0 .. 7 with the
+ * split operator) or re-tunes (-1). */
+#define PBSM3D_PRECOND_MODES 8
 int pbsm3d_precond_tuning(const pbsm3d_ctx* ctx, int which, int32_t* mode, float* ms);
 int pbsm3d_precond_select(pbsm3d_ctx* ctx, int which, int mode);
 
