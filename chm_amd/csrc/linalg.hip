@@ -9,6 +9,10 @@
 #include "kernels.hpp"
 
 namespace pb {
+
+// zero operand of the CSR column kernel's gathers for absent neighbours
+template <typename T>
+__device__ const T kZeroVal = T(0);
 namespace {
 
 constexpr int kRowsPerBlock = kBlock;  // one row per thread
@@ -93,7 +97,12 @@ __global__ void __launch_bounds__(kBlock) k_spmv_col(int64_t F, int L, const int
             if (z > 0) acc = fma(val[base - 1], xb, acc);
 #pragma unroll
             for (int f = 0; f < 3; ++f)
-                if (nb[f] >= 0) acc = fma(val[base + pos[f]], x[(int64_t)z * F + nb[f]], acc);
+            {
+                // unconditional loads (absent neighbours read a zero), so they issue together
+                const bool on = nb[f] >= 0;
+                acc = fma(*(on ? &val[base + pos[f]] : &kZeroVal<T>), *(on ? &x[(int64_t)z * F + nb[f]] : &kZeroVal<T>),
+                          acc);
+            }
             T xa = 0;
             if (z < L - 1) {
                 xa = x[r + F];

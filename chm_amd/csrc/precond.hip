@@ -28,6 +28,15 @@ __device__ __forceinline__ int nib(uint32_t slot, int k) { return (slot >> (4 * 
 
 // coefficient load; NT: with the non-temporal hint (read once per launch, so they need
 // not displace the neighbour values the gathers reuse from L2)
+// Operand of the gathers for an absent or masked neighbour: 0 * 0 is added instead of
+// branching round the load, so every load of a column is unconditional and the compiler
+// issues them together (a branch per neighbour put a full memory round trip per gather).
+__device__ const double kZeroOperand[2] = {0.0, 0.0};
+template <typename T>
+__device__ __forceinline__ const T* zero_ptr() {
+    return reinterpret_cast<const T*>(kZeroOperand);
+}
+
 template <bool NT, typename T>
 __device__ __forceinline__ T ldc(const T* p) {
     if constexpr (NT)
@@ -128,7 +137,10 @@ __global__ void __launch_bounds__(kBlock) k_dilu_fwd(int64_t F, int L, int64_t k
         T acc = rin[(int64_t)z * F + i];
 #pragma unroll
         for (int f = 0; f < 3; ++f)
-            if (np_[f] >= 0) acc -= lat[f * N + q] * ynb[(int64_t)z * F + np_[f]];
+        {
+            const bool on = np_[f] >= 0;
+            acc -= *(on ? &lat[f * N + q] : zero_ptr<T>()) * *(on ? &ynb[(int64_t)z * F + np_[f]] : zero_ptr<T>());
+        }
         return acc;
     };
     if constexpr (LM > 0) {
@@ -213,7 +225,10 @@ __global__ void __launch_bounds__(kBlock) k_dilu_bwd(int64_t F, int L, int64_t k
         T acc = 0;
 #pragma unroll
         for (int f = 0; f < 3; ++f)
-            if (np_[f] >= 0) acc += lat[f * N + q] * done[(int64_t)z * F + np_[f]];
+        {
+            const bool on = np_[f] >= 0;
+            acc += *(on ? &lat[f * N + q] : zero_ptr<T>()) * *(on ? &done[(int64_t)z * F + np_[f]] : zero_ptr<T>());
+        }
         return acc;
     };
     if constexpr (LM > 0) {
@@ -287,7 +302,11 @@ __global__ void __launch_bounds__(kBlock) k_eis_bwd(int64_t F, int L, int64_t k0
             T acc = 0;
 #pragma unroll
             for (int f = 0; f < 3; ++f)
-                if (np_[f] >= 0) acc += ldc<NT>(&lat[f * N + q]) * tnb[(int64_t)z * F + np_[f]];
+            {
+                const bool on = np_[f] >= 0;
+                acc += ldc<NT>(on ? &lat[f * N + q] : zero_ptr<T>()) *
+                       *(on ? &tnb[(int64_t)z * F + np_[f]] : zero_ptr<T>());
+            }
             av[z] = acc;
             va[z] = z < L - 1 ? ldc<NT>(&vert[N + q]) : T(0);
             dv[z] = ldc<NT>(&dinv[q]);
@@ -325,19 +344,24 @@ __device__ __forceinline__ void eis_fwd_column(int64_t F, int L, int64_t k, cons
         lo[f] = (n >= 0 && low) ? n : -1;
         up[f] = (n >= 0 && !low) ? n : -1;
     }
-    T wv[LM], dv[LM], vb[LM], lv[LM], tv[LM], ev[LM];
+    T wv[LM], dv[LM], vb[LM], lv[LM], tv[LM], ev[LM], dwv[LM];
 #pragma unroll
     for (int z = 0; z < LM; ++z)
         if (z < L) {
             const int64_t q = (int64_t)z * F + k;
             wv[z] = w[q];
+            if (DOTS) dwv[z] = dw[q];  // loaded with the rest of the column, not in the final loop
             dv[z] = ldc<NT>(&dinv[q]);
             vb[z] = z > 0 ? ldc<NT>(&vert[q]) : T(0);
             T acc = 0;
             if (!FUSE) {
 #pragma unroll
                 for (int f = 0; f < 3; ++f)
-                    if (lo[f] >= 0) acc += ldc<NT>(&lat[f * N + q]) * vnb[(int64_t)z * F + lo[f]];
+                {
+                    const bool on = lo[f] >= 0;
+                    acc += ldc<NT>(on ? &lat[f * N + q] : zero_ptr<T>()) *
+                           *(on ? &vnb[(int64_t)z * F + lo[f]] : zero_ptr<T>());
+                }
             }
             lv[z] = acc;
             if (WITH_T) {
@@ -355,7 +379,11 @@ __device__ __forceinline__ void eis_fwd_column(int64_t F, int L, int64_t k, cons
                 T acc = 0;
 #pragma unroll
                 for (int f = 0; f < 3; ++f)
-                    if (up[f] >= 0) acc += ldc<NT>(&lat[f * N + q]) * tnb[(int64_t)z * F + up[f]];
+                {
+                    const bool on = up[f] >= 0;
+                    acc += ldc<NT>(on ? &lat[f * N + q] : zero_ptr<T>()) *
+                           *(on ? &tnb[(int64_t)z * F + up[f]] : zero_ptr<T>());
+                }
                 av[z] = acc;
                 va[z] = z < L - 1 ? ldc<NT>(&vert[N + q]) : T(0);
             }
@@ -385,7 +413,7 @@ __device__ __forceinline__ void eis_fwd_column(int64_t F, int L, int64_t k, cons
             o[q] = oz;
             prev = vz;
             if (DOTS) {
-                d1 = fma((double)dw[q], (double)oz, d1);
+                d1 = fma((double)dwv[z], (double)oz, d1);
                 d2 = fma((double)oz, (double)oz, d2);
             }
         }
@@ -613,7 +641,8 @@ __global__ void __launch_bounds__(kBlock) k_pos_spmv(int64_t F, int L, const int
 #pragma unroll
         for (int f = 0; f < 3; ++f) {
             const int32_t n = nbp[f * F + k];
-            if (n >= 0) acc = fma(lat[f * N + q], x[lay + n], acc);
+            const bool on = n >= 0;
+            acc = fma(*(on ? &lat[f * N + q] : zero_ptr<T>()), *(on ? &x[lay + n] : zero_ptr<T>()), acc);
         }
         if (z < L - 1) acc = fma(vert[N + q], x[q + F], acc);
         if (RESID) {
@@ -675,7 +704,11 @@ __global__ void __launch_bounds__(B) k_pos_spmv_col(int64_t F, int L, const int3
                 if (z > 0) acc = fma(vert[q], xv[z > 0 ? z - 1 : 0], acc);
 #pragma unroll
                 for (int f = 0; f < 3; ++f)
-                    if (np_[f] >= 0) acc = fma(lat[f * N + q], x[(int64_t)z * F + np_[f]], acc);
+                {
+                    const bool on = np_[f] >= 0;
+                    acc = fma(*(on ? &lat[f * N + q] : zero_ptr<T>()), *(on ? &x[(int64_t)z * F + np_[f]] : zero_ptr<T>()),
+                              acc);
+                }
                 if (z < L - 1) acc = fma(vert[N + q], xv[z + 1 < LM ? z + 1 : z], acc);
                 y[q] = acc;
                 if (DOTS) {
@@ -706,7 +739,8 @@ __global__ void __launch_bounds__(B) k_pos_spmv_row(int64_t F, int L, const int3
 #pragma unroll
         for (int f = 0; f < 3; ++f) {
             const int32_t n = nbp[f * F + k];
-            if (n >= 0) acc = fma(lat[f * N + q], x[lay + n], acc);
+            const bool on = n >= 0;
+            acc = fma(*(on ? &lat[f * N + q] : zero_ptr<T>()), *(on ? &x[lay + n] : zero_ptr<T>()), acc);
         }
         if (z < L - 1) acc = fma(vert[N + q], x[q + F], acc);
         y[q] = acc;
