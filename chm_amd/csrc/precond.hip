@@ -276,14 +276,15 @@ __global__ void __launch_bounds__(kBlock) k_dilu_bwd(int64_t F, int L, int64_t k
 
 // t = D~^-1 (w - U t) for one colour; tnb reads t at the (higher-colour)
 // neighbour positions, t is written at the colour's own positions
-template <typename T, int LM, bool XCD, bool NT>
-__global__ void __launch_bounds__(kBlock) k_eis_bwd(int64_t F, int L, int64_t k0, int64_t k1,
+template <typename T, int LM, bool EX, bool XCD, bool NT>
+__global__ void __launch_bounds__(kBlock) k_eis_bwd(int64_t F, int Lrt, int64_t k0, int64_t k1,
                                                     const int32_t* __restrict__ nbp,
                                                     const uint8_t* __restrict__ lowmask, const T* __restrict__ lat,
                                                     const T* __restrict__ vert, const T* __restrict__ dinv,
                                                     const T* __restrict__ w, const T* __restrict__ tnb,
                                                     T* __restrict__ t, const double* __restrict__ stop) {
     if (stop && *stop != 0.0) return;  // a finished BiCGStab run (uniform over the grid)
+    const int L = EX ? LM : Lrt;  // EX: the layer count is the unroll bound, no guards
     const int64_t k = k0 + logical_block<XCD>((k1 - k0 + kBlock - 1) / kBlock) * kBlock + threadIdx.x;
     if (k >= k1) return;
     const int64_t N = F * L;
@@ -326,14 +327,15 @@ __global__ void __launch_bounds__(kBlock) k_eis_bwd(int64_t F, int L, int64_t k0
 // o = w - acc, v = D~^-1 o (without t: the left transform b^ = D~ (D~ + L)^-1 b).
 // FUSE: colour 0 (no lower neighbours), whose backward sweep runs first in the
 // same thread and whose t stays in registers.
-template <typename T, int LM, bool WITH_T, bool FUSE, bool DOTS, bool NT>
-__device__ __forceinline__ void eis_fwd_column(int64_t F, int L, int64_t k, const int32_t* __restrict__ nbp,
+template <typename T, int LM, bool EX, bool WITH_T, bool FUSE, bool DOTS, bool NT>
+__device__ __forceinline__ void eis_fwd_column(int64_t F, int Lrt, int64_t k, const int32_t* __restrict__ nbp,
                                                const uint8_t* __restrict__ lowmask, const T* __restrict__ lat,
                                                const T* __restrict__ vert, const T* __restrict__ dinv,
                                                const T* __restrict__ E, const T* __restrict__ w,
                                                const T* __restrict__ t, const T* __restrict__ tnb,
                                                const T* __restrict__ vnb, T* __restrict__ v, T* __restrict__ o,
                                                const T* __restrict__ dw, double& d1, double& d2) {
+    const int L = EX ? LM : Lrt;  // EX: the layer count is the unroll bound, no guards
     const int64_t N = F * L;
     const uint32_t lm = lowmask[k];
     int32_t lo[3], up[3];
@@ -420,7 +422,7 @@ __device__ __forceinline__ void eis_fwd_column(int64_t F, int L, int64_t k, cons
 }
 
 // DOTS: per-block partial sums (dw . o, o . o) into partial[2 * blockIdx.x] (FP64).
-template <typename T, int LM, bool XCD, bool WITH_T, bool FUSE, bool DOTS, bool NT>
+template <typename T, int LM, bool EX, bool XCD, bool WITH_T, bool FUSE, bool DOTS, bool NT>
 __global__ void __launch_bounds__(kBlock) k_eis_fwd(int64_t F, int L, int64_t k0, int64_t k1,
                                                     const int32_t* __restrict__ nbp,
                                                     const uint8_t* __restrict__ lowmask, const T* __restrict__ lat,
@@ -433,7 +435,7 @@ __global__ void __launch_bounds__(kBlock) k_eis_fwd(int64_t F, int L, int64_t k0
     if (stop && *stop != 0.0) return;  // a finished BiCGStab run (uniform over the grid)
     const int64_t k = k0 + logical_block<XCD>((k1 - k0 + kBlock - 1) / kBlock) * kBlock + threadIdx.x;
     double d1 = 0, d2 = 0;
-    if (k < k1) eis_fwd_column<T, LM, WITH_T, FUSE, DOTS, NT>(F, L, k, nbp, lowmask, lat, vert, dinv, E, w, t, tnb, vnb,
+    if (k < k1) eis_fwd_column<T, LM, EX, WITH_T, FUSE, DOTS, NT>(F, L, k, nbp, lowmask, lat, vert, dinv, E, w, t, tnb, vnb,
                                                            v, o, dw, d1, d2);
     if constexpr (DOTS) {
         __shared__ double s_red[kBlock / 64];
@@ -998,22 +1000,22 @@ bool tune_dilu_xcd(hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFa
 namespace {
 // one forward launch of colour c; with dw the block partials of (dw . o, o . o) go to
 // partial[2 * (nb + block)] and nb advances by the launch's block count
-template <typename T, int LM, bool XCD, bool NT, bool WITH_T, bool FUSE>
+template <typename T, int LM, bool EX, bool XCD, bool NT, bool WITH_T, bool FUSE>
 void launch_fwd(hipStream_t s, dim3 g, int64_t F, int L, const ColorOrder& co, int c, DiluFactors<T>& d, const T* w,
                 T* o, const T* dw, double* partial, int64_t& nb, const double* stop) {
     if (dw) {
-        hipLaunchKernelGGL((k_eis_fwd<T, LM, XCD, WITH_T, FUSE, true, NT>), g, dim3(kBlock), 0, s, F, L, co.off[c],
+        hipLaunchKernelGGL((k_eis_fwd<T, LM, EX, XCD, WITH_T, FUSE, true, NT>), g, dim3(kBlock), 0, s, F, L, co.off[c],
                            co.off[c + 1], co.nbp.p, co.lowmask.p, d.lat.p, d.vert.p, d.dinv.p, d.E.p, w, d.t.p,
                            d.t.p, d.y.p, d.y.p, o, dw, partial + 2 * nb, stop);
         nb += g.x;
     } else {
-        hipLaunchKernelGGL((k_eis_fwd<T, LM, XCD, WITH_T, FUSE, false, NT>), g, dim3(kBlock), 0, s, F, L, co.off[c],
+        hipLaunchKernelGGL((k_eis_fwd<T, LM, EX, XCD, WITH_T, FUSE, false, NT>), g, dim3(kBlock), 0, s, F, L, co.off[c],
                            co.off[c + 1], co.nbp.p, co.lowmask.p, d.lat.p, d.vert.p, d.dinv.p, d.E.p, w, d.t.p,
                            d.t.p, d.y.p, d.y.p, o, dw, partial, stop);
     }
 }
 
-template <typename T, int LM, bool XCD, bool NT>
+template <typename T, int LM, bool EX, bool XCD, bool NT>
 int64_t split_apply_t(hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFactors<T>& d, const T* w, T* o,
                       const T* dw, double* partial, const double* stop) {
     int64_t nb = 0;
@@ -1022,16 +1024,16 @@ int64_t split_apply_t(hipStream_t s, const ColorOrder& co, int64_t F, int L, Dil
     // backward sweep, colours top .. 1: t
     for (int c = top; c >= 1; --c) {
         if (co.off[c + 1] == co.off[c]) continue;
-        hipLaunchKernelGGL((k_eis_bwd<T, LM, XCD, NT>), blocks(c), dim3(kBlock), 0, s, F, L, co.off[c], co.off[c + 1],
+        hipLaunchKernelGGL((k_eis_bwd<T, LM, EX, XCD, NT>), blocks(c), dim3(kBlock), 0, s, F, L, co.off[c], co.off[c + 1],
                            co.nbp.p, co.lowmask.p, d.lat.p, d.vert.p, d.dinv.p, w, d.t.p, d.t.p, stop);
     }
     // colour 0: backward and forward sweeps in one pass
     if (co.off[1] > co.off[0])
-        launch_fwd<T, LM, XCD, NT, true, true>(s, blocks(0), F, L, co, 0, d, w, o, dw, partial, nb, stop);
+        launch_fwd<T, LM, EX, XCD, NT, true, true>(s, blocks(0), F, L, co, 0, d, w, o, dw, partial, nb, stop);
     // forward sweep, colours 1 .. top
     for (int c = 1; c <= top; ++c) {
         if (co.off[c + 1] == co.off[c]) continue;
-        launch_fwd<T, LM, XCD, NT, true, false>(s, blocks(c), F, L, co, c, d, w, o, dw, partial, nb, stop);
+        launch_fwd<T, LM, EX, XCD, NT, true, false>(s, blocks(c), F, L, co, c, d, w, o, dw, partial, nb, stop);
     }
     return nb;
 }
@@ -1040,7 +1042,7 @@ void split_lsolve_t(hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluF
     for (int c = 0; c < co.ncolor; ++c) {
         if (co.off[c + 1] == co.off[c]) continue;
         const dim3 g((unsigned)launch_blocks<XCD>((co.off[c + 1] - co.off[c] + kBlock - 1) / kBlock));
-        hipLaunchKernelGGL((k_eis_fwd<T, LM, XCD, false, false, false, NT>), g, dim3(kBlock), 0, s, F, L, co.off[c],
+        hipLaunchKernelGGL((k_eis_fwd<T, LM, false, XCD, false, false, false, NT>), g, dim3(kBlock), 0, s, F, L, co.off[c],
                            co.off[c + 1], co.nbp.p, co.lowmask.p, d.lat.p, d.vert.p, d.dinv.p, d.E.p, b, d.t.p,
                            d.t.p, d.y.p, d.y.p, bhat, (const T*)nullptr, (double*)nullptr, (const double*)nullptr);
     }
@@ -1050,7 +1052,7 @@ void split_rsolve_t(hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluF
     for (int c = co.ncolor - 1; c >= 0; --c) {
         if (co.off[c + 1] == co.off[c]) continue;
         const dim3 g((unsigned)launch_blocks<XCD>((co.off[c + 1] - co.off[c] + kBlock - 1) / kBlock));
-        hipLaunchKernelGGL((k_eis_bwd<T, LM, XCD, NT>), g, dim3(kBlock), 0, s, F, L, co.off[c], co.off[c + 1],
+        hipLaunchKernelGGL((k_eis_bwd<T, LM, false, XCD, NT>), g, dim3(kBlock), 0, s, F, L, co.off[c], co.off[c + 1],
                            co.nbp.p, co.lowmask.p, d.lat.p, d.vert.p, d.dinv.p, xhat, x, x, (const double*)nullptr);
     }
 }
@@ -1062,7 +1064,8 @@ int64_t split_dispatch(int which, hipStream_t s, const ColorOrder& co, int64_t F
 #define PB_SPLIT(LMV)                                                           \
     {                                                                           \
         if (which == 0)                                                         \
-            nb = split_apply_t<T, LMV, XCD, NT>(s, co, F, L, d, in, out, dw, partial, stop); \
+            nb = L == LMV ? split_apply_t<T, LMV, true, XCD, NT>(s, co, F, L, d, in, out, dw, partial, stop) \
+                          : split_apply_t<T, LMV, false, XCD, NT>(s, co, F, L, d, in, out, dw, partial, stop); \
         else if (which == 1)                                                    \
             split_lsolve_t<T, LMV, XCD, NT>(s, co, F, L, d, in, out);           \
         else                                                                    \
