@@ -259,6 +259,114 @@ __global__ void k_denorm(int64_t n, const float* __restrict__ Yp, const float* _
     Y[t] = Yp[r * MlpDims::OUTP + c] * sd[c] + mu[c];
 }
 
+// Inference of the whole network in one launch: a workgroup walks 64-sample tiles
+// (persistent grid) and a tile's activations ping-pong between two 64 x 128 BF16 LDS
+// buffers, so nothing but the 16 input features and the 11 outputs of a sample touches
+// HBM (the layer-by-layer path writes and re-reads 3 x 256 MB of activations at 1M
+// faces).  The BF16 weights (73.7 KB) are read through the vector cache, where every
+// workgroup of a CU shares them.  Eight waves; in a 128-wide layer wave w computes rows
+// 16 (w % 4) .. +16 and columns 64 (w / 4) .. +64 as four 16 x 16 MFMA blocks, K in
+// steps of 16 in increasing order -- the operand layout, accumulation order and epilogue
+// (+ bias, ReLU, BF16 rounding) of k_gemm_bf16, so the output equals the layer-by-layer
+// forward bit for bit.  LDS: 40.4 KB per workgroup.
+constexpr int kInferRows = 64;
+constexpr int kInferThreads = 512;
+constexpr int kAs = MlpDims::H + 4;    // LDS row stride (BF16) of the activation tiles
+constexpr int kAs0 = MlpDims::IN + 4;  // of the input tile
+
+struct InferArgs {
+    int64_t n;
+    const float* X;        // [n x IN] raw features
+    const uint16_t* W[4];  // BF16 [dout x din]
+    const float* b[4];
+    const float* norm;     // x_mean[IN], x_std[IN], y_mean[OUT], y_std[OUT]
+    float* Y;              // [n x OUT] raw outputs
+};
+
+// one layer on a 64-row tile: in (LDS, row stride ins, K columns) x W^T (global BF16
+// [dout x K]) + bias, dout = 128 (hidden) or 16 (output layer: waves 0-3 only); BF16
+// (+ ReLU) into outb or FP32 into outf
+__device__ __forceinline__ void infer_layer(const uint16_t* __restrict__ in, int ins, int K,
+                                            const uint16_t* __restrict__ W, const float* __restrict__ bias, int dout,
+                                            bool relu, uint16_t* __restrict__ outb, float* __restrict__ outf) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r0 = 16 * (wave & 3);
+    const int c0 = 64 * (wave >> 2);
+    if (c0 >= dout) return;
+    const int nblk = min(4, (dout - c0) / 16);  // 16-column MFMA blocks of this wave
+    f32x4 acc[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[b] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        bf16x4 a;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) a[j] = (short)in[(r0 + (lane & 15)) * ins + k0 + 4 * (lane >> 4) + j];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            if (b < nblk) {
+                bf16x4 w;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) w[j] = (short)W[(c0 + 16 * b + (lane & 15)) * K + k0 + 4 * (lane >> 4) + j];
+                acc[b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, w, acc[b], 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        if (b < nblk) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int row = r0 + 4 * (lane >> 4) + j;
+                const int col = c0 + 16 * b + (lane & 15);
+                float v = acc[b][j] + bias[col];
+                if (outf) {
+                    outf[row * MlpDims::OUTP + col] = v;
+                } else {
+                    if (relu) v = fmaxf(v, 0.f);
+                    outb[row * kAs + col] = to_bf16(v);
+                }
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kInferThreads) k_mlp_infer(InferArgs p) {
+    constexpr int H = MlpDims::H, IN = MlpDims::IN, OUT = MlpDims::OUT, OUTP = MlpDims::OUTP;
+    __shared__ uint16_t xin[kInferRows * kAs0];
+    __shared__ uint16_t act0[kInferRows * kAs];
+    __shared__ uint16_t act1[kInferRows * kAs];
+    __shared__ float yout[kInferRows * OUTP];
+    const int tid = threadIdx.x;
+    const float* xmu = p.norm;
+    const float* xsd = p.norm + IN;
+    const float* ymu = p.norm + 2 * IN;
+    const float* ysd = p.norm + 2 * IN + OUT;
+    const int64_t ntiles = (p.n + kInferRows - 1) / kInferRows;
+    for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int64_t row0 = tile * kInferRows;
+        __syncthreads();  // the previous tile's outputs are consumed
+        for (int e = tid; e < kInferRows * IN; e += kInferThreads) {
+            const int r = e / IN, c = e % IN;
+            const int64_t g = row0 + r;
+            xin[r * kAs0 + c] = g < p.n ? to_bf16((p.X[g * IN + c] - xmu[c]) / xsd[c]) : (uint16_t)0;
+        }
+        __syncthreads();
+        infer_layer(xin, kAs0, IN, p.W[0], p.b[0], H, true, act0, nullptr);
+        __syncthreads();
+        infer_layer(act0, kAs, H, p.W[1], p.b[1], H, true, act1, nullptr);
+        __syncthreads();
+        infer_layer(act1, kAs, H, p.W[2], p.b[2], H, true, act0, nullptr);
+        __syncthreads();
+        infer_layer(act0, kAs, H, p.W[3], p.b[3], OUTP, false, nullptr, yout);
+        __syncthreads();
+        for (int e = tid; e < kInferRows * OUT; e += kInferThreads) {
+            const int r = e / OUT, c = e % OUT;
+            const int64_t g = row0 + r;
+            if (g < p.n) p.Y[g * OUT + c] = yout[r * OUTP + c] * ysd[c] + ymu[c];
+        }
+    }
+}
+
 __global__ void k_sample_idx(int64_t nb, int64_t n, uint64_t key, int64_t* idx) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nb) return;
@@ -348,7 +456,7 @@ const float* Mlp::xsd() const { return norm.p + MlpDims::IN; }
 const float* Mlp::ymu() const { return norm.p + 2 * MlpDims::IN; }
 const float* Mlp::ysd() const { return norm.p + 2 * MlpDims::IN + MlpDims::OUT; }
 
-void Mlp::forward_prepared(int64_t n) {
+void Mlp::forward_prepared(int64_t n, bool keep_z) {
     hipStream_t s = stream;
     const uint16_t* in = Xb.p;
     for (int l = 0; l < 4; ++l) {
@@ -363,7 +471,7 @@ void Mlp::forward_prepared(int64_t n) {
         g.transB = 1;
         g.bias = bvec[l].p;
         if (l < 3) {
-            g.C = Z[l].p;
+            g.C = keep_z ? Z[l].p : nullptr;  // pre-activations: only the backward pass reads them
             g.ldc = MlpDims::H;
             g.relu = 1;
             g.Cb = Ab[l].p;
@@ -465,8 +573,43 @@ double Mlp::train(const float* X, const float* Y, int64_t n, int epochs, int bat
     return loss;
 }
 
+bool Mlp::fused_inference_ok() {
+    if (fused_ok < 0) {
+        int cus = 0;
+        PB_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+        infer_blocks = 2 * (cus > 0 ? cus : 256);  // two workgroups per CU (104 VGPRs, 8 waves each)
+        fused_ok = 1;
+    }
+    return fused_ok == 1;
+}
+
+void Mlp::predict_fused(const float* X, float* Y, int64_t n, hipStream_t s) {
+    InferArgs a;
+    a.n = n;
+    a.X = X;
+    for (int l = 0; l < 4; ++l) {
+        a.W[l] = Wb[l].p;
+        a.b[l] = bvec[l].p;
+    }
+    a.norm = norm.p;
+    a.Y = Y;
+    const int64_t tiles = (n + kInferRows - 1) / kInferRows;
+    const unsigned grid = (unsigned)std::min<int64_t>(tiles, infer_blocks);
+    hipLaunchKernelGGL(k_mlp_infer, dim3(grid), dim3(kInferThreads), 0, s, a);
+    PB_HIP(hipGetLastError());
+}
+
 void Mlp::predict(const float* X, float* Y, int64_t n, hipStream_t s) {
-    // runs on the caller's stream, chunked so that the workspace stays bounded
+    if (n <= 0) return;
+    if (fused_inference_ok()) {
+        predict_fused(X, Y, n, s);
+        return;
+    }
+    // layer by layer: runs on the caller's stream, chunked so that the workspace stays bounded
+    predict_layers(X, Y, n, s);
+}
+
+void Mlp::predict_layers(const float* X, float* Y, int64_t n, hipStream_t s) {
     const hipStream_t own = stream;
     stream = s;
     const int64_t chunk = 1 << 20;
@@ -474,7 +617,7 @@ void Mlp::predict(const float* X, float* Y, int64_t n, hipStream_t s) {
         const int64_t m = std::min(chunk, n - o);
         ensure(m);
         prepare(X + o * MlpDims::IN, nullptr, m, nullptr);
-        forward_prepared(m);
+        forward_prepared(m, false);
         hipLaunchKernelGGL(k_denorm, dim3(g1(m * MlpDims::OUT)), dim3(256), 0, stream, m, Yp.p, ymu(), ysd(),
                            Y + o * MlpDims::OUT);
     }

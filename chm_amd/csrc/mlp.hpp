@@ -20,7 +20,12 @@ class Mlp {
     ~Mlp();
     // X: [n x IN] f32, Y: [n x OUT] f32 device arrays (raw units; normalised inside)
     double train(const float* X, const float* Y, int64_t n, int epochs, int batch, float lr, uint64_t seed);
+    // inference: the fused one-launch kernel (or layer by layer through the training
+    // GEMM; the two agree bit for bit)
     void predict(const float* X, float* Y, int64_t n, hipStream_t s);
+    void predict_fused(const float* X, float* Y, int64_t n, hipStream_t s);
+    void predict_layers(const float* X, float* Y, int64_t n, hipStream_t s);
+    bool fused_inference_ok();
     double gradients(const float* Xh, const float* Yh, int64_t n, float* grads_host);  // MSE gradients, host arrays
     static int64_t param_count();
     void get_params(float* host, bool grads);
@@ -29,11 +34,14 @@ class Mlp {
 
     int device = 0;
     hipStream_t stream = nullptr;
+    int fused_ok = -1;       // fused inference set up on this device (-1: not yet)
+    int infer_blocks = 512;  // persistent grid of the fused inference (two workgroups per CU)
 
   private:
     void ensure(int64_t n);
     void prepare(const float* X, const float* Y, int64_t n, const int64_t* rows);
-    void forward_prepared(int64_t n);
+    // keep_z: also store the FP32 pre-activations of the hidden layers (for backward)
+    void forward_prepared(int64_t n, bool keep_z = true);
     double backward_prepared(int64_t n, float* dY_f32);
     void adam_step(float lr);
     const float* xmu() const;

@@ -178,6 +178,20 @@ int pbsm3d_mlp_predict(pbsm3d_mlp* h, const float* X, float* Y, int64_t n) {
     });
 }
 
+int pbsm3d_mlp_predict_path(pbsm3d_mlp* h, const float* X, float* Y, int64_t n, int path) {
+    if (!h || !X || !Y || n <= 0 || path < 0 || path > 2) return PBSM3D_EINVAL;
+    if (path == 1 && !h->m->fused_inference_ok()) return PBSM3D_EINVAL;
+    return guard(h, [&] {
+        if (path == 0)
+            h->m->predict(X, Y, n, h->m->stream);
+        else if (path == 1)
+            h->m->predict_fused(X, Y, n, h->m->stream);
+        else
+            h->m->predict_layers(X, Y, n, h->m->stream);
+        PB_HIP(hipStreamSynchronize(h->m->stream));
+    });
+}
+
 int pbsm3d_mlp_gradients(pbsm3d_mlp* h, const float* X, const float* Y, int64_t n, float* grads, double* loss) {
     if (!h || !X || !Y || n <= 0) return PBSM3D_EINVAL;
     return guard(h, [&] {

@@ -54,6 +54,7 @@ EXPORTED = (
     "pbsm3d_mlp_set_normalization",
     "pbsm3d_mlp_train",
     "pbsm3d_mlp_predict",
+    "pbsm3d_mlp_predict_path",
     "pbsm3d_mlp_gradients",
     "pbsm3d_mlp_last_error",
     "pbsm3d_training_pairs",
@@ -148,6 +149,7 @@ def lib():
         L.pbsm3d_mlp_set_normalization.argtypes = [vp, fp]
         L.pbsm3d_mlp_train.argtypes = [vp, vp, vp, C.c_int64, C.c_int32, C.c_int32, C.c_float, C.c_uint64, dp]
         L.pbsm3d_mlp_predict.argtypes = [vp, vp, vp, C.c_int64]
+        L.pbsm3d_mlp_predict_path.argtypes = [vp, vp, vp, C.c_int64, C.c_int]
         L.pbsm3d_mlp_gradients.argtypes = [vp, fp, fp, C.c_int64, fp, dp]
         L.pbsm3d_mlp_last_error.argtypes = [vp]
         L.pbsm3d_mlp_last_error.restype = C.c_char_p
@@ -373,9 +375,11 @@ class Mlp:
                                         epochs, batch, lr, seed, C.byref(loss)))
         return loss.value
 
-    def predict(self, X, Y):
+    def predict(self, X, Y, path: int = 0):
+        """Y = net(X) on device tensors; path 0 default, 1 fused one-launch kernel, 2 layer by layer."""
         self._f32(X, Y)
-        self._ck(lib().pbsm3d_mlp_predict(self._h, C.c_void_p(X.data_ptr()), C.c_void_p(Y.data_ptr()), X.shape[0]))
+        self._ck(lib().pbsm3d_mlp_predict_path(self._h, C.c_void_p(X.data_ptr()), C.c_void_p(Y.data_ptr()),
+                                               X.shape[0], path))
 
     def gradients(self, X: np.ndarray, Y: np.ndarray):
         """(loss, flat gradient) of the normalised MSE on one batch of host arrays."""

@@ -290,6 +290,11 @@ int pbsm3d_mlp_set_normalization(pbsm3d_mlp* net, const float* host_norm);
 int pbsm3d_mlp_train(pbsm3d_mlp* net, const float* X, const float* Y, int64_t n, int32_t epochs, int32_t batch,
                      float lr, uint64_t seed, double* final_loss);
 int pbsm3d_mlp_predict(pbsm3d_mlp* net, const float* X, float* Y, int64_t n);
+/* The same through a chosen kernel path: 0 = as pbsm3d_mlp_predict (the fused
+ * path), 1 = the fused one-launch kernel (all four layers per 64-sample tile,
+ * activations in LDS), 2 = layer by layer through the training GEMM.  Paths 1
+ * and 2 give the same outputs bit for bit. */
+int pbsm3d_mlp_predict_path(pbsm3d_mlp* net, const float* X, float* Y, int64_t n, int path);
 /* host arrays: MSE loss and its gradient w.r.t. all parameters on one batch */
 int pbsm3d_mlp_gradients(pbsm3d_mlp* net, const float* X, const float* Y, int64_t n, float* host_grads, double* loss);
 const char* pbsm3d_mlp_last_error(const pbsm3d_mlp* net);

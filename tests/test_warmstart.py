@@ -72,6 +72,14 @@ def test_network_gradients_match_reference():
     net.predict(Xd, P)
     ref = mlp_oracle.predict(p, X, norm, bf16=True)
     assert np.abs(P.cpu().numpy() - ref).max() <= 1e-2 * np.abs(ref).max()
+    # the fused one-launch inference kernel and the layer-by-layer GEMMs agree bit for bit
+    # (same MFMA operand layout, K order, epilogue and BF16 rounding), also on a ragged n
+    P1 = torch.empty_like(P)
+    P2 = torch.empty_like(P)
+    net.predict(Xd, P1, path=1)
+    net.predict(Xd, P2, path=2)
+    assert torch.equal(P1, P2)
+    assert torch.equal(P, P1)
 
 
 @pytest.mark.gpu
