@@ -130,6 +130,23 @@ class WindConfig(C.Structure):
     ]
 
 
+class SnowConfig(C.Structure):
+    """pbsm3d_snow_config of include/pbsm3d_snow.h."""
+
+    _fields_ = [
+        ("drift_density", C.c_double),
+        ("max_active_layer", C.c_double),
+        ("ic_rho", C.c_double),
+        ("max_h2o_vol", C.c_double),
+        ("mass_threshold", C.c_double),
+    ]
+
+
+# snowcover state fields (include/pbsm3d_snow.h), field-major [SNOW_FIELDS x F]
+SNOW_FIELDS = ("z_s", "m_s", "rho", "layer_count", "z_s_0", "z_s_l", "m_s_0", "m_s_l", "T_s", "T_s_0", "T_s_l",
+               "cc_s", "cc_s_0", "cc_s_l", "h2o_total", "h2o_vol", "h2o", "h2o_max", "h2o_sat")
+
+
 class Stats(C.Structure):
     _fields_ = [
         ("suspension_present", C.c_int32),

@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "PBSM3D.hpp"
+#include "snobal.hpp"
 #include "fetchr.hpp"
 #include "scale_wind_vert.hpp"
 
@@ -88,8 +89,16 @@ static int run(int argc, char** argv) {
     // with use_fetchr=1 / use_scale_wind_vert=1 those modules provide "fetch" and
     // "U_2m_above_srf" each step, run before PBSM3D as CHM's core orders them by
     // their depends / provides
+    // use_snobal=1: the snobal module (drift part) provides "swe" and "snowdepthavg" from
+    // the previous step's drift_mass before PBSM3D runs, as CHM orders snobal (provides
+    // swe) before PBSM3D (depends on swe)
     std::unique_ptr<fetchr> fetch_mod;
     std::unique_ptr<scale_wind_vert> wind_mod;
+    std::unique_ptr<snobal> snow_mod;
+    if (cfg.get("use_snobal", false)) {
+        snow_mod = std::make_unique<snobal>(cfg);
+        snow_mod->init(m);
+    }
     if (cfg.get("use_scale_wind_vert", false)) {
         wind_mod = std::make_unique<scale_wind_vert>(cfg);
         wind_mod->init(m);
@@ -101,6 +110,7 @@ static int run(int argc, char** argv) {
     PBSM3D mod{cfg};
     mod.init(m);
     for (int s = 0; s < nsteps; ++s) {
+        if (snow_mod) snow_mod->run(m);
         if (wind_mod) wind_mod->run(m);
         if (fetch_mod) fetch_mod->run(m);
         mod.run(m);
@@ -111,6 +121,10 @@ static int run(int argc, char** argv) {
     };
     if (fetch_mod) dump("fetch", out_path + ".fetch");
     if (wind_mod) dump("U_2m_above_srf", out_path + ".u2");
+    if (snow_mod) {
+        dump("swe", out_path + ".swe");
+        dump("snowdepthavg", out_path + ".sd");
+    }
     const pbsm3d_stats st = mod.last_stats();
     std::ofstream out(out_path, std::ios::binary);
     for (int k = 0; k < 8; ++k)

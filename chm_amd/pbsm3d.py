@@ -46,6 +46,14 @@ EXPORTED = (
     "pbsm3d_wind_run",
     "pbsm3d_wind_destroy",
     "pbsm3d_wind_last_error",
+    "pbsm3d_snow_config_default",
+    "pbsm3d_snow_create",
+    "pbsm3d_snow_drift",
+    "pbsm3d_snow_outputs",
+    "pbsm3d_snow_get_state",
+    "pbsm3d_snow_set_state",
+    "pbsm3d_snow_destroy",
+    "pbsm3d_snow_last_error",
     "pbsm3d_mlp_create",
     "pbsm3d_mlp_destroy",
     "pbsm3d_mlp_param_count",
@@ -139,6 +147,17 @@ def lib():
         L.pbsm3d_wind_destroy.restype = None
         L.pbsm3d_wind_last_error.argtypes = [vp]
         L.pbsm3d_wind_last_error.restype = C.c_char_p
+        L.pbsm3d_snow_config_default.argtypes = [C.POINTER(abi.SnowConfig)]
+        L.pbsm3d_snow_config_default.restype = None
+        L.pbsm3d_snow_create.argtypes = [C.POINTER(vp), C.c_int, C.c_int64, dp, C.POINTER(abi.SnowConfig)]
+        L.pbsm3d_snow_drift.argtypes = [vp, vp, C.c_int]
+        L.pbsm3d_snow_outputs.argtypes = [vp, vp, vp, C.c_int]
+        L.pbsm3d_snow_get_state.argtypes = [vp, dp]
+        L.pbsm3d_snow_set_state.argtypes = [vp, dp]
+        L.pbsm3d_snow_destroy.argtypes = [vp]
+        L.pbsm3d_snow_destroy.restype = None
+        L.pbsm3d_snow_last_error.argtypes = [vp]
+        L.pbsm3d_snow_last_error.restype = C.c_char_p
         fp = C.POINTER(C.c_float)
         L.pbsm3d_mlp_create.argtypes = [C.POINTER(vp), C.c_int, C.c_uint64]
         L.pbsm3d_mlp_destroy.argtypes = [vp]
@@ -511,6 +530,78 @@ class Wind:
     def close(self):
         if getattr(self, "_h", None):
             lib().pbsm3d_wind_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+def snow_config(**overrides) -> abi.SnowConfig:
+    c = abi.SnowConfig()
+    lib().pbsm3d_snow_config_default(C.byref(c))
+    for k, v in overrides.items():
+        if not hasattr(c, k):
+            raise KeyError(k)
+        setattr(c, k, v)
+    return c
+
+
+class Snow:
+    """Snowcover update from PBSM3D's drift_mass on the GPU (the drift consumer of the
+    snobal module); provides swe and snowdepthavg."""
+
+    def __init__(self, n_faces: int, swe0=None, cfg: abi.SnowConfig | None = None, device: int = 0):
+        L = lib()
+        self._keep = abi.Keep()
+        self.n_faces = int(n_faces)
+        cfg = cfg if cfg is not None else snow_config()
+        h = C.c_void_p()
+        rc = L.pbsm3d_snow_create(C.byref(h), device, self.n_faces,
+                                  self._keep.dptr(swe0) if swe0 is not None else None, C.byref(cfg))
+        if rc != 0:
+            raise Pbsm3dError(rc, L.pbsm3d_snow_last_error(None).decode())
+        self._h = h
+
+    def _ck(self, rc):
+        if rc != 0:
+            raise Pbsm3dError(rc, lib().pbsm3d_snow_last_error(self._h).decode())
+
+    def drift(self, drift_mass):
+        """Apply one step's drift_mass: numpy array (host) or float64 CUDA tensor (device)."""
+        if hasattr(drift_mass, "data_ptr"):
+            self._ck(lib().pbsm3d_snow_drift(self._h, C.c_void_p(drift_mass.data_ptr()), 1))
+        else:
+            d = np.ascontiguousarray(drift_mass, dtype=np.float64)
+            self._ck(lib().pbsm3d_snow_drift(self._h, d.ctypes.data_as(C.c_void_p), 0))
+
+    def outputs(self, device: bool = False):
+        """(swe, snowdepthavg): numpy arrays, or float64 CUDA tensors with device=True."""
+        if device:
+            import torch
+
+            swe = torch.empty(self.n_faces, dtype=torch.float64, device="cuda:0")
+            sd = torch.empty_like(swe)
+            self._ck(lib().pbsm3d_snow_outputs(self._h, C.c_void_p(swe.data_ptr()), C.c_void_p(sd.data_ptr()), 1))
+            return swe, sd
+        swe, sd = np.zeros(self.n_faces), np.zeros(self.n_faces)
+        self._ck(lib().pbsm3d_snow_outputs(self._h, swe.ctypes.data_as(C.c_void_p), sd.ctypes.data_as(C.c_void_p),
+                                           0))
+        return swe, sd
+
+    def get_state(self) -> np.ndarray:
+        """[len(abi.SNOW_FIELDS) x F] snowcover state."""
+        st = np.zeros((len(abi.SNOW_FIELDS), self.n_faces))
+        self._ck(lib().pbsm3d_snow_get_state(self._h, st.ctypes.data_as(C.POINTER(C.c_double))))
+        return st
+
+    def set_state(self, state):
+        st = np.ascontiguousarray(state, dtype=np.float64)
+        assert st.shape == (len(abi.SNOW_FIELDS), self.n_faces)
+        self._ck(lib().pbsm3d_snow_set_state(self._h, st.ctypes.data_as(C.POINTER(C.c_double))))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pbsm3d_snow_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -64,6 +64,10 @@ def lib():
         L.oracle_expint_e1.restype = C.c_double
         L.oracle_scale_wind.argtypes = [C.c_int64, ip, dp, dp, dp, dp, C.POINTER(abi.WindConfig), dp, dp, dp]
         L.oracle_scale_wind.restype = None
+        L.oracle_snow_init.argtypes = [C.c_int64, dp, C.POINTER(abi.SnowConfig), dp]
+        L.oracle_snow_init.restype = None
+        L.oracle_snow_drift.argtypes = [C.c_int64, C.POINTER(abi.SnowConfig), dp, dp]
+        L.oracle_snow_drift.restype = None
         L.oracle_config_default.argtypes = [C.POINTER(abi.Config)]
         L.oracle_config_default.restype = None
         L.oracle_max_threads.restype = C.c_int
@@ -213,6 +217,23 @@ def scale_wind(neigh, cx, cy, canopy, lai, cfg, U_R, snowdepthavg=None):
                             keep.dptr(canopy), keep.dptr(lai), C.byref(cfg), keep.dptr(U_R), keep.dptr(snowdepthavg),
                             keep.dptr(out))
     return out
+
+
+def snow_init(n_faces: int, swe0, cfg) -> np.ndarray:
+    """snobal's initial snowcover state [len(abi.SNOW_FIELDS) x F] (swe0 None: no initial condition)."""
+    keep = abi.Keep()
+    st = np.zeros((len(abi.SNOW_FIELDS), n_faces))
+    lib().oracle_snow_init(n_faces, keep.dptr(swe0) if swe0 is not None else None, C.byref(cfg), keep.dptr(st))
+    return st
+
+
+def snow_drift(state, cfg, drift_mass) -> np.ndarray:
+    """The snowcover state after one step's drift_mass (a new array)."""
+    keep = abi.Keep()
+    st = np.array(state, dtype=np.float64, copy=True, order="C")
+    lib().oracle_snow_drift(st.shape[1], C.byref(cfg), keep.dptr(np.ascontiguousarray(drift_mass, dtype=np.float64)),
+                            st.ctypes.data_as(C.POINTER(C.c_double)))
+    return st
 
 
 def face_colors(neigh: np.ndarray):

@@ -128,3 +128,45 @@ def test_scale_wind_vert_module_feeds_pbsm3d(tmp_path, oracle):
     want = oracle.scale_wind(m.neigh, np.ascontiguousarray(c[:, 0]), np.ascontiguousarray(c[:, 1]), None, None, cfg,
                              fo["U_R"], fo["snowdepthavg"])
     np.testing.assert_allclose(got, want, rtol=1e-12)
+
+
+@pytest.mark.gpu
+def test_snobal_module_closes_the_loop(tmp_path):
+    """The C++ snobal module (drift part; keys drift_density / IC_rho) runs before PBSM3D
+    each step, as CHM orders a provider of swe before its consumer: it takes the previous
+    step's drift_mass and provides swe and snowdepthavg.  Three steps equal the same chain
+    driven through the C-ABI (pbsm3d_snow_* then pbsm3d_step) bit for bit."""
+    from chm_amd import pbsm3d as pb
+
+    print("[case] host snobal module", flush=True)
+    m = meshgen.synthetic_mesh(600, seed=9)
+    F = m.n_faces
+    p = tmp_path / "m.json"
+    meshgen.save_mesh_json(m, str(p))
+    fo = meshgen.forcing_c2(F, seed=9)
+    fo["swe"] = np.random.default_rng(3).uniform(0.0, 60.0, F)
+    fbin = tmp_path / "f.bin"
+    np.concatenate([fo[k] for k in abi.FORCING_FIELDS]).astype("<f8").tofile(fbin)
+    obin = tmp_path / "o.bin"
+    r = subprocess.run([_bin(), "run", str(p), str(fbin), "3", str(obin), "nLayer=6", "use_snobal=1",
+                        "IC_rho=250", "drift_density=280"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    got_swe = np.fromfile(str(obin) + ".swe", dtype="<f8")
+    got_sd = np.fromfile(str(obin) + ".sd", dtype="<f8")
+    got = np.fromfile(obin, dtype="<f8").reshape(8, F)
+    sn = pb.Snow(F, fo["swe"], pb.snow_config(ic_rho=250.0, drift_density=280.0))
+    ctx = pb.Context(pb.face_geometry(m), pb.default_config(nLayer=6))
+    f2 = dict(fo)
+    drift = np.full(F, -9999.0)
+    for _ in range(3):
+        sn.drift(drift)
+        f2["swe"], f2["snowdepthavg"] = sn.outputs()
+        out, _ = ctx.step(f2)
+        drift = out["drift_mass"]
+    np.testing.assert_array_equal(got_swe, f2["swe"])
+    np.testing.assert_array_equal(got_sd, f2["snowdepthavg"])
+    for k, name in enumerate(abi.OUT_FIELDS):
+        np.testing.assert_array_equal(got[k], out[name], err_msg=name)
+    assert np.abs(f2["swe"] - fo["swe"]).max() > 0  # drift changed the snowcover
+    ctx.close()
+    sn.close()
