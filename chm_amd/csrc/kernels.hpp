@@ -276,9 +276,10 @@ void bicg_xr_dev(hipStream_t s, int64_t n, const T* ph, const T* sh, const T* sv
 // stage 0 start (rho, rr), 1 alpha from rh.v, 2 s.s, 3 omega from (s.t, t.t), 4 new (rho, rr);
 // sc[SC_FLAG] != 0 marks a breakdown (1: rh.v = 0, 2: omega = 0, 3: non-finite residual).
 // Stage 0 clears sc[SC_STOP]; stage 4 sets it when a breakdown was flagged, sqrt(rr) <=
-// stop_abs or rr is not finite.  Stages 1-4 do nothing once it is set.
+// stop_abs or rr is not finite.  Stages 1-4 do nothing once it is set.  host_slot (may be NULL):
+// device-visible pinned host memory that receives sc[0 .. SC_COUNT) after the update.
 void bicg_stage(hipStream_t s, const double* partial, int64_t nblk, int stride, int ncols, int stage, double* sc,
-                double stop_abs = 0.0);
+                double stop_abs = 0.0, double* host_slot = nullptr);
 
 // conversions / vector updates
 template <typename T>

@@ -249,6 +249,7 @@ void BicgWork<T>::ensure(int64_t n_) {
     partial.alloc(std::max<size_t>(4 * (size_t)kReduceBlocks, 2 * (size_t)pos_spmv_dots_max_blocks(n)));
     dsc.alloc(SC_COUNT);
     hsc.alloc(2 * SC_COUNT);
+    PB_HIP(hipHostGetDevicePointer((void**)&hsc_dev, hsc.p, 0));
     for (hipEvent_t& e : ev)
         if (!e) PB_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
 }
@@ -343,10 +344,9 @@ int bicg_run(hipStream_t s, BicgWork<T>& w, const CsrOp<T>& A, const PcFn<T>& M,
             ProfScope ps(A.prof, s, PBSM3D_K_ORTHO);
             bicg_stage(s, w.partial.p, nb, 2, 2, 3, sc);  // omega
             bicg_xr_dev<T>(s, n, ph, sh, w.s.p, w.t.p, w.rh.p, sc, x, w.r.p, w.partial.p, nblk);
-            bicg_stage(s, w.partial.p, nblk, 2, 2, 4, sc, stop_abs);  // rho, rr, stop test
+            // rho, rr, the stop test, and the scalars into host slot it % 2 (no copy launch)
+            bicg_stage(s, w.partial.p, nblk, 2, 2, 4, sc, stop_abs, w.hsc_dev + SC_COUNT * (it & 1));
         }
-        // the scalars of iteration it into host slot it % 2
-        PB_HIP(hipMemcpyAsync(h + SC_COUNT * (it & 1), sc, sizeof(double) * SC_COUNT, hipMemcpyDeviceToHost, s));
         PB_HIP(hipEventRecord(w.ev[it & 1], s));
     };
     if (maxit <= 0) return 0;

@@ -119,7 +119,8 @@ struct BicgWork {
     int64_t n = 0;
     DevBuf<T> r, rh, p, v, s, t, ph, sh;
     DevBuf<double> partial, dsc;
-    HostBuf<double> hsc;  // two slots of the scalars: iterations k and k+1 in flight
+    HostBuf<double> hsc;        // two slots of the scalars (iterations k and k+1 in flight)
+    double* hsc_dev = nullptr;  // hsc in the device address space (the stage kernel writes it)
     hipEvent_t ev[2] = {};
     BicgWork() = default;
     BicgWork(const BicgWork&) = delete;

@@ -458,7 +458,8 @@ __global__ void __launch_bounds__(kBlock) k_bicg_xr_dev(int64_t n, const T* __re
 constexpr int kStageThreads = 1024;
 __global__ void __launch_bounds__(kStageThreads) k_bicg_stage(const double* __restrict__ partial, int64_t nblk,
                                                               int stride, int ncols, int stage,
-                                                              double* __restrict__ sc, double stop_abs) {
+                                                              double* __restrict__ sc, double stop_abs,
+                                                              double* __restrict__ host_slot) {
     __shared__ double s_red[kStageThreads / 64];
     if (stage != 0 && sc[SC_STOP] != 0.0) return;  // uniform over the block
     double sum[2] = {0, 0};
@@ -518,6 +519,8 @@ __global__ void __launch_bounds__(kStageThreads) k_bicg_stage(const double* __re
             break;
         }
     }
+    if (host_slot)  // the scalars straight into the host-visible slot the host waits on
+        for (int k = 0; k < SC_COUNT; ++k) host_slot[k] = sc[k];
 }
 
 template <typename T>
@@ -662,9 +665,9 @@ void bicg_xr_dev(hipStream_t s, int64_t n, const T* ph, const T* sh, const T* sv
     hipLaunchKernelGGL((k_bicg_xr_dev<T>), dim3(nblk), dim3(kBlock), 0, s, n, ph, sh, sv, tv, rh, sc, x, r, partial);
 }
 void bicg_stage(hipStream_t s, const double* partial, int64_t nblk, int stride, int ncols, int stage, double* sc,
-                double stop_abs) {
+                double stop_abs, double* host_slot) {
     hipLaunchKernelGGL(k_bicg_stage, dim3(1), dim3(kStageThreads), 0, s, partial, nblk, stride, ncols, stage, sc,
-                       stop_abs);
+                       stop_abs, host_slot);
 }
 
 template <typename T>
