@@ -6,6 +6,8 @@
 //
 // Reductions are two-stage and deterministic: every block writes its partial
 // sums, then reduce_cols adds the partials in a fixed order.
+#include <initializer_list>
+
 #include "kernels.hpp"
 
 namespace pb {
@@ -396,15 +398,32 @@ __global__ void __launch_bounds__(kBlock) k_axpy_dots(int64_t n, double a, const
 }
 
 // ---- BiCGStab with device-resident scalars (slots pb::BicgSc) ----
+// The vector kernels walk their vectors four elements per thread and step (16-byte loads
+// for FP32) in a grid-stride loop, then the n % 4 tail one element per thread; the
+// per-element arithmetic is the same either way.
+template <typename T>
+using Vec4 = T __attribute__((ext_vector_type(4)));
+
 // p = r + beta (p - omega v), beta = (rho / rho_prev) (alpha / omega) of the previous step
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_bicg_p_dev(int64_t n, const T* __restrict__ r, const T* __restrict__ v,
                                                        const double* __restrict__ sc, T* __restrict__ p) {
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n || sc[SC_STOP] != 0.0) return;
+    if (sc[SC_STOP] != 0.0) return;  // uniform over the grid
     const double omega = sc[SC_OMEGA];
     const double beta = (sc[SC_RHO] / sc[SC_RHO_PREV]) * (sc[SC_ALPHA] / omega);
-    p[k] = (T)fma(beta, fma(-omega, (double)v[k], (double)p[k]), (double)r[k]);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n4 = n / 4;
+    for (int64_t i = t0; i < n4; i += stride) {
+        const Vec4<T> rv = reinterpret_cast<const Vec4<T>*>(r)[i];
+        const Vec4<T> vv = reinterpret_cast<const Vec4<T>*>(v)[i];
+        Vec4<T> pv = reinterpret_cast<const Vec4<T>*>(p)[i];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) pv[j] = (T)fma(beta, fma(-omega, (double)vv[j], (double)pv[j]), (double)rv[j]);
+        reinterpret_cast<Vec4<T>*>(p)[i] = pv;
+    }
+    for (int64_t k = 4 * n4 + t0; k < n; k += stride)
+        p[k] = (T)fma(beta, fma(-omega, (double)v[k], (double)p[k]), (double)r[k]);
 }
 
 // s = r - alpha v, partial[b] = s.s
@@ -417,7 +436,20 @@ __global__ void __launch_bounds__(kBlock) k_bicg_s_dev(int64_t n, const T* __res
     const double alpha = sc[SC_ALPHA];
     double ss = 0;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n4 = n / 4;
+    for (int64_t i = t0; i < n4; i += stride) {
+        const Vec4<T> vv = reinterpret_cast<const Vec4<T>*>(v)[i];
+        const Vec4<T> rv = reinterpret_cast<const Vec4<T>*>(r)[i];
+        Vec4<T> o;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            o[j] = (T)fma(-alpha, (double)vv[j], (double)rv[j]);
+            ss = fma((double)o[j], (double)o[j], ss);
+        }
+        reinterpret_cast<Vec4<T>*>(s_out)[i] = o;
+    }
+    for (int64_t k = 4 * n4 + t0; k < n; k += stride) {
         const T o = (T)fma(-alpha, (double)v[k], (double)r[k]);
         s_out[k] = o;
         ss = fma((double)o, (double)o, ss);
@@ -438,7 +470,27 @@ __global__ void __launch_bounds__(kBlock) k_bicg_xr_dev(int64_t n, const T* __re
     const double alpha = sc[SC_ALPHA], omega = sc[SC_OMEGA];
     double d1 = 0, d2 = 0;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n4 = n / 4;
+    for (int64_t i = t0; i < n4; i += stride) {
+        const Vec4<T> phv = reinterpret_cast<const Vec4<T>*>(ph)[i];
+        const Vec4<T> shv = reinterpret_cast<const Vec4<T>*>(sh)[i];
+        const Vec4<T> svv = reinterpret_cast<const Vec4<T>*>(sv)[i];
+        const Vec4<T> tvv = reinterpret_cast<const Vec4<T>*>(tv)[i];
+        const Vec4<T> rhv = reinterpret_cast<const Vec4<T>*>(rh)[i];
+        Vec4<T> xv = reinterpret_cast<const Vec4<T>*>(x)[i];
+        Vec4<T> o;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            xv[j] = (T)fma(omega, (double)shv[j], fma(alpha, (double)phv[j], (double)xv[j]));
+            o[j] = (T)fma(-omega, (double)tvv[j], (double)svv[j]);
+            d1 = fma((double)rhv[j], (double)o[j], d1);
+            d2 = fma((double)o[j], (double)o[j], d2);
+        }
+        reinterpret_cast<Vec4<T>*>(x)[i] = xv;
+        reinterpret_cast<Vec4<T>*>(r)[i] = o;
+    }
+    for (int64_t k = 4 * n4 + t0; k < n; k += stride) {
         x[k] = (T)fma(omega, (double)sh[k], fma(alpha, (double)ph[k], (double)x[k]));
         const T o = (T)fma(-omega, (double)tv[k], (double)sv[k]);
         r[k] = o;
@@ -650,18 +702,29 @@ void axpy_dots(hipStream_t s, int64_t n, double a, const T* x, const T* y, T* ou
                int nblk) {
     hipLaunchKernelGGL((k_axpy_dots<T>), dim3(nblk), dim3(kBlock), 0, s, n, a, x, y, out, rh, partial);
 }
+// the BiCGStab vector kernels read and write four elements at a time
+template <typename T>
+void check_vec4_aligned(std::initializer_list<const T*> ptrs) {
+    for (const T* p : ptrs)
+        if (reinterpret_cast<uintptr_t>(p) % (4 * sizeof(T)) != 0)
+            throw Error(PBSM3D_EINVAL, "BiCGStab vectors must be aligned to four elements");
+}
 template <typename T>
 void bicg_p_dev(hipStream_t s, int64_t n, const T* r, const T* v, const double* sc, T* p) {
-    hipLaunchKernelGGL((k_bicg_p_dev<T>), dim3(grid_for(n)), dim3(kBlock), 0, s, n, r, v, sc, p);
+    check_vec4_aligned<T>({r, v, p});
+    hipLaunchKernelGGL((k_bicg_p_dev<T>), dim3(grid_for((n + 3) / 4, kBlock, 8192)), dim3(kBlock), 0, s, n, r, v, sc,
+                       p);
 }
 template <typename T>
 void bicg_s_dev(hipStream_t s, int64_t n, const T* v, const T* r, const double* sc, T* s_out, double* partial,
                 int nblk) {
+    check_vec4_aligned<T>({v, r, s_out});
     hipLaunchKernelGGL((k_bicg_s_dev<T>), dim3(nblk), dim3(kBlock), 0, s, n, v, r, sc, s_out, partial);
 }
 template <typename T>
 void bicg_xr_dev(hipStream_t s, int64_t n, const T* ph, const T* sh, const T* sv, const T* tv, const T* rh,
                  const double* sc, T* x, T* r, double* partial, int nblk) {
+    check_vec4_aligned<T>({ph, sh, sv, tv, rh, x, r});
     hipLaunchKernelGGL((k_bicg_xr_dev<T>), dim3(nblk), dim3(kBlock), 0, s, n, ph, sh, sv, tv, rh, sc, x, r, partial);
 }
 void bicg_stage(hipStream_t s, const double* partial, int64_t nblk, int stride, int ncols, int stage, double* sc,
