@@ -53,6 +53,40 @@ def member_forcing(mesh: meshgen.Mesh, m: Member, step: int) -> dict:
     return fo
 
 
+def member_forcing_device(mesh: meshgen.Mesh, m: Member, steps: int, device: str) -> list[dict]:
+    """member_forcing for steps 0 .. steps-1 as device tensors, built on the GPU: the
+    step's wind speed and direction come from the same seeded advected fields
+    (meshgen.SmoothField modes, evaluated with torch in FP64), the other fields from
+    the step's host draw, which does not depend on the step."""
+    import torch
+
+    base = {k: torch.tensor(v, device=device) for k, v in member_forcing(mesh, m, 0).items()}
+    c = mesh.vertex[mesh.elem].mean(axis=1)
+    cx = torch.tensor(c[:, 0], device=device)
+    cy = torch.tensor(c[:, 1], device=device)
+    fields = (meshgen.SmoothField(m.seed, stream=meshgen.S_FIELD),
+              meshgen.SmoothField(m.seed + 7919, stream=meshgen.S_FIELD))
+
+    def value(sf, xs, ys):
+        acc = torch.zeros_like(xs)
+        for k in range(sf.n):
+            acc += torch.cos(float(sf.kx[k]) * xs + float(sf.ky[k]) * ys + float(sf.phase[k]))
+        return acc * math.sqrt(2.0 / sf.n)
+
+    z0 = meshgen.Z0_SNOW
+    lift = torch.log((meshgen.Z_U_R - (base["snowdepthavg"] + z0)) / z0) / math.log((2.0 - z0) / z0)
+    out = []
+    for s in range(steps):
+        xs, ys = cx - 150.0 * s, cy - 40.0 * s  # meshgen.forcing_advected's advection
+        u2 = torch.clamp(10.0 * torch.exp(0.25 * value(fields[0], xs, ys)), 4.0, 20.0)
+        f = dict(base)
+        f["U_2m_above_srf"] = u2 * m.wind_scale
+        f["U_R"] = u2 * lift * m.wind_scale
+        f["vw_dir"] = 270.0 + 20.0 * value(fields[1], xs, ys)
+        out.append(f)
+    return out
+
+
 def build(members: list[Member], precision=abi.MIXED, krylov=abi.KRYLOV_BICGSTAB_SPLIT, warm_net=None,
           device: int = 0):
     """Meshes, geometry and contexts of all members (set-up, not timed)."""
@@ -79,10 +113,10 @@ def run(built, steps: int, workers: int = 16, on_device: bool = True, device: in
     dev = f"cuda:{device}"
     forcings = []
     for m, mesh, ctx in built:
-        fs = [member_forcing(mesh, m, s) for s in range(steps)]
         if on_device:
-            fs = [{k: torch.tensor(v, device=dev) for k, v in f.items()} for f in fs]
-        forcings.append(fs)
+            forcings.append(member_forcing_device(mesh, m, steps, dev))
+        else:
+            forcings.append([member_forcing(mesh, m, s) for s in range(steps)])
     outs = [{k: torch.zeros(ctx.F, dtype=torch.float64, device=dev) for k in abi.OUT_FIELDS} if on_device else None
             for _, _, ctx in built]
     torch.cuda.synchronize()

@@ -2,7 +2,7 @@
 import numpy as np
 import pytest
 
-from chm_amd import abi, ensemble
+from chm_amd import abi, ensemble, meshgen
 
 
 def test_member_draws_follow_the_configuration():
@@ -32,3 +32,19 @@ def test_concurrent_members_match_sequential():
     for x, y in zip(ra["per_member"], rb["per_member"]):
         assert x["susp_iters"] == y["susp_iters"] and x["last_resid"] <= 1e-8
     print("[ensemble]", {k: v for k, v in ra.items() if k != "per_member"}, flush=True)
+
+
+def test_device_forcing_matches_host_forcing():
+    """The ensemble's device-built forcing (torch, FP64) equals member_forcing's numpy
+    fields step by step (same seeded advected modes) to rounding."""
+    import torch
+
+    m = ensemble.draw_members(2, seed=3, min_points=2000, max_points=4000)[1]
+    mesh = meshgen.synthetic_mesh(m.n_points, seed=m.seed)
+    dev = ensemble.member_forcing_device(mesh, m, 4, "cpu")
+    for s in range(4):
+        host = ensemble.member_forcing(mesh, m, s)
+        assert set(dev[s]) == set(host)
+        for k, v in host.items():
+            np.testing.assert_allclose(dev[s][k].numpy(), v, rtol=1e-14, atol=0, err_msg=f"{k} step {s}")
+    assert not torch.equal(dev[0]["vw_dir"], dev[3]["vw_dir"])  # the fields move with the steps
