@@ -278,8 +278,11 @@ void bicg_xr_dev(hipStream_t s, int64_t n, const T* ph, const T* sh, const T* sv
 // Stage 0 clears sc[SC_STOP]; stage 4 sets it when a breakdown was flagged, sqrt(rr) <=
 // stop_abs or rr is not finite.  Stages 1-4 do nothing once it is set.  host_slot (may be NULL):
 // device-visible pinned host memory that receives sc[0 .. SC_COUNT) after the update.
+// ss_partial (may be NULL): the ss_nblk partials of s.s, summed into sc[SC_SS] before the
+// stage itself (stage 2 folded into stage 3).
 void bicg_stage(hipStream_t s, const double* partial, int64_t nblk, int stride, int ncols, int stage, double* sc,
-                double stop_abs = 0.0, double* host_slot = nullptr);
+                double stop_abs = 0.0, double* host_slot = nullptr, const double* ss_partial = nullptr,
+                int64_t ss_nblk = 0);
 
 // conversions / vector updates
 template <typename T>

@@ -248,6 +248,7 @@ void BicgWork<T>::ensure(int64_t n_) {
     // two partial sums per block of the fused SpMV + dot launches, or of the vector kernels
     partial.alloc(std::max<size_t>(4 * (size_t)kReduceBlocks, 2 * (size_t)pos_spmv_dots_max_blocks(n)));
     dsc.alloc(SC_COUNT);
+    ss_partial.alloc(kReduceBlocks);
     hsc.alloc(2 * SC_COUNT);
     PB_HIP(hipHostGetDevicePointer((void**)&hsc_dev, hsc.p, 0));
     for (hipEvent_t& e : ev)
@@ -329,8 +330,7 @@ int bicg_run(hipStream_t s, BicgWork<T>& w, const CsrOp<T>& A, const PcFn<T>& M,
         {
             ProfScope ps(A.prof, s, PBSM3D_K_ORTHO);
             bicg_stage(s, w.partial.p, nb, 2, 2, 1, sc);  // alpha
-            bicg_s_dev<T>(s, n, w.v.p, w.r.p, sc, w.s.p, w.partial.p, nblk);
-            bicg_stage(s, w.partial.p, nblk, 1, 1, 2, sc);
+            bicg_s_dev<T>(s, n, w.v.p, w.r.p, sc, w.s.p, w.ss_partial.p, nblk);  // s.s summed in stage 3
         }
         if (pc) {
             ProfScope ps(A.prof, s, PBSM3D_K_PRECOND);
@@ -342,7 +342,7 @@ int bicg_run(hipStream_t s, BicgWork<T>& w, const CsrOp<T>& A, const PcFn<T>& M,
         }
         {
             ProfScope ps(A.prof, s, PBSM3D_K_ORTHO);
-            bicg_stage(s, w.partial.p, nb, 2, 2, 3, sc);  // omega
+            bicg_stage(s, w.partial.p, nb, 2, 2, 3, sc, 0.0, nullptr, w.ss_partial.p, nblk);  // s.s, omega
             bicg_xr_dev<T>(s, n, ph, sh, w.s.p, w.t.p, w.rh.p, sc, x, w.r.p, w.partial.p, nblk);
             // rho, rr, the stop test, and the scalars into host slot it % 2 (no copy launch)
             bicg_stage(s, w.partial.p, nblk, 2, 2, 4, sc, stop_abs, w.hsc_dev + SC_COUNT * (it & 1));

@@ -511,7 +511,8 @@ constexpr int kStageThreads = 1024;
 __global__ void __launch_bounds__(kStageThreads) k_bicg_stage(const double* __restrict__ partial, int64_t nblk,
                                                               int stride, int ncols, int stage,
                                                               double* __restrict__ sc, double stop_abs,
-                                                              double* __restrict__ host_slot) {
+                                                              double* __restrict__ host_slot,
+                                                              const double* __restrict__ ss_partial, int64_t ss_nblk) {
     __shared__ double s_red[kStageThreads / 64];
     if (stage != 0 && sc[SC_STOP] != 0.0) return;  // uniform over the block
     double sum[2] = {0, 0};
@@ -519,6 +520,12 @@ __global__ void __launch_bounds__(kStageThreads) k_bicg_stage(const double* __re
         double a = 0;
         for (int64_t b = threadIdx.x; b < nblk; b += blockDim.x) a += partial[b * stride + c];
         sum[c] = block_sum(a, s_red);
+    }
+    if (ss_partial) {  // stage 2 folded into this launch: s.s from its own partials first
+        double a = 0;
+        for (int64_t b = threadIdx.x; b < ss_nblk; b += blockDim.x) a += ss_partial[b];
+        a = block_sum(a, s_red);
+        if (threadIdx.x == 0) sc[SC_SS] = a;
     }
     if (threadIdx.x != 0) return;
     const double a = sum[0], b = sum[1];
@@ -728,9 +735,9 @@ void bicg_xr_dev(hipStream_t s, int64_t n, const T* ph, const T* sh, const T* sv
     hipLaunchKernelGGL((k_bicg_xr_dev<T>), dim3(nblk), dim3(kBlock), 0, s, n, ph, sh, sv, tv, rh, sc, x, r, partial);
 }
 void bicg_stage(hipStream_t s, const double* partial, int64_t nblk, int stride, int ncols, int stage, double* sc,
-                double stop_abs, double* host_slot) {
+                double stop_abs, double* host_slot, const double* ss_partial, int64_t ss_nblk) {
     hipLaunchKernelGGL(k_bicg_stage, dim3(1), dim3(kStageThreads), 0, s, partial, nblk, stride, ncols, stage, sc,
-                       stop_abs, host_slot);
+                       stop_abs, host_slot, ss_partial, ss_nblk);
 }
 
 template <typename T>
