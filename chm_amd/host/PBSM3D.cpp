@@ -96,7 +96,7 @@ void PBSM3D::init(mesh& domain) {
         face* f = domain->face_at(i);
         for (int j = 0; j < 3; ++j) {
             face* n = f->neighbor(j);
-            neigh[3 * i + j] = n ? (int32_t)n->cell_global_id : -1;
+            neigh[3 * i + j] = n ? (int32_t)n->id() : -1;  // the library's faces are face_at(0..F-1)
             const Vector_2 m = f->edge_unit_normal(j);
             nx[3 * i + j] = m.x();
             ny[3 * i + j] = m.y();

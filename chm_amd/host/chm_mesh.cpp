@@ -1,5 +1,7 @@
 #include "chm_mesh.hpp"
 
+#include "h5_file.hpp"
+
 #include <cctype>
 #include <fstream>
 #include <sstream>
@@ -154,7 +156,154 @@ std::shared_ptr<triangulation> triangulation::from_json_file(const std::string& 
     if (!sc.key("neigh")) throw std::runtime_error("mesh JSON without neigh");
     sc.numbers(n);
     std::vector<int32_t> ei(e.begin(), e.end()), ni(n.begin(), n.end());
+    if (sc.key("is_geographic")) {
+        sc.ws();
+        const char c = sc.p < txt.size() ? txt[sc.p] : '0';
+        if (c == 't' || (c >= '1' && c <= '9'))
+            throw std::runtime_error(path + ": geographic meshes are not supported (planar UTM geometry only)");
+    }
     return std::make_shared<triangulation>(std::move(v), std::move(ei), std::move(ni));
+}
+
+std::vector<std::string> triangulation::parameter_names() const {
+    std::vector<std::string> names;
+    for (const auto& kv : params_) names.push_back(kv.first);
+    return names;
+}
+
+// ---------------------------------------------------------------------------
+// HDF5 mesh (triangulation.cpp:724-1036): root attributes named "/mesh/version"
+// (absent: 1.0.0), "/mesh/proj4", "/mesh/is_geographic", "/mesh/is_partition"
+// (absent: false), "/mesh/partition_method"; datasets /mesh/local_sizes (read
+// unless is_partition), /mesh/cell_global_id, /mesh/owner (versions after
+// 2.0.0), /mesh/vertex (n x 3 doubles), /mesh/elem and /mesh/neighbor (F x 3
+// ints, -1 = none; global ids when is_partition).  The checks and their order
+// follow the reference loader.
+// ---------------------------------------------------------------------------
+namespace {
+
+std::vector<int32_t> to_int32(const h5::Array& a, const std::string& what) {
+    std::vector<int32_t> out(a.values.size());
+    for (size_t i = 0; i < out.size(); ++i) {
+        const double v = a.values[i];
+        if (!(v >= -2147483648.0 && v <= 2147483647.0) || v != static_cast<double>(static_cast<int64_t>(v)))
+            throw std::runtime_error(what + " holds a value that is not a 32-bit integer");
+        out[i] = static_cast<int32_t>(v);
+    }
+    return out;
+}
+
+void parse_version(const std::string& s, int& major, int& minor, int& patch) {
+    const std::string v = s.empty() ? "1.0.0" : s;
+    const size_t a = v.find('.'), b = a == std::string::npos ? a : v.find('.', a + 1);
+    if (b == std::string::npos) throw std::runtime_error("malformed mesh version '" + v + "'");
+    major = std::stoi(v.substr(0, a));
+    minor = std::stoi(v.substr(a + 1, b - a - 1));
+    patch = std::stoi(v.substr(b + 1));
+}
+
+}  // namespace
+
+std::shared_ptr<triangulation> triangulation::from_hdf5(const std::string& mesh_path,
+                                                        const std::vector<std::string>& param_paths) {
+    h5::File file(mesh_path);
+    mesh_info info;
+    if (file.has_attribute("/", "/mesh/version")) info.version = file.attribute_string("/", "/mesh/version");
+    int major = 1, minor = 0, patch = 0;
+    parse_version(info.version, major, minor, patch);
+    info.version = std::to_string(major) + "." + std::to_string(minor) + "." + std::to_string(patch);
+    info.proj4 = file.attribute_string("/", "/mesh/proj4");
+    info.is_geographic = file.attribute_number("/", "/mesh/is_geographic") != 0;
+    if (file.has_attribute("/", "/mesh/is_partition"))
+        info.is_partition = file.attribute_number("/", "/mesh/is_partition") != 0;
+    if (!info.is_partition && !(major >= 1)) throw std::runtime_error(mesh_path + ": h5 mesh version too old");
+    if (info.is_partition && !(major >= 3 && minor >= 0))
+        throw std::runtime_error(mesh_path + ": partitioned mesh version too old");
+    try {
+        info.partition_method = file.attribute_string("/", "/mesh/partition_method");
+        if (!info.is_partition) info.local_sizes = to_int32(file.read("/mesh/local_sizes"), "/mesh/local_sizes");
+    } catch (const h5::error&) {
+        // an absent field gets the reference's message; a corrupt file its own error
+        const bool lacks = !file.has_attribute("/", "/mesh/partition_method") ||
+                           (!info.is_partition && !file.exists("/mesh/local_sizes"));
+        if (!lacks) throw;
+        throw std::runtime_error(mesh_path +
+                                 ": the h5 lacks /mesh/partition_method or /mesh/local_sizes (written by an older "
+                                 "partition / meshpermutation tool)");
+    }
+    if (info.is_partition && info.partition_method != "metis")
+        throw std::runtime_error(mesh_path + ": partitioned meshes must be built with the metis method");
+    if (info.is_geographic)
+        throw std::runtime_error(mesh_path + ": geographic meshes are not supported (planar UTM geometry only)");
+
+    const std::vector<int32_t> gid = to_int32(file.read("/mesh/cell_global_id"), "/mesh/cell_global_id");
+    if (info.version == "1.0.0" || info.version == "1.2.0" || info.version == "2.0.0")
+        info.owner.assign(gid.size(), 0);
+    else
+        info.owner = to_int32(file.read("/mesh/owner"), "/mesh/owner");
+
+    const h5::Array va = file.read("/mesh/vertex");
+    if (va.dims.size() != 1 || va.components != 3) throw std::runtime_error(mesh_path + ": /mesh/vertex is not n x 3");
+    std::vector<double> vertex = va.values;
+    const int64_t nv = static_cast<int64_t>(va.dims[0]);
+
+    const h5::Array ea = file.read("/mesh/elem");
+    if (ea.dims.size() != 1 || ea.components != 3) throw std::runtime_error(mesh_path + ": /mesh/elem is not F x 3");
+    std::vector<int32_t> elem = to_int32(ea, "/mesh/elem");
+    const size_t F = ea.dims[0];
+    if (gid.size() < F || info.owner.size() < F)
+        throw std::runtime_error(mesh_path + ": fewer cell_global_id / owner entries than faces");
+    for (int32_t k : elem)
+        if (k < 0 || k >= nv) throw std::runtime_error(mesh_path + ": elem names a vertex out of range");
+
+    const h5::Array na = file.read("/mesh/neighbor");
+    if (na.dims.size() != 1 || na.components != 3) throw std::runtime_error(mesh_path + ": /mesh/neighbor is not F x 3");
+    if (na.dims[0] != F)
+        throw std::runtime_error(mesh_path + ": Expected: " + std::to_string(F) + " neighborlists, got: " +
+                                 std::to_string(na.dims[0]));
+    std::vector<int32_t> neigh = to_int32(na, "/mesh/neighbor");
+    if (info.is_partition) {
+        // neighbours are global ids; map them to positions in this file.  An id the
+        // file does not hold maps to 0, as the reference's std::map operator[] does.
+        std::map<int32_t, int32_t> local;
+        for (size_t i = 0; i < F; ++i) local[gid[i]] = static_cast<int32_t>(i);
+        for (int32_t& k : neigh)
+            if (k != -1) k = local[k];
+    }
+    for (size_t i = 0; i < F; ++i) {
+        for (int j = 0; j < 3; ++j) {
+            const int32_t k = neigh[3 * i + j];
+            if (k < -1 || k >= static_cast<int32_t>(F))
+                throw std::runtime_error(mesh_path + ": Face " + std::to_string(i) + " has out of bound neighbors.");
+            if (k != -1 && gid[k] == gid[i])
+                throw std::runtime_error(mesh_path + ": Face" + std::to_string(j) + " is face! (face " +
+                                         std::to_string(i) + ")");
+        }
+    }
+
+    auto t = std::make_shared<triangulation>(std::move(vertex), std::move(elem), std::move(neigh));
+    for (size_t i = 0; i < F; ++i) t->faces_[i]->cell_global_id = static_cast<size_t>(gid[i]);
+    t->info_ = std::move(info);
+
+    // Parameters (load_hdf5_parameters, triangulation.cpp:1234-1330): every dataset of
+    // /parameters, in name order, read as doubles; face i takes value i.
+    for (const std::string& pp : param_paths) {
+        h5::File pf(pp);
+        for (const std::string& name : pf.members("/parameters")) {
+            const h5::Array a = pf.read("/parameters/" + name);
+            if (a.components != 1 || a.values.size() < F)
+                throw std::runtime_error(pp + ": parameter " + name + " holds fewer values than the mesh has faces");
+            t->set_parameter(name, std::vector<double>(a.values.begin(), a.values.begin() + F));
+        }
+    }
+    return t;
+}
+
+std::shared_ptr<triangulation> triangulation::from_file(const std::string& path,
+                                                        const std::vector<std::string>& param_paths) {
+    if (path.size() >= 3 && path.compare(path.size() - 3, 3, ".h5") == 0) return from_hdf5(path, param_paths);
+    if (!param_paths.empty()) throw std::runtime_error("parameter files go with an .h5 mesh");
+    return from_json_file(path);
 }
 
 }  // namespace chm

@@ -7,6 +7,8 @@
 //   face::operator[](name)     per-face variables, -9999 until written
 //                              (timeseries/variablestorage.hpp:269-272)
 //   triangulation::from_json   triangulation.cpp:206-356 (.mesh JSON schema)
+//   triangulation::from_hdf5   triangulation.cpp:724-1036 (`_mesh.h5` topology)
+//                              and :1234-1330 (`_param.h5` parameters)
 // Variables are stored per name as one contiguous array over faces, so the
 // module can hand whole fields to the device without a gather.
 #pragma once
@@ -60,11 +62,33 @@ class face {
     size_t cell_local_id;
 };
 
+// What a mesh file says about itself besides the topology.
+struct mesh_info {
+    std::string version = "1.0.0";  // a file without one counts as 1.0.0 (triangulation.hpp:971-986)
+    std::string proj4;
+    std::string partition_method;
+    bool is_geographic = false;
+    bool is_partition = false;
+    std::vector<int32_t> local_sizes;  // faces per MPI rank the file was partitioned for
+    std::vector<int32_t> owner;        // owning rank per face (0 for versions up to 2.0.0)
+};
+
 class triangulation {
   public:
     // vertex xyz (3*nv), elem (3*F, CCW), neigh (3*F, -1 = boundary)
     triangulation(std::vector<double> vertex, std::vector<int32_t> elem, std::vector<int32_t> neigh);
     static std::shared_ptr<triangulation> from_json_file(const std::string& path);
+    // mesh_path: a `_mesh.h5`; param_paths: `_param.h5` files whose /parameters
+    // datasets become per-face parameters.  Planar (UTM) meshes only: a
+    // geographic mesh raises, as the module's geometry is planar.
+    static std::shared_ptr<triangulation> from_hdf5(const std::string& mesh_path,
+                                                    const std::vector<std::string>& param_paths = {});
+    // .h5 -> from_hdf5(path, params), anything else -> from_json_file (params must be empty)
+    static std::shared_ptr<triangulation> from_file(const std::string& path,
+                                                    const std::vector<std::string>& param_paths = {});
+
+    const mesh_info& info() const { return info_; }
+    std::vector<std::string> parameter_names() const;
 
     size_t size_faces() const { return faces_.size(); }
     size_t size_global_faces() const { return faces_.size(); }
@@ -87,6 +111,7 @@ class triangulation {
     std::vector<double> vertex_;
     std::vector<int32_t> elem_, neigh_;
     std::vector<std::unique_ptr<face>> faces_;
+    mesh_info info_;
     std::map<std::string, std::vector<double>> params_;
     std::map<std::string, std::vector<double>> vars_;
 };

@@ -1,10 +1,14 @@
 // Driver of the PBSM3D module the way the reference's core drives a module
 // (construct from config, init(mesh), run(mesh) per timestep, checkpoint):
 //
-//   pbsm3d_host check <mesh.json>
+//   pbsm3d_host check <mesh.json | mesh.h5> [param.h5 ...]
 //       host-only checks: depends/provides, config errors, face geometry of the
 //       module's face API against pbsm3d_face_geometry.  Prints one JSON line.
-//   pbsm3d_host run <mesh.json> <forcing.bin> <nsteps> <out.bin> [key=value ...]
+//   pbsm3d_host mesh <mesh.json | mesh.h5> [param.h5 ...]
+//       the mesh as loaded (metadata, vertex, elem, neigh, cell_global_id,
+//       parameters) as one JSON object.
+//   pbsm3d_host run <mesh.json | mesh.h5> <forcing.bin> <nsteps> <out.bin> [key=value ...]
+//       (param=<file.h5> adds a parameter file to an .h5 mesh; repeatable)
 //       forcing.bin holds 8*F doubles (U_R, U_2m_above_srf, vw_dir, swe, t, rh,
 //       snowdepthavg, fetch); runs nsteps timesteps and writes the 8 output
 //       fields (Qsalt, Qsusp, Qsubl, Qsubl_mass, sum_subl, drift_mass, sum_drift,
@@ -36,8 +40,8 @@ static std::string join(const std::vector<std::string>& v) {
     return s + "]";
 }
 
-static int check(const std::string& mesh_path) {
-    mesh m = triangulation::from_json_file(mesh_path);
+static int check(const std::string& mesh_path, const std::vector<std::string>& params) {
+    mesh m = triangulation::from_file(mesh_path, params);
     PBSM3D mod{config_file()};
     bool threw = false;
     try {
@@ -49,7 +53,8 @@ static int check(const std::string& mesh_path) {
     }
     const size_t F = m->size_faces();
     std::vector<double> nx(3 * F), ny(3 * F), el(3 * F), ar(F), cx(F), cy(F), cz(F);
-    int rc = pbsm3d_face_geometry((int64_t)F, m->vertex().data(), m->elem().data(), nullptr, nx.data(), ny.data(),
+    const double* area_param = m->has_parameter("area") ? m->parameter("area").data() : nullptr;
+    int rc = pbsm3d_face_geometry((int64_t)F, m->vertex().data(), m->elem().data(), area_param, nx.data(), ny.data(),
                                   el.data(), ar.data(), cx.data(), cy.data(), cz.data());
     double worst = 0;
     for (size_t i = 0; i < F; ++i) {
@@ -73,12 +78,15 @@ static int run(int argc, char** argv) {
     const std::string mesh_path = argv[2], forcing_path = argv[3], out_path = argv[5];
     const int nsteps = std::atoi(argv[4]);
     config_file cfg;
+    std::vector<std::string> params;
     for (int a = 6; a < argc; ++a) {
         std::string kv = argv[a];
         const size_t eq = kv.find('=');
-        if (eq != std::string::npos) cfg.put(kv.substr(0, eq), kv.substr(eq + 1));
+        if (eq == std::string::npos) continue;
+        if (kv.substr(0, eq) == "param") params.push_back(kv.substr(eq + 1));
+        else cfg.put(kv.substr(0, eq), kv.substr(eq + 1));
     }
-    mesh m = triangulation::from_json_file(mesh_path);
+    mesh m = triangulation::from_file(mesh_path, params);
     const size_t F = m->size_faces();
     std::vector<double> buf(8 * F);
     std::ifstream in(forcing_path, std::ios::binary);
@@ -133,7 +141,7 @@ static int run(int argc, char** argv) {
     // checkpoint -> restore into a fresh module -> one more step on both
     checkpoint_store chk;
     mod.checkpoint(m, chk);
-    mesh m2 = triangulation::from_json_file(mesh_path);
+    mesh m2 = triangulation::from_file(mesh_path, params);
     for (int k = 0; k < 8; ++k) m2->variable(kForcing[k]) = m->variable(kForcing[k]);
     PBSM3D mod2{cfg};
     mod2.init(m2);
@@ -148,11 +156,71 @@ static int run(int argc, char** argv) {
     return 0;
 }
 
+template <typename T>
+static void json_array(const std::vector<T>& v) {
+    std::printf("[");
+    for (size_t i = 0; i < v.size(); ++i) std::printf(i ? ",%.17g" : "%.17g", static_cast<double>(v[i]));
+    std::printf("]");
+}
+
+static std::string json_string(const std::string& v) {
+    std::string o = "\"";
+    for (unsigned char c : v) {
+        if (c == '"' || c == '\\') {
+            o += '\\';
+            o += static_cast<char>(c);
+        } else if (c < 0x20 || c >= 0x7f) {
+            char buf[8];
+            std::snprintf(buf, sizeof buf, "\\u%04x", c);
+            o += buf;
+        } else {
+            o += static_cast<char>(c);
+        }
+    }
+    return o + "\"";
+}
+
+static int dump_mesh(const std::string& mesh_path, const std::vector<std::string>& params) {
+    mesh m = triangulation::from_file(mesh_path, params);
+    const mesh_info& in = m->info();
+    std::vector<double> gid(m->size_faces());
+    for (size_t i = 0; i < gid.size(); ++i) gid[i] = static_cast<double>(m->face_at(i)->cell_global_id);
+    std::printf("{\"faces\": %zu, \"version\": %s, \"proj4\": %s, \"partition_method\": %s, "
+                "\"is_partition\": %s, \"local_sizes\": ",
+                m->size_faces(), json_string(in.version).c_str(), json_string(in.proj4).c_str(),
+                json_string(in.partition_method).c_str(), in.is_partition ? "true" : "false");
+    json_array(in.local_sizes);
+    std::printf(", \"owner\": ");
+    json_array(in.owner);
+    std::printf(", \"vertex\": ");
+    json_array(m->vertex());
+    std::printf(", \"elem\": ");
+    json_array(m->elem());
+    std::printf(", \"neigh\": ");
+    json_array(m->neigh());
+    std::printf(", \"cell_global_id\": ");
+    json_array(gid);
+    std::printf(", \"parameters\": {");
+    const std::vector<std::string> names = m->parameter_names();
+    for (size_t k = 0; k < names.size(); ++k) {
+        std::printf(k ? ", %s: " : "%s: ", json_string(names[k]).c_str());
+        json_array(m->parameter(names[k]));
+    }
+    std::printf("}}\n");
+    return 0;
+}
+
 int main(int argc, char** argv) {
     try {
-        if (argc >= 3 && std::string(argv[1]) == "check") return check(argv[2]);
-        if (argc >= 6 && std::string(argv[1]) == "run") return run(argc, argv);
-        std::fprintf(stderr, "usage: pbsm3d_host check <mesh.json> | run <mesh.json> <forcing.bin> <nsteps> <out.bin> [k=v...]\n");
+        const std::string cmd = argc >= 2 ? argv[1] : "";
+        if (argc >= 3 && (cmd == "check" || cmd == "mesh")) {
+            const std::vector<std::string> params(argv + 3, argv + argc);
+            return cmd == "check" ? check(argv[2], params) : dump_mesh(argv[2], params);
+        }
+        if (argc >= 6 && cmd == "run") return run(argc, argv);
+        std::fprintf(stderr,
+                     "usage: pbsm3d_host check|mesh <mesh.json|mesh.h5> [param.h5...] | run <mesh> <forcing.bin> "
+                     "<nsteps> <out.bin> [k=v...] [param=<param.h5>...]\n");
         return 2;
     } catch (const module_error& e) {
         std::fprintf(stderr, "module_error (%d): %s\n", e.code, e.what());

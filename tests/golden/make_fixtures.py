@@ -9,8 +9,16 @@
   config-1 mesh (L = 5, 10) and the granger1m mesh (L = 10).
 * tiny_fields.npz -- inputs and outputs of one step on a 12-point mesh solved
   by dense Gaussian elimination (no Krylov tolerance involved).
+* slope_mesh.h5, slope_param.h5 -- the reference's HDF5 test mesh and its
+  parameter file, copied as data (functional_tests/mesh_versioning/
+  slope.metis_mesh.h5, slope.metis_param.h5).
+* slope.npz -- vertex/elem/neigh of the same mesh in its .mesh JSON form and
+  the area of its .param JSON (functional_tests/mesh_versioning/slope.mesh,
+  slope.param), read as JSON data.  The .h5 holds these faces reordered: face
+  i of the .h5 is face id[i] of the JSON mesh (id: the parameter file's "id").
 
 Run from the repository root:  python tests/golden/make_fixtures.py
+(or  python tests/golden/make_fixtures.py slope  for item 4 alone)
 """
 from __future__ import annotations
 
@@ -27,6 +35,7 @@ sys.path.insert(0, ROOT)
 
 REF_MESH = "/root/reference/test_data/meshes/granger1m.mesh"
 REF_PARAM = "/root/reference/test_data/meshes/granger1m.param"
+REF_SLOPE = "/root/reference/functional_tests/mesh_versioning"
 
 
 def brute_force_pattern(n_faces: int, n_layer: int, neigh: np.ndarray):
@@ -54,6 +63,17 @@ def pattern_sha(rp: np.ndarray, col: np.ndarray) -> str:
     h.update(np.ascontiguousarray(rp, dtype="<i4").tobytes())
     h.update(np.ascontiguousarray(col, dtype="<i4").tobytes())
     return h.hexdigest()
+
+
+def slope():
+    import shutil
+
+    from chm_amd import meshgen
+
+    shutil.copyfile(os.path.join(REF_SLOPE, "slope.metis_mesh.h5"), os.path.join(HERE, "slope_mesh.h5"))
+    shutil.copyfile(os.path.join(REF_SLOPE, "slope.metis_param.h5"), os.path.join(HERE, "slope_param.h5"))
+    m = meshgen.load_mesh_json(os.path.join(REF_SLOPE, "slope.mesh"), os.path.join(REF_SLOPE, "slope.param"))
+    np.savez_compressed(os.path.join(HERE, "slope.npz"), vertex=m.vertex, elem=m.elem, neigh=m.neigh, area=m.area)
 
 
 def main():
@@ -92,6 +112,9 @@ def main():
     )
     print(json.dumps(pats, indent=1))
 
+    # 4. the HDF5 test mesh and its JSON form
+    slope()
+
 
 if __name__ == "__main__":
-    main()
+    slope() if sys.argv[1:] == ["slope"] else main()
