@@ -475,11 +475,15 @@ def main():
     dom = max(rooflines, key=lambda n: rooflines[n]["ms_total"]) if rooflines else None
     roofline = rooflines.get(dom)
 
-    cpu = None
+    cpu = cpu1 = None
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         log(f"[bench] CPU baseline: oracle on {args.cpu_faces} faces, {threads} threads")
         cpu = cpu_baseline(g, forcing, L, args.cpu_faces, threads)
+        if threads != 1:
+            # one core, as the reference's default build runs (no OpenMP, one MPI rank)
+            log(f"[bench] CPU baseline: oracle on {args.cpu_faces} faces, 1 thread")
+            cpu1 = cpu_baseline(g, forcing, L, args.cpu_faces, 1)
 
     out = {
         "metric": METRIC,
@@ -507,6 +511,7 @@ def main():
         "roofline": roofline,
         "kernel_rooflines": rooflines,
         "cpu_baseline": cpu,
+        "cpu_baseline_one_core": cpu1,
     }
     print(json.dumps(out), flush=True)
 
