@@ -264,25 +264,28 @@ enum BicgSc {
 // p = r + beta (p - omega v) with beta from the scalars of the previous step
 template <typename T>
 void bicg_p_dev(hipStream_t s, int64_t n, const T* r, const T* v, const double* sc, T* p);
-// s = r - alpha v, partial[b] = s.s (nblk blocks)
+// alpha = rho / (rh.v) and s = r - alpha v, partial[b] = s.s (nblk blocks).  dots: the
+// operator's dots_nb per-block pairs (rh.v, v.v); every workgroup sums them in the same fixed
+// order and workgroup 0 records rh.v and alpha in sc (no single-workgroup stage launch).
+// partial must not alias dots.
 template <typename T>
-void bicg_s_dev(hipStream_t s, int64_t n, const T* v, const T* r, const double* sc, T* s_out, double* partial,
-                int nblk);
-// x += alpha ph + omega sh; r = s - omega t; partial[2b] = rh.r, partial[2b+1] = r.r
+void bicg_s_dev(hipStream_t s, int64_t n, const T* v, const T* r, double* sc, T* s_out, double* partial, int nblk,
+                const double* dots, int64_t dots_nb);
+// omega from the operator's pairs (s.t, t.t) and k_bicg_s_dev's ss_nb partials of s.s (summed
+// like alpha above), then x += alpha ph + omega sh; r = s - omega t; partial[2b] = rh.r,
+// partial[2b+1] = r.r.  partial must not alias dots or ss_part.
 template <typename T>
-void bicg_xr_dev(hipStream_t s, int64_t n, const T* ph, const T* sh, const T* sv, const T* tv, const T* rh,
-                 const double* sc, T* x, T* r, double* partial, int nblk);
-// one block sums ncols (<= 2) partial columns in a fixed order and updates the scalars:
-// stage 0 start (rho, rr), 1 alpha from rh.v, 2 s.s, 3 omega from (s.t, t.t), 4 new (rho, rr);
-// sc[SC_FLAG] != 0 marks a breakdown (1: rh.v = 0, 2: omega = 0, 3: non-finite residual).
-// Stage 0 clears sc[SC_STOP]; stage 4 sets it when a breakdown was flagged, sqrt(rr) <=
-// stop_abs or rr is not finite.  Stages 1-4 do nothing once it is set.  host_slot (may be NULL):
-// device-visible pinned host memory that receives sc[0 .. SC_COUNT) after the update.
-// ss_partial (may be NULL): the ss_nblk partials of s.s, summed into sc[SC_SS] before the
-// stage itself (stage 2 folded into stage 3).
-void bicg_stage(hipStream_t s, const double* partial, int64_t nblk, int stride, int ncols, int stage, double* sc,
-                double stop_abs = 0.0, double* host_slot = nullptr, const double* ss_partial = nullptr,
-                int64_t ss_nblk = 0);
+void bicg_xr_dev(hipStream_t s, int64_t n, const T* ph, const T* sh, const T* sv, const T* tv, const T* rh, double* sc,
+                 T* x, T* r, double* partial, int nblk, const double* dots, int64_t dots_nb, const double* ss_part,
+                 int64_t ss_nb);
+// one block sums the pairs (rh.r, r.r) of nblk partials (stride doubles apart) in a fixed order:
+// stage 0 starts a run (rho, rr; clears sc[SC_STOP]), stage 1 ends an iteration: new rho and
+// rr, and sc[SC_STOP] set when a breakdown was flagged (sc[SC_FLAG] 1: rh.v = 0, 2: omega = 0,
+// 3: non-finite residual), sqrt(rr) <= stop_abs or rr is not finite.  Stage 1 does nothing once
+// sc[SC_STOP] is set.  host_slot (may be NULL): device-visible pinned host memory that
+// receives sc[0 .. SC_COUNT) after the update.
+void bicg_stage(hipStream_t s, const double* partial, int64_t nblk, int stride, int stage, double* sc,
+                double stop_abs = 0.0, double* host_slot = nullptr);
 
 // conversions / vector updates
 template <typename T>

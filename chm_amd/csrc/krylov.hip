@@ -19,6 +19,8 @@ inline int off_h() { return 0; }
 inline int off_ss(int m) { return m + 2; }
 inline int off_y(int m) { return m + 4; }
 
+constexpr int kVecBlocks = 512;  // two workgroups per CU
+
 inline int reduce_blocks(int64_t n) {
     int64_t b = (n + kBlock - 1) / kBlock;
     if (b > kReduceBlocks) b = kReduceBlocks;
@@ -249,6 +251,7 @@ void BicgWork<T>::ensure(int64_t n_) {
     partial.alloc(std::max<size_t>(4 * (size_t)kReduceBlocks, 2 * (size_t)pos_spmv_dots_max_blocks(n)));
     dsc.alloc(SC_COUNT);
     ss_partial.alloc(kReduceBlocks);
+    xr_partial.alloc(2 * (size_t)kReduceBlocks);
     hsc.alloc(2 * SC_COUNT);
     PB_HIP(hipHostGetDevicePointer((void**)&hsc_dev, hsc.p, 0));
     for (hipEvent_t& e : ev)
@@ -282,6 +285,9 @@ int bicg_run(hipStream_t s, BicgWork<T>& w, const CsrOp<T>& A, const PcFn<T>& M,
              double stop_abs, int maxit, double* res_out, bool stop_rel = false) {
     const int64_t n = A.n;
     const int nblk = reduce_blocks(n);
+    // The s and x/r kernels: every workgroup first sums the operator's dot partials itself,
+    // so they run as few workgroups as fill the chip (grid-stride over the vectors).
+    const int nvec = std::min(nblk, kVecBlocks);
     const int spmv_id = A.prof_class >= 0 ? A.prof_class : (sizeof(T) == 8 ? PBSM3D_K_SPMV64 : PBSM3D_K_SPMV32);
     const bool pc = static_cast<bool>(M);
     T* ph = pc ? w.ph.p : w.p.p;  // the preconditioned search directions
@@ -304,7 +310,7 @@ int bicg_run(hipStream_t s, BicgWork<T>& w, const CsrOp<T>& A, const PcFn<T>& M,
     }
     PB_HIP(hipMemcpyAsync(w.rh.p, w.r.p, n * sizeof(T), hipMemcpyDeviceToDevice, s));
     mdot<T>(s, n, w.rh.p, 1, w.r.p, w.partial.p, nblk);
-    bicg_stage(s, w.partial.p, nblk, 2, 2, 0, sc);
+    bicg_stage(s, w.partial.p, nblk, 2, 0, sc);
     fetch();
     double rr = std::sqrt(h[SC_RR]);
     *res_out = rr;
@@ -329,8 +335,7 @@ int bicg_run(hipStream_t s, BicgWork<T>& w, const CsrOp<T>& A, const PcFn<T>& M,
         }
         {
             ProfScope ps(A.prof, s, PBSM3D_K_ORTHO);
-            bicg_stage(s, w.partial.p, nb, 2, 2, 1, sc);  // alpha
-            bicg_s_dev<T>(s, n, w.v.p, w.r.p, sc, w.s.p, w.ss_partial.p, nblk);  // s.s summed in stage 3
+            bicg_s_dev<T>(s, n, w.v.p, w.r.p, sc, w.s.p, w.ss_partial.p, nvec, w.partial.p, nb);  // alpha, s, s.s
         }
         if (pc) {
             ProfScope ps(A.prof, s, PBSM3D_K_PRECOND);
@@ -342,10 +347,11 @@ int bicg_run(hipStream_t s, BicgWork<T>& w, const CsrOp<T>& A, const PcFn<T>& M,
         }
         {
             ProfScope ps(A.prof, s, PBSM3D_K_ORTHO);
-            bicg_stage(s, w.partial.p, nb, 2, 2, 3, sc, 0.0, nullptr, w.ss_partial.p, nblk);  // s.s, omega
-            bicg_xr_dev<T>(s, n, ph, sh, w.s.p, w.t.p, w.rh.p, sc, x, w.r.p, w.partial.p, nblk);
+            // omega, x, r, (rh.r, r.r)
+            bicg_xr_dev<T>(s, n, ph, sh, w.s.p, w.t.p, w.rh.p, sc, x, w.r.p, w.xr_partial.p, nvec, w.partial.p, nb,
+                           w.ss_partial.p, nvec);
             // rho, rr, the stop test, and the scalars into host slot it % 2 (no copy launch)
-            bicg_stage(s, w.partial.p, nblk, 2, 2, 4, sc, stop_abs, w.hsc_dev + SC_COUNT * (it & 1));
+            bicg_stage(s, w.xr_partial.p, nvec, 2, 1, sc, stop_abs, w.hsc_dev + SC_COUNT * (it & 1));
         }
         PB_HIP(hipEventRecord(w.ev[it & 1], s));
     };

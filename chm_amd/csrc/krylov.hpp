@@ -120,6 +120,7 @@ struct BicgWork {
     DevBuf<T> r, rh, p, v, s, t, ph, sh;
     DevBuf<double> partial, dsc;
     DevBuf<double> ss_partial;  // per-block s.s of k_bicg_s_dev (kept apart from the operator's dots)
+    DevBuf<double> xr_partial;  // per-block (rh.r, r.r) of k_bicg_xr_dev (read by the closing stage)
     HostBuf<double> hsc;        // two slots of the scalars (iterations k and k+1 in flight)
     double* hsc_dev = nullptr;  // hsc in the device address space (the stage kernel writes it)
     hipEvent_t ev[2] = {};

@@ -426,14 +426,80 @@ __global__ void __launch_bounds__(kBlock) k_bicg_p_dev(int64_t n, const T* __res
         p[k] = (T)fma(beta, fma(-omega, (double)v[k], (double)p[k]), (double)r[k]);
 }
 
-// s = r - alpha v, partial[b] = s.s
+// Column c of nblk per-block partials (stride doubles apart), summed in a fixed order:
+// every workgroup that calls this gets the same value, in thread 0.  Thread t adds entries
+// t, t + blockDim.x, ... in turn; the loads of eight turns are issued before their adds, so
+// the latency of one L2 round trip is paid once per eight entries, not per entry.
+__device__ __forceinline__ double sum_partials(const double* __restrict__ partial, int64_t nblk, int stride, int c,
+                                               double* s_red) {
+    double a = 0;
+    const int64_t step = blockDim.x;
+    int64_t b = threadIdx.x;
+    for (; b + 7 * step < nblk; b += 8 * step) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = partial[(b + u * step) * stride + c];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) a += v[u];
+    }
+    for (; b < nblk; b += step) a += partial[b * stride + c];
+    return block_sum(a, s_red);
+}
+// Both columns of nblk (x, y) pairs in one pass, each summed in the order sum_partials uses.
+__device__ __forceinline__ void sum_pairs(const double* __restrict__ partial, int64_t nblk, double* s_red, double& x,
+                                          double& y) {
+    double a = 0, c = 0;
+    const int64_t step = blockDim.x;
+    const double2* pr = reinterpret_cast<const double2*>(partial);
+    int64_t b = threadIdx.x;
+    for (; b + 7 * step < nblk; b += 8 * step) {
+        double2 v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = pr[b + u * step];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            a += v[u].x;
+            c += v[u].y;
+        }
+    }
+    for (; b < nblk; b += step) {
+        a += pr[b].x;
+        c += pr[b].y;
+    }
+    x = block_sum(a, s_red);
+    y = block_sum(c, s_red);
+}
+
+// alpha = rho / (rh . v) from the operator's partials (rh.v in column 0 of stride-2 pairs),
+// computed by every workgroup alike and recorded by workgroup 0; breakdown (rh.v = 0 or not
+// finite): alpha = 0 and flag 1.  Returns alpha to every thread.
+__device__ double bicg_alpha(const double* __restrict__ dots, int64_t dots_nb, double* __restrict__ sc,
+                             double* s_red, double* s_bc) {
+    const double rhv = sum_partials(dots, dots_nb, 2, 0, s_red);
+    if (threadIdx.x == 0) {
+        const bool ok = rhv != 0.0 && isfinite(rhv);
+        const double alpha = ok ? sc[SC_RHO] / rhv : 0.0;
+        *s_bc = alpha;
+        if (blockIdx.x == 0) {
+            sc[SC_RHV] = rhv;
+            sc[SC_ALPHA] = alpha;
+            if (!ok) sc[SC_FLAG] = 1;  // breakdown: the caller restarts
+        }
+    }
+    __syncthreads();
+    return *s_bc;
+}
+
+// s = r - alpha v, partial[b] = s.s; alpha from the operator's rh.v partials
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_bicg_s_dev(int64_t n, const T* __restrict__ v, const T* __restrict__ r,
-                                                       const double* __restrict__ sc, T* __restrict__ s_out,
-                                                       double* __restrict__ partial) {
+                                                       double* __restrict__ sc, T* __restrict__ s_out,
+                                                       double* __restrict__ partial, const double* __restrict__ dots,
+                                                       int64_t dots_nb) {
     __shared__ double s_red[kBlock / 64];
+    __shared__ double s_bc;
     if (sc[SC_STOP] != 0.0) return;  // uniform over the grid
-    const double alpha = sc[SC_ALPHA];
+    const double alpha = bicg_alpha(dots, dots_nb, sc, s_red, &s_bc);
     double ss = 0;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -458,16 +524,55 @@ __global__ void __launch_bounds__(kBlock) k_bicg_s_dev(int64_t n, const T* __res
     if (threadIdx.x == 0) partial[blockIdx.x] = t;
 }
 
+// omega from the operator's (s.t, t.t) partials and the s.s partials of k_bicg_s_dev: the
+// minimal-residual value, enlarged when the angle between t and s is wide -- |cos(t, s)| <
+// 0.7 -> omega *= 0.7 / |cos| (Sleijpen & van der Vorst 1995), which keeps omega away from 0
+// where plain BiCGStab stagnates.  Computed by every workgroup alike, recorded by workgroup
+// 0; omega = 0 or non-finite dots: flag 2.  Returns omega to every thread.
+__device__ double bicg_omega(const double* __restrict__ dots, int64_t dots_nb, const double* __restrict__ ss_part,
+                             int64_t ss_nb, double* __restrict__ sc, double* s_red, double* s_bc) {
+    double st, tt;
+    sum_pairs(dots, dots_nb, s_red, st, tt);
+    const double ss = sum_partials(ss_part, ss_nb, 1, 0, s_red);
+    if (threadIdx.x == 0) {
+        double omega = 0;
+        int flag = 0;
+        if (tt > 0.0 && isfinite(st) && isfinite(tt)) {
+            omega = st / tt;
+            const double cs = st / sqrt(tt * ss);
+            if (isfinite(cs) && fabs(cs) < 0.7 && fabs(cs) > 0.0) omega *= 0.7 / fabs(cs);
+            if (omega == 0.0) flag = -2;  // only when nothing else is flagged
+        } else {
+            flag = 2;
+        }
+        *s_bc = omega;
+        if (blockIdx.x == 0) {
+            sc[SC_ST] = st;
+            sc[SC_TT] = tt;
+            sc[SC_SS] = ss;
+            sc[SC_OMEGA] = omega;
+            if (flag == 2 || (flag == -2 && sc[SC_FLAG] == 0.0)) sc[SC_FLAG] = 2;
+        }
+    }
+    __syncthreads();
+    return *s_bc;
+}
+
 // x += alpha ph + omega sh;  r = s - omega t;  partial[2b] = rh.r, partial[2b+1] = r.r
+// (omega from the operator's s.t, t.t partials and the s.s partials, see bicg_omega)
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_bicg_xr_dev(int64_t n, const T* __restrict__ ph, const T* __restrict__ sh,
                                                         const T* __restrict__ sv, const T* __restrict__ tv,
-                                                        const T* __restrict__ rh, const double* __restrict__ sc,
+                                                        const T* __restrict__ rh, double* __restrict__ sc,
                                                         T* __restrict__ x, T* __restrict__ r,
-                                                        double* __restrict__ partial) {
+                                                        double* __restrict__ partial, const double* __restrict__ dots,
+                                                        int64_t dots_nb, const double* __restrict__ ss_part,
+                                                        int64_t ss_nb) {
     __shared__ double s_red[kBlock / 64];
+    __shared__ double s_bc;
     if (sc[SC_STOP] != 0.0) return;  // uniform over the grid
-    const double alpha = sc[SC_ALPHA], omega = sc[SC_OMEGA];
+    const double alpha = sc[SC_ALPHA];
+    const double omega = bicg_omega(dots, dots_nb, ss_part, ss_nb, sc, s_red, &s_bc);
     double d1 = 0, d2 = 0;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -505,78 +610,37 @@ __global__ void __launch_bounds__(kBlock) k_bicg_xr_dev(int64_t n, const T* __re
     }
 }
 
-// Sums ncols (<= 2) columns of partial (stride, fixed order) in one block and
+// Sums the (rh.r, r.r) pairs of partial (stride, fixed order) in one block and
 // updates the scalars of the given BiCGStab stage (see bicg_stage()).
 constexpr int kStageThreads = 1024;
 __global__ void __launch_bounds__(kStageThreads) k_bicg_stage(const double* __restrict__ partial, int64_t nblk,
-                                                              int stride, int ncols, int stage,
-                                                              double* __restrict__ sc, double stop_abs,
-                                                              double* __restrict__ host_slot,
-                                                              const double* __restrict__ ss_partial, int64_t ss_nblk) {
+                                                              int stride, int stage, double* __restrict__ sc,
+                                                              double stop_abs, double* __restrict__ host_slot) {
     __shared__ double s_red[kStageThreads / 64];
     if (stage != 0 && sc[SC_STOP] != 0.0) return;  // uniform over the block
-    double sum[2] = {0, 0};
-    for (int c = 0; c < ncols; ++c) {
-        double a = 0;
-        for (int64_t b = threadIdx.x; b < nblk; b += blockDim.x) a += partial[b * stride + c];
-        sum[c] = block_sum(a, s_red);
-    }
-    if (ss_partial) {  // stage 2 folded into this launch: s.s from its own partials first
-        double a = 0;
-        for (int64_t b = threadIdx.x; b < ss_nblk; b += blockDim.x) a += ss_partial[b];
-        a = block_sum(a, s_red);
-        if (threadIdx.x == 0) sc[SC_SS] = a;
+    double a, b;  // rh.r, r.r
+    if (stride == 2) {
+        sum_pairs(partial, nblk, s_red, a, b);
+    } else {
+        a = sum_partials(partial, nblk, stride, 0, s_red);
+        b = sum_partials(partial, nblk, stride, 1, s_red);
     }
     if (threadIdx.x != 0) return;
-    const double a = sum[0], b = sum[1];
-    switch (stage) {
-        case 0:  // rho = rh.r, rr = r.r of the starting residual
-            sc[SC_RHO] = a;
-            sc[SC_RR] = b;
-            sc[SC_RHO_PREV] = 1;
-            sc[SC_ALPHA] = 1;
-            sc[SC_OMEGA] = 1;
-            sc[SC_FLAG] = 0;
-            sc[SC_STOP] = 0;
-            break;
-        case 1:  // a = rh.v -> alpha
-            sc[SC_RHV] = a;
-            if (a != 0.0 && isfinite(a)) {
-                sc[SC_ALPHA] = sc[SC_RHO] / a;
-            } else {
-                sc[SC_ALPHA] = 0;
-                sc[SC_FLAG] = 1;  // breakdown: the caller restarts
-            }
-            break;
-        case 2:  // a = s.s
-            sc[SC_SS] = a;
-            break;
-        case 3:  // a = s.t, b = t.t -> omega
-            sc[SC_ST] = a;
-            sc[SC_TT] = b;
-            if (b > 0.0 && isfinite(a) && isfinite(b)) {
-                // minimal-residual omega, enlarged when the angle between t and s is wide:
-                // |cos(t, s)| < 0.7 -> omega *= 0.7 / |cos| (Sleijpen & van der Vorst 1995),
-                // which keeps omega away from 0 where plain BiCGStab stagnates
-                double omega = a / b;
-                const double cs = a / sqrt(b * sc[SC_SS]);
-                if (isfinite(cs) && fabs(cs) < 0.7 && fabs(cs) > 0.0) omega *= 0.7 / fabs(cs);
-                sc[SC_OMEGA] = omega;
-            } else {
-                sc[SC_OMEGA] = 0;
-                sc[SC_FLAG] = 2;
-            }
-            if (sc[SC_OMEGA] == 0.0 && sc[SC_FLAG] == 0) sc[SC_FLAG] = 2;
-            break;
-        default: {  // a = rh.r, b = r.r of the new residual
-            sc[SC_RHO_PREV] = sc[SC_RHO];
-            sc[SC_RHO] = a;
-            sc[SC_RR] = b;
-            if (!isfinite(b)) sc[SC_FLAG] = 3;
-            const double rr = sqrt(b);
-            if (sc[SC_FLAG] != 0.0 || rr <= stop_abs || !isfinite(rr)) sc[SC_STOP] = 1;
-            break;
-        }
+    if (stage == 0) {  // the starting residual
+        sc[SC_RHO] = a;
+        sc[SC_RR] = b;
+        sc[SC_RHO_PREV] = 1;
+        sc[SC_ALPHA] = 1;
+        sc[SC_OMEGA] = 1;
+        sc[SC_FLAG] = 0;
+        sc[SC_STOP] = 0;
+    } else {  // the new residual of an iteration, and the stop test
+        sc[SC_RHO_PREV] = sc[SC_RHO];
+        sc[SC_RHO] = a;
+        sc[SC_RR] = b;
+        if (!isfinite(b)) sc[SC_FLAG] = 3;
+        const double rr = sqrt(b);
+        if (sc[SC_FLAG] != 0.0 || rr <= stop_abs || !isfinite(rr)) sc[SC_STOP] = 1;
     }
     if (host_slot)  // the scalars straight into the host-visible slot the host waits on
         for (int k = 0; k < SC_COUNT; ++k) host_slot[k] = sc[k];
@@ -723,21 +787,29 @@ void bicg_p_dev(hipStream_t s, int64_t n, const T* r, const T* v, const double* 
                        p);
 }
 template <typename T>
-void bicg_s_dev(hipStream_t s, int64_t n, const T* v, const T* r, const double* sc, T* s_out, double* partial,
-                int nblk) {
+void bicg_s_dev(hipStream_t s, int64_t n, const T* v, const T* r, double* sc, T* s_out, double* partial, int nblk,
+                const double* dots, int64_t dots_nb) {
     check_vec4_aligned<T>({v, r, s_out});
-    hipLaunchKernelGGL((k_bicg_s_dev<T>), dim3(nblk), dim3(kBlock), 0, s, n, v, r, sc, s_out, partial);
+    if (partial == dots) throw Error(PBSM3D_EINVAL, "s.s partials must not overwrite the dots they follow");
+    hipLaunchKernelGGL((k_bicg_s_dev<T>), dim3(nblk), dim3(kBlock), 0, s, n, v, r, sc, s_out, partial, dots, dots_nb);
 }
 template <typename T>
-void bicg_xr_dev(hipStream_t s, int64_t n, const T* ph, const T* sh, const T* sv, const T* tv, const T* rh,
-                 const double* sc, T* x, T* r, double* partial, int nblk) {
+void bicg_xr_dev(hipStream_t s, int64_t n, const T* ph, const T* sh, const T* sv, const T* tv, const T* rh, double* sc,
+                 T* x, T* r, double* partial, int nblk, const double* dots, int64_t dots_nb, const double* ss_part,
+                 int64_t ss_nb) {
     check_vec4_aligned<T>({ph, sh, sv, tv, rh, x, r});
-    hipLaunchKernelGGL((k_bicg_xr_dev<T>), dim3(nblk), dim3(kBlock), 0, s, n, ph, sh, sv, tv, rh, sc, x, r, partial);
+    if (partial == dots || partial == ss_part)
+        throw Error(PBSM3D_EINVAL, "x/r partials must not overwrite the partials they read");
+    if (reinterpret_cast<uintptr_t>(dots) % 16 != 0) throw Error(PBSM3D_EINVAL, "dot pairs must be 16-byte aligned");
+    hipLaunchKernelGGL((k_bicg_xr_dev<T>), dim3(nblk), dim3(kBlock), 0, s, n, ph, sh, sv, tv, rh, sc, x, r, partial,
+                       dots, dots_nb, ss_part, ss_nb);
 }
-void bicg_stage(hipStream_t s, const double* partial, int64_t nblk, int stride, int ncols, int stage, double* sc,
-                double stop_abs, double* host_slot, const double* ss_partial, int64_t ss_nblk) {
-    hipLaunchKernelGGL(k_bicg_stage, dim3(1), dim3(kStageThreads), 0, s, partial, nblk, stride, ncols, stage, sc,
-                       stop_abs, host_slot, ss_partial, ss_nblk);
+void bicg_stage(hipStream_t s, const double* partial, int64_t nblk, int stride, int stage, double* sc,
+                double stop_abs, double* host_slot) {
+    if (stride == 2 && reinterpret_cast<uintptr_t>(partial) % 16 != 0)
+        throw Error(PBSM3D_EINVAL, "dot pairs must be 16-byte aligned");
+    hipLaunchKernelGGL(k_bicg_stage, dim3(1), dim3(kStageThreads), 0, s, partial, nblk, stride, stage, sc, stop_abs,
+                       host_slot);
 }
 
 template <typename T>
@@ -779,9 +851,10 @@ void copy_f64(hipStream_t s, int64_t n, const double* a, double* b) {
                               const T*, const T*, T*, double*);                                                    \
     template void axpy_dots<T>(hipStream_t, int64_t, double, const T*, const T*, T*, const T*, double*, int);      \
     template void bicg_p_dev<T>(hipStream_t, int64_t, const T*, const T*, const double*, T*);                     \
-    template void bicg_s_dev<T>(hipStream_t, int64_t, const T*, const T*, const double*, T*, double*, int);       \
-    template void bicg_xr_dev<T>(hipStream_t, int64_t, const T*, const T*, const T*, const T*, const T*,          \
-                                 const double*, T*, T*, double*, int);
+    template void bicg_s_dev<T>(hipStream_t, int64_t, const T*, const T*, double*, T*, double*, int,             \
+                                const double*, int64_t);                                                          \
+    template void bicg_xr_dev<T>(hipStream_t, int64_t, const T*, const T*, const T*, const T*, const T*, double*, \
+                                 T*, T*, double*, int, const double*, int64_t, const double*, int64_t);
 PB_INST(double)
 PB_INST(float)
 #undef PB_INST
