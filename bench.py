@@ -24,6 +24,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import statistics
 import os
 import sys
 import time
@@ -150,11 +151,14 @@ def run_variant(pb, g, dev_forcing, dev_outs, cfg, net, steps, warmup, torch, sp
     stats = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    marks = [t0]
     for _ in range(steps):
-        _, st = ctx.step(dev_forcing, outs=dev_outs, on_device=True)
+        _, st = ctx.step(dev_forcing, outs=dev_outs, on_device=True)  # returns with the step complete
         stats.append(st)
+        marks.append(time.perf_counter())
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    step_ms = [1000.0 * (b - a) for a, b in zip(marks, marks[1:])]
     prof = ctx.profile_read(reset=True)
     ctx.profile(True)
     for _ in range(BREAKDOWN_STEPS):
@@ -166,7 +170,7 @@ def run_variant(pb, g, dev_forcing, dev_outs, cfg, net, steps, warmup, torch, sp
     tuning = {"spmv_fp64": ctx.spmv_tuning(0), "spmv_fp32": ctx.spmv_tuning(1),
               "dilu_fp64": ctx.precond_tuning(0), "dilu_fp32": ctx.precond_tuning(1)}
     ctx.close()
-    return (t1 - t0), stats, prof, breakdown, nnz, tuning
+    return (t1 - t0), stats, prof, breakdown, nnz, tuning, step_ms
 
 
 BREAKDOWN_STEPS = 2
@@ -338,12 +342,14 @@ def main():
     for name in names:
         cfg, learned = variant_config(pb, abi, name, L, krylov_mixed)
         log(f"[bench] variant {name}: warmup {args.warmup}, steps {args.steps}")
-        secs, stats, prof, breakdown, nnz, tuning = run_variant(pb, g, dev_forcing, dev_outs, cfg, net if learned else None, args.steps,
-                                                     args.warmup, torch, args.spmv_shape, args.dilu_mode)
+        secs, stats, prof, breakdown, nnz, tuning, step_ms = run_variant(
+            pb, g, dev_forcing, dev_outs, cfg, net if learned else None, args.steps, args.warmup, torch,
+            args.spmv_shape, args.dilu_mode)
         st = stats[-1]
         variants[name] = {
             "steps_per_s": args.steps / secs,
             "ms_per_step": 1000.0 * secs / args.steps,
+            "ms_per_step_median": round(statistics.median(step_ms), 3),
             "krylov": {abi.KRYLOV_BICGSTAB: "bicgstab", abi.KRYLOV_BICGSTAB_SPLIT: "bicgstab_split"}.get(
                 cfg.krylov, f"gmres({cfg.restart})"),
             "precision": "fp64" if cfg.precision == abi.FP64 else "fp32-inner/fp64-refinement",
