@@ -92,3 +92,27 @@ def test_context_calls_reject_null_context():
     assert L.pbsm3d_profile_classes(None, 1) == abi.PBSM3D_EINVAL
     assert L.pbsm3d_precond_select(None, 0, 1) == abi.PBSM3D_EINVAL
     assert L.pbsm3d_spmv_select(None, 0, 1) == abi.PBSM3D_EINVAL
+
+
+def test_empty_inputs_rejected_without_gpu():
+    """A mesh with no faces is refused by every entry point before any device call."""
+    import ctypes as C
+
+    from chm_amd import pbsm3d
+
+    L = pbsm3d.lib()
+    assert L.pbsm3d_face_geometry(0, None, None, None, *([None] * 7)) == abi.PBSM3D_EINVAL
+    assert L.pbsm3d_face_colors(0, None, None) == abi.PBSM3D_EINVAL
+    h = C.c_void_p()
+    x = np.zeros(1)
+    nb = np.full(3, -1, dtype=np.int32)
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    assert L.pbsm3d_snow_create(C.byref(h), 0, 0, None, C.byref(abi.SnowConfig())) == abi.PBSM3D_EINVAL
+    assert L.pbsm3d_fetch_create(C.byref(h), 0, 0, dp(x), dp(x), dp(x), None,
+                                 C.byref(abi.FetchConfig())) == abi.PBSM3D_EINVAL
+    assert L.pbsm3d_wind_create(C.byref(h), 0, 0, nb.ctypes.data_as(C.POINTER(C.c_int32)), dp(x), dp(x), None, None,
+                                C.byref(abi.WindConfig())) == abi.PBSM3D_EINVAL
+    assert h.value is None
+    m = abi.Mesh()
+    m.n_faces = 0
+    assert L.pbsm3d_create(C.byref(h), C.byref(m), C.byref(pbsm3d.default_config()), 0) == abi.PBSM3D_EINVAL
