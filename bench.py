@@ -480,6 +480,19 @@ def main():
             rooflines[kname]["traffic_source"] = traffic["source"]
     dom = max(rooflines, key=lambda n: rooflines[n]["ms_total"]) if rooflines else None
     roofline = rooflines.get(dom)
+    # BiCGStab vector work (p; alpha + s; omega + x, r; the rho / stop stage), from the untimed
+    # steps with every class bracketed: 15 n-vectors of traffic per iteration, suspension
+    # (n = N, the inner precision) and deposition (n = F, FP64)
+    bd = head.get(f"kernel_breakdown_{BREAKDOWN_STEPS}_steps", {}).get("ortho")
+    if head["krylov"] == "bicgstab_split" and bd and bd["ms_total"] > 0:
+        algo = BREAKDOWN_STEPS * 15 * (head["susp_iters"] * N * s_bytes + head["dep_iters"] * F * 8)
+        achieved = algo / (bd["ms_total"] * 1e-3) / 1e9
+        rooflines["ortho"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                              "kernel": "ortho (BiCGStab vector kernels k_bicg_p_dev / k_bicg_s_dev / k_bicg_xr_dev "
+                                        "and k_bicg_stage, suspension and deposition)",
+                              "algorithmic_bytes": algo, "ms_total": bd["ms_total"], "brackets": bd["launches"],
+                              "source": f"kernel_breakdown_{BREAKDOWN_STEPS}_steps (HIP events, every class bracketed)"}
 
     cpu = cpu1 = None
     if not args.no_cpu_baseline:
