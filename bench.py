@@ -279,7 +279,7 @@ def main():
     ap.add_argument("--train-epochs", type=int, default=8)
     ap.add_argument("--gain-seeds", default="100,101,102,103",
                     help="held-out forcing draws on which the learned start is compared with the cold start")
-    ap.add_argument("--cpu-faces", type=int, default=200000)
+    ap.add_argument("--cpu-faces", type=int, default=400000)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fetch", action="store_true", help="skip timing the fetch (fetchr) kernel")
