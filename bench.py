@@ -493,6 +493,10 @@ def main():
                                         "and k_bicg_stage, suspension and deposition)",
                               "algorithmic_bytes": algo, "ms_total": bd["ms_total"], "brackets": bd["launches"],
                               "source": f"kernel_breakdown_{BREAKDOWN_STEPS}_steps (HIP events, every class bracketed)"}
+        tv = traffic.get(f"bicg_vectors_fp{8 * s_bytes}", {}).get("bytes_per_iteration")
+        if tv:  # measured HBM bytes of one suspension iteration's vector kernels, against 15 N s
+            rooflines["ortho"]["traffic_per_suspension_iteration"] = {"measured": tv, "algorithmic": 15 * N * s_bytes,
+                                                                      "source": traffic["source"]}
 
     cpu = cpu1 = None
     if not args.no_cpu_baseline:
