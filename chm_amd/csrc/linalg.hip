@@ -499,14 +499,32 @@ __global__ void __launch_bounds__(kBlock) k_bicg_s_dev(int64_t n, const T* __res
     __shared__ double s_red[kBlock / 64];
     __shared__ double s_bc;
     if (sc[SC_STOP] != 0.0) return;  // uniform over the grid
-    const double alpha = bicg_alpha(dots, dots_nb, sc, s_red, &s_bc);
-    double ss = 0;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t n4 = n / 4;
+    // FP64: operands loaded one step ahead, the first step's while alpha is summed (measured
+    // faster for FP64 vectors and slightly slower for FP32, which keeps the plain loop)
+    constexpr bool kAhead = sizeof(T) == 8;
+    Vec4<T> v_next = {}, r_next = {};
+    if (kAhead && t0 < n4) {
+        v_next = reinterpret_cast<const Vec4<T>*>(v)[t0];
+        r_next = reinterpret_cast<const Vec4<T>*>(r)[t0];
+    }
+    const double alpha = bicg_alpha(dots, dots_nb, sc, s_red, &s_bc);
+    double ss = 0;
     for (int64_t i = t0; i < n4; i += stride) {
-        const Vec4<T> vv = reinterpret_cast<const Vec4<T>*>(v)[i];
-        const Vec4<T> rv = reinterpret_cast<const Vec4<T>*>(r)[i];
+        Vec4<T> vv, rv;
+        if constexpr (kAhead) {
+            vv = v_next;
+            rv = r_next;
+            if (i + stride < n4) {
+                v_next = reinterpret_cast<const Vec4<T>*>(v)[i + stride];
+                r_next = reinterpret_cast<const Vec4<T>*>(r)[i + stride];
+            }
+        } else {
+            vv = reinterpret_cast<const Vec4<T>*>(v)[i];
+            rv = reinterpret_cast<const Vec4<T>*>(r)[i];
+        }
         Vec4<T> o;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
@@ -572,18 +590,29 @@ __global__ void __launch_bounds__(kBlock) k_bicg_xr_dev(int64_t n, const T* __re
     __shared__ double s_bc;
     if (sc[SC_STOP] != 0.0) return;  // uniform over the grid
     const double alpha = sc[SC_ALPHA];
-    const double omega = bicg_omega(dots, dots_nb, ss_part, ss_nb, sc, s_red, &s_bc);
-    double d1 = 0, d2 = 0;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t n4 = n / 4;
+    // FP64: operands one step ahead, the first step's arriving while omega is summed (as in
+    // k_bicg_s_dev, FP32 keeps the plain loop)
+    constexpr bool kAhead = sizeof(T) == 8;
+    Vec4<T> nx[6] = {};
+    auto load = [&](int64_t i) {
+        nx[0] = reinterpret_cast<const Vec4<T>*>(ph)[i];
+        nx[1] = reinterpret_cast<const Vec4<T>*>(sh)[i];
+        nx[2] = reinterpret_cast<const Vec4<T>*>(sv)[i];
+        nx[3] = reinterpret_cast<const Vec4<T>*>(tv)[i];
+        nx[4] = reinterpret_cast<const Vec4<T>*>(rh)[i];
+        nx[5] = reinterpret_cast<const Vec4<T>*>(x)[i];
+    };
+    if (kAhead && t0 < n4) load(t0);
+    const double omega = bicg_omega(dots, dots_nb, ss_part, ss_nb, sc, s_red, &s_bc);
+    double d1 = 0, d2 = 0;
     for (int64_t i = t0; i < n4; i += stride) {
-        const Vec4<T> phv = reinterpret_cast<const Vec4<T>*>(ph)[i];
-        const Vec4<T> shv = reinterpret_cast<const Vec4<T>*>(sh)[i];
-        const Vec4<T> svv = reinterpret_cast<const Vec4<T>*>(sv)[i];
-        const Vec4<T> tvv = reinterpret_cast<const Vec4<T>*>(tv)[i];
-        const Vec4<T> rhv = reinterpret_cast<const Vec4<T>*>(rh)[i];
-        Vec4<T> xv = reinterpret_cast<const Vec4<T>*>(x)[i];
+        if (!kAhead) load(i);
+        const Vec4<T> phv = nx[0], shv = nx[1], svv = nx[2], tvv = nx[3], rhv = nx[4];
+        Vec4<T> xv = nx[5];
+        if (kAhead && i + stride < n4) load(i + stride);
         Vec4<T> o;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
