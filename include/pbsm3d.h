@@ -110,8 +110,9 @@ typedef struct pbsm3d_config {
     int32_t block_size;           /* reserved (256)                                */
     int32_t krylov;               /* PBSM3D_KRYLOV_GMRES (reference) or _BICGSTAB  */
     double inner_rtol;            /* MIXED: relative residual each FP32 inner solve
-                                     aims for; 0 = default (1e-3 for the split
-                                     BiCGStab, 1e-4 otherwise)                     */
+                                     aims for; 0 = default (split BiCGStab: 2e-3
+                                     from x0 = 0, 1e-3 from the learned start;
+                                     1e-4 otherwise)                               */
 } pbsm3d_config;
 
 /* Krylov method of the suspension solve (the deposition solve is GMRES(restart)) */
