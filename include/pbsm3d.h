@@ -111,7 +111,7 @@ typedef struct pbsm3d_config {
     int32_t krylov;               /* PBSM3D_KRYLOV_GMRES (reference) or _BICGSTAB  */
     double inner_rtol;            /* MIXED: relative residual each FP32 inner solve
                                      aims for; 0 = default (split BiCGStab: 2e-3
-                                     from x0 = 0, 1e-3 from the learned start;
+                                     from x0 = 0, 5e-4 from the learned start;
                                      1e-4 otherwise)                               */
 } pbsm3d_config;
 
