@@ -83,6 +83,7 @@ void PBSM3D::init(mesh& domain) {
     c_.restart = cfg.get("solver_restart", 30);
     c_.precision = cfg.get("mixed_precision", false) ? PBSM3D_MIXED : PBSM3D_FP64;
     c_.krylov = cfg.get("solver_bicgstab", false) ? PBSM3D_KRYLOV_BICGSTAB : PBSM3D_KRYLOV_GMRES;
+    if (cfg.get("solver_bicgstab_split", false)) c_.krylov = PBSM3D_KRYLOV_BICGSTAB_SPLIT;  // the fastest solver
     c_.inner_rtol = cfg.get("solver_inner_rtol", 0.0);
     device_ = cfg.get("device", 0);
 
