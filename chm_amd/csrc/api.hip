@@ -119,6 +119,30 @@ void build_pattern(int64_t F, int L, const int32_t* neigh_fm, std::vector<int32_
         }
 }
 
+// The face adjacency the kernels assume (a triangulation's dual graph): indices in
+// range, no face its own neighbour, no neighbour listed twice, and every link
+// reciprocated by exactly one edge of the neighbour (the D-ILU factorisation reads
+// the neighbour's coupling back through that edge).
+void check_neighbours(int64_t F, const int32_t* neigh) {
+    for (int64_t k = 0; k < 3 * F; ++k)
+        if (neigh[k] < -1 || neigh[k] >= F) throw Error(PBSM3D_EINVAL, "neighbour index out of range");
+    for (int64_t i = 0; i < F; ++i) {
+        const int32_t* nb = neigh + 3 * i;
+        for (int j = 0; j < 3; ++j) {
+            const int32_t n = nb[j];
+            if (n < 0) continue;
+            if (n == i) throw Error(PBSM3D_EINVAL, "face " + std::to_string(i) + " is its own neighbour");
+            for (int q = j + 1; q < 3; ++q)
+                if (nb[q] == n) throw Error(PBSM3D_EINVAL, "face " + std::to_string(i) + " lists a neighbour twice");
+            int back = 0;
+            for (int g = 0; g < 3; ++g) back += neigh[3 * (int64_t)n + g] == (int32_t)i;
+            if (back != 1)
+                throw Error(PBSM3D_EINVAL, "neighbour link " + std::to_string(i) + " -> " + std::to_string(n) +
+                                               " is not reciprocated by exactly one edge");
+        }
+    }
+}
+
 void validate(const pbsm3d_mesh* m, const pbsm3d_config* c) {
     if (!m || !c) throw Error(PBSM3D_EINVAL, "null mesh or config");
     if (m->n_faces <= 0) throw Error(PBSM3D_EINVAL, "mesh has no faces");
@@ -144,9 +168,7 @@ void validate(const pbsm3d_mesh* m, const pbsm3d_config* c) {
         throw Error(PBSM3D_EINVAL, "preconditioner not available on the GPU path");
     if (c->dep_precond != PBSM3D_PC_NONE && c->dep_precond != PBSM3D_PC_JACOBI && c->dep_precond != PBSM3D_PC_MCILU0)
         throw Error(PBSM3D_EINVAL, "deposition preconditioner not available on the GPU path");
-    const int64_t F = m->n_faces;
-    for (int64_t k = 0; k < 3 * F; ++k)
-        if (m->neigh[k] < -1 || m->neigh[k] >= F) throw Error(PBSM3D_EINVAL, "neighbour index out of range");
+    check_neighbours(m->n_faces, m->neigh);
 }
 
 // edge arrays: face-major [3*i+j] (ABI) -> j-major [j*F+i] (device)
@@ -176,8 +198,9 @@ float ms_between(hipEvent_t a, hipEvent_t b) {
 // from x0 = 0, 2e-3 took 5 % less solve time than 1e-3 (1258 vs 1337 iterations); from the
 // learned start, which begins near 0.2 ||b||, tighter targets win (2e-4 .. 2e-3: 1297, 1277,
 // 1250, 1225, 1283, 1287, 1320, 1366 iterations for 2, 3, 4, 5, 7, 10, 15, 20 e-4).
+constexpr double kSplitInnerCold = 2e-3, kSplitInnerLearned = 5e-4;
 static double split_inner_rtol_default(const pbsm3d_ctx* c, const pbsm3d_config& cfg) {
-    return cfg.warm_start == PBSM3D_WS_LEARNED && c->warm != nullptr ? 5e-4 : 2e-3;
+    return cfg.warm_start == PBSM3D_WS_LEARNED && c->warm != nullptr ? kSplitInnerLearned : kSplitInnerCold;
 }
 
 // Suspension solve with the multicolour D-ILU(0) preconditioner, carried out in
@@ -237,7 +260,8 @@ SolveInfo solve_in_position_order(pbsm3d_ctx* c, const pbsm3d_config& cfg, int64
             Ah.prof_class = PBSM3D_K_PRECOND;
             si = bicgstab_split<float>(s, c->bw32, c->ws32, A, Ah, c->bp.p, c->xp.p, c->r_s.p, c->split32[0].p,
                                        c->split32[1].p, c->split32[2].p, cfg.max_iters, cfg.tol,
-                                       cfg.inner_rtol > 0 ? cfg.inner_rtol : split_inner_rtol_default(c, cfg));
+                                       cfg.inner_rtol > 0 ? cfg.inner_rtol : split_inner_rtol_default(c, cfg),
+                                       cfg.inner_rtol > 0 ? cfg.inner_rtol : kSplitInnerCold);
         }
     } else if (cfg.precision == PBSM3D_FP64) {
         PcFn<double> M = [c, F, L](hipStream_t st, const double* r, double* z) {
@@ -388,8 +412,15 @@ int pbsm3d_create(pbsm3d_ctx** out, const pbsm3d_mesh* m, const pbsm3d_config* c
             if (cfg->use_R94_lambda) {
                 if (m->lai) upload(c->lai, m->lai, F, s); else fill(c->lai, F, 0.0, s);
             } else {
-                if (m->stalk_number) upload(c->stalk_n, m->stalk_number, F, s); else fill(c->stalk_n, F, 1.0, s);
-                if (m->stalk_diameter) upload(c->stalk_d, m->stalk_diameter, F, s); else fill(c->stalk_d, F, 0.8, s);
+                // both stalk attributes, or the defaults N = 1, dv = 0.8 when either is
+                // missing (PBSM3D.cpp:302-309: a stalk_number already read is replaced too)
+                if (m->stalk_number && m->stalk_diameter) {
+                    upload(c->stalk_n, m->stalk_number, F, s);
+                    upload(c->stalk_d, m->stalk_diameter, F, s);
+                } else {
+                    fill(c->stalk_n, F, 1.0, s);
+                    fill(c->stalk_d, F, 0.8, s);
+                }
             }
         }
         if (m->is_water) upload(c->water, m->is_water, F, s);
@@ -564,8 +595,15 @@ int pbsm3d_step(pbsm3d_ctx* c, const pbsm3d_forcing* fo, int32_t dt_s, pbsm3d_ou
         a.pc_im32 = c->colf.im32.p;
         a.pc_invdiag = c->invdiag_s.p;
         a.fscal = c->fscal.p;
-        c->last_vw_dir = fdev[2];
-        c->last_t = fdev[4];
+        // the features of pbsm3d_training_pairs read this step's vw_dir and t later: keep
+        // context-owned copies (the caller's device forcing may be freed or reused by then)
+        for (int k : {2, 4}) {
+            double* keepk = c->forcing_stage.p + (size_t)k * F;
+            if (fdev[k] != keepk)
+                PB_HIP(hipMemcpyAsync(keepk, fdev[k], F * sizeof(double), hipMemcpyDeviceToDevice, s));
+        }
+        c->last_vw_dir = c->forcing_stage.p + 2 * (size_t)F;
+        c->last_t = c->forcing_stage.p + 4 * (size_t)F;
         {
             ProfScope ps(&c->prof, s, PBSM3D_K_ASSEMBLE);
             hipLaunchKernelGGL(k_assemble, dim3(grid_for(F)), dim3(kBlock), 0, s, a);
@@ -761,9 +799,8 @@ int pbsm3d_spmv_tuning(const pbsm3d_ctx* c, int which, int32_t* chosen, float* m
 
 int pbsm3d_face_colors(int64_t n_faces, const int32_t* neigh, int32_t* color) {
     if (n_faces <= 0 || !neigh || !color) return PBSM3D_EINVAL;
-    for (int64_t k = 0; k < 3 * n_faces; ++k)
-        if (neigh[k] < -1 || neigh[k] >= n_faces) return PBSM3D_EINVAL;
     try {
+        check_neighbours(n_faces, neigh);
         return face_colors(n_faces, neigh, color);
     } catch (const std::exception& e) {
         set_global_error(e.what());

@@ -21,8 +21,6 @@ constexpr double kRhoIce = 917.0;   // PhysConst.h:39
 constexpr double kZUR = 50.0;       // Atmosphere.h:31
 constexpr double kLMax = 40.0;      // PBSM3D.cpp:227
 
-__device__ __forceinline__ bool chm_nan(double v) { return v == -9999.0 || isnan(v); }
-
 // stdDryAirDensity of MeteoIO: standard-atmosphere pressure / (R_dry * T)
 __device__ __forceinline__ double dry_air_density(double z, double T) {
     const double lapse = 0.0065, R0 = 6356766.0, Rd = 287.058;
@@ -54,10 +52,10 @@ __global__ void __launch_bounds__(kBlock) k_assemble(AssembleArgs a) {
     if (a.use_exp_fetch || a.use_tanh_fetch) fetch = a.fetch ? a.fetch[i] : 1000.0;
     const double uref = a.U_R[i];
     double sd = a.snowdepthavg[i];
-    sd = chm_nan(sd) ? 0 : sd;
+    sd = chm_missing(sd) ? 0 : sd;
     const double u2 = a.U_2m[i];
     double swe = a.swe[i];
-    swe = chm_nan(swe) ? 0 : swe;
+    swe = chm_missing(swe) ? 0 : swe;
     double hdiff = 0;
     if (a.enable_veg) hdiff = fmax(0.0, a.canopy[i] - sd);
 
@@ -294,7 +292,7 @@ __global__ void __launch_bounds__(kBlock) k_post(PostArgs a) {
     for (int z = 0; z < a.L; ++z) {
         const int64_t r = (int64_t)z * a.F + i;
         double c = a.x[r];
-        c = c < 0 || chm_nan(c) ? 0 : c;
+        c = c < 0 || chm_missing(c) ? 0 : c;
         Qsusp += c * a.u_z[r] * a.dz;
         Qsubl += a.csubl[r] * c * a.dz;
     }
@@ -327,7 +325,7 @@ __global__ void __launch_bounds__(kBlock) k_deposition(DepositionArgs a) {
         if (!(um > 0) && n >= 0) {
             qt = a.Qsusp[n];
             qs = a.Qsalt[n];
-            if (chm_nan(qs)) qs = 0;
+            if (chm_missing(qs)) qs = 0;
         }
         if (n >= 0) {
             const double wj = a.dep_w[j * F + i];  // eps*E/dx, precomputed at init
@@ -347,9 +345,9 @@ __global__ void __launch_bounds__(kBlock) k_drift(DriftArgs a) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.F) return;
     const double q = a.qdep[i];
-    double mass = (chm_nan(q) ? 0 : q) * a.dt;
+    double mass = (chm_missing(q) ? 0 : q) * a.dt;
     double swe = a.swe[i];
-    swe = chm_nan(swe) ? 0 : swe;
+    swe = chm_missing(swe) ? 0 : swe;
     if (mass < 0 && fabs(mass) > swe) {
         a.more[i] = 1;
         mass = -swe;

@@ -54,7 +54,7 @@ struct pbsm3d_ctx {
     // learned warm start
     pb::Mlp* warm = nullptr;          // not owned
     pb::DevBuf<float> feat, pred;     // [F x 16], [F x 11]
-    const double* last_vw_dir = nullptr;  // forcing pointer of the current step (device)
+    const double* last_vw_dir = nullptr;  // this step's vw_dir and t, context-owned copies (device)
     const double* last_t = nullptr;
     pb::Profiler prof;
     std::string err;

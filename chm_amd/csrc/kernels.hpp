@@ -5,6 +5,10 @@
 
 namespace pb {
 
+// module_base::is_nan (src/modules/module_base.hpp:471-479): a face variable is
+// missing when it lies within 1e-5 of the -9999 default or is NaN.
+__device__ __forceinline__ bool chm_missing(double v) { return fabs(v - -9999.0) < 1e-5 || isnan(v); }
+
 // wavefront-wide sum (64 lanes)
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {

@@ -406,7 +406,7 @@ void add_into(hipStream_t s, int64_t n, double a, const float* d, double* x) { a
 template <typename T>
 SolveInfo bicgstab_split(hipStream_t s, BicgWork<T>& w, GmresWork<T>& gw, const CsrOp<double>& A64,
                          const CsrOp<T>& Ahat, const double* b, double* x, double* r, T* work1, T* work2, T* work3,
-                         int maxit, double tol, double inner_rtol) {
+                         int maxit, double tol, double inner_rtol, double inner_rtol_cold) {
     SolveInfo info;
     const int64_t n = A64.n;
     w.ensure(n);
@@ -417,7 +417,10 @@ SolveInfo bicgstab_split(hipStream_t s, BicgWork<T>& w, GmresWork<T>& gw, const 
         return info;
     }
     double beta = residual_norm(s, gw, A64, x, b, r);
-    if (drop_worse_start(s, info, beta, nb, x, n)) beta = residual_norm(s, gw, A64, x, b, r);
+    if (drop_worse_start(s, info, beta, nb, x, n)) {
+        beta = residual_norm(s, gw, A64, x, b, r);
+        if (inner_rtol_cold > 0) inner_rtol = inner_rtol_cold;  // the start is x0 = 0 after all
+    }
     const int pclass = Ahat.prof_class >= 0 ? Ahat.prof_class : PBSM3D_K_PRECOND;
     while (true) {
         info.resid = beta / nb;
@@ -450,10 +453,10 @@ SolveInfo bicgstab_split(hipStream_t s, BicgWork<T>& w, GmresWork<T>& gw, const 
 
 template SolveInfo bicgstab_split<double>(hipStream_t, BicgWork<double>&, GmresWork<double>&, const CsrOp<double>&,
                                           const CsrOp<double>&, const double*, double*, double*, double*, double*,
-                                          double*, int, double, double);
+                                          double*, int, double, double, double);
 template SolveInfo bicgstab_split<float>(hipStream_t, BicgWork<float>&, GmresWork<float>&, const CsrOp<double>&,
                                          const CsrOp<float>&, const double*, double*, double*, float*, float*, float*,
-                                         int, double, double);
+                                         int, double, double, double);
 
 SolveInfo bicgstab_mixed(hipStream_t s, BicgWork<float>& w, GmresWork<float>& gw, const CsrOp<double>& A64,
                          const CsrOp<float>& A32, const PcFn<float>& M32, const double* b, double* x, double* r,

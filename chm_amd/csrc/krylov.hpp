@@ -141,11 +141,12 @@ SolveInfo bicgstab_solve(hipStream_t s, BicgWork<double>& w, GmresWork<double>& 
 // transforms r / ||r|| with Ahat's left factor, runs unpreconditioned BiCGStab on
 // Ahat (storage T) from zero to a relative inner_rtol, maps the correction back
 // with the right factor and adds it to x in FP64.  Ahat must have .split set;
-// work1, work2 are n-vectors of T.
+// work1, work2 are n-vectors of T.  inner_rtol_cold (> 0) replaces inner_rtol when
+// the given start is dropped for x0 = 0 (a warm start worse than zero).
 template <typename T>
 SolveInfo bicgstab_split(hipStream_t s, BicgWork<T>& w, GmresWork<T>& gw, const CsrOp<double>& A64,
                          const CsrOp<T>& Ahat, const double* b, double* x, double* r, T* work1, T* work2, T* work3,
-                         int maxit, double tol, double inner_rtol);
+                         int maxit, double tol, double inner_rtol, double inner_rtol_cold = 0.0);
 // FP64 refinement around FP32 BiCGStab runs (FP64 residuals, FP64 dot accumulation)
 SolveInfo bicgstab_mixed(hipStream_t s, BicgWork<float>& w, GmresWork<float>& gw, const CsrOp<double>& A64,
                          const CsrOp<float>& A32, const PcFn<float>& M32, const double* b, double* x, double* r,

@@ -158,7 +158,7 @@ __global__ void __launch_bounds__(kBlock) k_snow_drift(int64_t F, SnowParams c, 
     Pack p;
     load(st, F, i, p);
     double mass = drift[i];
-    mass = (fabs(mass - -9999.0) < 1e-5 || isnan(mass)) ? 0 : mass;  // module_base::is_nan
+    mass = chm_missing(mass) ? 0 : mass;
     const double transport_density = mass < 0. ? p.rho : c.drift_density;
     adj_snow(p, c, mass / transport_density, mass);
     store(st, F, i, p);
@@ -177,7 +177,7 @@ __global__ void __launch_bounds__(kBlock) k_snow_init(int64_t F, SnowParams c, c
     p.z_s = 0.;
     if (swe0) {
         const double s = swe0[i];
-        if (!(fabs(s - -9999.0) < 1e-5 || isnan(s))) {
+        if (!chm_missing(s)) {
             p.rho = c.ic_rho;
             p.T_s = -10 + kFreeze;
             p.T_s_0 = -15. + kFreeze;

@@ -77,7 +77,7 @@ __global__ void __launch_bounds__(kBlock) k_point_scale(WindArgs a) {
     if (!a.ignore_canopy && a.canopy) Z_CanTop = a.canopy[i];
     const double Z_CanBot = Z_CanTop / 2.0;
     double snowdepthavg = a.snowdepthavg ? a.snowdepthavg[i] : 0.0;
-    if (fabs(snowdepthavg - -9999.0) < 1e-5 || isnan(snowdepthavg)) snowdepthavg = 0.0;  // module_base::is_nan
+    if (chm_missing(snowdepthavg)) snowdepthavg = 0.0;
     const double Z_2m_above_srf = snowdepthavg + 2.0;
     if (Z_2m_above_srf >= Z_R) {
         a.out[i] = U_R;

@@ -37,6 +37,7 @@ PBSM3D::PBSM3D(config_file cfg_) : module_base("PBSM3D", parallel::domain, std::
     use_R94_lambda = cfg.get("use_R94_lambda", true);
     provides("blowingsnow_probability");
     debug_output = cfg.get("debug_output", false);
+    checkpoint_sum_subl = cfg.get("checkpoint_sum_subl", false);
     provides("Qsubl");
     provides("Qsubl_mass");
     provides("sum_subl");
@@ -85,6 +86,10 @@ void PBSM3D::init(mesh& domain) {
     c_.krylov = cfg.get("solver_bicgstab", false) ? PBSM3D_KRYLOV_BICGSTAB : PBSM3D_KRYLOV_GMRES;
     if (cfg.get("solver_bicgstab_split", false)) c_.krylov = PBSM3D_KRYLOV_BICGSTAB_SPLIT;  // the fastest solver
     c_.inner_rtol = cfg.get("solver_inner_rtol", 0.0);
+    // launch shapes of the D-ILU sweeps / position-order SpMV (pbsm3d_precond_select,
+    // pbsm3d_spmv_select); -1 lets the library time them at the first solve
+    split_mode_ = cfg.get("solver_sweep_mode", -1);
+    spmv_variant_ = cfg.get("solver_spmv_variant", -1);
     device_ = cfg.get("device", 0);
 
     const size_t F = domain->size_faces();
@@ -125,9 +130,13 @@ void PBSM3D::init(mesh& domain) {
                     sn.assign(F, 1.0);  // PBSM3D.cpp:303-309 defaults when stalk data are absent
                     sd.assign(F, 0.8);
                 }
+                // both attributes or neither: a face missing either keeps the defaults
+                // N = 1, dv = 0.8, even when stalk_number was read (PBSM3D.cpp:302-309)
                 try {
-                    sn[i] = f->veg_attribute("stalk_number");
-                    sd[i] = f->veg_attribute("stalk_diameter");
+                    const double n_i = f->veg_attribute("stalk_number");
+                    const double d_i = f->veg_attribute("stalk_diameter");
+                    sn[i] = n_i;
+                    sd[i] = d_i;
                 } catch (const std::out_of_range&) {
                 }
             }
@@ -152,6 +161,9 @@ void PBSM3D::init(mesh& domain) {
     if (ctx_) pbsm3d_destroy(ctx_);
     ctx_ = nullptr;
     check(pbsm3d_create(&ctx_, &m, &c_, device_), nullptr);
+    const int which = c_.precision == PBSM3D_MIXED ? 1 : 0;
+    if (split_mode_ >= 0) check(pbsm3d_precond_select(ctx_, which, split_mode_), ctx_);
+    if (spmv_variant_ >= 0) check(pbsm3d_spmv_select(ctx_, which, spmv_variant_), ctx_);
 }
 
 // PBSM3D.cpp:400-1748: one timestep, fields read from and written to the face variables.
@@ -182,9 +194,16 @@ void PBSM3D::run(mesh& domain) {
     check(pbsm3d_step(ctx_, &fo, dt, &o, &stats_), ctx_);
 }
 
-// PBSM3D.cpp:1753-1773: sum_drift by local face index.
+// PBSM3D.cpp:1753-1773: sum_drift by local face index; with checkpoint_sum_subl also
+// the module's sum_subl accumulator (opt-in: not part of the reference's checkpoint).
 void PBSM3D::checkpoint(mesh& domain, checkpoint_store& chkpt) {
     chkpt.vars["PBSM3D:sum_drift"] = domain->variable("sum_drift");
+    if (checkpoint_sum_subl) {
+        if (!ctx_) throw module_error("PBSM3D: checkpoint() before init()");
+        std::vector<double> ss(domain->size_faces());
+        check(pbsm3d_get_state(ctx_, nullptr, ss.data()), ctx_);
+        chkpt.vars["PBSM3D:sum_subl"] = std::move(ss);
+    }
 }
 
 void PBSM3D::load_checkpoint(mesh& domain, checkpoint_store& chkpt) {
@@ -193,6 +212,15 @@ void PBSM3D::load_checkpoint(mesh& domain, checkpoint_store& chkpt) {
         throw module_error("PBSM3D: checkpoint has no PBSM3D:sum_drift of the right size");
     domain->variable("sum_drift") = it->second;
     if (ctx_) check(pbsm3d_set_state(ctx_, it->second.data(), nullptr), ctx_);
+    if (checkpoint_sum_subl) {
+        auto ss = chkpt.vars.find("PBSM3D:sum_subl");
+        if (ss == chkpt.vars.end() || ss->second.size() != domain->size_faces())
+            throw module_error("PBSM3D: checkpoint_sum_subl is set but the checkpoint has no PBSM3D:sum_subl "
+                               "of the right size");
+        if (!ctx_) throw module_error("PBSM3D: load_checkpoint() before init()");
+        check(pbsm3d_set_state(ctx_, nullptr, ss->second.data()), ctx_);
+        domain->variable("sum_subl") = ss->second;
+    }
 }
 
 }  // namespace chm

@@ -27,12 +27,16 @@ class PBSM3D : public module_base {
     // config keys read in the constructor (PBSM3D.cpp:123-145)
     bool use_exp_fetch, use_tanh_fetch, use_PomLi_probability, z0_ustar_coupling;
     bool use_subgrid_topo, use_subgrid_topo_V2, use_R94_lambda, debug_output;
+    // extension key checkpoint_sum_subl (default false): the checkpoint also carries
+    // sum_subl, which the reference keeps in module data only and does not save
+    bool checkpoint_sum_subl;
 
   private:
     pbsm3d_config c_{};
     pbsm3d_ctx* ctx_ = nullptr;
     pbsm3d_stats stats_{};
     int device_ = 0;
+    int split_mode_ = -1, spmv_variant_ = -1;  // fixed launch shapes (-1: the library tunes)
 };
 
 }  // namespace chm

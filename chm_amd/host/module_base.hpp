@@ -6,6 +6,7 @@
 // CHM_THROW_EXCEPTION(module_error, ...) in src/exception.hpp:133).
 #pragma once
 
+#include <cmath>
 #include <map>
 #include <sstream>
 #include <stdexcept>
@@ -79,8 +80,8 @@ class module_base {
     void provides(const std::string& v) { provides_.push_back(v); }
     // an input used when some module provides it (module_base.hpp:319-341)
     void optional(const std::string& v) { optional_.push_back(v); }
-    // module_base::is_nan (module_base.hpp:471-479): -9999 or NaN
-    static bool is_nan(double v) { return v == -9999.0 || v != v; }
+    // module_base::is_nan (module_base.hpp:471-479): within 1e-5 of -9999, or NaN
+    static bool is_nan(double v) { return std::fabs(v - -9999.0) < 1e-5 || v != v; }
     config_file cfg;
 
   private:

@@ -149,10 +149,13 @@ static int run(int argc, char** argv) {
     mod.run(m);
     mod2.run(m2);
     const bool same = m->variable("sum_drift") == m2->variable("sum_drift");
+    const bool same_subl = m->variable("sum_subl") == m2->variable("sum_subl");
     std::printf("{\"faces\": %zu, \"susp_iters\": %d, \"dep_iters\": %d, \"susp_resid\": %.3e, \"dep_resid\": %.3e, "
-                "\"suspension_present\": %d, \"deposition_present\": %d, \"restart_identical\": %s}\n",
+                "\"suspension_present\": %d, \"deposition_present\": %d, \"susp_refinements\": %d, "
+                "\"restart_identical\": %s, \"restart_sum_subl_identical\": %s, \"checkpoint_keys\": %zu}\n",
                 F, st.susp_iters, st.dep_iters, st.susp_resid, st.dep_resid, st.suspension_present,
-                st.deposition_present, same ? "true" : "false");
+                st.deposition_present, st.susp_refinements, same ? "true" : "false", same_subl ? "true" : "false",
+                chk.vars.size());
     return 0;
 }
 

@@ -49,7 +49,8 @@ extern "C" {
 #define PBSM3D_PC_NONE 0
 #define PBSM3D_PC_JACOBI 1
 #define PBSM3D_PC_COLUMN 2   /* per-face vertical tridiagonal block (Thomas, nLayer unknowns) */
-#define PBSM3D_PC_ILU0 3     /* ILU(0) on the CSR, level-scheduled */
+#define PBSM3D_PC_ILU0 3     /* ILU(0) on the CSR in the natural order (CPU oracle only; the GPU
+                                path uses the multicolour form PBSM3D_PC_MCILU0) */
 #define PBSM3D_PC_ILUT 4     /* ILUT(1e-4, fill 3) as the reference's Ifpack2 setup (CPU oracle only) */
 #define PBSM3D_PC_MCILU0 5   /* D-ILU(0) in the multicolour order (face colour, layer, face): faces are
                                 greedily coloured so that no two neighbours share a colour; each colour is
@@ -72,7 +73,9 @@ typedef struct pbsm3d_mesh {
     const double* canopy_height;  /* [F] veg_attribute("CanopyHeight") or NULL     */
     const double* lai;            /* [F] veg_attribute("LAI") or NULL              */
     const double* stalk_number;   /* [F] veg_attribute("stalk_number") or NULL     */
-    const double* stalk_diameter; /* [F] veg_attribute("stalk_diameter") or NULL   */
+    const double* stalk_diameter; /* [F] veg_attribute("stalk_diameter") or NULL;
+                                     either NULL: N = 1, dv = 0.8 on every face
+                                     (PBSM3D.cpp:302-309)                          */
     const uint8_t* is_water;      /* [F] module_base::is_water(face) or NULL       */
 } pbsm3d_mesh;
 

@@ -73,6 +73,10 @@ def lib():
         L.oracle_max_threads.restype = C.c_int
         L.oracle_set_ilut.argtypes = [C.c_double, C.c_double]
         L.oracle_set_ilut.restype = None
+        L.oracle_std_dry_air_density.argtypes = [C.c_double, C.c_double]
+        L.oracle_std_dry_air_density.restype = C.c_double
+        L.oracle_is_nan.argtypes = [C.c_double]
+        L.oracle_is_nan.restype = C.c_int
         _lib = L
     return _lib
 
@@ -203,6 +207,14 @@ def fetch(cx, cy, cz, canopy, cfg, vw_dir):
                        keep.dptr(canopy) if canopy is not None else None, C.byref(cfg), keep.dptr(vw_dir),
                        keep.dptr(out))
     return out
+
+
+def std_dry_air_density(altitude: float, temperature_k: float) -> float:
+    return lib().oracle_std_dry_air_density(float(altitude), float(temperature_k))
+
+
+def is_nan(v: float) -> bool:
+    return lib().oracle_is_nan(float(v)) == 1
 
 
 def expint_e1(x: float) -> float:

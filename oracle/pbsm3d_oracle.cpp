@@ -18,10 +18,14 @@
 //   * NearestNeighborProblem src/math/LinearAlgebra.cpp:31-275 (CSR pattern with
 //                       sorted columns, zeroSystem, sumInto, Solve, getRhsMax)
 //
-// Third-party arithmetic restated from published algorithms (parity unpinned:
-// nothing in the reference pins these values, see the tests' docstrings):
+// Third-party arithmetic restated from published algorithms:
 //   * MeteoIO Atmosphere::stdDryAirDensity (PBSM3D.cpp:784-786): standard
-//     atmosphere pressure over (R_dry * T).
+//     atmosphere pressure over (R_dry * T).  MeteoIO is not in the reference
+//     tree; the restatement is pinned against the U.S. Standard Atmosphere 1976
+//     table (pressure and density at geometric altitudes 0 - 5 km and at the
+//     11 km geopotential tropopause) in tests/test_oracle.py.
+// Parity unpinned (nothing in the reference or in published tables fixes the
+// iterates; only the stopping rule is matched, see the tests' docstrings):
 //   * Belos GMRES (restart 30, max 1000 iterations, tol 1e-8 on ||b-Ax||/||b||,
 //     right preconditioning) -- restarted GMRES of Saad & Schultz (1986) with
 //     classical Gram-Schmidt plus one re-orthogonalisation pass.
@@ -71,7 +75,8 @@ constexpr double kEarthR0 = 6356766.0;
 
 thread_local std::string g_err;
 
-inline bool chm_is_nan(double v) { return v == -9999.0 || std::isnan(v); }  // module_base.hpp:471-479
+// module_base::is_nan (module_base.hpp:471-479): within 1e-5 of the -9999 default, or NaN
+inline bool chm_is_nan(double v) { return std::fabs(v - -9999.0) < 1e-5 || std::isnan(v); }
 
 double std_dry_air_density(double altitude, double temperature) {
     const double expo = kGravity / (kLapse * kRDry);
@@ -817,8 +822,10 @@ int oracle_create(oracle_ctx** out, const pbsm3d_mesh* m, const pbsm3d_config* c
             if (cfg->use_R94_lambda) {
                 c->lai[i] = m->lai ? m->lai[i] : 0.0;
             } else {
-                c->stalk_n[i] = m->stalk_number ? m->stalk_number[i] : 1.0;
-                c->stalk_d[i] = m->stalk_diameter ? m->stalk_diameter[i] : 0.8;
+                // both stalk attributes or the defaults N = 1, dv = 0.8 (PBSM3D.cpp:302-309)
+                const bool both = m->stalk_number && m->stalk_diameter;
+                c->stalk_n[i] = both ? m->stalk_number[i] : 1.0;
+                c->stalk_d[i] = both ? m->stalk_diameter[i] : 0.8;
             }
         }
     } else {
@@ -1345,6 +1352,14 @@ int oracle_precond_apply(int64_t n, const int32_t* rp, const int32_t* col, const
     M->apply(r, z);
     return 0;
 }
+
+// MeteoIO stdDryAirDensity as restated above (PBSM3D.cpp:784-786), for the table checks
+double oracle_std_dry_air_density(double altitude, double temperature) {
+    return std_dry_air_density(altitude, temperature);
+}
+
+// module_base::is_nan as restated above (module_base.hpp:471-479)
+int oracle_is_nan(double v) { return chm_is_nan(v) ? 1 : 0; }
 
 void oracle_set_ilut(double tau, double level_of_fill) {
     g_ilut_tau = tau;

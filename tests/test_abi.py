@@ -116,3 +116,37 @@ def test_empty_inputs_rejected_without_gpu():
     m = abi.Mesh()
     m.n_faces = 0
     assert L.pbsm3d_create(C.byref(h), C.byref(m), C.byref(pbsm3d.default_config()), 0) == abi.PBSM3D_EINVAL
+
+
+def test_bad_neighbour_links_rejected_without_gpu():
+    """The kernels read a neighbour's coupling back through the edge that points to the
+    face, so pbsm3d_create and pbsm3d_face_colors refuse adjacency that is not a
+    triangulation's: a face that is its own neighbour, a neighbour listed twice, a link
+    not reciprocated (or reciprocated twice).  Refused before any device call."""
+    import ctypes as C
+
+    from chm_amd import pbsm3d
+
+    m = meshgen.synthetic_mesh(60, seed=2)
+    g0 = pbsm3d.face_geometry(m)
+    nb0 = g0.neigh.reshape(-1, 3)
+    i = int(np.flatnonzero((nb0 >= 0).all(axis=1))[0])  # an interior face
+    n = int(nb0[i, 0])
+    bad = []
+    a = nb0.copy(); a[i, 1] = i; bad.append(("self", a))
+    a = nb0.copy(); a[i, 1] = a[i, 0]; bad.append(("twice", a))
+    a = nb0.copy(); a[n, list(a[n]).index(i)] = -1; bad.append(("one-way", a))
+    a = nb0.copy(); a[i, 0] = -1; bad.append(("one-way back", a))
+    L = pbsm3d.lib()
+    col = np.zeros(m.n_faces, dtype=np.int32)
+    for name, a in bad:
+        g = pbsm3d.face_geometry(m)
+        g.neigh = np.ascontiguousarray(a.reshape(-1).astype(np.int32))
+        with pytest.raises(pbsm3d.Pbsm3dError) as e:
+            pbsm3d.Context(g, pbsm3d.default_config())
+        assert e.value.code == abi.PBSM3D_EINVAL, name
+        assert L.pbsm3d_face_colors(m.n_faces, g.neigh.ctypes.data_as(C.POINTER(C.c_int32)),
+                                    col.ctypes.data_as(C.POINTER(C.c_int32))) == abi.PBSM3D_EINVAL, name
+    # the unmodified mesh colours fine
+    assert L.pbsm3d_face_colors(m.n_faces, g0.neigh.ctypes.data_as(C.POINTER(C.c_int32)),
+                                col.ctypes.data_as(C.POINTER(C.c_int32))) >= 3

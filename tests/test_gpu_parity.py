@@ -181,6 +181,50 @@ def test_multistep_state_and_checkpoint(pb, oracle, mesh_small):
     np.testing.assert_array_equal(a["sum_subl"], b["sum_subl"])
 
 
+def test_missing_value_band(pb, oracle, mesh_small):
+    """module_base::is_nan (module_base.hpp:471-479) on the GPU: swe and snowdepthavg within
+    1e-5 of -9999 (and NaN) count as 0 in the saltation test, the layer heights and the drift
+    cap (PBSM3D.cpp:447, 469, 1723), exactly as zeros do; swe just outside the band
+    (-9999 +- 2e-5) is a value, as in the reference.  The other is_nan sites (the clamped
+    concentration, a neighbour's Qsalt, qdep) read values the step computes itself, so no
+    forcing reaches them with -9999; they share the one device helper with these."""
+    _print_case("missing_value_band")
+    F = mesh_small.n_faces
+    g = pb.face_geometry(mesh_small)
+    fo = meshgen.forcing_c1(F, seed=4)
+    rng = np.random.default_rng(4)
+    pick = rng.choice(F, F // 4, replace=False)
+    band = rng.choice([-9999.0, -9999.0 + 5e-6, -9999.0 - 5e-6, -9999.0 + 9e-6, -9999.0 - 9e-6, np.nan], len(pick))
+    half = len(pick) // 2
+    zero = dict(fo, swe=fo["swe"].copy(), snowdepthavg=fo["snowdepthavg"].copy())
+    zero["swe"][pick] = 0.0
+    zero["snowdepthavg"][pick[:half]] = 0.0
+    miss = dict(fo, swe=fo["swe"].copy(), snowdepthavg=fo["snowdepthavg"].copy())
+    miss["swe"][pick] = band
+    miss["snowdepthavg"][pick[:half]] = band[:half]
+    out = {}
+    for name, f in (("zero", zero), ("missing", miss)):
+        ctx = pb.Context(g, pb.default_config(nLayer=4, tol=1e-12))
+        out[name], st = ctx.step(f, want_fields=True)
+        assert st["suspension_present"] == 1 and st["susp_resid"] <= 1e-12
+    for k in abi.OUT_FIELDS:
+        np.testing.assert_array_equal(out["missing"][k], out["zero"][k], err_msg=k)
+    assert (out["missing"]["pbsm_more_than_avail"][pick] == 1).any()
+    # just outside the band: a (negative) swe value, against the oracle
+    rest = np.setdiff1d(np.arange(F), pick)[:20]
+    outside = dict(miss, swe=miss["swe"].copy())
+    outside["swe"][rest] = np.where(np.arange(len(rest)) % 2 == 0, -9999.0 + 2e-5, -9999.0 - 2e-5)
+    ctx = pb.Context(g, pb.default_config(nLayer=4, tol=1e-12))
+    orc = oracle.Oracle(g, oracle.oracle_config(nLayer=4, tol=1e-13))
+    gg, _ = ctx.step(outside, want_fields=True)
+    og, _ = orc.step(outside)
+    np.testing.assert_allclose(gg["Qsalt"], og["Qsalt"], rtol=1e-12, atol=0)
+    np.testing.assert_array_equal(gg["pbsm_more_than_avail"], og["pbsm_more_than_avail"])
+    for k in ("conc", "Qsusp", "qdep", "drift_mass", "sum_drift"):
+        assert rel_err(gg[k], og[k]) < 1e-8, (k, rel_err(gg[k], og[k]))
+    assert not np.array_equal(gg["drift_mass"], out["missing"]["drift_mass"])
+
+
 def test_device_resident_forcing(pb, mesh_small):
     _print_case("device_forcing")
     import torch

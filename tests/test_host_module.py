@@ -53,8 +53,13 @@ def test_reference_mesh_json_roundtrip(tmp_path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("keys", [(), ("mixed_precision=1", "solver_bicgstab_split=1")])
+@pytest.mark.parametrize("keys", [(), ("mixed_precision=1", "solver_bicgstab_split=1", "solver_sweep_mode=1",
+                                       "solver_spmv_variant=1", "checkpoint_sum_subl=1")])
 def test_module_run_matches_abi(tmp_path, keys):
+    """Three steps of the C++ module equal three steps through the C ABI bit for bit; the
+    split mixed solver with its launch shapes pinned on both sides (the module keys
+    solver_sweep_mode / solver_spmv_variant), and the checkpoint restore continues
+    identically -- with checkpoint_sum_subl also in sum_subl."""
     from chm_amd import pbsm3d as pb
 
     print(f"[case] host module run {keys}", flush=True)
@@ -71,17 +76,20 @@ def test_module_run_matches_abi(tmp_path, keys):
     assert r.returncode == 0, r.stderr
     st = json.loads(r.stdout)
     assert st["restart_identical"] is True and st["susp_resid"] <= 1e-8
+    assert st["restart_sum_subl_identical"] is bool(keys) and st["checkpoint_keys"] == (2 if keys else 1)
     got = np.fromfile(obin, dtype="<f8").reshape(8, F)
     extra = dict(precision=abi.MIXED, krylov=abi.KRYLOV_BICGSTAB_SPLIT) if keys else {}
     ctx = pb.Context(pb.face_geometry(m), pb.default_config(nLayer=6, **extra))
+    if keys:
+        ctx.precond_select(1, 1)
+        ctx.spmv_select(1, 1)
     for _ in range(3):
-        ref, _ = ctx.step(fo)
+        ref, rst = ctx.step(fo)
+    assert st["susp_iters"] == rst["susp_iters"] and st["susp_refinements"] == rst["susp_refinements"]
+    if keys:
+        assert st["susp_refinements"] > 0
     for k, name in enumerate(abi.OUT_FIELDS):
-        if keys:  # each process tunes its own sweep launch shapes, which may order the dot sums apart
-            scale = max(1.0, float(np.abs(ref[name]).max()))
-            np.testing.assert_allclose(got[k], ref[name], rtol=0, atol=1e-4 * scale, err_msg=name)
-        else:
-            np.testing.assert_array_equal(got[k], ref[name], err_msg=name)
+        np.testing.assert_array_equal(got[k], ref[name], err_msg=name)
 
 
 @pytest.mark.gpu

@@ -121,15 +121,80 @@ def test_saltation_known_answers(oracle, mesh_small):
     assert (tr["saltation"] == 1).all()
     np.testing.assert_allclose(tr["hs"], 0.08436 * 0.43429448190325176 ** 1.27, rtol=1e-14)
     assert (tr["z0"] == 0.01).all()
-    # c_salt = rho_f/(3.29 u*) (1 - u*_th^2/u*^2), rho_f of the standard atmosphere at the centroid
-    z = g.center_z
-    p = 101325.0 * (1 - 0.0065 * 6356766.0 * z / (288.15 * (6356766.0 + z))) ** (9.80665 / (0.0065 * 287.058))
-    rho = p / (287.058 * 263.15)
+    # c_salt = rho_f/(3.29 u*) (1 - u*_th^2/u*^2), rho_f the dry-air density of the standard
+    # atmosphere at the centroid (pinned to the 1976 table in test_std_dry_air_density_us1976)
+    rho = np.array([oracle.std_dry_air_density(z, 263.15) for z in g.center_z])
     cs = rho / (3.29 * 0.43429448190325176) * (1 - (0.29552845528455285 / 0.43429448190325176) ** 2)
     np.testing.assert_allclose(tr["c_salt"], cs, rtol=1e-13)
-    assert abs(rho.min() - 101325.0 / (287.058 * 263.15) * 0.77) < 0.2  # 200-2000 m band, sanity
     # Qsalt = c_salt * 2.8 u*_th * hs
     np.testing.assert_allclose(outs["Qsalt"], cs * 2.8 * 0.29552845528455285 * tr["hs"], rtol=1e-13)
+
+
+# U.S. Standard Atmosphere 1976, geometric altitude Z (m): pressure (Pa), temperature (K),
+# density (kg m-3) as tabulated (five significant digits), and the 11 km geopotential
+# tropopause (Z = 11,019.07 m): 22,632.06 Pa, 216.65 K.
+_US1976 = [(0.0, 1.01325e5, 288.150, 1.2250), (1000.0, 8.9876e4, 281.651, 1.1117),
+           (2000.0, 7.9501e4, 275.154, 1.0066), (3000.0, 7.0121e4, 268.659, 0.90925),
+           (5000.0, 5.4048e4, 255.676, 0.73643)]
+_TROPOPAUSE = (6356766.0 * 11000.0 / (6356766.0 - 11000.0), 22632.06, 216.65)
+
+
+def test_std_dry_air_density_us1976(oracle):
+    """MeteoIO's stdDryAirDensity (PBSM3D.cpp:784-786; MeteoIO is not in the reference
+    tree) restated as p_std(z) / (R_dry T), checked against the published 1976 standard
+    atmosphere: p = rho(z, T) * R_dry * T must reproduce the tabulated pressure (3e-5:
+    the table's five digits plus MeteoIO's R_dry = 287.058 against the table's 287.0531)
+    and rho(z, T_table) the tabulated density (7e-5).  A wrong lapse rate, exponent, sea
+    level temperature, Earth radius (geometric vs geopotential height) or gas constant
+    moves one of these by more (shown below on perturbed restatements)."""
+    R = 287.058
+    for z, p_tab, t_tab, rho_tab in _US1976:
+        rho = oracle.std_dry_air_density(z, t_tab)
+        assert abs(rho * R * t_tab - p_tab) / p_tab < 3e-5, (z, rho * R * t_tab, p_tab)
+        assert abs(rho - rho_tab) / rho_tab < 7e-5, (z, rho, rho_tab)
+        # the density is linear in 1/T at fixed altitude
+        np.testing.assert_allclose(oracle.std_dry_air_density(z, 2 * t_tab), rho / 2, rtol=1e-15)
+    zt, pt, tt = _TROPOPAUSE
+    assert abs(oracle.std_dry_air_density(zt, tt) * R * tt - pt) / pt < 3e-5
+    # the checks discriminate: plausible mistakes in the formula miss the table by more
+    def p_variant(z, lapse=0.0065, t0=288.15, r0=6356766.0, rgas=287.058, g0=9.80665):
+        h = z if r0 is None else r0 * z / (r0 + z)
+        return 101325.0 * (1 - lapse * h / t0) ** (g0 / (lapse * rgas))
+    for kw in (dict(r0=None), dict(lapse=0.0060), dict(t0=273.15), dict(rgas=287.0), dict(g0=9.81)):
+        worst = max(abs(p_variant(z, **kw) - p_tab) / p_tab for z, p_tab, _, _ in _US1976 + [(zt, pt, 0, 0)])
+        assert worst > 6e-5, (kw, worst)
+
+
+def test_is_nan_band(oracle):
+    """module_base::is_nan (module_base.hpp:471-479): a value counts as missing within
+    1e-5 of -9999 (either side) or when NaN; just outside the band it is a value."""
+    for v in (-9999.0, -9999.0 + 5e-6, -9999.0 - 5e-6, -9999.0 + 9.9e-6, float("nan")):
+        assert oracle.is_nan(v), v
+    for v in (-9999.0 + 2e-5, -9999.0 - 2e-5, -9998.0, 0.0, 9999.0, float("inf"), -float("inf")):
+        assert not oracle.is_nan(v), v
+
+
+def test_missing_swe_and_depth_act_as_zero(oracle, mesh_small):
+    """swe and snowdepthavg within the -9999 band enter the step as 0 (PBSM3D.cpp:447, 469,
+    1723): the fields equal those of a forcing with zeros in those faces."""
+    F = mesh_small.n_faces
+    fo = meshgen.forcing_c1(F, seed=4)
+    rng = np.random.default_rng(4)
+    pick = rng.choice(F, F // 5, replace=False)
+    band = rng.choice([-9999.0, -9999.0 + 5e-6, -9999.0 - 5e-6, -9999.0 + 9e-6, np.nan], len(pick))
+    zero = dict(fo, swe=fo["swe"].copy(), snowdepthavg=fo["snowdepthavg"].copy())
+    zero["swe"][pick] = 0.0
+    zero["snowdepthavg"][pick[: len(pick) // 2]] = 0.0
+    miss = dict(fo, swe=fo["swe"].copy(), snowdepthavg=fo["snowdepthavg"].copy())
+    miss["swe"][pick] = band
+    miss["snowdepthavg"][pick[: len(pick) // 2]] = band[: len(pick) // 2]
+    _, oz, sz, _ = _run(oracle, mesh_small, zero, nLayer=4, tol=1e-13)
+    _, om, sm, _ = _run(oracle, mesh_small, miss, nLayer=4, tol=1e-13)
+    assert sz["suspension_present"] == sm["suspension_present"] == 1
+    for k in abi.OUT_FIELDS:
+        np.testing.assert_array_equal(om[k], oz[k], err_msg=k)
+    # eroding faces whose swe counts as 0 hit the more-than-available cap (PBSM3D.cpp:1724-1728)
+    assert (om["pbsm_more_than_avail"][pick] == 1).any()
 
 
 def test_tanh_fetch_factor_is_constant(oracle, mesh_small):
