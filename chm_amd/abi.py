@@ -76,6 +76,10 @@ class Config(C.Structure):
         ("block_size", C.c_int32),
         ("krylov", C.c_int32),
         ("inner_rtol", C.c_double),
+        ("use_PomLi_probability", C.c_int32),
+        ("z0_ustar_coupling", C.c_int32),
+        ("rouault_diffusion_coef", C.c_int32),
+        ("reserved_cfg", C.c_int32),
     ]
 
 
@@ -89,6 +93,7 @@ class Forcing(C.Structure):
         ("rh", _dp),
         ("snowdepthavg", _dp),
         ("fetch", _dp),
+        ("p_snow_hours", _dp),
         ("on_device", C.c_int32),
     ]
 
@@ -103,6 +108,7 @@ class Out(C.Structure):
         ("drift_mass", _dp),
         ("sum_drift", _dp),
         ("pbsm_more_than_avail", _dp),
+        ("blowingsnow_probability", _dp),
         ("conc", _dp),
         ("qdep", _dp),
         ("on_device", C.c_int32),
@@ -173,8 +179,9 @@ class Stats(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_ if not name.startswith("reserved")}
 
 
-OUT_FIELDS = ("Qsalt", "Qsusp", "Qsubl", "Qsubl_mass", "sum_subl", "drift_mass", "sum_drift", "pbsm_more_than_avail")
-FORCING_FIELDS = ("U_R", "U_2m_above_srf", "vw_dir", "swe", "t", "rh", "snowdepthavg", "fetch")
+OUT_FIELDS = ("Qsalt", "Qsusp", "Qsubl", "Qsubl_mass", "sum_subl", "drift_mass", "sum_drift", "pbsm_more_than_avail",
+              "blowingsnow_probability")
+FORCING_FIELDS = ("U_R", "U_2m_above_srf", "vw_dir", "swe", "t", "rh", "snowdepthavg", "fetch", "p_snow_hours")
 
 
 def default_config(**overrides) -> Config:
@@ -206,6 +213,9 @@ def default_config(**overrides) -> Config:
         block_size=256,
         krylov=KRYLOV_GMRES,
         inner_rtol=0.0,
+        use_PomLi_probability=0,
+        z0_ustar_coupling=0,
+        rouault_diffusion_coef=0,
     )
     for k, v in overrides.items():
         if not hasattr(c, k):

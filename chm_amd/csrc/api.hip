@@ -445,11 +445,12 @@ int pbsm3d_create(pbsm3d_ctx** out, const pbsm3d_mesh* m, const pbsm3d_config* c
         const int64_t N = F * c->L;
         c->u_z.alloc(N);
         c->csubl.alloc(N);
-        for (auto* b : {&c->Qsalt, &c->Qsusp, &c->Qsubl, &c->Qsubl_mass, &c->drift_mass, &c->more}) fill(*b, F, -9999.0, s);
+        for (auto* b : {&c->Qsalt, &c->Qsusp, &c->Qsubl, &c->Qsubl_mass, &c->drift_mass, &c->more, &c->blowprob})
+            fill(*b, F, -9999.0, s);
         fill(c->sum_subl, F, 0.0, s);
         fill(c->sum_drift, F, 0.0, s);  // PBSM3D.cpp:391
         c->salt_flag.alloc(F);
-        c->forcing_stage.alloc(8 * F);
+        c->forcing_stage.alloc(9 * F);
         if (cfg->precond == PBSM3D_PC_COLUMN) {
             c->colf.lo.alloc(N);
             c->colf.cp.alloc(N);
@@ -524,10 +525,13 @@ int pbsm3d_step(pbsm3d_ctx* c, const pbsm3d_forcing* fo, int32_t dt_s, pbsm3d_ou
             throw Error(PBSM3D_EINVAL, "forcing pointer is null");
         if ((cfg.use_exp_fetch || cfg.use_tanh_fetch) && !fo->fetch)
             throw Error(PBSM3D_EINVAL, "fetch forcing required by the fetch mode");
+        if (cfg.use_PomLi_probability && !fo->p_snow_hours)
+            throw Error(PBSM3D_EINVAL, "p_snow_hours forcing required by use_PomLi_probability");
 
-        const double* fin[8] = {fo->U_R, fo->U_2m_above_srf, fo->vw_dir, fo->swe, fo->t, fo->rh, fo->snowdepthavg, fo->fetch};
-        const double* fdev[8];
-        for (int k = 0; k < 8; ++k) {
+        const double* fin[9] = {fo->U_R, fo->U_2m_above_srf, fo->vw_dir, fo->swe,  fo->t,
+                                fo->rh,  fo->snowdepthavg,   fo->fetch,  fo->p_snow_hours};
+        const double* fdev[9];
+        for (int k = 0; k < 9; ++k) {
             if (!fin[k]) {
                 fdev[k] = nullptr;
             } else if (fo->on_device) {
@@ -553,6 +557,9 @@ int pbsm3d_step(pbsm3d_ctx* c, const pbsm3d_forcing* fo, int32_t dt_s, pbsm3d_ou
         a.use_tanh_fetch = cfg.use_tanh_fetch;
         a.use_exp_fetch = cfg.use_exp_fetch;
         a.use_R94_lambda = cfg.use_R94_lambda;
+        a.use_PomLi_probability = cfg.use_PomLi_probability;
+        a.z0_ustar_coupling = cfg.z0_ustar_coupling;
+        a.rouault_diffusion_coef = cfg.rouault_diffusion_coef;
         a.settling_velocity = cfg.settling_velocity;
         a.settling_scale = cfg.settling_scale;
         a.min_sd_trans = cfg.min_sd_trans;
@@ -579,6 +586,8 @@ int pbsm3d_step(pbsm3d_ctx* c, const pbsm3d_forcing* fo, int32_t dt_s, pbsm3d_ou
         a.rh = fdev[5];
         a.snowdepthavg = fdev[6];
         a.fetch = fdev[7];
+        a.p_snow_hours = fdev[8];
+        a.blowing_prob = c->blowprob.p;
         a.rp = c->susp.rp.p;
         a.val = c->susp.val.p;
         a.rhs = c->susp.rhs.p;
@@ -751,6 +760,7 @@ int pbsm3d_step(pbsm3d_ctx* c, const pbsm3d_forcing* fo, int32_t dt_s, pbsm3d_ou
             cp(out->drift_mass, c->drift_mass, F);
             cp(out->sum_drift, c->sum_drift, F);
             cp(out->pbsm_more_than_avail, c->more, F);
+            cp(out->blowingsnow_probability, c->blowprob, F);
             cp(out->conc, c->susp.x, N);
             cp(out->qdep, c->dep.x, F);
         }

@@ -38,6 +38,71 @@ __device__ __forceinline__ void downwind(double bearing, double& x, double& y) {
     y = -sin(th);
 }
 
+// u* of a saltation candidate with the z0 feedback of Li & Pomeroy 2000 eq. 5
+// (PBSM3D.cpp:691-716): the root of g(u) = u2 k / ln(2 / (c u^2 + lambda/2)) - u.
+// The bracket is searched as boost::math::tools::bracket_and_solve_root does with
+// guess 1, factor 1, rising = false and 500 evaluations: from g(1) >= 0 the
+// bracket walks up (u *= factor), else down (u /= factor); the factor doubles
+// after 32 evaluations, then after 16, 8, 4, 2 and at every evaluation after
+// that; a downward walk below the smallest normal number returns the bracket
+// (0, u), and running out of evaluations is a failure (no saltation).  Inside the
+// bracket the root is taken to full precision by bisection: the reference's
+// TOMS 748 stops when the bracket is narrower than 1e-8 and takes its midpoint,
+// which lies within 5e-9 of the same root.  ok = false: no bracket.
+__device__ double ustar_coupled(double u2, double lambda, bool& ok) {
+    auto g = [&](double u) { return u2 * kKappa / log(2.0 / (0.6131702345e-2 * u * u + .5 * lambda)) - u; };
+    auto sgn = [](double v) { return v == 0 ? 0 : (signbit(v) ? -1 : 1); };  // boost::math::sign
+    const int max_iter = 500;
+    double a = 1.0, b = 1.0, fa = g(a), fb = fa, factor = 1.0;
+    int count = max_iter - 1, step = 32;
+    ok = true;
+    if (!(fa < 0)) {  // (fa < 0) == rising with rising = false: the root lies above
+        while (sgn(fb) == sgn(fa)) {
+            if (count == 0) { ok = false; return 0.0; }
+            if ((max_iter - count) % step == 0) {
+                factor *= 2;
+                if (step > 1) step /= 2;
+            }
+            a = b;
+            fa = fb;
+            b *= factor;
+            fb = g(b);
+            --count;
+        }
+    } else {
+        while (sgn(fb) == sgn(fa)) {
+            if (fabs(a) < 2.2250738585072014e-308) return a / 2.0;  // bracket (0, a)
+            if (count == 0) { ok = false; return 0.0; }
+            if ((max_iter - count) % step == 0) {
+                factor *= 2;
+                if (step > 1) step /= 2;
+            }
+            b = a;
+            fb = fa;
+            a /= factor;
+            fa = g(a);
+            --count;
+        }
+    }
+    if (fa == 0) return a;
+    if (fb == 0) return b;
+    // bisection on [a, b] (a < b, g(a) and g(b) of opposite signs) to the last bit
+    double lo = a, hi = b, glo = fa;
+    for (int it = 0; it < 200; ++it) {
+        const double mid = lo + (hi - lo) / 2.0;
+        if (mid <= lo || mid >= hi) break;
+        const double gm = g(mid);
+        if (gm == 0) return mid;
+        if (sgn(gm) == sgn(glo)) {
+            lo = mid;
+            glo = gm;
+        } else {
+            hi = mid;
+        }
+    }
+    return lo + (hi - lo) / 2.0;
+}
+
 }  // namespace
 
 __global__ void __launch_bounds__(kBlock) k_assemble(AssembleArgs a) {
@@ -70,8 +135,19 @@ __global__ void __launch_bounds__(kBlock) k_assemble(AssembleArgs a) {
             lambda = 0.5 * (a.enable_veg ? a.lai[i] : 0.0) * hdiff;
         else
             lambda = (a.enable_veg ? a.stalk_n[i] * a.stalk_d[i] : 0.0) * hdiff;
-        ustar = u2 * kKappa / log(2.0 / 0.0002);
-        if (ustar >= uth) salt = true;
+        if (a.z0_ustar_coupling) {
+            // a failed search leaves u* at its placeholder 1.3, which the threshold test below
+            // still sees (PBSM3D.cpp:647, 710-714)
+            bool root_ok = true;
+            const double r = ustar_coupled(u2, lambda, root_ok);
+            if (root_ok) ustar = r;
+        } else {
+            ustar = u2 * kKappa / log(2.0 / 0.0002);
+        }
+        if (ustar >= uth) {
+            salt = true;
+            if (a.z0_ustar_coupling) z0 = 0.6131702345e-2 * ustar * ustar + .5 * lambda;  // Li & Pomeroy eq. 4
+        }
     }
     if (!salt) {
         z0 = a.z0_snow;
@@ -109,6 +185,24 @@ __global__ void __launch_bounds__(kBlock) k_assemble(AssembleArgs a) {
             c_salt *= 1.0 - exp(-3.0 * fetch / 500.0);
         else if (a.use_tanh_fetch && fetch <= 300.)
             c_salt *= 0.5 * tanh(0.1333333333e-1 * 300.0 - 2.0) + 0.5;  // uses fetch_ref, PBSM3D.cpp:839-845
+        if (a.use_PomLi_probability) {
+            // probability of blowing snow, Essery, Li & Pomeroy 1999 eqs 10-14 (PBSM3D.cpp:848-865),
+            // with the 10 m wind log-scaled from the reference height (PBSM3D.cpp:451-463)
+            const double z10 = 10. + sd;
+            const double u10 = z10 < kZUR ? uref * log((z10 - (sd + a.z0_snow)) / a.z0_snow) /
+                                                log((kZUR - (sd + a.z0_snow)) / a.z0_snow)
+                                          : uref;
+            const double A = a.p_snow_hours[i];
+            const double u_mean = 11.2 + 0.365 * T + 0.00706 * T * T + 0.9 * log(A);
+            const double delta = 0.145 * T + 0.00196 * T * T + 4.3;
+            const double Nst = (a.enable_veg && !a.use_R94_lambda) ? a.stalk_n[i] : 0.0;
+            const double dst = (a.enable_veg && !a.use_R94_lambda) ? a.stalk_d[i] : 0.0;
+            const double z0v = (Nst * dst * hdiff) / 2.0;
+            const double us = u10 / sqrt((1 + 340.0 * z0v));
+            const double Pu10 = 1.0 / (1.0 + exp((sqrt(kPi) * (u_mean - us)) / delta));
+            a.blowing_prob[i] = Pu10;
+            c_salt *= Pu10;
+        }
         const double uhs = 2.8 * uth;
         Qsalt = c_salt * uhs * hs;
         double mass = 0;
@@ -182,7 +276,10 @@ __global__ void __launch_bounds__(kBlock) k_assemble(AssembleArgs a) {
         const double csubl = a.do_sublimation ? dmdt / mm : 0.0;
 
         const double l = kKappa * (czl + z0) * kLMax / (kKappa * (czl + z0) + kLMax);
-        const double Kv = a.snow_diffusion_const * ustar * l;
+        // Rouault et al. 1991 coefficient in place of snow_diffusion_const (PBSM3D.cpp:1164-1172)
+        const double dcoef = a.rouault_diffusion_coef ? 1.0 / (1.0 + (1.0 * w * w) / (1.56 * ustar * ustar))
+                                                      : a.snow_diffusion_const;
+        const double Kv = dcoef * ustar * l;
         const double a3 = area * Kv / dz;  // alpha[3] == alpha[4]
         const double ks = u_z / vnorm;
         const double ux = vx * ks, uy = vy * ks;

@@ -23,6 +23,7 @@ struct pbsm3d_ctx {
     pb::DevBuf<uint8_t> water;
     pb::System susp, dep;
     pb::DevBuf<double> u_z, csubl, Qsalt, Qsusp, Qsubl, Qsubl_mass, sum_subl, drift_mass, sum_drift, more;
+    pb::DevBuf<double> blowprob;  // blowingsnow_probability (use_PomLi_probability)
     pb::DevBuf<double> fscal;  // [3F] per-face c_salt, hs, u* of the last step
     pb::DevBuf<uint8_t> salt_flag;
     pb::DevBuf<double> forcing_stage;

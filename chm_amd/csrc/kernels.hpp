@@ -56,6 +56,7 @@ struct AssembleArgs {
     // config
     int do_fixed_settling, do_sublimation, do_lateral_diff, enable_veg;
     int use_tanh_fetch, use_exp_fetch, use_R94_lambda;
+    int use_PomLi_probability, z0_ustar_coupling, rouault_diffusion_coef;
     double settling_velocity, settling_scale, min_sd_trans, cutoff, snow_diffusion_const, z0_snow;
     // geometry, edges j-major ([j*F + i])
     const double *nx, *ny, *elen, *area, *cz;
@@ -64,12 +65,12 @@ struct AssembleArgs {
     const double *canopy, *lai, *stalk_n, *stalk_d;
     const uint8_t* is_water;
     // forcing
-    const double *U_R, *U_2m, *vw_dir, *swe, *t, *rh, *snowdepthavg, *fetch;
+    const double *U_R, *U_2m, *vw_dir, *swe, *t, *rh, *snowdepthavg, *fetch, *p_snow_hours;
     // suspension system
     const int32_t* rp;
     double *val, *rhs, *x;
     // per-cell / per-face outputs
-    double *u_z, *csubl, *Qsalt;
+    double *u_z, *csubl, *Qsalt, *blowing_prob;
     uint8_t* salt_flag;
     double* fscal;  // [3F] c_salt, hs, u* (features of the learned warm start), may be null
     // fused preconditioner set-up (nullable)

@@ -46,11 +46,11 @@ PBSM3D::PBSM3D(config_file cfg_) : module_base("PBSM3D", parallel::domain, std::
     provides("Qsalt");
     provides("sum_drift");
 
-    // options whose reference code paths rely on GSL / Boost root finding
-    // (PBSM3D.cpp:486-716, 848-865) are not part of the GPU step
-    if (use_PomLi_probability || z0_ustar_coupling || use_subgrid_topo || use_subgrid_topo_V2 || debug_output)
-        throw module_error("PBSM3D: use_PomLi_probability, z0_ustar_coupling, use_subgrid_topo(_V2) and "
-                           "debug_output are not available on the GPU path");
+    // the subgrid-topography options rely on GSL quadrature and distributions
+    // (PBSM3D.cpp:486-645) and debug_output adds ~30 diagnostic fields per face and
+    // layer: neither is part of the GPU step
+    if (use_subgrid_topo || use_subgrid_topo_V2 || debug_output)
+        throw module_error("PBSM3D: use_subgrid_topo(_V2) and debug_output are not available on the GPU path");
 }
 
 PBSM3D::~PBSM3D() {
@@ -70,12 +70,16 @@ void PBSM3D::init(mesh& domain) {
     c_.min_sd_trans = cfg.get("min_sd_trans", 0.1);
     c_.cutoff = cfg.get("cutoff", 0.3);
     c_.snow_diffusion_const = cfg.get("snow_diffusion_const", 0.3);
-    if (cfg.get("rouault_diffusion_coef", false) || cfg.get("iterative_subl", false))
-        throw module_error("PBSM3D: rouault_diffusion_coef and iterative_subl are not available on the GPU path");
+    c_.rouault_diffusion_coef = cfg.get("rouault_diffusion_coef", false);
+    // iterative_subl: a Newton solve for the particle temperature (PBSM3D.cpp:1062-1100)
+    if (cfg.get("iterative_subl", false))
+        throw module_error("PBSM3D: iterative_subl is not available on the GPU path");
     c_.enable_veg = cfg.get("enable_veg", true);
     c_.use_tanh_fetch = use_tanh_fetch;
     c_.use_exp_fetch = use_exp_fetch;
     c_.use_R94_lambda = use_R94_lambda;
+    c_.use_PomLi_probability = use_PomLi_probability;
+    c_.z0_ustar_coupling = z0_ustar_coupling;
     // extension keys (defaults reproduce the reference)
     c_.z0_snow = cfg.get("z0_snow", 0.01);
     c_.settling_scale = cfg.get("settling_scale", 1.0);
@@ -180,6 +184,7 @@ void PBSM3D::run(mesh& domain) {
     fo.rh = domain->variable("rh").data();
     fo.snowdepthavg = domain->variable("snowdepthavg").data();
     fo.fetch = (use_exp_fetch || use_tanh_fetch) ? domain->variable("fetch").data() : nullptr;
+    fo.p_snow_hours = use_PomLi_probability ? domain->variable("p_snow_hours").data() : nullptr;
     fo.on_device = 0;
     pbsm3d_out o{};
     o.Qsalt = domain->variable("Qsalt").data();
@@ -190,6 +195,7 @@ void PBSM3D::run(mesh& domain) {
     o.drift_mass = domain->variable("drift_mass").data();
     o.sum_drift = domain->variable("sum_drift").data();
     o.pbsm_more_than_avail = domain->variable("pbsm_more_than_avail").data();
+    o.blowingsnow_probability = domain->variable("blowingsnow_probability").data();
     o.on_device = 0;
     check(pbsm3d_step(ctx_, &fo, dt, &o, &stats_), ctx_);
 }

@@ -30,9 +30,11 @@
 
 using namespace chm;
 
-static const char* kForcing[8] = {"U_R", "U_2m_above_srf", "vw_dir", "swe", "t", "rh", "snowdepthavg", "fetch"};
-static const char* kOut[8] = {"Qsalt", "Qsusp", "Qsubl", "Qsubl_mass", "sum_subl", "drift_mass", "sum_drift",
-                              "pbsm_more_than_avail"};
+constexpr int kNForcing = 9, kNOut = 9;
+static const char* kForcing[kNForcing] = {"U_R", "U_2m_above_srf", "vw_dir", "swe", "t", "rh", "snowdepthavg", "fetch",
+                                          "p_snow_hours"};
+static const char* kOut[kNOut] = {"Qsalt", "Qsusp", "Qsubl", "Qsubl_mass", "sum_subl", "drift_mass", "sum_drift",
+                                  "pbsm_more_than_avail", "blowingsnow_probability"};
 
 static std::string join(const std::vector<std::string>& v) {
     std::string s = "[";
@@ -88,11 +90,11 @@ static int run(int argc, char** argv) {
     }
     mesh m = triangulation::from_file(mesh_path, params);
     const size_t F = m->size_faces();
-    std::vector<double> buf(8 * F);
+    std::vector<double> buf(kNForcing * F);
     std::ifstream in(forcing_path, std::ios::binary);
     in.read(reinterpret_cast<char*>(buf.data()), buf.size() * sizeof(double));
     if (!in) throw std::runtime_error("short forcing file");
-    for (int k = 0; k < 8; ++k) m->variable(kForcing[k]).assign(buf.begin() + k * F, buf.begin() + (k + 1) * F);
+    for (int k = 0; k < kNForcing; ++k) m->variable(kForcing[k]).assign(buf.begin() + k * F, buf.begin() + (k + 1) * F);
 
     // with use_fetchr=1 / use_scale_wind_vert=1 those modules provide "fetch" and
     // "U_2m_above_srf" each step, run before PBSM3D as CHM's core orders them by
@@ -135,14 +137,14 @@ static int run(int argc, char** argv) {
     }
     const pbsm3d_stats st = mod.last_stats();
     std::ofstream out(out_path, std::ios::binary);
-    for (int k = 0; k < 8; ++k)
+    for (int k = 0; k < kNOut; ++k)
         out.write(reinterpret_cast<const char*>(m->variable(kOut[k]).data()), F * sizeof(double));
 
     // checkpoint -> restore into a fresh module -> one more step on both
     checkpoint_store chk;
     mod.checkpoint(m, chk);
     mesh m2 = triangulation::from_file(mesh_path, params);
-    for (int k = 0; k < 8; ++k) m2->variable(kForcing[k]) = m->variable(kForcing[k]);
+    for (int k = 0; k < kNForcing; ++k) m2->variable(kForcing[k]) = m->variable(kForcing[k]);
     PBSM3D mod2{cfg};
     mod2.init(m2);
     mod2.load_checkpoint(m2, chk);

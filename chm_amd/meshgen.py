@@ -183,7 +183,7 @@ def synthetic_mesh(n_points: int, seed: int = 0, extent: float = 1.0e4, reorder:
 Z_U_R = 50.0  # reference wind height, src/physics/Atmosphere.h:31
 Z0_SNOW = 0.01  # src/physics/Snow.h:31
 
-FORCING_FIELDS = ("U_R", "U_2m_above_srf", "vw_dir", "swe", "t", "rh", "snowdepthavg", "fetch")
+FORCING_FIELDS = ("U_R", "U_2m_above_srf", "vw_dir", "swe", "t", "rh", "snowdepthavg", "fetch", "p_snow_hours")
 
 
 def u_ref_from_u2(u2: np.ndarray, snow_depth: np.ndarray) -> np.ndarray:
@@ -197,7 +197,8 @@ def u_ref_from_u2(u2: np.ndarray, snow_depth: np.ndarray) -> np.ndarray:
 
 
 def forcing_c1(n_faces: int, seed: int = 0) -> dict:
-    """Uniform 10 m/s at 2 m, direction N(270, 20) deg, -10 C, 70 % RH, 0.5 m snow."""
+    """Uniform 10 m/s at 2 m, direction N(270, 20) deg, -10 C, 70 % RH, 0.5 m snow, fetch
+    1000 m and 24 h since snowfall (read only by use_PomLi_probability)."""
     f = n_faces
     sd = np.full(f, 0.5)
     u2 = np.full(f, 10.0)
@@ -210,6 +211,7 @@ def forcing_c1(n_faces: int, seed: int = 0) -> dict:
         "rh": np.full(f, 70.0),
         "snowdepthavg": sd,
         "fetch": np.full(f, 1000.0),
+        "p_snow_hours": np.full(f, 24.0),
     }
 
 

@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define PBSM3D_ABI_VERSION 5
+#define PBSM3D_ABI_VERSION 6
 
 /* return codes */
 #define PBSM3D_OK 0
@@ -116,6 +116,20 @@ typedef struct pbsm3d_config {
                                      aims for; 0 = default (split BiCGStab: 2e-3
                                      from x0 = 0, 5e-4 from the learned start;
                                      1e-4 otherwise)                               */
+    /* reference keys that are off by default */
+    int32_t use_PomLi_probability;  /* 0 (PBSM3D.cpp:124, 848-865): c_salt scaled by the
+                                       probability of blowing snow of Essery, Li & Pomeroy
+                                       1999 eqs 10-14 from the forcing's p_snow_hours; the
+                                       probability is the blowingsnow_probability output   */
+    int32_t z0_ustar_coupling;      /* 0 (PBSM3D.cpp:125, 691-737): u* of saltation
+                                       candidates from the Li & Pomeroy 2000 eq. 5 fixed
+                                       point u* = u2 k / ln(2 / (c u*^2 + lambda/2)), and
+                                       z0 = c u*^2 + lambda/2 while saltating (a bracketing
+                                       root search per face)                              */
+    int32_t rouault_diffusion_coef; /* 0 (PBSM3D.cpp:258, 1164-1172): vertical diffusivity
+                                       coefficient 1 / (1 + w^2 / (1.56 u*^2)) per cell in
+                                       place of snow_diffusion_const                        */
+    int32_t reserved_cfg;
 } pbsm3d_config;
 
 /* Krylov method of the suspension solve (the deposition solve is GMRES(restart)) */
@@ -137,6 +151,8 @@ typedef struct pbsm3d_forcing {
     const double* rh;             /* [F] %                                         */
     const double* snowdepthavg;   /* [F] m (-9999/NaN -> 0)                        */
     const double* fetch;          /* [F] m, or NULL when no fetch mode is on       */
+    const double* p_snow_hours;   /* [F] hours since snowfall, read by
+                                     use_PomLi_probability only (else may be NULL)  */
     int32_t on_device;            /* 1: all pointers are device (HBM) pointers     */
 } pbsm3d_forcing;
 
@@ -150,6 +166,8 @@ typedef struct pbsm3d_out {
     double* drift_mass;
     double* sum_drift;
     double* pbsm_more_than_avail;
+    double* blowingsnow_probability; /* written on saltating faces with
+                                        use_PomLi_probability, else kept (-9999 at first) */
     /* solver fields for parity checks, NULL to skip                               */
     double* conc;   /* [nLayer*F] suspension solution (after no clamping)          */
     double* qdep;   /* [F] deposition solution                                     */

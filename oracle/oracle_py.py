@@ -38,6 +38,7 @@ def lib():
         L.oracle_destroy.argtypes = [vp]
         L.oracle_destroy.restype = None
         L.oracle_set_solver.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.oracle_set_replay.argtypes = [vp, dp, dp]
         L.oracle_system_nnz.argtypes = [vp, C.c_int]
         L.oracle_system_nnz.restype = C.c_int64
         L.oracle_export_system.argtypes = [vp, C.c_int, ip, ip, dp, dp]
@@ -77,6 +78,8 @@ def lib():
         L.oracle_std_dry_air_density.restype = C.c_double
         L.oracle_is_nan.argtypes = [C.c_double]
         L.oracle_is_nan.restype = C.c_int
+        L.oracle_ustar_coupled.argtypes = [C.c_double, C.c_double, C.POINTER(C.c_int)]
+        L.oracle_ustar_coupled.restype = C.c_double
         _lib = L
     return _lib
 
@@ -128,6 +131,12 @@ class Oracle:
 
     def set_solver(self, direct=None, threads=None, precond=-1, dep_precond=-1):
         lib().oracle_set_solver(self._h, int(bool(direct)), int(threads or 1), precond, dep_precond)
+
+    def set_replay(self, conc=None, qdep=None):
+        """The following steps take conc / qdep (another solver's solutions) instead of
+        solving; None: solve as usual."""
+        keep = abi.Keep()
+        lib().oracle_set_replay(self._h, keep.dptr(conc), keep.dptr(qdep))
 
     def step(self, forcing: dict, dt: int = 900, want_fields: bool = True):
         keep = abi.Keep()
@@ -215,6 +224,13 @@ def std_dry_air_density(altitude: float, temperature_k: float) -> float:
 
 def is_nan(v: float) -> bool:
     return lib().oracle_is_nan(float(v)) == 1
+
+
+def ustar_coupled(u2: float, lam: float):
+    """(u*, bracketed) of the z0_ustar_coupling root search for one face."""
+    ok = C.c_int(0)
+    r = lib().oracle_ustar_coupled(float(u2), float(lam), C.byref(ok))
+    return r, ok.value == 1
 
 
 def expint_e1(x: float) -> float:

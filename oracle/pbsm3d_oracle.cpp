@@ -95,6 +95,73 @@ double log_scale_wind(double u, double z_in, double z_out, double sd, double z0)
     return u * std::log((z_out - (sd + z0)) / z0) / std::log((z_in - (sd + z0)) / z0);
 }
 
+// boost::math::sign: 0, or -1 / +1 from the sign bit
+int boost_sign(double v) { return v == 0 ? 0 : (std::signbit(v) ? -1 : 1); }
+
+// u* with the z0 feedback of Li & Pomeroy 2000 eq. 5 (PBSM3D.cpp:691-716): the root
+// of g(u) = u2 k / ln(2 / (0.6131702345e-2 u^2 + lambda/2)) - u, bracketed as
+// boost::math::tools::bracket_and_solve_root(g, guess 1, factor 1, rising false,
+// max_iter 500) brackets it (Boost.Math's documented algorithm: walk up from the guess
+// when g(guess) >= 0, else down, multiplying / dividing by the factor, which doubles
+// after 32 evaluations and then ever sooner; a downward walk below DBL_MIN returns
+// (0, a); running out of evaluations throws, *ok = false).  The reference then runs
+// TOMS 748 to a bracket narrower than 1e-8 and takes its midpoint; the restatement
+// bisects the bracket to the last bit (the two differ by < 5e-9: parity with Boost's
+// iterates is unpinned, the root itself is pinned in tests/test_oracle.py).
+double ustar_li_pomeroy(double u2, double lambda, bool* ok) {
+    auto g = [&](double u) { return u2 * kKappa / std::log(2.0 / (0.6131702345e-2 * u * u + .5 * lambda)) - u; };
+    const std::uintmax_t max_iter = 500;
+    double factor = 1.0;
+    double a = 1.0, b = 1.0, fa = g(a), fb = fa;
+    std::uintmax_t count = max_iter - 1;
+    int step = 32;
+    *ok = true;
+    if ((fa < 0) == false) {
+        while (boost_sign(fb) == boost_sign(fa)) {
+            if (count == 0) { *ok = false; return 0; }
+            if ((max_iter - count) % step == 0) {
+                factor *= 2;
+                if (step > 1) step /= 2;
+            }
+            a = b;
+            fa = fb;
+            b *= factor;
+            fb = g(b);
+            --count;
+        }
+    } else {
+        while (boost_sign(fb) == boost_sign(fa)) {
+            if (std::fabs(a) < 2.2250738585072014e-308) return 0 + (a - 0) / 2.0;
+            if (count == 0) { *ok = false; return 0; }
+            if ((max_iter - count) % step == 0) {
+                factor *= 2;
+                if (step > 1) step /= 2;
+            }
+            b = a;
+            fb = fa;
+            a /= factor;
+            fa = g(a);
+            --count;
+        }
+    }
+    if (fa == 0) return a;
+    if (fb == 0) return b;
+    double lo = a, hi = b, glo = fa;
+    for (int it = 0; it < 200; ++it) {
+        const double mid = lo + (hi - lo) / 2.0;
+        if (mid <= lo || mid >= hi) break;
+        const double gm = g(mid);
+        if (gm == 0) return mid;
+        if (boost_sign(gm) == boost_sign(glo)) {
+            lo = mid;
+            glo = gm;
+        } else {
+            hi = mid;
+        }
+    }
+    return lo + (hi - lo) / 2.0;
+}
+
 void neg_bearing_to_cartesian(double bearing, double* x, double* y) {
     double h = 450. - bearing;
     if (h > 360.) h = h - 360.;
@@ -690,9 +757,14 @@ struct oracle_ctx {
     std::vector<double> u_z, csubl;  // [L*F]
     // face variables / state
     std::vector<double> Qsalt, Qsusp, Qsubl, Qsubl_mass, sum_subl, drift_mass, sum_drift, more_than_avail;
+    std::vector<double> blowing_prob;
     // solver options
     int direct = 0;
     int threads = 1;
+    // replay: the next step takes these as the suspension / deposition solutions instead
+    // of solving (a checker feeds another solver's x to get the rest of the step and the
+    // deposition system that follows from it); empty = solve
+    std::vector<double> replay_s, replay_d;
     std::string err;
 };
 
@@ -843,7 +915,8 @@ int oracle_create(oracle_ctx** out, const pbsm3d_mesh* m, const pbsm3d_config* c
     for (auto* v : {&c->ustar, &c->hs, &c->c_salt, &c->z0, &c->saltation, &c->uth}) v->assign(F, 0.0);
     c->u_z.assign(N, 0.0);
     c->csubl.assign(N, 0.0);
-    for (auto* v : {&c->Qsalt, &c->Qsusp, &c->Qsubl, &c->Qsubl_mass, &c->drift_mass, &c->more_than_avail})
+    for (auto* v : {&c->Qsalt, &c->Qsusp, &c->Qsubl, &c->Qsubl_mass, &c->drift_mass, &c->more_than_avail,
+                    &c->blowing_prob})
         v->assign(F, -9999.0);  // face variables default to -9999 (variablestorage.hpp:269-272)
     c->sum_subl.assign(F, 0.0);
     c->sum_drift.assign(F, 0.0);  // PBSM3D.cpp:391
@@ -852,6 +925,12 @@ int oracle_create(oracle_ctx** out, const pbsm3d_mesh* m, const pbsm3d_config* c
 }
 
 void oracle_destroy(oracle_ctx* c) { delete c; }
+
+int oracle_set_replay(oracle_ctx* c, const double* x_susp, const double* x_dep) {
+    if (x_susp) c->replay_s.assign(x_susp, x_susp + c->F * c->L); else c->replay_s.clear();
+    if (x_dep) c->replay_d.assign(x_dep, x_dep + c->F); else c->replay_d.clear();
+    return 0;
+}
 
 int oracle_set_solver(oracle_ctx* c, int direct, int threads, int precond, int dep_precond) {
     c->direct = direct;
@@ -862,6 +941,7 @@ int oracle_set_solver(oracle_ctx* c, int direct, int threads, int precond, int d
 }
 
 int oracle_step(oracle_ctx* c, const pbsm3d_forcing* fo, int32_t dt_i, pbsm3d_out* out, pbsm3d_stats* st) {
+    if (c->cfg.use_PomLi_probability && !fo->p_snow_hours) return PBSM3D_EINVAL;
 #ifdef _OPENMP
     omp_set_num_threads(c->threads);
 #endif
@@ -909,10 +989,20 @@ int oracle_step(oracle_ctx* c, const pbsm3d_forcing* fo, int32_t dt_i, pbsm3d_ou
                 lambda = 0.5 * c->lai[i] * height_diff;
             else
                 lambda = c->stalk_n[i] * c->stalk_d[i] * height_diff;
-            ustar = u2 * kKappa / std::log(2.0 / 0.0002);
+            if (cfg.z0_ustar_coupling) {
+                // PBSM3D.cpp:691-716: a failed search keeps the placeholder u* = 1.3
+                bool ok = true;
+                const double r = ustar_li_pomeroy(u2, lambda, &ok);
+                if (ok) ustar = r;
+            } else {
+                ustar = u2 * kKappa / std::log(2.0 / 0.0002);
+            }
             if (ustar >= uth) {
                 saltation = true;
-                z0 = z0snow;
+                if (cfg.z0_ustar_coupling)
+                    z0 = 0.6131702345e-2 * ustar * ustar + .5 * lambda;  // PBSM3D.cpp:727-737
+                else
+                    z0 = z0snow;
             }
         }
         if (!saltation) {
@@ -945,6 +1035,20 @@ int oracle_step(oracle_ctx* c, const pbsm3d_forcing* fo, int32_t dt_i, pbsm3d_ou
                 const double fetch_ref = 300;
                 const double Lc = 0.5 * std::tanh(0.1333333333e-1 * fetch_ref - 2.0) + 0.5;
                 c_salt *= Lc;
+            }
+            if (cfg.use_PomLi_probability) {
+                // PBSM3D.cpp:848-865, Essery, Li & Pomeroy 1999 eqs 10-14; u10 from
+                // PBSM3D.cpp:451-463 (log_scale_wind with the snow roughness)
+                const double z10 = 10. + snow_depth;
+                const double u10 = z10 < kZUR ? log_scale_wind(uref, kZUR, z10, snow_depth, z0snow) : uref;
+                const double A = fo->p_snow_hours[i];
+                const double u_mean = 11.2 + 0.365 * T + 0.00706 * T * T + 0.9 * std::log(A);
+                const double delta = 0.145 * T + 0.00196 * T * T + 4.3;
+                const double z0v = (c->stalk_n[i] * c->stalk_d[i] * height_diff) / 2.0;
+                const double us = u10 / std::sqrt((1 + 340.0 * z0v));
+                const double Pu10 = 1.0 / (1.0 + std::exp((std::sqrt(kPi) * (u_mean - us)) / delta));
+                c->blowing_prob[i] = Pu10;
+                c_salt *= Pu10;
             }
             const double uhs = 2.8 * uth;
             Qsalt = c_salt * uhs * hs;
@@ -1035,7 +1139,11 @@ int oracle_step(oracle_ctx* c, const pbsm3d_forcing* fo, int32_t dt_i, pbsm3d_ou
             }
             const double l = kKappa * (czl + z0) * kLMax / (kKappa * (czl + z0) + kLMax);
             const double w = omega;
-            const double diffusion_coeff = cfg.snow_diffusion_const;
+            double diffusion_coeff = cfg.snow_diffusion_const;
+            if (cfg.rouault_diffusion_coef) {  // PBSM3D.cpp:1166-1172
+                const double c2 = 1.0;
+                diffusion_coeff = 1.0 / (1.0 + (c2 * w * w) / (1.56 * ustar * ustar));
+            }
             K[3] = K[4] = diffusion_coeff * ustar * l;
             alpha[3] = A[3] * K[3] / dz;
             alpha[4] = A[4] * K[4] / dz;
@@ -1132,7 +1240,9 @@ int oracle_step(oracle_ctx* c, const pbsm3d_forcing* fo, int32_t dt_i, pbsm3d_ou
     stats.suspension_present = suspension_present;
     int rc = 0;
     if (suspension_present) {
-        if (c->direct) {
+        if (!c->replay_s.empty()) {
+            c->x_s = c->replay_s;
+        } else if (c->direct) {
             if (!dense_solve(c->susp, c->rhs_s.data(), c->x_s.data())) rc = PBSM3D_ENOCONV;
             stats.susp_iters = 0;
         } else {
@@ -1211,7 +1321,9 @@ int oracle_step(oracle_ctx* c, const pbsm3d_forcing* fo, int32_t dt_i, pbsm3d_ou
     stats.deposition_present = deposition_present;
     auto t_post = std::chrono::steady_clock::now();
     if (deposition_present) {
-        if (c->direct) {
+        if (!c->replay_d.empty()) {
+            c->x_d = c->replay_d;
+        } else if (c->direct) {
             if (!dense_solve(c->dep, c->rhs_d.data(), c->x_d.data())) rc = PBSM3D_ENOCONV;
         } else {
             auto M = make_precond(cfg.dep_precond, F, 1);
@@ -1261,6 +1373,7 @@ int oracle_step(oracle_ctx* c, const pbsm3d_forcing* fo, int32_t dt_i, pbsm3d_ou
         cp(out->drift_mass, c->drift_mass);
         cp(out->sum_drift, c->sum_drift);
         cp(out->pbsm_more_than_avail, c->more_than_avail);
+        cp(out->blowingsnow_probability, c->blowing_prob);
         cp(out->conc, c->x_s);
         cp(out->qdep, c->x_d);
     }
@@ -1360,6 +1473,13 @@ double oracle_std_dry_air_density(double altitude, double temperature) {
 
 // module_base::is_nan as restated above (module_base.hpp:471-479)
 int oracle_is_nan(double v) { return chm_is_nan(v) ? 1 : 0; }
+
+double oracle_ustar_coupled(double u2, double lambda, int* ok) {
+    bool b = true;
+    const double r = ustar_li_pomeroy(u2, lambda, &b);
+    *ok = b ? 1 : 0;
+    return r;
+}
 
 void oracle_set_ilut(double tau, double level_of_fill) {
     g_ilut_tau = tau;

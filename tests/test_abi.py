@@ -28,7 +28,7 @@ def test_library_exports_every_declared_symbol():
     assert len(names) >= 10
     for n in names:
         assert hasattr(L, n), n
-    assert L.pbsm3d_abi_version() == 5
+    assert L.pbsm3d_abi_version() == 6
     assert set(pbsm3d.EXPORTED) <= set(names)
 
 
@@ -54,6 +54,8 @@ def test_config_defaults_match_reference():
     assert (c.use_tanh_fetch, c.use_exp_fetch, c.use_R94_lambda) == (1, 0, 1)
     assert (c.tol, c.max_iters, c.restart) == (1e-8, 1000, 30)
     assert c.z0_snow == 0.01 and c.settling_scale == 1.0
+    # PBSM3D.cpp:124-125, 258: off by default
+    assert (c.use_PomLi_probability, c.z0_ustar_coupling, c.rouault_diffusion_coef) == (0, 0, 0)
     d = abi.default_config()
     for name, _ in abi.Config._fields_:
         assert getattr(c, name) == getattr(d, name), name

@@ -254,7 +254,10 @@ def test_nonconvergence_raises(pb, mesh_small):
 
 def test_options_parity(pb, oracle, mesh_small):
     """Non-default config keys: fixed settling, no lateral diffusion, exp fetch,
-    no sublimation, vegetation with stalk lambda, water faces, z0/settling extension keys."""
+    no sublimation, vegetation with stalk lambda, water faces, z0/settling extension keys,
+    and the reference's optional keys use_PomLi_probability (with R94 and stalk lambda),
+    z0_ustar_coupling (the kernel's bracketing root search, both lambda forms) and
+    rouault_diffusion_coef (computed and fixed settling velocity)."""
     _print_case("options")
     F = mesh_small.n_faces
     rng = np.random.default_rng(7)
@@ -264,9 +267,13 @@ def test_options_parity(pb, oracle, mesh_small):
     g.is_water = (rng.random(F) < 0.05).astype(np.uint8)
     fo = meshgen.forcing_c2(F, seed=5)
     fo["fetch"] = rng.uniform(50, 800, F)
+    fo["p_snow_hours"] = rng.uniform(1.0, 300.0, F)
     cases = [dict(do_fixed_settling=1, settling_velocity=0.3), dict(do_lateral_diff=0),
              dict(use_exp_fetch=1, use_tanh_fetch=0), dict(do_sublimation=0), dict(use_R94_lambda=0),
-             dict(z0_snow=0.003, settling_scale=1.2), dict(enable_veg=0)]
+             dict(z0_snow=0.003, settling_scale=1.2), dict(enable_veg=0),
+             dict(use_PomLi_probability=1), dict(use_PomLi_probability=1, use_R94_lambda=0),
+             dict(z0_ustar_coupling=1), dict(z0_ustar_coupling=1, use_R94_lambda=0),
+             dict(rouault_diffusion_coef=1), dict(rouault_diffusion_coef=1, do_fixed_settling=1, settling_velocity=0.3)]
     for kw in cases:
         ctx = pb.Context(g, pb.default_config(nLayer=5, tol=1e-12, **kw))
         orc = oracle.Oracle(g, oracle.oracle_config(nLayer=5, tol=1e-13, **kw))
@@ -276,6 +283,11 @@ def test_options_parity(pb, oracle, mesh_small):
         _, _, vo, _ = orc.system(0)
         assert _row_rel(vg, vo, rp) < 1e-12, kw
         np.testing.assert_allclose(gg["Qsalt"], og["Qsalt"], rtol=1e-12, atol=0, err_msg=str(kw))
+        np.testing.assert_allclose(gg["blowingsnow_probability"], og["blowingsnow_probability"], rtol=1e-12, atol=0,
+                                   err_msg=str(kw))
+        if kw.get("use_PomLi_probability"):
+            p = gg["blowingsnow_probability"]
+            assert ((p > 0) & (p < 1)).sum() > F // 4, kw
         for k in ("conc", "Qsusp", "drift_mass"):
             assert rel_err(gg[k], og[k]) < 1e-7, (kw, k)
 

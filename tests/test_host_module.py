@@ -52,14 +52,19 @@ def test_reference_mesh_json_roundtrip(tmp_path):
     assert d["faces"] == 985 and d["geometry_max_abs_diff"] == 0.0
 
 
+_OPTION_KEYS = ("use_PomLi_probability", "rouault_diffusion_coef", "z0_ustar_coupling")
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("keys", [(), ("mixed_precision=1", "solver_bicgstab_split=1", "solver_sweep_mode=1",
-                                       "solver_spmv_variant=1", "checkpoint_sum_subl=1")])
+                                       "solver_spmv_variant=1", "checkpoint_sum_subl=1"),
+                                  tuple(f"{k}=1" for k in _OPTION_KEYS)])
 def test_module_run_matches_abi(tmp_path, keys):
     """Three steps of the C++ module equal three steps through the C ABI bit for bit; the
     split mixed solver with its launch shapes pinned on both sides (the module keys
-    solver_sweep_mode / solver_spmv_variant), and the checkpoint restore continues
-    identically -- with checkpoint_sum_subl also in sum_subl."""
+    solver_sweep_mode / solver_spmv_variant), the reference's optional keys (PomLi
+    probability from p_snow_hours, Rouault diffusivity, z0 / u* coupling), and the
+    checkpoint restore continues identically -- with checkpoint_sum_subl also in sum_subl."""
     from chm_amd import pbsm3d as pb
 
     print(f"[case] host module run {keys}", flush=True)
@@ -68,6 +73,7 @@ def test_module_run_matches_abi(tmp_path, keys):
     p = tmp_path / "m.json"
     meshgen.save_mesh_json(m, str(p))
     fo = meshgen.forcing_c2(F, seed=9)
+    fo["p_snow_hours"] = np.random.default_rng(9).uniform(1.0, 300.0, F)
     fbin = tmp_path / "f.bin"
     np.concatenate([fo[k] for k in abi.FORCING_FIELDS]).astype("<f8").tofile(fbin)
     obin = tmp_path / "o.bin"
@@ -75,19 +81,23 @@ def test_module_run_matches_abi(tmp_path, keys):
                        text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     st = json.loads(r.stdout)
+    mixed = "mixed_precision=1" in keys
     assert st["restart_identical"] is True and st["susp_resid"] <= 1e-8
-    assert st["restart_sum_subl_identical"] is bool(keys) and st["checkpoint_keys"] == (2 if keys else 1)
-    got = np.fromfile(obin, dtype="<f8").reshape(8, F)
-    extra = dict(precision=abi.MIXED, krylov=abi.KRYLOV_BICGSTAB_SPLIT) if keys else {}
+    assert st["restart_sum_subl_identical"] is mixed and st["checkpoint_keys"] == (2 if mixed else 1)
+    got = np.fromfile(obin, dtype="<f8").reshape(len(abi.OUT_FIELDS), F)
+    extra = dict(precision=abi.MIXED, krylov=abi.KRYLOV_BICGSTAB_SPLIT) if mixed else {}
+    extra.update({k: 1 for k in _OPTION_KEYS if f"{k}=1" in keys})
     ctx = pb.Context(pb.face_geometry(m), pb.default_config(nLayer=6, **extra))
-    if keys:
+    if mixed:
         ctx.precond_select(1, 1)
         ctx.spmv_select(1, 1)
     for _ in range(3):
         ref, rst = ctx.step(fo)
     assert st["susp_iters"] == rst["susp_iters"] and st["susp_refinements"] == rst["susp_refinements"]
-    if keys:
+    if mixed:
         assert st["susp_refinements"] > 0
+    if "use_PomLi_probability=1" in keys:
+        assert ((ref["blowingsnow_probability"] > 0) & (ref["blowingsnow_probability"] < 1)).any()
     for k, name in enumerate(abi.OUT_FIELDS):
         np.testing.assert_array_equal(got[k], ref[name], err_msg=name)
 
@@ -167,7 +177,7 @@ def test_snobal_module_closes_the_loop(tmp_path):
     assert r.returncode == 0, r.stderr
     got_swe = np.fromfile(str(obin) + ".swe", dtype="<f8")
     got_sd = np.fromfile(str(obin) + ".sd", dtype="<f8")
-    got = np.fromfile(obin, dtype="<f8").reshape(8, F)
+    got = np.fromfile(obin, dtype="<f8").reshape(len(abi.OUT_FIELDS), F)
     sn = pb.Snow(F, fo["swe"], pb.snow_config(ic_rho=250.0, drift_density=280.0))
     ctx = pb.Context(pb.face_geometry(m), pb.default_config(nLayer=6))
     f2 = dict(fo)

@@ -172,7 +172,7 @@ def test_module_step_on_h5_mesh(tmp_path):
     assert r.returncode == 0, r.stderr
     st = json.loads(r.stdout)
     assert st["faces"] == F and st["restart_identical"] is True
-    got = np.fromfile(obin, dtype="<f8").reshape(8, F)
+    got = np.fromfile(obin, dtype="<f8").reshape(len(abi.OUT_FIELDS), F)
     ctx = pb.Context(pb.face_geometry(m), pb.default_config(nLayer=5))
     for _ in range(2):
         ref, _ = ctx.step(fo)

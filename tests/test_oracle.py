@@ -238,6 +238,128 @@ def test_layer_wind_log_profile(oracle, mesh_small):
     assert (np.diff(u, axis=0) > 0).all()
 
 
+def test_ustar_coupling_root(oracle):
+    """z0_ustar_coupling (PBSM3D.cpp:691-716): u* is the root of Li & Pomeroy 2000 eq. 5,
+    u* = u2 k / ln(2 / (0.6131702345e-2 u*^2 + lambda / 2)), bracketed from the guess 1
+    downward when g(1) < 0 and upward otherwise.  The equation has a second root near
+    u* ~ 18 m/s (where the log's argument approaches 1); the search must return the one
+    on the guess's side, as scipy's brentq on that side finds it.  With lambda / 2 > 2
+    the log is negative for every u*, no bracket exists and the downward walk ends in
+    Boost's escape bracket (0, a < DBL_MIN): u* ~ 0."""
+    from scipy.optimize import brentq
+
+    def g(u, u2, lam):
+        return u2 * 0.4 / np.log(2.0 / (0.6131702345e-2 * u * u + 0.5 * lam)) - u
+
+    for u2, lam, side in ((10.0, 0.0, (1e-6, 1.0)), (6.0, 0.1, (1e-6, 1.0)), (4.0, 0.24, (1e-6, 1.0)),
+                          (20.0, 0.0, (1.0, 8.0)), (25.0, 0.3, (1.0, 8.0))):
+        want = brentq(g, *side, args=(u2, lam), xtol=1e-15, rtol=1e-15)
+        got, ok = oracle.ustar_coupled(u2, lam)
+        assert ok
+        assert abs(got - want) <= 4e-15 * want, (u2, lam, got, want)
+        assert abs(g(got, u2, lam)) < 1e-13
+        assert (g(1.0, u2, lam) < 0) == (got < 1.0)  # the root on the guess's side
+    tiny, ok = oracle.ustar_coupled(10.0, 5.0)
+    assert ok and 0 <= tiny < 1e-300
+
+
+def test_z0_ustar_coupling_step(oracle, mesh_small):
+    """Through the step: with vegetation (lambda = LAI hd / 2) saltating faces take the
+    coupled u* and z0 = c u*^2 + lambda / 2 (PBSM3D.cpp:727-737, floored at Z0_SNOW);
+    u* >= u*_th on exactly those faces, and the faces that do not saltate keep the
+    reference-height u* with Z0_SNOW."""
+    F = mesh_small.n_faces
+    rng = np.random.default_rng(12)
+    g = oracle.face_geometry(mesh_small)
+    g.canopy_height = rng.uniform(0.4, 0.75, F)  # hd = canopy - 0.5 m snow in [0, 0.25] <= cutoff
+    g.lai = rng.uniform(0.0, 3.0, F)
+    fo = meshgen.forcing_c2(F, seed=12)
+    o = oracle.Oracle(g, oracle.oracle_config(nLayer=4, z0_ustar_coupling=1))
+    o.step(fo)
+    tr = o.face_trace()
+    hd = np.maximum(0.0, g.canopy_height - fo["snowdepthavg"])
+    lam = 0.5 * g.lai * hd
+    root = np.array([oracle.ustar_coupled(u, lm)[0] for u, lm in zip(fo["U_2m_above_srf"], lam)])
+    # every face is a candidate (hd <= cutoff, depth >= min_sd_trans); those whose root
+    # passes the threshold saltate with it (a face whose c_salt then comes out negative
+    # drops its saltation flag but keeps u* and z0, PBSM3D.cpp:814-824)
+    cand = root >= tr["uth"]
+    assert cand.any() and (~cand).any()
+    assert not (tr["saltation"][~cand] == 1).any()
+    np.testing.assert_array_equal(tr["ustar"][cand], root[cand])
+    np.testing.assert_allclose(tr["z0"][cand], np.maximum(0.01, 0.6131702345e-2 * root[cand] ** 2 + 0.5 * lam[cand]),
+                               rtol=1e-15)
+    assert (tr["z0"][cand] > 0.01).any()
+    np.testing.assert_allclose(tr["ustar"][~cand], np.maximum(0.01, 0.4 * fo["U_R"][~cand] / math.log(5000.0)),
+                               rtol=1e-15)
+    assert (tr["z0"][~cand] == 0.01).all()
+
+
+def test_pomli_probability(oracle, mesh_small):
+    """use_PomLi_probability (PBSM3D.cpp:848-865): c_salt, hence Qsalt, is multiplied by
+    the probability of blowing snow P = 1 / (1 + exp(sqrt(pi) (u_mean - u*) / delta)) of
+    Essery, Li & Pomeroy 1999 (eqs 10-13: u_mean = 11.2 + 0.365 T + 0.00706 T^2 +
+    0.9 ln A, delta = 0.145 T + 0.00196 T^2 + 4.3, A the hours since snowfall, u* the 10 m
+    wind, log-scaled from U_R over snow, over sqrt(1 + 340 z0v), z0v = 0 without stalks).
+    Pinned by the logistic's fixed points: A chosen so that u_mean equals the 10 m wind
+    gives P = 1/2 exactly; u_mean = u10 -/+ delta / sqrt(pi) gives 1 / (1 + e^(-/+1)); and
+    the two sides sum to one.  Faces that do not saltate keep the -9999 default."""
+    F = mesh_small.n_faces
+    fo = meshgen.forcing_c2(F, seed=13)
+    T, sd = fo["t"], fo["snowdepthavg"]
+    z10 = 10.0 + sd
+    u10 = fo["U_R"] * np.log((z10 - (sd + 0.01)) / 0.01) / np.log((50.0 - (sd + 0.01)) / 0.01)
+    base = 11.2 + 0.365 * T + 0.00706 * T * T
+    delta = 0.145 * T + 0.00196 * T * T + 4.3
+    _, off, _, _ = _run(oracle, mesh_small, fo, nLayer=3)
+    salt = off["Qsalt"] > 0
+    assert salt.sum() > F // 3
+    probs = {}
+    for name, shift, want in (("mid", 0.0, 0.5), ("lo", -1.0, 1 / (1 + 1 / math.e)), ("hi", 1.0, 1 / (1 + math.e))):
+        u_mean = u10 + shift * delta / math.sqrt(math.pi)
+        A = np.exp((u_mean - base) / 0.9)
+        _, on, _, _ = _run(oracle, mesh_small, dict(fo, p_snow_hours=A), nLayer=3, use_PomLi_probability=1)
+        p = on["blowingsnow_probability"]
+        np.testing.assert_allclose(p[salt], want, rtol=1e-12, err_msg=name)
+        assert (p[~salt] == -9999.0).all()
+        np.testing.assert_allclose(on["Qsalt"][salt], want * off["Qsalt"][salt], rtol=1e-12, err_msg=name)
+        probs[name] = p[salt]
+    np.testing.assert_allclose(probs["lo"] + probs["hi"], 1.0, rtol=1e-12)
+    # the key needs the p_snow_hours forcing
+    g = oracle.face_geometry(mesh_small)
+    with pytest.raises(RuntimeError):
+        oracle.Oracle(g, oracle.oracle_config(nLayer=3, use_PomLi_probability=1)).step(
+            {k: v for k, v in fo.items() if k != "p_snow_hours"})
+
+
+def test_rouault_diffusion_coefficient(oracle, mesh_small):
+    """rouault_diffusion_coef (PBSM3D.cpp:1164-1172): the vertical diffusivity uses
+    1 / (1 + w^2 / (1.56 u*^2)) in place of snow_diffusion_const.  With a fixed settling
+    velocity w the interior coupling to the cell above is area w + alpha3 with
+    alpha3 = area K / dz (K = coef u* l), so alpha3 from the exported system, divided by
+    alpha3 of a run with snow_diffusion_const = 1, is the coefficient itself; w = 0 makes
+    it exactly 1."""
+    F = mesh_small.n_faces
+    L = 4
+    fo = meshgen.forcing_c2(F, seed=14)
+    g = oracle.face_geometry(mesh_small)
+    a3 = {}
+    for name, kw in (("rouault", dict(rouault_diffusion_coef=1)), ("one", dict(snow_diffusion_const=1.0))):
+        o = oracle.Oracle(g, oracle.oracle_config(nLayer=L, do_fixed_settling=1, settling_velocity=0.4, **kw))
+        o.step(fo)
+        rp, col, val, _ = o.system(0)
+        r = F + np.arange(F)  # layer 1: an interior layer for L = 4
+        above = np.array([val[rp[k]:rp[k + 1]][col[rp[k]:rp[k + 1]] == k + F][0] for k in r])
+        a3[name] = above - g.area * 0.4
+        ustar = o.face_trace()["ustar"]
+    np.testing.assert_allclose(a3["rouault"] / a3["one"], 1.0 / (1.0 + 0.4 ** 2 / (1.56 * ustar ** 2)), rtol=1e-9)
+    for name, kw in (("rouault", dict(rouault_diffusion_coef=1)), ("one", dict(snow_diffusion_const=1.0))):
+        o = oracle.Oracle(g, oracle.oracle_config(nLayer=L, do_fixed_settling=1, settling_velocity=0.0, **kw))
+        o.step(fo)
+        a3[name] = o.system(0)[2]
+    np.testing.assert_array_equal(a3["rouault"], a3["one"])
+
+
 # ---------------------------------------------------------------------------
 # suspension / deposition assembly -- conservation invariants
 # ---------------------------------------------------------------------------
