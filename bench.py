@@ -183,36 +183,49 @@ def timed_classes(pb, cfg):
     return ("spmv64" if cfg.precision == pb.abi.FP64 else "spmv32", "precond")
 
 
-def cpu_baseline(g_full, forcing, n_layer, faces, threads):
-    """Times the CPU oracle (GMRES(30) + ILUT(1e-4, 3), the reference's solver
-    setup) on a Hilbert-contiguous patch of the same mesh; steps/s is scaled
-    to the full face count by the face ratio."""
+def cpu_baseline(g_full, forcing, n_layer, faces, threads, steps):
+    """Times the CPU oracle (GMRES(30) + ILUT(1e-4, 3), the reference's solver setup) on
+    the same mesh and forcing: `steps` consecutive 900 s steps after init (the CSR
+    patterns and the solver set-up of init are not timed, as in the reference's
+    timestep loop), steps/s from the median step.  faces > 0 restricts it to a
+    Hilbert-contiguous patch of that many faces and scales steps/s by the face ratio
+    (a quicker, flattering sample); 0 = the whole mesh, no scaling."""
     import numpy as np
 
     from chm_amd import abi
     from oracle import oracle_py
 
     F = g_full.n_faces
-    fs = min(faces, F)
-    nb = g_full.neigh.reshape(F, 3)[:fs].copy()
-    nb[nb >= fs] = -1
-    sl3 = slice(0, 3 * fs)
-    gs = abi.Geometry(nb, g_full.normal_x[sl3], g_full.normal_y[sl3], g_full.edge_length[sl3], g_full.area[:fs],
-                      g_full.center_x[:fs], g_full.center_y[:fs], g_full.center_z[:fs])
-    fo = {k: np.ascontiguousarray(v[:fs]) for k, v in forcing.items()}
+    fs = min(faces, F) if faces > 0 else F
+    if fs < F:
+        nb = g_full.neigh.reshape(F, 3)[:fs].copy()
+        nb[nb >= fs] = -1
+        sl3 = slice(0, 3 * fs)
+        gs = abi.Geometry(nb, g_full.normal_x[sl3], g_full.normal_y[sl3], g_full.edge_length[sl3], g_full.area[:fs],
+                          g_full.center_x[:fs], g_full.center_y[:fs], g_full.center_z[:fs])
+        fo = {k: np.ascontiguousarray(v[:fs]) for k, v in forcing.items()}
+    else:
+        gs, fo = g_full, forcing
     orc = oracle_py.Oracle(gs, oracle_py.oracle_config(nLayer=n_layer), threads=threads)
-    t0 = time.perf_counter()
-    _, st = orc.step(fo, want_fields=False)
-    dt = time.perf_counter() - t0
+    times, st = [], None
+    for k in range(steps):
+        t0 = time.perf_counter()
+        _, st = orc.step(fo, want_fields=False)
+        times.append(time.perf_counter() - t0)
+        log(f"[bench] CPU baseline step {k + 1}/{steps}: {times[-1]:.1f} s on {threads} threads")
     orc.close()
+    dt = statistics.median(times)
+    where = (f"the whole mesh ({F} faces) x {n_layer} layers" if fs == F else
+             f"a {fs}-face Hilbert-contiguous patch of the mesh x {n_layer} layers, steps/s scaled by faces {fs}/{F}")
     return {
         "value": (1.0 / dt) * (fs / F),
         "unit": "steps/s",
         "cores": threads,
         "kind": "port",
-        "sample": (f"one 900 s step of the CPU FP64 restatement (GMRES(30) + ILUT(1e-4, fill 3), tol 1e-8) on a "
-                   f"{fs}-face Hilbert-contiguous patch of the same mesh x {n_layer} layers, {dt:.1f} s on {threads} "
-                   f"threads, scaled by faces {fs}/{F}; susp/dep iterations {st['susp_iters']}/{st['dep_iters']}"),
+        "sample": (f"{steps} consecutive 900 s steps of the CPU FP64 restatement (GMRES(30) + ILUT(1e-4, fill 3), "
+                   f"tol 1e-8, OpenMP assembly / SpMV / dots, serial ILUT) on {where}, init not timed; median "
+                   f"step {dt:.1f} s (steps {', '.join(f'{t:.1f}' for t in times)} s) on {threads} threads; "
+                   f"susp/dep iterations {st['susp_iters']}/{st['dep_iters']}"),
     }
 
 
@@ -279,13 +292,17 @@ def main():
     ap.add_argument("--train-epochs", type=int, default=8)
     ap.add_argument("--gain-seeds", default="100,101,102,103",
                     help="held-out forcing draws on which the learned start is compared with the cold start")
-    ap.add_argument("--cpu-faces", type=int, default=400000)
+    ap.add_argument("--cpu-faces", type=int, default=0,
+                    help="CPU baseline on a patch of this many faces, scaled (default 0: the whole mesh)")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-one-core", action="store_true",
+                    help="also time the CPU baseline on one thread (the reference's default build)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fetch", action="store_true", help="skip timing the fetch (fetchr) kernel")
     ap.add_argument("--spmv-shape", type=int, default=-1, help="pin the SpMV launch shape (default: tuned)")
-    ap.add_argument("--ensemble", type=int, default=8, help="parameter-ensemble members to time (0: skip)")
-    ap.add_argument("--ensemble-steps", type=int, default=3)
+    ap.add_argument("--ensemble", type=int, default=64, help="parameter-ensemble members to time (0: skip)")
+    ap.add_argument("--ensemble-steps", type=int, default=24)
     ap.add_argument("--host-io-steps", type=int, default=3, help="steps timed with host forcing / outputs (0: skip)")
     ap.add_argument("--dilu-mode", type=int, default=-1, choices=list(range(-1, 8)),
                     help="pin the D-ILU sweep mode: bit 0 XCD workgroup mapping; split operator only: bit 1 "
@@ -397,29 +414,48 @@ def main():
         fctx.close()
         log(f"[bench] fetch: {fetch_info}")
 
-    # parameter ensemble: uneven members run concurrently (one stream each) vs one at a time
+    # parameter ensemble: every member advanced by one pbsm3d_batch_step per step (members of
+    # equal nLayer in one union, per-member BiCGStab scalars), against the members run one at
+    # a time and concurrently (one context and stream each, host threads)
     ens = None
     if args.ensemble > 0:
         from chm_amd import ensemble
 
         log(f"[bench] ensemble: {args.ensemble} members x {args.ensemble_steps} steps")
         members = ensemble.draw_members(args.ensemble, seed=0)
+        t_set = time.perf_counter()
+        meshes = ensemble.member_meshes(members, workers=16)
+        batch = ensemble.build_batch(members, meshes, precision=abi.MIXED, warm_net=net)
+        t_set = time.perf_counter() - t_set
+        ensemble.run_batch(batch, members, meshes, 1)  # first step: launch shapes of the transforms warm up
+        bat = ensemble.run_batch(batch, members, meshes, args.ensemble_steps)
+        info = batch.info()
+        batch.close()
+        del batch
+        ref_steps = min(args.ensemble_steps, 2)
         built = ensemble.build(members, precision=abi.MIXED, krylov=krylov_mixed, warm_net=net)
         ensemble.run(built, 1)  # first steps include the launch-shape tuning
-        seq = ensemble.run(built, args.ensemble_steps, workers=1)
+        seq = ensemble.run(built, ref_steps, workers=1)
         # at most 16 host threads: the process has 4 hardware queues, and more streams in
         # flight only contend (64 threads for 64 members ran at 0.7x the rate of 16)
-        par = ensemble.run(built, args.ensemble_steps, workers=min(len(built), 16))
-        ens = {"members": len(built), "steps": args.ensemble_steps,
-               "faces": [r["faces"] for r in par["per_member"]], "layers": [r["layers"] for r in par["per_member"]],
+        par = ensemble.run(built, ref_steps, workers=min(len(built), 16))
+        ens = {"members": len(members), "steps": args.ensemble_steps,
+               "faces": [mesh.n_faces for mesh in meshes], "layers": [m.n_layer for m in members],
+               "member_steps_per_s_batched": round(bat["member_steps_per_s"], 3),
+               "cell_steps_per_s_batched": round(bat["cell_steps_per_s"], 1),
                "member_steps_per_s_concurrent": round(par["member_steps_per_s"], 3),
                "member_steps_per_s_one_at_a_time": round(seq["member_steps_per_s"], 3),
-               "cell_steps_per_s_concurrent": round(par["cell_steps_per_s"], 1),
-               "speedup_concurrent": round(par["member_steps_per_s"] / seq["member_steps_per_s"], 3),
-               "max_resid": max(r["last_resid"] for r in par["per_member"]),
-               "host_threads": min(len(built), 16),
-               "note": "mixed precision, learned start on 10-layer members; members on separate HIP streams "
-                       "driven by host threads (not graph-captured); forcing built on the GPU"}
+               "speedup_batched_vs_one_at_a_time": round(bat["member_steps_per_s"] / seq["member_steps_per_s"], 3),
+               "speedup_batched_vs_concurrent": round(bat["member_steps_per_s"] / par["member_steps_per_s"], 3),
+               "max_resid_batched": bat["max_resid"],
+               "max_resid_concurrent": max(r["last_resid"] for r in par["per_member"]),
+               "unions": bat["unions"], "batched_launches_last_step": info["launches"],
+               "batched_lockstep_iterations_last_step": info["iterations"],
+               "reference_steps": ref_steps, "setup_s": round(t_set, 1),
+               "note": "mixed precision, learned start on the 10-layer members, forcing built on the GPU; batched: "
+                       "one pbsm3d_batch_step per step (members of equal nLayer in one union system, one launch "
+                       "per colour sweep and per BiCGStab kernel for all of them); concurrent: one context and "
+                       "HIP stream per member on 16 host threads; one at a time: the same contexts in turn"}
         for _, _, ctx in built:
             ctx.close()
         log(f"[bench] ensemble: {ens}")
@@ -501,12 +537,12 @@ def main():
     cpu = cpu1 = None
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        log(f"[bench] CPU baseline: oracle on {args.cpu_faces} faces, {threads} threads")
-        cpu = cpu_baseline(g, forcing, L, args.cpu_faces, threads)
-        if threads != 1:
+        log(f"[bench] CPU baseline: oracle, {args.cpu_faces or F} faces, {threads} threads, {args.cpu_steps} steps")
+        cpu = cpu_baseline(g, forcing, L, args.cpu_faces, threads, args.cpu_steps)
+        if args.cpu_one_core and threads != 1:
             # one core, as the reference's default build runs (no OpenMP, one MPI rank)
-            log(f"[bench] CPU baseline: oracle on {args.cpu_faces} faces, 1 thread")
-            cpu1 = cpu_baseline(g, forcing, L, args.cpu_faces, 1)
+            log(f"[bench] CPU baseline: oracle, {args.cpu_faces or F} faces, 1 thread")
+            cpu1 = cpu_baseline(g, forcing, L, args.cpu_faces, 1, args.cpu_steps)
 
     out = {
         "metric": METRIC,

@@ -21,6 +21,78 @@ void throw_hip(hipError_t e, const char* what, const char* file, int line) {
 }
 }  // namespace pb
 
+namespace pb {
+// The argument block of k_assemble for this context and step (forcing already on the device).
+AssembleArgs assemble_args(pbsm3d_ctx* c, const double* const fdev[9], int32_t dt_s) {
+    const int64_t F = c->F;
+    const int L = c->L;
+    const pbsm3d_config& cfg = c->cfg;
+    AssembleArgs a{};
+    a.F = F;
+    a.L = L;
+    a.dz = c->dz;
+    a.dt = (double)dt_s;
+    a.do_fixed_settling = cfg.do_fixed_settling;
+    a.do_sublimation = cfg.do_sublimation;
+    a.do_lateral_diff = cfg.do_lateral_diff;
+    a.enable_veg = c->enable_veg ? 1 : 0;
+    a.use_tanh_fetch = cfg.use_tanh_fetch;
+    a.use_exp_fetch = cfg.use_exp_fetch;
+    a.use_R94_lambda = cfg.use_R94_lambda;
+    a.use_PomLi_probability = cfg.use_PomLi_probability;
+    a.z0_ustar_coupling = cfg.z0_ustar_coupling;
+    a.rouault_diffusion_coef = cfg.rouault_diffusion_coef;
+    a.settling_velocity = cfg.settling_velocity;
+    a.settling_scale = cfg.settling_scale;
+    a.min_sd_trans = cfg.min_sd_trans;
+    a.cutoff = cfg.cutoff;
+    a.snow_diffusion_const = cfg.snow_diffusion_const;
+    a.z0_snow = cfg.z0_snow;
+    a.nx = c->nx.p;
+    a.ny = c->ny.p;
+    a.elen = c->elen.p;
+    a.area = c->area.p;
+    a.cz = c->cz.p;
+    a.neigh = c->neigh.p;
+    a.slots = c->slots.p;
+    a.canopy = c->canopy.p;
+    a.lai = c->lai.p;
+    a.stalk_n = c->stalk_n.p;
+    a.stalk_d = c->stalk_d.p;
+    a.is_water = c->water.p;
+    a.U_R = fdev[0];
+    a.U_2m = fdev[1];
+    a.vw_dir = fdev[2];
+    a.swe = fdev[3];
+    a.t = fdev[4];
+    a.rh = fdev[5];
+    a.snowdepthavg = fdev[6];
+    a.fetch = fdev[7];
+    a.p_snow_hours = fdev[8];
+    a.blowing_prob = c->blowprob.p;
+    a.rp = c->susp.rp.p;
+    a.val = c->susp.val.p;
+    a.rhs = c->susp.rhs.p;
+    a.x = c->susp.x.p;
+    a.u_z = c->u_z.p;
+    a.csubl = c->csubl.p;
+    a.Qsalt = c->Qsalt.p;
+    a.salt_flag = c->salt_flag.p;
+    a.pc_lo = c->colf.lo.p;
+    a.pc_cp = c->colf.cp.p;
+    a.pc_im = c->colf.im.p;
+    a.pc_lo32 = c->colf.lo32.p;
+    a.pc_cp32 = c->colf.cp32.p;
+    a.pc_im32 = c->colf.im32.p;
+    a.pc_invdiag = c->invdiag_s.p;
+    a.fscal = c->fscal.p;
+    a.z0_face = c->z0_face.p;
+    a.settle_face = c->settle_face.p;
+    return a;
+}
+
+}  // namespace pb
+
 namespace {
 
 std::mutex g_err_mu;
@@ -545,65 +617,7 @@ int pbsm3d_step(pbsm3d_ctx* c, const pbsm3d_forcing* fo, int32_t dt_s, pbsm3d_ou
         PB_HIP(hipEventRecord(c->ev[0], s));
 
         // ---- coefficients + suspension assembly (+ fused preconditioner set-up)
-        AssembleArgs a{};
-        a.F = F;
-        a.L = L;
-        a.dz = c->dz;
-        a.dt = (double)dt_s;
-        a.do_fixed_settling = cfg.do_fixed_settling;
-        a.do_sublimation = cfg.do_sublimation;
-        a.do_lateral_diff = cfg.do_lateral_diff;
-        a.enable_veg = c->enable_veg ? 1 : 0;
-        a.use_tanh_fetch = cfg.use_tanh_fetch;
-        a.use_exp_fetch = cfg.use_exp_fetch;
-        a.use_R94_lambda = cfg.use_R94_lambda;
-        a.use_PomLi_probability = cfg.use_PomLi_probability;
-        a.z0_ustar_coupling = cfg.z0_ustar_coupling;
-        a.rouault_diffusion_coef = cfg.rouault_diffusion_coef;
-        a.settling_velocity = cfg.settling_velocity;
-        a.settling_scale = cfg.settling_scale;
-        a.min_sd_trans = cfg.min_sd_trans;
-        a.cutoff = cfg.cutoff;
-        a.snow_diffusion_const = cfg.snow_diffusion_const;
-        a.z0_snow = cfg.z0_snow;
-        a.nx = c->nx.p;
-        a.ny = c->ny.p;
-        a.elen = c->elen.p;
-        a.area = c->area.p;
-        a.cz = c->cz.p;
-        a.neigh = c->neigh.p;
-        a.slots = c->slots.p;
-        a.canopy = c->canopy.p;
-        a.lai = c->lai.p;
-        a.stalk_n = c->stalk_n.p;
-        a.stalk_d = c->stalk_d.p;
-        a.is_water = c->water.p;
-        a.U_R = fdev[0];
-        a.U_2m = fdev[1];
-        a.vw_dir = fdev[2];
-        a.swe = fdev[3];
-        a.t = fdev[4];
-        a.rh = fdev[5];
-        a.snowdepthavg = fdev[6];
-        a.fetch = fdev[7];
-        a.p_snow_hours = fdev[8];
-        a.blowing_prob = c->blowprob.p;
-        a.rp = c->susp.rp.p;
-        a.val = c->susp.val.p;
-        a.rhs = c->susp.rhs.p;
-        a.x = c->susp.x.p;
-        a.u_z = c->u_z.p;
-        a.csubl = c->csubl.p;
-        a.Qsalt = c->Qsalt.p;
-        a.salt_flag = c->salt_flag.p;
-        a.pc_lo = c->colf.lo.p;
-        a.pc_cp = c->colf.cp.p;
-        a.pc_im = c->colf.im.p;
-        a.pc_lo32 = c->colf.lo32.p;
-        a.pc_cp32 = c->colf.cp32.p;
-        a.pc_im32 = c->colf.im32.p;
-        a.pc_invdiag = c->invdiag_s.p;
-        a.fscal = c->fscal.p;
+        AssembleArgs a = assemble_args(c, fdev, dt_s);
         // the features of pbsm3d_training_pairs read this step's vw_dir and t later: keep
         // context-owned copies (the caller's device forcing may be freed or reused by then)
         for (int k : {2, 4}) {

@@ -128,7 +128,8 @@ __global__ void __launch_bounds__(kBlock) k_assemble(AssembleArgs a) {
     const double T = a.t[i];
     const double uth = 0.35 + (1.0 / 150.0) * T + (1.0 / 8200.0) * T * T;
     bool salt = false;
-    double ustar = 1.3, lambda = 0, z0 = a.z0_snow;
+    const double z0_snow = a.z0_face ? a.z0_face[i] : a.z0_snow;
+    double ustar = 1.3, lambda = 0, z0 = z0_snow;
     const bool water = a.is_water ? a.is_water[i] != 0 : false;
     if (hdiff <= a.cutoff && sd >= a.min_sd_trans && !water) {
         if (a.use_R94_lambda)
@@ -150,10 +151,10 @@ __global__ void __launch_bounds__(kBlock) k_assemble(AssembleArgs a) {
         }
     }
     if (!salt) {
-        z0 = a.z0_snow;
+        z0 = z0_snow;
         ustar = fmax(0.01, kKappa * uref / log(kZUR / z0));
     }
-    z0 = fmax(a.z0_snow, z0);
+    z0 = fmax(z0_snow, z0);
     ustar = fmax(0.01, ustar);
     const double hs = salt ? 0.08436 * pow(ustar, 1.27) : 0.0;
 
@@ -189,8 +190,8 @@ __global__ void __launch_bounds__(kBlock) k_assemble(AssembleArgs a) {
             // probability of blowing snow, Essery, Li & Pomeroy 1999 eqs 10-14 (PBSM3D.cpp:848-865),
             // with the 10 m wind log-scaled from the reference height (PBSM3D.cpp:451-463)
             const double z10 = 10. + sd;
-            const double u10 = z10 < kZUR ? uref * log((z10 - (sd + a.z0_snow)) / a.z0_snow) /
-                                                log((kZUR - (sd + a.z0_snow)) / a.z0_snow)
+            const double u10 = z10 < kZUR ? uref * log((z10 - (sd + z0_snow)) / z0_snow) /
+                                                log((kZUR - (sd + z0_snow)) / z0_snow)
                                           : uref;
             const double A = a.p_snow_hours[i];
             const double u_mean = 11.2 + 0.365 * T + 0.00706 * T * T + 0.9 * log(A);
@@ -261,7 +262,7 @@ __global__ void __launch_bounds__(kBlock) k_assemble(AssembleArgs a) {
         const double r_z = pow((3.0 * mm) / (4 * kPi * kRhoIce), 0.3333333);
         const double xrz = 0.005 * pow(u_z, 1.36);
         double w = a.do_fixed_settling ? a.settling_velocity : 1.1e7 * pow(r_z, 1.8);
-        w *= a.settling_scale;
+        w *= a.settle_face ? a.settle_face[i] : a.settling_scale;
         const double Vr = w + 3.0 * xrz * cos(kPi / 4.0);
         const double Re = 2.0 * r_z * Vr / 1.88e-5;
         const double Nu = 1.79 + 0.606 * pow(Re, 0.5);
@@ -441,6 +442,7 @@ __global__ void __launch_bounds__(kBlock) k_deposition(DepositionArgs a) {
 __global__ void __launch_bounds__(kBlock) k_drift(DriftArgs a) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.F) return;
+    if (a.face_member && !a.member_on[a.face_member[i]]) return;  // that member had no deposition
     const double q = a.qdep[i];
     double mass = (chm_missing(q) ? 0 : q) * a.dt;
     double swe = a.swe[i];

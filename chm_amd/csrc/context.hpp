@@ -24,6 +24,7 @@ struct pbsm3d_ctx {
     pb::System susp, dep;
     pb::DevBuf<double> u_z, csubl, Qsalt, Qsusp, Qsubl, Qsubl_mass, sum_subl, drift_mass, sum_drift, more;
     pb::DevBuf<double> blowprob;  // blowingsnow_probability (use_PomLi_probability)
+    pb::DevBuf<double> z0_face, settle_face;  // per-face z0_snow / settling_scale of a batch union (else empty)
     pb::DevBuf<double> fscal;  // [3F] per-face c_salt, hs, u* of the last step
     pb::DevBuf<uint8_t> salt_flag;
     pb::DevBuf<double> forcing_stage;
@@ -68,6 +69,9 @@ struct pbsm3d_ctx {
 };
 
 namespace pb {
+// the argument block of k_assemble for this context and step (fdev: the nine forcing
+// fields on the device, NULL where absent)
+AssembleArgs assemble_args(pbsm3d_ctx* c, const double* const fdev[9], int32_t dt_s);
 // features of the current step into c->feat (needs the assembled step)
 void warm_features(pbsm3d_ctx* c, const double* vw_dir, const double* t);
 // x0 of the suspension (which = 0) or deposition (which = 1) solve from c->pred

@@ -58,6 +58,9 @@ struct AssembleArgs {
     int use_tanh_fetch, use_exp_fetch, use_R94_lambda;
     int use_PomLi_probability, z0_ustar_coupling, rouault_diffusion_coef;
     double settling_velocity, settling_scale, min_sd_trans, cutoff, snow_diffusion_const, z0_snow;
+    // per-face values of z0_snow / settling_scale (a batch of parameter-ensemble members held
+    // as one union), or null: the scalars above
+    const double *z0_face, *settle_face;
     // geometry, edges j-major ([j*F + i])
     const double *nx, *ny, *elen, *area, *cz;
     const int32_t* neigh;
@@ -100,6 +103,9 @@ struct DriftArgs {
     const double *qdep, *swe;
     const uint8_t* salt_flag;
     double *drift_mass, *sum_drift, *more;
+    // batch: face i belongs to member face_member[i], whose drift runs only when
+    // member_on[member] != 0 (its deposition was present); null: every face
+    const int32_t *face_member, *member_on;
 };
 
 __global__ void k_assemble(AssembleArgs a);
@@ -216,6 +222,26 @@ void split_op(int which, hipStream_t s, const ColorOrder& co, int64_t F, int L, 
 template <typename T>
 int64_t split_apply_dots(hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFactors<T>& d, const T* in, T* out,
                          const T* dw, double* partial, int mode, const double* stop = nullptr);
+// Workgroup tables of a batch of independent systems held as one union (batch.hip):
+// the union's faces are member-major, so each colour's positions are too, and the
+// sweeps of colour c run as nblk[c] workgroups, tab[c][b] = {k0, k1, member, 0} (faces
+// [k0, k1) of one member, at most kBlock).  The forward launches of one application
+// (colour 0 first, then 1 .. top) write their DOTS partial pairs at pair offset
+// part_off[c] + b.
+constexpr int kMaxColors = 8;
+struct SweepTables {
+    int ncolor = 0;
+    DevBuf<int4> tab[kMaxColors];
+    int64_t nblk[kMaxColors] = {};
+    int64_t part_off[kMaxColors] = {};
+    int64_t total = 0;  // partial pairs of one application
+};
+// split_op(0) of a batch: every workgroup returns at entry when stop[member * stop_stride]
+// != 0 (that member's run has finished; stop may be NULL); with dw, the partial pairs
+// (dw.out, out.out) of each workgroup go to partial[2 (part_off[c] + b)].  L <= 16.
+template <typename T>
+void split_apply_batch(hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFactors<T>& d, const SweepTables& st,
+                       const T* in, T* out, const T* dw, double* partial, const double* stop, int stop_stride);
 constexpr int kSplitModes = 8;
 // times split_op(0) in each mode (HIP events on the stream) and returns the fastest;
 // ms_out[kSplitModes] receives the averages (ms), if given.

@@ -276,17 +276,13 @@ __global__ void __launch_bounds__(kBlock) k_dilu_bwd(int64_t F, int L, int64_t k
 
 // t = D~^-1 (w - U t) for one colour; tnb reads t at the (higher-colour)
 // neighbour positions, t is written at the colour's own positions
-template <typename T, int LM, bool EX, bool XCD, bool NT>
-__global__ void __launch_bounds__(kBlock) k_eis_bwd(int64_t F, int Lrt, int64_t k0, int64_t k1,
-                                                    const int32_t* __restrict__ nbp,
-                                                    const uint8_t* __restrict__ lowmask, const T* __restrict__ lat,
-                                                    const T* __restrict__ vert, const T* __restrict__ dinv,
-                                                    const T* __restrict__ w, const T* __restrict__ tnb,
-                                                    T* __restrict__ t, const double* __restrict__ stop) {
-    if (stop && *stop != 0.0) return;  // a finished BiCGStab run (uniform over the grid)
+template <typename T, int LM, bool EX, bool NT>
+__device__ __forceinline__ void eis_bwd_column(int64_t F, int Lrt, int64_t k, const int32_t* __restrict__ nbp,
+                                               const uint8_t* __restrict__ lowmask, const T* __restrict__ lat,
+                                               const T* __restrict__ vert, const T* __restrict__ dinv,
+                                               const T* __restrict__ w, const T* __restrict__ tnb,
+                                               T* __restrict__ t) {
     const int L = EX ? LM : Lrt;  // EX: the layer count is the unroll bound, no guards
-    const int64_t k = k0 + logical_block<XCD>((k1 - k0 + kBlock - 1) / kBlock) * kBlock + threadIdx.x;
-    if (k >= k1) return;
     const int64_t N = F * L;
     const uint32_t lm = lowmask[k];
     int32_t np_[3];
@@ -320,6 +316,19 @@ __global__ void __launch_bounds__(kBlock) k_eis_bwd(int64_t F, int Lrt, int64_t 
             next = dv[z] * (wv[z] - (av[z] + va[z] * next));
             t[(int64_t)z * F + k] = next;
         }
+}
+
+template <typename T, int LM, bool EX, bool XCD, bool NT>
+__global__ void __launch_bounds__(kBlock) k_eis_bwd(int64_t F, int Lrt, int64_t k0, int64_t k1,
+                                                    const int32_t* __restrict__ nbp,
+                                                    const uint8_t* __restrict__ lowmask, const T* __restrict__ lat,
+                                                    const T* __restrict__ vert, const T* __restrict__ dinv,
+                                                    const T* __restrict__ w, const T* __restrict__ tnb,
+                                                    T* __restrict__ t, const double* __restrict__ stop) {
+    if (stop && *stop != 0.0) return;  // a finished BiCGStab run (uniform over the grid)
+    const int64_t k = k0 + logical_block<XCD>((k1 - k0 + kBlock - 1) / kBlock) * kBlock + threadIdx.x;
+    if (k >= k1) return;
+    eis_bwd_column<T, LM, EX, NT>(F, Lrt, k, nbp, lowmask, lat, vert, dinv, w, tnb, t);
 }
 
 // Forward sweep of one colour: acc = L v (lower neighbours from vnb, the cell
@@ -448,6 +457,54 @@ __global__ void __launch_bounds__(kBlock) k_eis_fwd(int64_t F, int L, int64_t k0
     }
 }
 
+
+// Table-driven forms for a batch of independent systems held as one union (members'
+// faces consecutive, so each colour's positions are member-major): workgroup b takes
+// faces [tab[b].x, tab[b].y) of one colour of member tab[b].z and returns at once when
+// that member's run has stopped (stop[member * stop_stride] != 0).  No workgroup spans
+// two members, so the DOTS partials of a workgroup belong to one member.
+template <typename T, int LM, bool EX, bool NT>
+__global__ void __launch_bounds__(kBlock) k_eis_bwd_tab(int64_t F, int Lrt, const int4* __restrict__ tab,
+                                                        const int32_t* __restrict__ nbp,
+                                                        const uint8_t* __restrict__ lowmask, const T* __restrict__ lat,
+                                                        const T* __restrict__ vert, const T* __restrict__ dinv,
+                                                        const T* __restrict__ w, const T* __restrict__ tnb,
+                                                        T* __restrict__ t, const double* __restrict__ stop,
+                                                        int stop_stride) {
+    const int4 e = tab[blockIdx.x];
+    if (stop && stop[(int64_t)e.z * stop_stride] != 0.0) return;  // uniform over the workgroup
+    const int64_t k = (int64_t)e.x + threadIdx.x;
+    if (k >= e.y) return;
+    eis_bwd_column<T, LM, EX, NT>(F, Lrt, k, nbp, lowmask, lat, vert, dinv, w, tnb, t);
+}
+
+template <typename T, int LM, bool EX, bool WITH_T, bool FUSE, bool DOTS, bool NT>
+__global__ void __launch_bounds__(kBlock) k_eis_fwd_tab(int64_t F, int L, const int4* __restrict__ tab,
+                                                        const int32_t* __restrict__ nbp,
+                                                        const uint8_t* __restrict__ lowmask, const T* __restrict__ lat,
+                                                        const T* __restrict__ vert, const T* __restrict__ dinv,
+                                                        const T* __restrict__ E, const T* __restrict__ w,
+                                                        const T* __restrict__ t, const T* __restrict__ tnb,
+                                                        const T* __restrict__ vnb, T* __restrict__ v,
+                                                        T* __restrict__ o, const T* __restrict__ dw,
+                                                        double* __restrict__ partial, const double* __restrict__ stop,
+                                                        int stop_stride) {
+    const int4 e = tab[blockIdx.x];
+    if (stop && stop[(int64_t)e.z * stop_stride] != 0.0) return;  // uniform over the workgroup
+    const int64_t k = (int64_t)e.x + threadIdx.x;
+    double d1 = 0, d2 = 0;
+    if (k < e.y) eis_fwd_column<T, LM, EX, WITH_T, FUSE, DOTS, NT>(F, L, k, nbp, lowmask, lat, vert, dinv, E, w, t, tnb,
+                                                                  vnb, v, o, dw, d1, d2);
+    if constexpr (DOTS) {
+        __shared__ double s_red[kBlock / 64];
+        const double a = block_sum(d1, s_red);
+        const double b = block_sum(d2, s_red);
+        if (threadIdx.x == 0) {
+            partial[2 * blockIdx.x] = a;
+            partial[2 * blockIdx.x + 1] = b;
+        }
+    }
+}
 
 // ---------------------------------------------------------------------------
 // Layer-parallel form of the split operator's sweeps.  A workgroup is 64 face
@@ -1161,6 +1218,61 @@ int64_t split_apply_dots(hipStream_t s, const ColorOrder& co, int64_t F, int L, 
     return split_mode_dispatch<T>(0, s, co, F, L, d, in, out, dw, partial, mode, stop);
 }
 
+namespace {
+template <typename T, int LM, bool EX>
+void split_apply_batch_t(hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFactors<T>& d,
+                         const SweepTables& st, const T* w, T* o, const T* dw, double* partial, const double* stop,
+                         int stop_stride) {
+    const int top = co.ncolor - 1;
+    for (int c = top; c >= 1; --c) {
+        if (st.nblk[c] == 0) continue;
+        hipLaunchKernelGGL((k_eis_bwd_tab<T, LM, EX, false>), dim3((unsigned)st.nblk[c]), dim3(kBlock), 0, s, F, L,
+                           st.tab[c].p, co.nbp.p, co.lowmask.p, d.lat.p, d.vert.p, d.dinv.p, w, d.t.p, d.t.p, stop,
+                           stop_stride);
+    }
+    for (int c = 0; c <= top; ++c) {
+        if (st.nblk[c] == 0) continue;
+        const dim3 g((unsigned)st.nblk[c]);
+        double* pc = dw ? partial + 2 * st.part_off[c] : nullptr;
+#define PB_FWD_TAB(FUSE, DOTS)                                                                                   \
+    hipLaunchKernelGGL((k_eis_fwd_tab<T, LM, EX, true, FUSE, DOTS, false>), g, dim3(kBlock), 0, s, F, L, st.tab[c].p, \
+                       co.nbp.p, co.lowmask.p, d.lat.p, d.vert.p, d.dinv.p, d.E.p, w, d.t.p, d.t.p, d.y.p, d.y.p, o, dw, \
+                       pc, stop, stop_stride)
+        if (c == 0) {
+            if (dw) PB_FWD_TAB(true, true); else PB_FWD_TAB(true, false);
+        } else {
+            if (dw) PB_FWD_TAB(false, true); else PB_FWD_TAB(false, false);
+        }
+#undef PB_FWD_TAB
+    }
+}
+}  // namespace
+
+template <typename T>
+void split_apply_batch(hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFactors<T>& d, const SweepTables& st,
+                       const T* in, T* out, const T* dw, double* partial, const double* stop, int stop_stride) {
+    if (co.ncolor > kMaxColors) throw Error(PBSM3D_EINVAL, "too many colours for the batch tables");
+#define PB_BATCH(LMV)                                                                                   \
+    {                                                                                                   \
+        if (L == LMV)                                                                                   \
+            split_apply_batch_t<T, LMV, true>(s, co, F, L, d, st, in, out, dw, partial, stop, stop_stride); \
+        else                                                                                            \
+            split_apply_batch_t<T, LMV, false>(s, co, F, L, d, st, in, out, dw, partial, stop, stop_stride); \
+    }
+    if (L <= 1)
+        PB_BATCH(1)
+    else if (L <= 5)
+        PB_BATCH(5)
+    else if (L <= 10)
+        PB_BATCH(10)
+    else if (L <= 16)
+        PB_BATCH(16)
+    else
+        throw Error(PBSM3D_EINVAL, "the split D-ILU operator needs nLayer <= 16");
+#undef PB_BATCH
+    PB_HIP(hipGetLastError());
+}
+
 template <typename T>
 int tune_split_mode(hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFactors<T>& d, const T* w, T* o,
                     float* ms_out) {
@@ -1315,6 +1427,12 @@ template void dilu_apply<double>(hipStream_t, const ColorOrder&, int64_t, int, D
                                  double*, bool, bool);
 template void dilu_apply<float>(hipStream_t, const ColorOrder&, int64_t, int, DiluFactors<float>&, const float*,
                                 float*, bool, bool);
+template void split_apply_batch<double>(hipStream_t, const ColorOrder&, int64_t, int, DiluFactors<double>&,
+                                        const SweepTables&, const double*, double*, const double*, double*,
+                                        const double*, int);
+template void split_apply_batch<float>(hipStream_t, const ColorOrder&, int64_t, int, DiluFactors<float>&,
+                                       const SweepTables&, const float*, float*, const float*, double*, const double*,
+                                       int);
 template void split_op<double>(int, hipStream_t, const ColorOrder&, int64_t, int, DiluFactors<double>&, const double*,
                                double*, int);
 template void split_op<float>(int, hipStream_t, const ColorOrder&, int64_t, int, DiluFactors<float>&, const float*,

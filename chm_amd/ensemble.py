@@ -106,6 +106,70 @@ def build(members: list[Member], precision=abi.MIXED, krylov=abi.KRYLOV_BICGSTAB
     return built
 
 
+def _mesh_of(args):
+    n_points, seed = args
+    return meshgen.synthetic_mesh(n_points, seed=seed)
+
+
+def member_meshes(members: list[Member], workers: int = 16) -> list[meshgen.Mesh]:
+    """The members' seeded meshes, generated in a process pool (set-up, not timed)."""
+    jobs = [(m.n_points, m.seed) for m in members]
+    if workers <= 1 or len(jobs) <= 1:
+        return [_mesh_of(j) for j in jobs]
+    import multiprocessing as mp
+
+    with concurrent.futures.ProcessPoolExecutor(max_workers=min(workers, len(jobs)),
+                                                mp_context=mp.get_context("spawn")) as ex:
+        return list(ex.map(_mesh_of, jobs))
+
+
+def member_config(m: Member, precision=abi.MIXED, krylov=abi.KRYLOV_BICGSTAB_SPLIT, warm: bool = False,
+                  **overrides) -> abi.Config:
+    cfg = pb.default_config(nLayer=m.n_layer, precision=precision, krylov=krylov, z0_snow=m.z0,
+                            settling_scale=m.settling_scale, **overrides)
+    if warm:
+        cfg.warm_start = abi.WS_LEARNED
+    return cfg
+
+
+def build_batch(members: list[Member], meshes, precision=abi.MIXED, warm_net=None, device: int = 0, **overrides):
+    """All members in one pbsm3d_batch (members of equal nLayer share one union system;
+    the learned start applies to the 10-layer members).  Set-up, not timed."""
+    geoms = [pb.face_geometry(mesh) for mesh in meshes]
+    cfgs = [member_config(m, precision, abi.KRYLOV_BICGSTAB_SPLIT, warm_net is not None, **overrides)
+            for m in members]
+    b = pb.Batch(geoms, cfgs, device=device)
+    if warm_net is not None:
+        b.set_warm_start(warm_net)
+    return b
+
+
+def run_batch(batch, members: list[Member], meshes, steps: int, device: int = 0) -> dict:
+    """`steps` timesteps of every member through one pbsm3d_batch_step call per step,
+    forcing and outputs resident in HBM."""
+    import torch
+
+    dev = f"cuda:{device}"
+    forcings = [member_forcing_device(mesh, m, steps, dev) for m, mesh in zip(members, meshes)]
+    outs = [{k: torch.zeros(mesh.n_faces, dtype=torch.float64, device=dev) for k in abi.OUT_FIELDS}
+            for mesh in meshes]
+    torch.cuda.synchronize()
+    iters = [0] * len(members)
+    worst = 0.0
+    t0 = time.perf_counter()
+    for s in range(steps):
+        _, st = batch.step([f[s] for f in forcings], outs=outs, on_device=True)
+        for k, x in enumerate(st):
+            iters[k] += x["susp_iters"]
+            worst = max(worst, x["susp_resid"], x["dep_resid"])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    cells = sum(mesh.n_faces * m.n_layer for m, mesh in zip(members, meshes))
+    return {"members": len(members), "steps": steps, "wall_s": wall, "member_steps_per_s": len(members) * steps / wall,
+            "cell_steps_per_s": cells * steps / wall, "max_resid": worst, "susp_iters": iters,
+            "unions": batch.info()["unions"], "outs": outs}
+
+
 def run(built, steps: int, workers: int = 16, on_device: bool = True, device: int = 0) -> dict:
     """Run every member for `steps` timesteps, members in parallel on `workers` host threads."""
     import torch

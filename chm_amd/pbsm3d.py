@@ -41,6 +41,14 @@ EXPORTED = (
     "pbsm3d_fetch_nearest",
     "pbsm3d_fetch_destroy",
     "pbsm3d_fetch_last_error",
+    "pbsm3d_batch_create",
+    "pbsm3d_batch_step",
+    "pbsm3d_batch_set_warm_start",
+    "pbsm3d_batch_info",
+    "pbsm3d_batch_get_state",
+    "pbsm3d_batch_set_state",
+    "pbsm3d_batch_destroy",
+    "pbsm3d_batch_last_error",
     "pbsm3d_wind_config_default",
     "pbsm3d_wind_create",
     "pbsm3d_wind_run",
@@ -128,6 +136,18 @@ def lib():
         L.pbsm3d_face_colors.argtypes = [C.c_int64, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
         L.pbsm3d_precond_tuning.argtypes = [vp, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_float)]
         L.pbsm3d_precond_select.argtypes = [vp, C.c_int, C.c_int]
+        L.pbsm3d_batch_create.argtypes = [C.POINTER(vp), C.c_int32, C.POINTER(abi.Mesh), C.POINTER(abi.Config),
+                                          C.c_int]
+        L.pbsm3d_batch_step.argtypes = [vp, C.POINTER(abi.Forcing), C.c_int32, C.POINTER(abi.Out),
+                                        C.POINTER(abi.Stats)]
+        L.pbsm3d_batch_set_warm_start.argtypes = [vp, vp]
+        L.pbsm3d_batch_info.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        L.pbsm3d_batch_get_state.argtypes = [vp, C.c_int32, dp, dp]
+        L.pbsm3d_batch_set_state.argtypes = [vp, C.c_int32, dp, dp]
+        L.pbsm3d_batch_destroy.argtypes = [vp]
+        L.pbsm3d_batch_destroy.restype = None
+        L.pbsm3d_batch_last_error.argtypes = [vp]
+        L.pbsm3d_batch_last_error.restype = C.c_char_p
         L.pbsm3d_fetch_config_default.argtypes = [C.POINTER(abi.FetchConfig)]
         L.pbsm3d_fetch_config_default.restype = None
         L.pbsm3d_fetch_create.argtypes = [C.POINTER(vp), C.c_int, C.c_int64, dp, dp, dp, dp,
@@ -476,6 +496,80 @@ class Fetch:
 
     def __del__(self):
         self.close()
+
+
+class Batch:
+    """Parameter-ensemble members advanced together (include/pbsm3d_batch.h): members of
+    equal nLayer share one union system whose sweeps and BiCGStab kernels are launched
+    once for all of them, with per-member BiCGStab scalars and stop flags."""
+
+    def __init__(self, geoms, cfgs, device: int = 0):
+        L = lib()
+        self.M = len(geoms)
+        if len(cfgs) != self.M:
+            raise ValueError("one config per member")
+        self._keep = abi.Keep()
+        meshes = (abi.Mesh * self.M)(*[g.struct(self._keep) for g in geoms])
+        self.cfgs = (abi.Config * self.M)(*cfgs)
+        self.F = [g.n_faces for g in geoms]
+        self.L = [int(c.nLayer) for c in cfgs]
+        h = C.c_void_p()
+        rc = L.pbsm3d_batch_create(C.byref(h), self.M, meshes, self.cfgs, device)
+        if rc != 0:
+            raise Pbsm3dError(rc, L.pbsm3d_batch_last_error(None).decode(errors="replace"))
+        self._h = h
+
+    def _check(self, rc):
+        if rc != 0:
+            raise Pbsm3dError(rc, lib().pbsm3d_batch_last_error(self._h).decode(errors="replace"))
+
+    def step(self, forcings, dt: int = 900, outs=None, want_fields: bool = False, on_device: bool = False):
+        """One timestep of every member: forcings[m] as Context.step takes it; returns
+        (outputs per member, stats per member)."""
+        keep = abi.Keep()
+        fs = (abi.Forcing * self.M)(*[abi.forcing_struct(f, keep, on_device=on_device) for f in forcings])
+        if outs is None:
+            outs = []
+            for m in range(self.M):
+                o = {k: np.zeros(self.F[m]) for k in abi.OUT_FIELDS}
+                if want_fields:
+                    o["conc"] = np.zeros(self.F[m] * self.L[m])
+                    o["qdep"] = np.zeros(self.F[m])
+                outs.append(o)
+        os_ = (abi.Out * self.M)(*[abi.out_struct(o, keep, on_device=any(hasattr(v, "data_ptr") for v in o.values()))
+                                   for o in outs])
+        st = (abi.Stats * self.M)()
+        self._check(lib().pbsm3d_batch_step(self._h, fs, int(dt), os_, st))
+        return outs, [s.as_dict() for s in st]
+
+    def set_warm_start(self, net):
+        self._check(lib().pbsm3d_batch_set_warm_start(self._h, net._h if net is not None else None))
+
+    def info(self) -> dict:
+        u, n, it = C.c_int32(0), C.c_int64(0), C.c_int64(0)
+        self._check(lib().pbsm3d_batch_info(self._h, C.byref(u), C.byref(n), C.byref(it)))
+        return {"unions": int(u.value), "launches": int(n.value), "iterations": int(it.value)}
+
+    def get_state(self, m: int):
+        a, b = np.zeros(self.F[m]), np.zeros(self.F[m])
+        keep = abi.Keep()
+        self._check(lib().pbsm3d_batch_get_state(self._h, m, keep.dptr(a), keep.dptr(b)))
+        return a, b
+
+    def set_state(self, m: int, sum_drift, sum_subl):
+        keep = abi.Keep()
+        self._check(lib().pbsm3d_batch_set_state(self._h, m, keep.dptr(sum_drift), keep.dptr(sum_subl)))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pbsm3d_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def wind_config(**overrides) -> abi.WindConfig:

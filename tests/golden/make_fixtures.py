@@ -8,7 +8,9 @@
   CSR patterns built by an independent set-based numpy builder for the
   config-1 mesh (L = 5, 10) and the granger1m mesh (L = 10).
 * tiny_fields.npz -- inputs and outputs of one step on a 12-point mesh solved
-  by dense Gaussian elimination (no Krylov tolerance involved).
+  by dense Gaussian elimination (no Krylov tolerance involved), written by the
+  oracle itself: a regression fixture for the oracle and the GPU path, not an
+  independent pin.
 * slope_mesh.h5, slope_param.h5 -- the reference's HDF5 test mesh and its
   parameter file, copied as data (functional_tests/mesh_versioning/
   slope.metis_mesh.h5, slope.metis_param.h5).

@@ -53,14 +53,25 @@ def test_pattern_and_assembly_c1(pb, oracle, mesh_c1):
     gold = json.load(open(os.path.join(GOLD, "patterns.json")))
     from tests.test_oracle import _pattern_sha
 
+    # the deposition right-hand side follows from each side's own suspension solution: the
+    # oracle, handed the GPU's solutions, assembles the same deposition system
+    orc_r = oracle.Oracle(g, oracle.oracle_config(nLayer=5, tol=1e-13))
+    orc_r.set_replay(gg["conc"], gg["qdep"])
+    rg, _ = orc_r.step(fo)
     for which in (0, 1):
         rp_g, col_g, val_g, rhs_g = ctx.system(which)
         rp_o, col_o, val_o, rhs_o = orc.system(which)
         assert np.array_equal(rp_g, rp_o) and np.array_equal(col_g, col_o)  # bit-exact pattern
         assert _pattern_sha(rp_g, col_g) == gold["c1_L5" if which == 0 else "c1_L1"]["sha256"]
+        assert _row_rel(val_g, val_o, rp_o) < 1e-12, which
         if which == 0:
-            assert _row_rel(val_g, val_o, rp_o) < 1e-12
             assert rel_err(rhs_g, rhs_o) < 1e-12
+        else:
+            _, _, val_r, rhs_r = orc_r.system(1)
+            assert _row_rel(val_g, val_r, rp_o) < 1e-12
+            assert rel_err(rhs_g, rhs_r) < 1e-12
+            for k in ("Qsusp", "Qsubl", "drift_mass"):
+                assert rel_err(gg[k], rg[k]) < 1e-12, k
     np.testing.assert_allclose(gg["Qsalt"], og["Qsalt"], rtol=1e-12, atol=0)
     assert gst["suspension_present"] == ost["suspension_present"] == 1
     assert gst["deposition_present"] == ost["deposition_present"] == 1
@@ -210,19 +221,36 @@ def test_missing_value_band(pb, oracle, mesh_small):
     for k in abi.OUT_FIELDS:
         np.testing.assert_array_equal(out["missing"][k], out["zero"][k], err_msg=k)
     assert (out["missing"]["pbsm_more_than_avail"][pick] == 1).any()
-    # just outside the band: a (negative) swe value, against the oracle
+    # just outside the band: a (negative) swe value, against the oracle.  Saltating faces whose
+    # swe counts as 0 (or is negative) keep or lose c_salt on the sign of the saltation
+    # divergence (PBSM3D.cpp:879-919), which is zero up to rounding (sum_j E_j m_j = 0), so the
+    # GPU and the CPU may round it to different signs: those faces are made calm here (no
+    # saltation), and swe then acts through the drift cap alone (PBSM3D.cpp:1720-1728).
     rest = np.setdiff1d(np.arange(F), pick)[:20]
-    outside = dict(miss, swe=miss["swe"].copy())
+    outside = dict(miss, swe=miss["swe"].copy(), U_2m_above_srf=miss["U_2m_above_srf"].copy())
     outside["swe"][rest] = np.where(np.arange(len(rest)) % 2 == 0, -9999.0 + 2e-5, -9999.0 - 2e-5)
+    outside["U_2m_above_srf"][np.concatenate([pick, rest])] = 2.0
     ctx = pb.Context(g, pb.default_config(nLayer=4, tol=1e-12))
     orc = oracle.Oracle(g, oracle.oracle_config(nLayer=4, tol=1e-13))
     gg, _ = ctx.step(outside, want_fields=True)
     og, _ = orc.step(outside)
     np.testing.assert_allclose(gg["Qsalt"], og["Qsalt"], rtol=1e-12, atol=0)
-    np.testing.assert_array_equal(gg["pbsm_more_than_avail"], og["pbsm_more_than_avail"])
-    for k in ("conc", "Qsusp", "qdep", "drift_mass", "sum_drift"):
+    for k in ("conc", "Qsusp", "qdep"):
         assert rel_err(gg[k], og[k]) < 1e-8, (k, rel_err(gg[k], og[k]))
-    assert not np.array_equal(gg["drift_mass"], out["missing"]["drift_mass"])
+    # the cap and its flag jump at qdep = 0 on faces whose swe counts as <= 0: a face whose
+    # deposition is zero to within the solve's rounding may fall on either side, so those
+    # faces are left out of the capped fields' comparison
+    calm = np.concatenate([pick, rest])
+    sure = np.ones(F, dtype=bool)
+    sure[calm[np.abs(og["qdep"][calm]) < 1e-9]] = False
+    np.testing.assert_array_equal(gg["pbsm_more_than_avail"][sure], og["pbsm_more_than_avail"][sure])
+    for k in ("drift_mass", "sum_drift"):
+        assert rel_err(gg[k][sure], og[k][sure]) < 1e-8, (k, rel_err(gg[k][sure], og[k][sure]))
+    # an eroding face with swe = -9999 +- 2e-5 is capped at mass = -swe (a value), not at 0
+    eroding = rest[gg["qdep"][rest] < -1e-9]
+    assert len(eroding) > 0
+    np.testing.assert_allclose(gg["drift_mass"][eroding], -outside["swe"][eroding], rtol=1e-15)
+    assert (gg["pbsm_more_than_avail"][eroding] == 1).all()
 
 
 def test_device_resident_forcing(pb, mesh_small):
