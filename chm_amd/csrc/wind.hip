@@ -11,6 +11,7 @@
 // fraction 5.1.22 by Lentz's method above.
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "../../include/pbsm3d_wind.h"
@@ -110,16 +111,28 @@ __device__ __forceinline__ double tps_basis(double sx, double sy, double ex, dou
     return -(log(dij) + c + expint_e1(dij));
 }
 
-__global__ void __launch_bounds__(kBlock) k_smooth(int64_t F, const int32_t* __restrict__ neigh,
+__global__ void k_clear_flag(double* flag) { *flag = 0.0; }
+
+// neigh[i * sf + j * sj] is neighbour j of face i (sf = 1, sj = F: the context's j-major
+// copy; sf = 3, sj = 1: the ABI's face-major array).  CHECKED: the indices come straight
+// from the caller's device memory, unvalidated: one outside [-1, F) counts as no
+// neighbour and sets *bad = 1.
+template <bool CHECKED>
+__global__ void __launch_bounds__(kBlock) k_smooth(int64_t F, const int32_t* __restrict__ neigh, int64_t sf, int64_t sj,
                                                    const double* __restrict__ cx, const double* __restrict__ cy,
-                                                   const double* __restrict__ pt, double* __restrict__ out) {
+                                                   const double* __restrict__ pt, double* __restrict__ out,
+                                                   double* __restrict__ bad) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= F) return;
     double sx[3], sy[3], sz[3];
     int n = 0;
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-        const int32_t nb = neigh[j * F + i];
+        int32_t nb = neigh[i * sf + j * sj];
+        if (CHECKED && (nb < -1 || nb >= F)) {
+            *bad = 1.0;
+            nb = -1;
+        }
         if (nb < 0) continue;
         sx[n] = cx[nb];
         sy[n] = cy[nb];
@@ -196,8 +209,31 @@ struct pbsm3d_wind_ctx {
     }
 };
 
+// Scratch of pbsm3d_wind_run_noalloc, which allocates nothing: the point-scaled wind of
+// every face (the smoothing kernel reads the neighbours' values back) and, after it, one
+// flag word for out-of-range neighbour indices.  32 MiB per GPU for the rest of the
+// process.  Calls that overlap on one GPU share it and race.
+static __device__ __align__(256) double scratch[PBSM3D_WIND_NOALLOC_MAX_FACES + 32];
+
 namespace {
 std::string g_wind_err;
+
+// The device address of `scratch` on `device` (current), looked up once per device: each
+// GPU holds its own copy at its own address, and the host-side name is not that address.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+double* noalloc_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices) throw Error(PBSM3D_EINVAL, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        PB_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[device] = static_cast<double*>(p);
+    }
+    return g_scratch_ptr[device];
+}
 
 template <typename Fn>
 int wind_guard(pbsm3d_wind_ctx* c, Fn&& fn) {
@@ -296,11 +332,46 @@ int pbsm3d_wind_run(pbsm3d_wind_ctx* c, const double* U_R, const double* snowdep
                    c->cfg.smooth ? c->point.p : dout};
         hipLaunchKernelGGL(k_point_scale, dim3(grid_for(F)), dim3(kBlock), 0, s, a);
         if (c->cfg.smooth)
-            hipLaunchKernelGGL(k_smooth, dim3(grid_for(F)), dim3(kBlock), 0, s, F, c->neigh.p, c->cx.p, c->cy.p,
-                               c->point.p, dout);
+            hipLaunchKernelGGL(k_smooth<false>, dim3(grid_for(F)), dim3(kBlock), 0, s, F, c->neigh.p, (int64_t)1, F,
+                               c->cx.p, c->cy.p, c->point.p, dout, (double*)nullptr);
         PB_HIP(hipGetLastError());
         if (!on_device) PB_HIP(hipMemcpyAsync(U2, dout, F * sizeof(double), hipMemcpyDeviceToHost, s));
         PB_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int pbsm3d_wind_run_noalloc(int device, int64_t F, const int32_t* neigh, const double* x, const double* y,
+                            const double* canopy, const double* lai, const pbsm3d_wind_config* cfg, const double* U_R,
+                            const double* snowdepthavg, double* U2) {
+    if (F <= 0 || !neigh || !x || !y || !cfg || !U_R || !U2 || !(cfg->z0_snow > 0)) {
+        g_wind_err = "invalid argument";
+        return PBSM3D_EINVAL;
+    }
+    if (F > PBSM3D_WIND_NOALLOC_MAX_FACES) {
+        g_wind_err = "pbsm3d_wind_run_noalloc: " + std::to_string(F) + " faces exceed the scratch of " +
+                     std::to_string((long long)PBSM3D_WIND_NOALLOC_MAX_FACES) + " faces";
+        return PBSM3D_EINVAL;
+    }
+    if (canopy && !lai && !cfg->ignore_canopy) {
+        g_wind_err = "LAI is required with canopy heights (scale_wind_vert.cpp:110)";
+        return PBSM3D_EINVAL;
+    }
+    return wind_guard(nullptr, [&] {
+        PB_HIP(hipSetDevice(device));
+        double* scr = noalloc_scratch(device);  // [max faces] point-scaled winds, then the flag word
+        double* bad = scr + PBSM3D_WIND_NOALLOC_MAX_FACES;
+        hipStream_t s = nullptr;  // the device's null stream: creating a stream is left to the caller's set-up
+        hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, s, bad);
+        WindArgs a{F, cfg->ignore_canopy, cfg->z0_snow, canopy, lai, U_R, snowdepthavg, cfg->smooth ? scr : U2};
+        hipLaunchKernelGGL(k_point_scale, dim3(grid_for(F)), dim3(kBlock), 0, s, a);
+        if (cfg->smooth)
+            hipLaunchKernelGGL(k_smooth<true>, dim3(grid_for(F)), dim3(kBlock), 0, s, F, neigh, (int64_t)3, (int64_t)1,
+                               x, y, scr, U2, bad);
+        PB_HIP(hipGetLastError());
+        PB_HIP(hipStreamSynchronize(s));
+        double flag = 0;
+        PB_HIP(hipMemcpy(&flag, bad, sizeof(double), hipMemcpyDeviceToHost));
+        if (flag != 0.0) throw Error(PBSM3D_EINVAL, "neighbour index out of range");
     });
 }
 

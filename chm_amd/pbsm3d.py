@@ -52,6 +52,7 @@ EXPORTED = (
     "pbsm3d_wind_config_default",
     "pbsm3d_wind_create",
     "pbsm3d_wind_run",
+    "pbsm3d_wind_run_noalloc",
     "pbsm3d_wind_destroy",
     "pbsm3d_wind_last_error",
     "pbsm3d_snow_config_default",
@@ -84,6 +85,7 @@ N_FEATURES, N_TARGETS = 16, 11
 # XCD workgroup mapping
 SPMV_VARIANTS = 10
 PRECOND_MODES = 8
+PBSM3D_WIND_NOALLOC_MAX_FACES = 4194304  # include/pbsm3d_wind.h
 
 KERNEL_CLASSES = ("assemble", "spmv64", "spmv32", "precond", "ortho", "residual", "deposition", "warmstart",
                   "factor")
@@ -163,6 +165,8 @@ def lib():
         L.pbsm3d_wind_create.argtypes = [C.POINTER(vp), C.c_int, C.c_int64, C.POINTER(C.c_int32), dp, dp, dp, dp,
                                          C.POINTER(abi.WindConfig)]
         L.pbsm3d_wind_run.argtypes = [vp, vp, vp, vp, C.c_int]
+        L.pbsm3d_wind_run_noalloc.argtypes = [C.c_int, C.c_int64, vp, vp, vp, vp, vp, C.POINTER(abi.WindConfig), vp,
+                                              vp, vp]
         L.pbsm3d_wind_destroy.argtypes = [vp]
         L.pbsm3d_wind_destroy.restype = None
         L.pbsm3d_wind_last_error.argtypes = [vp]
@@ -628,6 +632,22 @@ class Wind:
 
     def __del__(self):
         self.close()
+
+
+def wind_run_noalloc(neigh, center_x, center_y, U_R, out, snowdepthavg=None, canopy_height=None, lai=None,
+                     cfg: abi.WindConfig | None = None, device: int = 0) -> None:
+    """pbsm3d_wind_run_noalloc on device tensors (neigh int32 [F, 3], the rest float64 [F]):
+    writes U_2m_above_srf into ``out``; the library allocates nothing."""
+    L = lib()
+    cfg = cfg if cfg is not None else wind_config()
+
+    def p(t):
+        return C.c_void_p(t.data_ptr()) if t is not None else None
+
+    rc = L.pbsm3d_wind_run_noalloc(device, int(center_x.shape[0]), p(neigh), p(center_x), p(center_y),
+                                   p(canopy_height), p(lai), C.byref(cfg), p(U_R), p(snowdepthavg), p(out))
+    if rc != 0:
+        raise Pbsm3dError(rc, L.pbsm3d_wind_last_error(None).decode())
 
 
 def snow_config(**overrides) -> abi.SnowConfig:

@@ -44,6 +44,23 @@ int pbsm3d_wind_create(pbsm3d_wind_ctx** out, int device, int64_t n_faces, const
 int pbsm3d_wind_run(pbsm3d_wind_ctx* ctx, const double* U_R, const double* snowdepthavg, double* U_2m_above_srf,
                     int on_device);
 
+/* Allocation-free variant of pbsm3d_wind_create + pbsm3d_wind_run.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (a tight serving
+ * loop, a harness that forbids hipMalloc).  Every array is a device pointer the caller
+ * owns: neigh [3F] face-major as in pbsm3d_wind_create (indices are checked on the
+ * device: one outside [-1, F) makes the call return PBSM3D_EINVAL), centroids,
+ * optional canopy / lai / snowdepthavg (NULL as in pbsm3d_wind_create / _run), U_R in,
+ * U_2m_above_srf out.  The per-face intermediate lives in a fixed device array of the
+ * library (32 MiB per GPU, held until the process ends), so n_faces is at most
+ * PBSM3D_WIND_NOALLOC_MAX_FACES (else PBSM3D_EINVAL).  Work runs on the device's null
+ * stream and is complete on return.  Calls that overlap on one GPU share that array
+ * and race. */
+#define PBSM3D_WIND_NOALLOC_MAX_FACES 4194304
+int pbsm3d_wind_run_noalloc(int device, int64_t n_faces, const int32_t* neigh, const double* center_x,
+                            const double* center_y, const double* canopy_height, const double* lai,
+                            const pbsm3d_wind_config* cfg, const double* U_R, const double* snowdepthavg,
+                            double* U_2m_above_srf);
+
 void pbsm3d_wind_destroy(pbsm3d_wind_ctx* ctx);
 /* ctx NULL: the last pbsm3d_wind_create error */
 const char* pbsm3d_wind_last_error(const pbsm3d_wind_ctx* ctx);
