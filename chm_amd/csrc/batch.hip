@@ -13,10 +13,13 @@
 // runs is per member as well (its own ||b||, residual, inner target and convergence),
 // as the single-member solve does it (bicgstab_split in krylov.hip).
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <exception>
 #include <cstring>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/pbsm3d.h"
@@ -423,16 +426,30 @@ template <typename T>
 struct BatchWork {
     DevBuf<T> r, rh, p, v, sv, t;
     DevBuf<double> op_part, ss_part, xr_part, sc, tgt, fac;
-    HostBuf<double> hsc;  // two slots of every member's scalars, device-visible
+    HostBuf<double> hsc;  // six slots of every member's scalars, device-visible: 0 / 1 for
+                          // the stream-launched iterations, 2 .. 5 for the two captured graphs
     double* hsc_dev = nullptr;
     hipEvent_t ev[2] = {};
     int M = 0;
+    // two captured graphs of two iterations each (slots 2, 3 and 4, 5), replayed in turn;
+    // captured for one x and one set of factors
+    hipGraphExec_t pair[2] = {};
+    const void* pair_key[2] = {};
     BatchWork() = default;
     BatchWork(const BatchWork&) = delete;
     BatchWork& operator=(const BatchWork&) = delete;
     ~BatchWork() {
         for (hipEvent_t e : ev)
             if (e) (void)hipEventDestroy(e);
+        drop_graphs();
+    }
+    void drop_graphs() {
+        for (hipGraphExec_t& g : pair)
+            if (g) {
+                (void)hipGraphExecDestroy(g);
+                g = nullptr;
+            }
+        pair_key[0] = pair_key[1] = nullptr;
     }
     void ensure(const Segments& sg) {
         for (DevBuf<T>* b : {&r, &rh, &p, &v, &sv, &t}) b->alloc(sg.N);
@@ -442,7 +459,8 @@ struct BatchWork {
         sc.alloc((size_t)sg.M * BS_COUNT);
         tgt.alloc(sg.M);
         if (!hsc.p || M != sg.M) {
-            hsc.alloc((size_t)2 * sg.M * BS_COUNT);
+            drop_graphs();
+            hsc.alloc((size_t)6 * sg.M * BS_COUNT);
             PB_HIP(hipHostGetDevicePointer((void**)&hsc_dev, hsc.p, 0));
         }
         for (hipEvent_t& e : ev)
@@ -455,12 +473,51 @@ struct RunCount {
     int64_t launches = 0, iterations = 0;
 };
 
+// The launches of one batched BiCGStab iteration after the first (the first copies p = r):
+// p, v = Ahat p (with rh.v partials), alpha, s, t = Ahat s (with s.t, t.t), omega, x / r
+// (with rh.r, r.r), rho and the stop test, whose scalars go to host slot `slot`.
+template <typename T>
+void enqueue_iteration(hipStream_t s, Segments& sg, const ColorOrder& co, DiluFactors<T>& d, BatchWork<T>& w, T* x,
+                       int slot, bool first) {
+    const int M = sg.M;
+    const unsigned nch = (unsigned)sg.nchunk;
+    const double* stop = w.sc.p + BS_STOP;
+    if (first)
+        PB_HIP(hipMemcpyAsync(w.p.p, w.r.p, sg.N * sizeof(T), hipMemcpyDeviceToDevice, s));
+    else
+        hipLaunchKernelGGL((k_seg_p<T>), dim3(nch), dim3(kBlock), 0, s, sg.chunks.p, w.r.p, w.v.p, w.sc.p, w.p.p);
+    split_apply_batch<T>(s, co, sg.F, sg.L, d, sg.sweep, w.p.p, w.v.p, w.rh.p, w.op_part.p, stop, BS_COUNT);
+    hipLaunchKernelGGL(k_seg_stage, dim3(M), dim3(kStage), 0, s, (int)ST_ALPHA, sg.chunk_begin.p, sg.op_rng.p,
+                       w.op_part.p, (const double*)nullptr, w.sc.p, (const double*)nullptr, (double*)nullptr);
+    hipLaunchKernelGGL((k_seg_s<T>), dim3(nch), dim3(kBlock), 0, s, sg.chunks.p, w.v.p, w.r.p, w.sc.p, w.sv.p,
+                       w.ss_part.p);
+    split_apply_batch<T>(s, co, sg.F, sg.L, d, sg.sweep, w.sv.p, w.t.p, w.sv.p, w.op_part.p, stop, BS_COUNT);
+    hipLaunchKernelGGL(k_seg_stage, dim3(M), dim3(kStage), 0, s, (int)ST_OMEGA, sg.chunk_begin.p, sg.op_rng.p,
+                       w.op_part.p, w.ss_part.p, w.sc.p, (const double*)nullptr, (double*)nullptr);
+    hipLaunchKernelGGL((k_seg_xr<T>), dim3(nch), dim3(kBlock), 0, s, sg.chunks.p, w.p.p, w.sv.p, w.sv.p, w.t.p,
+                       w.rh.p, w.sc.p, x, w.r.p, w.xr_part.p);
+    hipLaunchKernelGGL(k_seg_stage, dim3(M), dim3(kStage), 0, s, (int)ST_RHO, sg.chunk_begin.p, sg.op_rng.p,
+                       w.xr_part.p, (const double*)nullptr, w.sc.p, (const double*)nullptr,
+                       w.hsc_dev + (size_t)M * BS_COUNT * slot);
+    PB_HIP(hipGetLastError());
+}
+
+// launches per iteration: p (or the copy), three stages, s, x/r and two operator applications
+inline int64_t iteration_launches(const Segments& sg, const ColorOrder& co) {
+    int nc = 0;
+    for (int c = 0; c < co.ncolor; ++c) nc += sg.sweep.nblk[c] > 0;
+    return 6 + 2 * (2 * nc - 1);
+}
+
 // Batched BiCGStab on the split operator Ahat from x = 0: each member runs until its own
 // ||r|| <= target[m] ||r0|| (target < 0: the member does not run).  iters[m] receives
 // the member's iterations; at most maxit lock-step iterations.
+// graphs: iterations 2, 3, ... run as replays of two captured graphs of two iterations each
+// (the same launches; at most maxit + 1 lock-step iterations then).
 template <typename T>
 void batch_bicg_run(hipStream_t s, Segments& sg, const ColorOrder& co, DiluFactors<T>& d, BatchWork<T>& w, const T* b,
-                    T* x, const std::vector<double>& target, int maxit, std::vector<int>& iters, RunCount& rc) {
+                    T* x, const std::vector<double>& target, int maxit, std::vector<int>& iters, RunCount& rc,
+                    bool graphs) {
     const int M = sg.M;
     const int64_t n = sg.N;
     const unsigned nch = (unsigned)sg.nchunk;
@@ -480,41 +537,63 @@ void batch_bicg_run(hipStream_t s, Segments& sg, const ColorOrder& co, DiluFacto
         return true;
     };
     if (all_stopped(w.hsc.p)) return;
-    const double* stop = w.sc.p + BS_STOP;
-    auto iterate = [&](int it) {
-        if (it == 1)
-            PB_HIP(hipMemcpyAsync(w.p.p, w.r.p, n * sizeof(T), hipMemcpyDeviceToDevice, s));
-        else
-            hipLaunchKernelGGL((k_seg_p<T>), dim3(nch), dim3(kBlock), 0, s, sg.chunks.p, w.r.p, w.v.p, w.sc.p, w.p.p);
-        split_apply_batch<T>(s, co, sg.F, sg.L, d, sg.sweep, w.p.p, w.v.p, w.rh.p, w.op_part.p, stop, BS_COUNT);
-        hipLaunchKernelGGL(k_seg_stage, dim3(M), dim3(kStage), 0, s, (int)ST_ALPHA, sg.chunk_begin.p, sg.op_rng.p,
-                           w.op_part.p, (const double*)nullptr, w.sc.p, (const double*)nullptr, (double*)nullptr);
-        hipLaunchKernelGGL((k_seg_s<T>), dim3(nch), dim3(kBlock), 0, s, sg.chunks.p, w.v.p, w.r.p, w.sc.p, w.sv.p,
-                           w.ss_part.p);
-        split_apply_batch<T>(s, co, sg.F, sg.L, d, sg.sweep, w.sv.p, w.t.p, w.sv.p, w.op_part.p, stop, BS_COUNT);
-        hipLaunchKernelGGL(k_seg_stage, dim3(M), dim3(kStage), 0, s, (int)ST_OMEGA, sg.chunk_begin.p, sg.op_rng.p,
-                           w.op_part.p, w.ss_part.p, w.sc.p, (const double*)nullptr, (double*)nullptr);
-        hipLaunchKernelGGL((k_seg_xr<T>), dim3(nch), dim3(kBlock), 0, s, sg.chunks.p, w.p.p, w.sv.p, w.sv.p, w.t.p,
-                           w.rh.p, w.sc.p, x, w.r.p, w.xr_part.p);
-        hipLaunchKernelGGL(k_seg_stage, dim3(M), dim3(kStage), 0, s, (int)ST_RHO, sg.chunk_begin.p, sg.op_rng.p,
-                           w.xr_part.p, (const double*)nullptr, w.sc.p, (const double*)nullptr,
-                           w.hsc_dev + (size_t)M * BS_COUNT * (it & 1));
-        PB_HIP(hipGetLastError());
-        PB_HIP(hipEventRecord(w.ev[it & 1], s));
-        int nc = 0;
-        for (int c = 0; c < co.ncolor; ++c) nc += sg.sweep.nblk[c] > 0;
-        rc.launches += 6 + 2 * (2 * nc - 1);  // p (or copy), 3 stages, s, x/r and two operator applications
+    const int64_t per_it = iteration_launches(sg, co);
+    const size_t slot_len = (size_t)M * BS_COUNT;
+    if (!graphs || maxit < 3) {
+        auto iterate = [&](int it) {
+            enqueue_iteration<T>(s, sg, co, d, w, x, it & 1, it == 1);
+            PB_HIP(hipEventRecord(w.ev[it & 1], s));
+            rc.launches += per_it;
+            rc.iterations += 1;
+        };
+        iterate(1);
+        for (int it = 1;; ++it) {
+            if (it < maxit) iterate(it + 1);
+            PB_HIP(hipEventSynchronize(w.ev[it & 1]));
+            if (all_stopped(w.hsc.p + slot_len * (it & 1)) || it >= maxit) break;
+        }
+    } else {
+        const void* key = static_cast<const void*>(x);
+        for (int g = 0; g < 2; ++g)
+            if (!w.pair[g] || w.pair_key[g] != key) {
+                if (w.pair[g]) (void)hipGraphExecDestroy(w.pair[g]);
+                w.pair[g] = nullptr;
+                hipGraph_t graph = nullptr;
+                PB_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                enqueue_iteration<T>(s, sg, co, d, w, x, 2 + 2 * g, false);
+                enqueue_iteration<T>(s, sg, co, d, w, x, 3 + 2 * g, false);
+                PB_HIP(hipStreamEndCapture(s, &graph));
+                const hipError_t ie = hipGraphInstantiate(&w.pair[g], graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+                PB_HIP(ie);
+                w.pair_key[g] = key;
+            }
+        enqueue_iteration<T>(s, sg, co, d, w, x, 1, true);  // iteration 1 (slot 1)
+        PB_HIP(hipEventRecord(w.ev[1], s));
+        rc.launches += per_it;
         rc.iterations += 1;
-    };
-    iterate(1);
-    for (int it = 1;; ++it) {
-        if (it < maxit) iterate(it + 1);
-        PB_HIP(hipEventSynchronize(w.ev[it & 1]));
-        if (all_stopped(w.hsc.p + (size_t)M * BS_COUNT * (it & 1)) || it >= maxit) break;
+        int launched = 0;  // pair k covers iterations 2k + 2, 2k + 3 and records ev[k & 1]
+        auto launch_pair = [&]() {
+            PB_HIP(hipGraphLaunch(w.pair[launched & 1], s));
+            PB_HIP(hipEventRecord(w.ev[launched & 1], s));
+            rc.launches += 2 * per_it;
+            rc.iterations += 2;
+            ++launched;
+        };
+        launch_pair();
+        PB_HIP(hipEventSynchronize(w.ev[1]));  // iteration 1 (recorded before pair 0 re-records ev[0])
+        if (!all_stopped(w.hsc.p + slot_len * 1))
+            for (int k = 0;; ++k) {
+                const int last = 2 * k + 3;  // the last iteration of pair k
+                if (last < maxit) launch_pair();
+                PB_HIP(hipEventSynchronize(w.ev[k & 1]));
+                if (all_stopped(w.hsc.p + slot_len * (3 + 2 * (k & 1))) || last >= maxit) break;
+            }
     }
     PB_HIP(hipStreamSynchronize(s));
     std::vector<double> h((size_t)M * BS_COUNT);
-    PB_HIP(hipMemcpy(h.data(), w.sc.p, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    PB_HIP(hipMemcpyAsync(h.data(), w.sc.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    PB_HIP(hipStreamSynchronize(s));
     for (int m = 0; m < M; ++m) iters[m] = (int)h[(size_t)m * BS_COUNT + BS_ITERS];
 }
 
@@ -535,7 +614,7 @@ void batch_split_solve(hipStream_t s, Segments& sg, const ColorOrder& co, DiluFa
                        DiluFactors<T>& dT, BatchWork<T>& w, const double* b, double* x, double* r, T* w1, T* w2,
                        T* w3, DevBuf<double>& spmv_part, const std::vector<int>& present, bool warm, int maxit,
                        double tol, std::vector<double> inner, const std::vector<double>& inner_cold,
-                       std::vector<MemberSolve>& out, RunCount& rc) {
+                       std::vector<MemberSolve>& out, RunCount& rc, bool graphs) {
     const int M = sg.M;
     const int64_t n = sg.N;
     const unsigned nch = (unsigned)sg.nchunk;
@@ -588,7 +667,7 @@ void batch_split_solve(hipStream_t s, Segments& sg, const ColorOrder& co, DiluFa
         hipLaunchKernelGGL((k_seg_scale<T>), dim3(nch), dim3(kBlock), 0, s, sg.chunks.p, r, w.fac.p, w1);
         split_op<T>(1, s, co, sg.F, sg.L, dT, w1, w2, 0);  // left transform of r / ||r||
         PB_HIP(hipMemsetAsync(w3, 0, n * sizeof(T), s));
-        batch_bicg_run<T>(s, sg, co, dT, w, w2, w3, target, left, it_run, rc);
+        batch_bicg_run<T>(s, sg, co, dT, w, w2, w3, target, left, it_run, rc, graphs);
         split_op<T>(2, s, co, sg.F, sg.L, dT, w3, w1, 0);  // correction back to the unknowns
         upload_factors(s, w.fac, a);
         hipLaunchKernelGGL((k_seg_axpy<T>), dim3(nch), dim3(kBlock), 0, s, sg.chunks.p, w.fac.p, w1, x);
@@ -645,6 +724,7 @@ struct pbsm3d_batch {
     std::vector<int> group_of, index_in_group;
     std::vector<int64_t> faces;
     pbsm3d_mlp* warm = nullptr;
+    bool graphs = true;  // iterations 2, 3, ... replayed from captured hipGraphs
     RunCount last;
     std::string err;
 };
@@ -780,7 +860,7 @@ void stage_group_forcing(BatchGroup& g, const pbsm3d_forcing* fo, const double* 
 
 template <typename T>
 void group_suspension_solve(pbsm3d_batch* B, BatchGroup& g, BatchWork<T>& w, DiluFactors<T>& dT, T* const* wk,
-                            const std::vector<int>& present, bool warm, std::vector<MemberSolve>& res) {
+                            const std::vector<int>& present, bool warm, std::vector<MemberSolve>& res, RunCount& rc) {
     pbsm3d_ctx* c = g.ctx;
     const pbsm3d_config& cfg = c->cfg;
     const int M = (int)g.members.size();
@@ -798,10 +878,11 @@ void group_suspension_solve(pbsm3d_batch* B, BatchGroup& g, BatchWork<T>& w, Dil
         }
     }
     batch_split_solve<T>(c->stream, g.susp, c->co, c->dl_s, dT, w, c->bp.p, c->xp.p, c->r_s.p, wk[0], wk[1], wk[2],
-                         g.spmv_part_s, present, warm, cfg.max_iters, cfg.tol, inner, cold, res, B->last);
+                         g.spmv_part_s, present, warm, cfg.max_iters, cfg.tol, inner, cold, res, rc, B->graphs);
 }
 
-void group_step(pbsm3d_batch* B, BatchGroup& g, const pbsm3d_forcing* fo, int32_t dt_s, pbsm3d_stats* st) {
+void group_step(pbsm3d_batch* B, BatchGroup& g, const pbsm3d_forcing* fo, int32_t dt_s, pbsm3d_stats* st,
+                RunCount& rc) {
     pbsm3d_ctx* c = g.ctx;
     hipStream_t s = c->stream;
     const pbsm3d_config& cfg = c->cfg;
@@ -849,10 +930,10 @@ void group_step(pbsm3d_batch* B, BatchGroup& g, const pbsm3d_forcing* fo, int32_
     if (any_present) {
         if (cfg.precision == PBSM3D_MIXED) {
             float* wk[3] = {c->split32[0].p, c->split32[1].p, c->split32[2].p};
-            group_suspension_solve<float>(B, g, g.w32, c->dl32, wk, present, warm, sres);
+            group_suspension_solve<float>(B, g, g.w32, c->dl32, wk, present, warm, sres, rc);
         } else {
             double* wk[3] = {c->split64[0].p, c->split64[1].p, c->split64[2].p};
-            group_suspension_solve<double>(B, g, g.w64, c->dl_s, wk, present, warm, sres);
+            group_suspension_solve<double>(B, g, g.w64, c->dl_s, wk, present, warm, sres, rc);
         }
     }
     for (int m = 0; m < M; ++m)
@@ -887,7 +968,7 @@ void group_step(pbsm3d_batch* B, BatchGroup& g, const pbsm3d_forcing* fo, int32_
         std::vector<double> zero(M, 0.0);
         batch_split_solve<double>(s, g.dep, c->co, c->dl_d, c->dl_d, g.wd, c->dsplit[0].p, c->dsplit[1].p, c->r_d.p,
                                   c->dsplit[2].p, c->dsplit[3].p, c->dsplit[4].p, g.spmv_part_d, dep_present, warm,
-                                  cfg.max_iters, cfg.tol, zero, zero, dres, B->last);
+                                  cfg.max_iters, cfg.tol, zero, zero, dres, rc, B->graphs);
         for (int m = 0; m < M; ++m)
             if (dep_present[m] && !dres[m].converged)
                 throw Error(PBSM3D_ENOCONV, "batch member " + std::to_string(g.members[m]) +
@@ -1043,20 +1124,36 @@ int pbsm3d_batch_step(pbsm3d_batch* B, const pbsm3d_forcing* forcings, int32_t d
     return batch_guard(B, [&] {
         PB_HIP(hipSetDevice(B->device));
         std::vector<pbsm3d_stats> st(B->M);
+        const size_t G = B->groups.size();
+        std::vector<RunCount> rc(G);
+        std::vector<std::exception_ptr> err(G);
+        const auto t0 = std::chrono::steady_clock::now();
+        // one host thread per union: each drives its own context and stream, so the unions'
+        // kernels overlap on the GPU and one union's host waits do not stall the others
+        auto work = [&](size_t k) {
+            try {
+                PB_HIP(hipSetDevice(B->device));
+                group_step(B, *B->groups[k], forcings, dt_s, st.data(), rc[k]);
+                if (outs) group_outputs(*B->groups[k], outs);
+            } catch (...) {
+                err[k] = std::current_exception();
+            }
+        };
+        if (G == 1) {
+            work(0);
+        } else {
+            std::vector<std::thread> th;
+            for (size_t k = 0; k < G; ++k) th.emplace_back(work, k);
+            for (auto& t : th) t.join();
+        }
+        for (auto& e : err)
+            if (e) std::rethrow_exception(e);
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         B->last = RunCount();
-        hipEvent_t e0 = nullptr, e1 = nullptr;
-        PB_HIP(hipEventCreate(&e0));
-        PB_HIP(hipEventCreate(&e1));
-        PB_HIP(hipEventRecord(e0, B->groups[0]->ctx->stream));
-        for (auto& g : B->groups) group_step(B, *g, forcings, dt_s, st.data());
-        PB_HIP(hipEventRecord(e1, B->groups.back()->ctx->stream));
-        if (outs)
-            for (auto& g : B->groups) group_outputs(*g, outs);
-        PB_HIP(hipEventSynchronize(e1));
-        float ms = 0;
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
+        for (const RunCount& r : rc) {
+            B->last.launches += r.launches;
+            B->last.iterations += r.iterations;
+        }
         if (stats)
             for (int m = 0; m < B->M; ++m) {
                 stats[m] = st[m];
@@ -1073,6 +1170,12 @@ int pbsm3d_batch_set_warm_start(pbsm3d_batch* B, pbsm3d_mlp* net) {
             const int rc = pbsm3d_set_warm_start(g->ctx, net);
             if (rc != PBSM3D_OK) return rc;
         }
+    return PBSM3D_OK;
+}
+
+int pbsm3d_batch_use_graphs(pbsm3d_batch* B, int enable) {
+    if (!B) return PBSM3D_EINVAL;
+    B->graphs = enable != 0;
     return PBSM3D_OK;
 }
 

@@ -44,6 +44,7 @@ EXPORTED = (
     "pbsm3d_batch_create",
     "pbsm3d_batch_step",
     "pbsm3d_batch_set_warm_start",
+    "pbsm3d_batch_use_graphs",
     "pbsm3d_batch_info",
     "pbsm3d_batch_get_state",
     "pbsm3d_batch_set_state",
@@ -143,6 +144,7 @@ def lib():
         L.pbsm3d_batch_step.argtypes = [vp, C.POINTER(abi.Forcing), C.c_int32, C.POINTER(abi.Out),
                                         C.POINTER(abi.Stats)]
         L.pbsm3d_batch_set_warm_start.argtypes = [vp, vp]
+        L.pbsm3d_batch_use_graphs.argtypes = [vp, C.c_int]
         L.pbsm3d_batch_info.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         L.pbsm3d_batch_get_state.argtypes = [vp, C.c_int32, dp, dp]
         L.pbsm3d_batch_set_state.argtypes = [vp, C.c_int32, dp, dp]
@@ -548,6 +550,10 @@ class Batch:
 
     def set_warm_start(self, net):
         self._check(lib().pbsm3d_batch_set_warm_start(self._h, net._h if net is not None else None))
+
+    def use_graphs(self, enable: bool = True):
+        """Replay the BiCGStab iterations from captured hipGraphs (default) or launch them."""
+        self._check(lib().pbsm3d_batch_use_graphs(self._h, 1 if enable else 0))
 
     def info(self) -> dict:
         u, n, it = C.c_int32(0), C.c_int64(0), C.c_int64(0)

@@ -44,6 +44,12 @@ int pbsm3d_batch_step(pbsm3d_batch* b, const pbsm3d_forcing* forcings, int32_t d
 /* attach (NULL: detach) the network of the learned warm start (nLayer-10 members) */
 int pbsm3d_batch_set_warm_start(pbsm3d_batch* b, pbsm3d_mlp* net);
 
+/* enable = 1 (default): the BiCGStab iterations after the first of every inner run are
+ * replayed from two captured hipGraphs of two iterations each (captured at the first run
+ * of each union and system); 0: every kernel launched on the stream.  The launches and
+ * the results are the same either way. */
+int pbsm3d_batch_use_graphs(pbsm3d_batch* b, int enable);
+
 /* number of unions (distinct nLayer values) and launches of the last step's
  * suspension solves: *launches counts every kernel launch of the batched BiCGStab
  * iterations (all unions), *iterations the lock-step iterations they ran */

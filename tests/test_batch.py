@@ -126,3 +126,28 @@ def test_batch_c5_shape():
     for f, v in worst.items():
         assert v < 5e-4, (f, v)
     torch.cuda.synchronize()
+
+
+def test_batch_graph_replays_match_stream_launches():
+    """The captured-graph replays of the BiCGStab iterations give the stream launches'
+    results bit for bit, step after step (each step replays the two graphs many times,
+    with the refinement loop's own launches and synchronisations between replays)."""
+    print("[case] batch graphs vs stream launches", flush=True)
+    members = _members(4, 11, 3000, 12000)
+    meshes = [meshgen.synthetic_mesh(m.n_points, seed=m.seed) for m in members]
+    runs = []
+    for graphs in (True, False):
+        b = ensemble.build_batch(members, meshes, precision=abi.MIXED)
+        b.use_graphs(graphs)
+        outs = []
+        for step in range(3):
+            fos = [ensemble.member_forcing(mesh, m, step) for m, mesh in zip(members, meshes)]
+            got, st = b.step(fos, want_fields=True)
+            outs.append((got, st))
+        runs.append(outs)
+        b.close()
+    for (ga, sa), (gb, sb) in zip(*runs):
+        for k in range(len(members)):
+            assert sa[k]["susp_iters"] == sb[k]["susp_iters"] and sa[k]["dep_iters"] == sb[k]["dep_iters"]
+            for f in ga[k]:
+                np.testing.assert_array_equal(ga[k][f], gb[k][f], err_msg=f"member {k} {f}")
