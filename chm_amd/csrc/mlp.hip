@@ -1,18 +1,19 @@
 // Per-face network of the learned warm start: 16 -> 128 -> 128 -> 128 -> 11,
 // ReLU, BF16 operands with FP32 accumulation on the matrix cores
-// (v_mfma_f32_16x16x16_bf16), trained on the GPU by its own forward and
-// backward kernels with Adam.
+// (gfx950's v_mfma_f32_16x16x32_bf16), trained on the GPU by its own forward
+// and backward kernels with Adam.
 //
 // Every layer is a GEMM with a fused epilogue.  Activations are stored
 // [samples x features] row-major; for a layer with weights W [out x in]
 //   forward   Z = A_in W^T + b,   A_out = relu(Z)        (M=samples, N=out, K=in)
 //   backward  dW = dZ^T A_in,     db = colsum(dZ)        (M=out, N=in, K=samples)
 //             dA_in = dZ W,       dZ_in = dA_in * [Z_in > 0]  (M=samples, N=in, K=out)
-// The GEMM stages 64x16 A and 16x64 B tiles of BF16 in LDS; each of the four
-// wavefronts owns a 32x32 quarter of the 64x64 output tile, i.e. 2x2 MFMA
-// 16x16 blocks, with the operand fragments read from LDS in the MFMA lane
-// layout (A: row = lane%16, k = 4*(lane/16)+j; B: col = lane%16, same k;
-// C: col = lane%16, row = 4*(lane/16)+j).
+// The GEMM stages 64x32 A and 32x64 B tiles of BF16 in LDS, both k-contiguous
+// (As[row][k], Bs[col][k]); each of the four wavefronts owns a 32x32 quarter of
+// the 64x64 output tile, i.e. 2x2 MFMA 16x16 blocks, with the operand fragments
+// read from LDS in the MFMA lane layout as one 16-byte load per lane
+// (A: row = lane%16, k = 8*(lane/16)+j, j < 8; B: col = lane%16, same k;
+// C: col = lane%16, row = 4*(lane/16)+j, j < 4).
 #include <cmath>
 #include <cstring>
 #include <random>
@@ -23,10 +24,15 @@
 namespace pb {
 namespace {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int TM = 64, TN = 64, TK = 16;
+constexpr int TM = 64, TN = 64, TK = 32;
+
+// 8 consecutive BF16 values (16 bytes, 16-byte aligned) as an MFMA operand fragment
+__device__ __forceinline__ bf16x8 frag8(const uint16_t* p) {
+    return *reinterpret_cast<const bf16x8*>(p);
+}
 
 __device__ __forceinline__ uint16_t to_bf16(float f) {
     uint32_t u = __float_as_uint(f);
@@ -53,8 +59,8 @@ struct GemmArgs {
 };
 
 __global__ void __launch_bounds__(256) k_gemm_bf16(GemmArgs g) {
-    __shared__ uint16_t As[TM][TK + 4];
-    __shared__ uint16_t Bs[TK][TN + 4];
+    __shared__ __align__(16) uint16_t As[TM][TK + 8];
+    __shared__ __align__(16) uint16_t Bs[TN][TK + 8];
     const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
     const int kb = blockIdx.z * g.ksplit;
     const int ke = min(g.K, kb + g.ksplit);
@@ -94,20 +100,16 @@ __global__ void __launch_bounds__(256) k_gemm_bf16(GemmArgs g) {
             uint16_t v = 0;
             if (gk < ke && gn < g.N)
                 v = g.transB ? g.B[(size_t)gn * g.ldb + gk] : g.B[(size_t)gk * g.ldb + gn];
-            Bs[r][c] = v;
+            Bs[c][r] = v;
         }
         __syncthreads();
 #pragma unroll
         for (int tm = 0; tm < 2; ++tm) {
-            bf16x4 a;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) a[j] = (short)As[wm + tm * 16 + (lane & 15)][4 * (lane >> 4) + j];
+            const bf16x8 a = frag8(&As[wm + tm * 16 + (lane & 15)][8 * (lane >> 4)]);
 #pragma unroll
             for (int tn = 0; tn < 2; ++tn) {
-                bf16x4 b;
-#pragma unroll
-                for (int j = 0; j < 4; ++j) b[j] = (short)Bs[4 * (lane >> 4) + j][wn + tn * 16 + (lane & 15)];
-                acc[tm][tn] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[tm][tn], 0, 0, 0);
+                const bf16x8 b = frag8(&Bs[wn + tn * 16 + (lane & 15)][8 * (lane >> 4)]);
+                acc[tm][tn] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[tm][tn], 0, 0, 0);
             }
         }
         __syncthreads();
@@ -228,6 +230,21 @@ __global__ void k_colsum(int64_t rows, int cols, const uint16_t* __restrict__ D,
     if (threadIdx.x == 0) out[c] = red[0];
 }
 
+// *acc += scale * (sum of the partials, in a fixed order): one workgroup
+__global__ void __launch_bounds__(256) k_loss_accum(int64_t nb, const float* __restrict__ partial, double scale,
+                                                    double* __restrict__ acc) {
+    __shared__ double red[256];
+    double s = 0;
+    for (int64_t k = threadIdx.x; k < nb; k += 256) s += partial[k];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *acc += red[0] * scale;
+}
+
 __global__ void k_adam(int64_t n, float* __restrict__ p, float* __restrict__ m, float* __restrict__ v,
                        const float* __restrict__ g, float lr, float b1, float b2, float eps, float c1, float c2,
                        uint16_t* __restrict__ pb, const uint8_t* __restrict__ keep) {
@@ -266,13 +283,13 @@ __global__ void k_denorm(int64_t n, const float* __restrict__ Yp, const float* _
 // faces).  The BF16 weights (73.7 KB) are read through the vector cache, where every
 // workgroup of a CU shares them.  Eight waves; in a 128-wide layer wave w computes rows
 // 16 (w % 4) .. +16 and columns 64 (w / 4) .. +64 as four 16 x 16 MFMA blocks, K in
-// steps of 16 in increasing order -- the operand layout, accumulation order and epilogue
+// steps of 32 in increasing order -- the operand layout, accumulation order and epilogue
 // (+ bias, ReLU, BF16 rounding) of k_gemm_bf16, so the output equals the layer-by-layer
-// forward bit for bit.  LDS: 40.4 KB per workgroup.
+// forward bit for bit.  LDS: 41.5 KB per workgroup.
 constexpr int kInferRows = 64;
 constexpr int kInferThreads = 512;
-constexpr int kAs = MlpDims::H + 4;    // LDS row stride (BF16) of the activation tiles
-constexpr int kAs0 = MlpDims::IN + 4;  // of the input tile
+constexpr int kAs = MlpDims::H + 8;    // LDS row stride (BF16) of the activation tiles (16-byte rows)
+constexpr int kAs0 = MlpDims::IN + 8;  // of the input tile
 
 struct InferArgs {
     int64_t n;
@@ -297,17 +314,16 @@ __device__ __forceinline__ void infer_layer(const uint16_t* __restrict__ in, int
     f32x4 acc[4];
 #pragma unroll
     for (int b = 0; b < 4; ++b) acc[b] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < K; k0 += 16) {
-        bf16x4 a;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) a[j] = (short)in[(r0 + (lane & 15)) * ins + k0 + 4 * (lane >> 4) + j];
+    for (int k0 = 0; k0 < K; k0 += 32) {
+        // K = 16 (the input layer): the upper half of the 32-wide step is zero padding
+        bf16x8 a = {};
+        if (k0 + 8 * (lane >> 4) < K) a = frag8(&in[(r0 + (lane & 15)) * ins + k0 + 8 * (lane >> 4)]);
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
             if (b < nblk) {
-                bf16x4 w;
-#pragma unroll
-                for (int j = 0; j < 4; ++j) w[j] = (short)W[(c0 + 16 * b + (lane & 15)) * K + k0 + 4 * (lane >> 4) + j];
-                acc[b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, w, acc[b], 0, 0, 0);
+                bf16x8 w = {};
+                if (k0 + 8 * (lane >> 4) < K) w = frag8(&W[(c0 + 16 * b + (lane & 15)) * K + k0 + 8 * (lane >> 4)]);
+                acc[b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, w, acc[b], 0, 0, 0);
             }
         }
     }
@@ -332,9 +348,9 @@ __device__ __forceinline__ void infer_layer(const uint16_t* __restrict__ in, int
 
 __global__ void __launch_bounds__(kInferThreads) k_mlp_infer(InferArgs p) {
     constexpr int H = MlpDims::H, IN = MlpDims::IN, OUT = MlpDims::OUT, OUTP = MlpDims::OUTP;
-    __shared__ uint16_t xin[kInferRows * kAs0];
-    __shared__ uint16_t act0[kInferRows * kAs];
-    __shared__ uint16_t act1[kInferRows * kAs];
+    __shared__ __align__(16) uint16_t xin[kInferRows * kAs0];
+    __shared__ __align__(16) uint16_t act0[kInferRows * kAs];
+    __shared__ __align__(16) uint16_t act1[kInferRows * kAs];
     __shared__ float yout[kInferRows * OUTP];
     const int tid = threadIdx.x;
     const float* xmu = p.norm;
@@ -449,6 +465,7 @@ void Mlp::ensure(int64_t n) {
     idx.alloc(n);
     partial.alloc(g1(n * MlpDims::OUTP) + 1);
     if (!splitws.p) splitws.alloc((size_t)kMaxSplits * MlpDims::H * MlpDims::H);
+    loss_dev.alloc(1);
 }
 
 const float* Mlp::xmu() const { return norm.p; }
@@ -485,7 +502,7 @@ void Mlp::forward_prepared(int64_t n, bool keep_z) {
     }
 }
 
-double Mlp::backward_prepared(int64_t n, float* dY_f32) {
+double Mlp::backward_prepared(int64_t n, float* dY_f32, double* loss_acc) {
     hipStream_t s = stream;
     const unsigned nb = g1(n * MlpDims::OUTP);
     hipLaunchKernelGGL(k_mse_grad, dim3(nb), dim3(256), 0, s, n, Yp.p, Tn.p, dYb.p, dY_f32, partial.p);
@@ -524,6 +541,11 @@ double Mlp::backward_prepared(int64_t n, float* dY_f32) {
             dcur = dZb[l - 1].p;
         }
     }
+    if (loss_acc) {  // training: the batch loss is added on the device, no host round trip per batch
+        hipLaunchKernelGGL(k_loss_accum, dim3(1), dim3(256), 0, s, (int64_t)nb, partial.p, 1.0 / (double)(n * MlpDims::OUT),
+                           loss_acc);
+        return 0.0;
+    }
     std::vector<float> hp(nb);
     PB_HIP(hipMemcpyAsync(hp.data(), partial.p, nb * sizeof(float), hipMemcpyDeviceToHost, s));
     PB_HIP(hipStreamSynchronize(s));
@@ -558,15 +580,18 @@ double Mlp::train(const float* X, const float* Y, int64_t n, int epochs, int bat
     const int64_t per_epoch = (n + batch - 1) / batch;
     double loss = 0;
     for (int e = 0; e < epochs; ++e) {
-        double acc = 0;
+        PB_HIP(hipMemsetAsync(loss_dev.p, 0, sizeof(double), stream));
         for (int64_t b = 0; b < per_epoch; ++b) {
             const uint64_t key = seed * 0x9E3779B97F4A7C15ull + (uint64_t)(e * per_epoch + b) * 0xD1B54A32D192ED03ull;
             hipLaunchKernelGGL(k_sample_idx, dim3(g1(batch)), dim3(256), 0, stream, (int64_t)batch, n, key, idx.p);
             prepare(X, Y, batch, idx.p);
             forward_prepared(batch);
-            acc += backward_prepared(batch, nullptr);
+            backward_prepared(batch, nullptr, loss_dev.p);
             adam_step(lr);
         }
+        double acc = 0;
+        PB_HIP(hipMemcpyAsync(&acc, loss_dev.p, sizeof(double), hipMemcpyDeviceToHost, stream));
+        PB_HIP(hipStreamSynchronize(stream));
         loss = acc / (double)per_epoch;
     }
     PB_HIP(hipStreamSynchronize(stream));

@@ -42,7 +42,9 @@ class Mlp {
     void prepare(const float* X, const float* Y, int64_t n, const int64_t* rows);
     // keep_z: also store the FP32 pre-activations of the hidden layers (for backward)
     void forward_prepared(int64_t n, bool keep_z = true);
-    double backward_prepared(int64_t n, float* dY_f32);
+    // loss_acc (device, may be null): the batch's mean squared error is added there on the
+    // device and 0 is returned; null: the loss is returned (a host round trip)
+    double backward_prepared(int64_t n, float* dY_f32, double* loss_acc = nullptr);
     void adam_step(float lr);
     const float* xmu() const;
     const float* xsd() const;
@@ -57,6 +59,7 @@ class Mlp {
     DevBuf<uint16_t> Xb, Ab[3], dZb[3], dYb;
     DevBuf<float> Z[3], Yp, Tn, partial;
     DevBuf<float> splitws;  // split-K partial products of the weight gradients
+    DevBuf<double> loss_dev;  // the epoch's summed batch losses (training)
     DevBuf<int64_t> idx;
     int64_t t_adam = 0;
 };
