@@ -548,16 +548,24 @@ int pbsm3d_create(pbsm3d_ctx** out, const pbsm3d_mesh* m, const pbsm3d_config* c
         c->r_s.alloc(N);
         c->r_d.alloc(F);
         const bool bicg = cfg->krylov == PBSM3D_KRYLOV_BICGSTAB || cfg->krylov == PBSM3D_KRYLOV_BICGSTAB_SPLIT;
+        // split work vectors: one zero pad slot past the end (the right transform's output is
+        // gathered by its own sweeps, see split_op)
         if (cfg->krylov == PBSM3D_KRYLOV_BICGSTAB_SPLIT && cfg->dep_precond == PBSM3D_PC_MCILU0) {
-            for (int k = 0; k < 5; ++k) c->dsplit[k].alloc(F);
+            for (int k = 0; k < 5; ++k) {
+                c->dsplit[k].alloc(F + 1);
+                zero_tail(c->dsplit[k], F);
+            }
             c->bwd64.ensure(F);
         }
         if (cfg->krylov == PBSM3D_KRYLOV_BICGSTAB_SPLIT) {
             for (int k = 0; k < 3; ++k) {
-                if (cfg->precision == PBSM3D_MIXED)
-                    c->split32[k].alloc(N);
-                else
-                    c->split64[k].alloc(N);
+                if (cfg->precision == PBSM3D_MIXED) {
+                    c->split32[k].alloc(N + 1);
+                    zero_tail(c->split32[k], N);
+                } else {
+                    c->split64[k].alloc(N + 1);
+                    zero_tail(c->split64[k], N);
+                }
             }
         }
         if (cfg->precision == PBSM3D_MIXED) {

@@ -172,24 +172,34 @@ struct ColorOrder {
     DevBuf<uint8_t> lowmask;   // [F] over positions: bit j = neighbour j has a lower colour
     std::vector<int32_t> h_color;
 };
-// D-ILU factors in position order; index q = z*F + k.
+// Zeroes elements [from, n) of a buffer (the pad slots below); at set-up only.
+template <typename T>
+inline void zero_tail(DevBuf<T>& b, size_t from) {
+    if (b.p && b.n > from) PB_HIP(hipMemset(b.p + from, 0, (b.n - from) * sizeof(T)));
+}
+// D-ILU factors in position order; index q = z*F + k.  lat, y and t carry one zero pad slot
+// past their last entry (lat[3N], y[N], t[N]): the split operator's sweeps read an absent
+// neighbour there instead of branching round the load.
 template <typename T>
 struct DiluFactors {
-    DevBuf<T> lat;   // [3N] coupling to neighbour j (row's own lateral entries)
+    DevBuf<T> lat;   // [3N + 1] coupling to neighbour j (row's own lateral entries), zero pad
     DevBuf<T> vert;  // [2N] coupling below (0) / above (1)
     DevBuf<T> dinv;  // [N]  1 / d~
     DevBuf<T> diag;  // [N]  a_rr (with lat and vert: the operator itself in position order)
     DevBuf<T> E;     // [N]  a_rr - d~ (Eisenstat split operator)
-    DevBuf<T> y;     // [N]  intermediate of the apply
-    DevBuf<T> t;     // [N]  intermediate of the split operator
+    DevBuf<T> y;     // [N + 1]  intermediate of the apply, zero pad
+    DevBuf<T> t;     // [N + 1]  intermediate of the split operator, zero pad
     void alloc(int64_t N) {
-        lat.alloc(3 * N);
+        lat.alloc(3 * N + 1);
         vert.alloc(2 * N);
         dinv.alloc(N);
         diag.alloc(N);
         E.alloc(N);
-        y.alloc(N);
-        t.alloc(N);
+        y.alloc(N + 1);
+        t.alloc(N + 1);
+        zero_tail(lat, 3 * N);
+        zero_tail(y, N);
+        zero_tail(t, N);
     }
 };
 void build_color_order(ColorOrder& co, int64_t F, const int32_t* neigh_fm, hipStream_t s);
@@ -210,7 +220,8 @@ bool tune_dilu_xcd(hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFa
 // Split (Eisenstat) form of the D-ILU preconditioned operator, position order (nLayer <= 16):
 // which = 0: out = Ahat in = D~ (D~ + L)^-1 A (D~ + U)^-1 in (one backward + one forward sweep);
 // 1: out = D~ (D~ + L)^-1 in (left transform of a right-hand side);
-// 2: out = (D~ + U)^-1 in (right transform back to the unknowns).  in and out must differ.
+// 2: out = (D~ + U)^-1 in (right transform back to the unknowns).  in and out must differ;
+// for which = 2, out must hold N + 1 entries with out[N] = 0 (the sweeps gather from it).
 // mode: bit 0 the XCD workgroup mapping, bit 1 the factor arrays loaded with the non-temporal hint,
 // bit 2 (which = 0) the layer-parallel sweeps (a workgroup per 64 face columns, a wave per layer).
 template <typename T>

@@ -282,8 +282,12 @@ __device__ __forceinline__ void eis_bwd_column(int64_t F, int Lrt, int64_t k, co
                                                const T* __restrict__ vert, const T* __restrict__ dinv,
                                                const T* __restrict__ w, const T* __restrict__ tnb,
                                                T* __restrict__ t) {
+    // 32-bit positions (the host checks 3 N + 1 < 2^32): every load is base + 32-bit offset
+    // from one uniform base.  An absent or masked neighbour reads the zero pad slots lat[3N]
+    // and tnb[N] instead of branching round the load.
     const int L = EX ? LM : Lrt;  // EX: the layer count is the unroll bound, no guards
-    const int64_t N = F * L;
+    const uint32_t Fu = (uint32_t)F, ku = (uint32_t)k;
+    const uint32_t N = Fu * (uint32_t)L;
     const uint32_t lm = lowmask[k];
     int32_t np_[3];
 #pragma unroll
@@ -295,14 +299,14 @@ __device__ __forceinline__ void eis_bwd_column(int64_t F, int Lrt, int64_t k, co
 #pragma unroll
     for (int z = 0; z < LM; ++z)
         if (z < L) {
-            const int64_t q = (int64_t)z * F + k;
+            const uint32_t q = (uint32_t)z * Fu + ku;
             T acc = 0;
 #pragma unroll
             for (int f = 0; f < 3; ++f)
             {
                 const bool on = np_[f] >= 0;
-                acc += ldc<NT>(on ? &lat[f * N + q] : zero_ptr<T>()) *
-                       *(on ? &tnb[(int64_t)z * F + np_[f]] : zero_ptr<T>());
+                acc += ldc<NT>(&lat[on ? (uint32_t)f * N + q : 3 * N]) *
+                       tnb[on ? (uint32_t)z * Fu + (uint32_t)np_[f] : N];
             }
             av[z] = acc;
             va[z] = z < L - 1 ? ldc<NT>(&vert[N + q]) : T(0);
@@ -314,7 +318,7 @@ __device__ __forceinline__ void eis_bwd_column(int64_t F, int Lrt, int64_t k, co
     for (int z = LM - 1; z >= 0; --z)
         if (z < L) {
             next = dv[z] * (wv[z] - (av[z] + va[z] * next));
-            t[(int64_t)z * F + k] = next;
+            t[(uint32_t)z * Fu + ku] = next;
         }
 }
 
@@ -344,8 +348,10 @@ __device__ __forceinline__ void eis_fwd_column(int64_t F, int Lrt, int64_t k, co
                                                const T* __restrict__ t, const T* __restrict__ tnb,
                                                const T* __restrict__ vnb, T* __restrict__ v, T* __restrict__ o,
                                                const T* __restrict__ dw, double& d1, double& d2) {
+    // 32-bit positions and the zero pad slots lat[3N], vnb[N], tnb[N] (see eis_bwd_column)
     const int L = EX ? LM : Lrt;  // EX: the layer count is the unroll bound, no guards
-    const int64_t N = F * L;
+    const uint32_t Fu = (uint32_t)F, ku = (uint32_t)k;
+    const uint32_t N = Fu * (uint32_t)L;
     const uint32_t lm = lowmask[k];
     int32_t lo[3], up[3];
 #pragma unroll
@@ -359,7 +365,7 @@ __device__ __forceinline__ void eis_fwd_column(int64_t F, int Lrt, int64_t k, co
 #pragma unroll
     for (int z = 0; z < LM; ++z)
         if (z < L) {
-            const int64_t q = (int64_t)z * F + k;
+            const uint32_t q = (uint32_t)z * Fu + ku;
             wv[z] = w[q];
             if (DOTS) dwv[z] = dw[q];  // loaded with the rest of the column, not in the final loop
             dv[z] = ldc<NT>(&dinv[q]);
@@ -370,8 +376,8 @@ __device__ __forceinline__ void eis_fwd_column(int64_t F, int Lrt, int64_t k, co
                 for (int f = 0; f < 3; ++f)
                 {
                     const bool on = lo[f] >= 0;
-                    acc += ldc<NT>(on ? &lat[f * N + q] : zero_ptr<T>()) *
-                           *(on ? &vnb[(int64_t)z * F + lo[f]] : zero_ptr<T>());
+                    acc += ldc<NT>(&lat[on ? (uint32_t)f * N + q : 3 * N]) *
+                           vnb[on ? (uint32_t)z * Fu + (uint32_t)lo[f] : N];
                 }
             }
             lv[z] = acc;
@@ -386,14 +392,14 @@ __device__ __forceinline__ void eis_fwd_column(int64_t F, int Lrt, int64_t k, co
 #pragma unroll
         for (int z = 0; z < LM; ++z)
             if (z < L) {
-                const int64_t q = (int64_t)z * F + k;
+                const uint32_t q = (uint32_t)z * Fu + ku;
                 T acc = 0;
 #pragma unroll
                 for (int f = 0; f < 3; ++f)
                 {
                     const bool on = up[f] >= 0;
-                    acc += ldc<NT>(on ? &lat[f * N + q] : zero_ptr<T>()) *
-                           *(on ? &tnb[(int64_t)z * F + up[f]] : zero_ptr<T>());
+                    acc += ldc<NT>(&lat[on ? (uint32_t)f * N + q : 3 * N]) *
+                           tnb[on ? (uint32_t)z * Fu + (uint32_t)up[f] : N];
                 }
                 av[z] = acc;
                 va[z] = z < L - 1 ? ldc<NT>(&vert[N + q]) : T(0);
@@ -410,7 +416,7 @@ __device__ __forceinline__ void eis_fwd_column(int64_t F, int Lrt, int64_t k, co
 #pragma unroll
     for (int z = 0; z < LM; ++z)
         if (z < L) {
-            const int64_t q = (int64_t)z * F + k;
+            const uint32_t q = (uint32_t)z * Fu + ku;
             const T acc = lv[z] + vb[z] * prev;
             T oz, vz;
             if (WITH_T) {
@@ -1206,15 +1212,23 @@ int64_t split_mode_dispatch(int which, hipStream_t s, const ColorOrder& co, int6
 }
 }  // namespace
 
+// the column sweeps index positions in 32 bits, pad slots included
+inline void check_split_size(int64_t F, int L) {
+    if ((uint64_t)3 * (uint64_t)F * (uint64_t)L + 1 >= (1ull << 32))
+        throw Error(PBSM3D_EINVAL, "the split D-ILU operator needs 3 * faces * nLayer < 2^32");
+}
+
 template <typename T>
 void split_op(int which, hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFactors<T>& d, const T* in, T* out,
               int mode) {
+    check_split_size(F, L);
     split_mode_dispatch<T>(which, s, co, F, L, d, in, out, nullptr, nullptr, mode, nullptr);
 }
 
 template <typename T>
 int64_t split_apply_dots(hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFactors<T>& d, const T* in, T* out,
                          const T* dw, double* partial, int mode, const double* stop) {
+    check_split_size(F, L);
     return split_mode_dispatch<T>(0, s, co, F, L, d, in, out, dw, partial, mode, stop);
 }
 
@@ -1252,6 +1266,7 @@ template <typename T>
 void split_apply_batch(hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFactors<T>& d, const SweepTables& st,
                        const T* in, T* out, const T* dw, double* partial, const double* stop, int stop_stride) {
     if (co.ncolor > kMaxColors) throw Error(PBSM3D_EINVAL, "too many colours for the batch tables");
+    check_split_size(F, L);
 #define PB_BATCH(LMV)                                                                                   \
     {                                                                                                   \
         if (L == LMV)                                                                                   \

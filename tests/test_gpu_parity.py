@@ -122,8 +122,17 @@ def test_fields_c1(pb, oracle, mesh_c1, precision, precond, krylov):
     ctx8 = pb.Context(g, pb.default_config(nLayer=5, precision=precision, precond=precond, tol=1e-8, krylov=krylov))
     g8, st8 = ctx8.step(fo, want_fields=True)
     assert st8["susp_resid"] <= 1e-8 and st8["dep_resid"] <= 1e-8
-    for k, bound in (("conc", 1e-4), ("Qsusp", 1e-4), ("qdep", 1e-3), ("drift_mass", 1e-3)):
-        assert rel_err(g8[k], og[k]) < bound, (k, rel_err(g8[k], og[k]))
+    errs = {k: rel_err(g8[k], og[k]) for k in ("conc", "Qsusp", "Qsubl", "qdep", "drift_mass")}
+    print(f"[c1] precision={precision} precond={precond} krylov={krylov} tol 1e-8 vs oracle 1e-13: "
+          + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()), flush=True)
+    # bounds: 2.5x the largest error measured over the krylov / precision variants of each
+    # preconditioner family (the solves are deterministic, so a drift past them is a change)
+    if precond == abi.PC_MCILU0:
+        bounds = {"conc": 4e-7, "Qsusp": 1e-6, "Qsubl": 1.2e-6, "qdep": 2.6e-5, "drift_mass": 2.6e-5}
+    else:
+        bounds = {"conc": 1e-6, "Qsusp": 2.5e-6, "Qsubl": 3e-6, "qdep": 1.6e-4, "drift_mass": 1.6e-4}
+    for k, bound in bounds.items():
+        assert errs[k] < bound, (k, errs[k])
     # deposition mass balance sum_i area_i qdep_i = sum_i rhs_i holds to the solve tolerance
     _, _, _, rhs = ctx8.system(1)
     # (|1^T r| <= sqrt(F) ||r|| <= sqrt(F) tol ||b||)
@@ -398,6 +407,12 @@ def test_c3_mixed_precision_96_steps(pb, oracle, mesh_c1):
     gpu = {p: pb.Context(g, pb.default_config(nLayer=5, precision=p, krylov=abi.KRYLOV_BICGSTAB_SPLIT
                                               if p == abi.MIXED else abi.KRYLOV_GMRES, tol=1e-8))
            for p in (abi.FP64, abi.MIXED)}
+    # launch shapes pinned: the iterates (hence the forward errors bounded below) do not then
+    # depend on which shapes a timing tuner picks on the box
+    for p, ctx in gpu.items():
+        which = 0 if p == abi.FP64 else 1
+        ctx.spmv_select(which, 1)
+        ctx.precond_select(which, 1)
     fields = ("Qsalt", "Qsusp", "Qsubl", "qdep", "drift_mass", "sum_drift", "sum_subl")
     worst = {p: {k: 0.0 for k in fields} for p in gpu}
     for step in range(96):
@@ -412,8 +427,14 @@ def test_c3_mixed_precision_96_steps(pb, oracle, mesh_c1):
     for p, name in ((abi.FP64, "fp64"), (abi.MIXED, "mixed")):
         print(f"[c3] {name} max over 96 steps of max|x - x_oracle| / max|x_oracle|: "
               + ", ".join(f"{k} {v:.2e}" for k, v in worst[p].items()), flush=True)
-        for k, bound in (("Qsalt", 1e-12), ("Qsusp", 1e-5), ("Qsubl", 1e-5), ("qdep", 1e-3), ("drift_mass", 1e-3),
-                         ("sum_drift", 1e-3), ("sum_subl", 1e-5)):
+        # FP64: within 1e-6 of the oracle; mixed: within twice the errors measured with these
+        # launch shapes (Qsusp 2.2e-7, Qsubl 3.2e-7, qdep / drift_mass 5.4e-7, sum_drift 1.2e-7,
+        # sum_subl 9.8e-8)
+        bounds = ({k: 1e-6 for k in fields} if p == abi.FP64 else
+                  {"Qsusp": 4.4e-7, "Qsubl": 6.4e-7, "qdep": 1.1e-6, "drift_mass": 1.1e-6, "sum_drift": 2.4e-7,
+                   "sum_subl": 2e-7})
+        bounds["Qsalt"] = 1e-12
+        for k, bound in bounds.items():
             assert worst[p][k] < bound, (name, k, worst[p][k])
 
 
