@@ -43,16 +43,17 @@ def spmv_bytes(nnz: int, n: int, f: int, s: int) -> int:
     return nnz * s + 2 * n * s + 12 * f
 
 
-def split_apply_bytes(nnz: int, n: int, f: int, f0: int, L: int, s: int) -> int:
+def split_apply_bytes(nnz: int, n: int, f: int, f0: int, L: int, s: int, f_top: int = 0) -> int:
     """Algorithmic bytes of one application of the split (Eisenstat) D-ILU operator with
     the BiCGStab dots fused in: every off-diagonal coupling once (upper ones in the
     backward sweep, lower ones in the forward sweep); per row of colours >= 1 the
     backward sweep reads w, 1/d~ and writes t, the forward sweep reads w, 1/d~, D - d~,
     t and the dot vector and writes v and the result (10 vectors); colour 0 (f0 faces)
-    does both sweeps in one pass with t in registers (6 vectors); the neighbour
-    positions and colour mask once per sweep (13 bytes per face and sweep)."""
+    does both sweeps in one pass with t in registers (6 vectors); the top colour (f_top
+    faces) does not write v, which no higher colour reads (one vector less); the
+    neighbour positions and colour mask once per sweep (13 bytes per face and sweep)."""
     n0 = f0 * L
-    return (nnz - n) * s + (10 * (n - n0) + 6 * n0) * s + 26 * f - 13 * f0
+    return (nnz - n) * s + (10 * (n - n0) + 6 * n0 - f_top * L) * s + 26 * f - 13 * f0
 
 
 def spmv_dots_bytes(nnz: int, n: int, f: int, s: int) -> int:
@@ -245,35 +246,18 @@ def train_network(pb, abi, meshgen, warmstart, g, F, draws, samples, epochs, tor
     return net, info
 
 
-def warm_start_gain(pb, abi, meshgen, g, F, L, krylov_mixed, net, seeds, torch):
+def warm_start_gain(pb, abi, meshgen, warmstart, g, F, L, krylov_mixed, net, seeds):
     """One step per held-out forcing draw (seeds disjoint from the training draws and the
     timed forcing) with the headline solver, cold start vs learned start: suspension
-    iterations, initial residual ||b - A x0|| / ||b|| and step time.  Each context first
-    steps once on the timed forcing so its launch-shape tuning is not counted."""
-    warm0 = meshgen.forcing_c2(F, seed=0)
-    out = {"seeds": seeds}
-    for label, learned in (("cold", False), ("learned", True)):
-        cfg = pb.default_config(nLayer=L, precision=abi.MIXED, krylov=krylov_mixed,
-                                warm_start=abi.WS_LEARNED if learned else abi.WS_ZERO)
-        ctx = pb.Context(g, cfg)
-        if learned:
-            ctx.set_warm_start(net)
-        ctx.step(warm0, want_fields=False)
-        its, r0, ms = [], [], []
-        for sd in seeds:
-            fo = meshgen.forcing_c2(F, seed=sd)
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            _, st = ctx.step(fo, want_fields=False)
-            torch.cuda.synchronize()
-            ms.append(round(1000.0 * (time.perf_counter() - t0), 2))
-            its.append(st["susp_iters"])
-            r0.append(round(st["susp_resid0"], 4))
-        ctx.close()
-        out[label] = {"susp_iters": its, "resid0": r0, "ms_per_step": ms}
-    out["iters_ratio_learned_to_cold"] = round(sum(out["learned"]["susp_iters"]) / max(1, sum(out["cold"]["susp_iters"])), 4)
-    out["time_ratio_learned_to_cold"] = round(sum(out["learned"]["ms_per_step"]) / max(1e-9, sum(out["cold"]["ms_per_step"])), 4)
-    out["note"] = "step times include host-side forcing upload (host arrays); iterations are the comparison"
+    iterations, initial residual ||b - A x0|| / ||b|| and step time with the forcing
+    resident in HBM, and the learned start with the learned coarse correction on 256
+    tiles.  Each context first steps once on the timed forcing so its launch-shape
+    tuning is not counted."""
+    cfg = pb.default_config(nLayer=L, precision=abi.MIXED, krylov=krylov_mixed)
+    out = warmstart.compare_starts(g, net, [meshgen.forcing_c2(F, seed=sd) for sd in seeds], cfg,
+                                   warmup_draw=meshgen.forcing_c2(F, seed=0),
+                                   coarse_tiles=256 if krylov_mixed == abi.KRYLOV_BICGSTAB_SPLIT else 0)
+    out["seeds"] = seeds
     return out
 
 
@@ -290,7 +274,7 @@ def main():
     ap.add_argument("--train-draws", type=int, default=8)
     ap.add_argument("--train-samples", type=int, default=4000000)
     ap.add_argument("--train-epochs", type=int, default=8)
-    ap.add_argument("--gain-seeds", default="100,101,102,103",
+    ap.add_argument("--gain-seeds", default="100,101,102,103,104,105,106,107",
                     help="held-out forcing draws on which the learned start is compared with the cold start")
     ap.add_argument("--cpu-faces", type=int, default=0,
                     help="CPU baseline on a patch of this many faces, scaled (default 0: the whole mesh)")
@@ -307,6 +291,9 @@ def main():
     ap.add_argument("--dilu-mode", type=int, default=-1, choices=list(range(-1, 8)),
                     help="pin the D-ILU sweep mode: bit 0 XCD workgroup mapping; split operator only: bit 1 "
                          "non-temporal factor loads, bit 2 layer-parallel sweeps (default: tuned)")
+    ap.add_argument("--study-draws", type=int, default=20000,
+                    help="learned-start study: FP64 solves of forcing draws on the 2,500-point mesh at 10 layers "
+                         "that train its network, before cold vs learned on 16 held-out draws (0: skip)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="run one streaming kernel (y = -x, float32) reading and writing this many MB first: "
                          "the calibration dispatch for rocprofv3 FETCH_SIZE / WRITE_SIZE")
@@ -474,9 +461,16 @@ def main():
     gain = None
     if net is not None and "mixed_learned" in names and args.gain_seeds:
         log("[bench] learned vs cold start on held-out forcing draws")
-        gain = warm_start_gain(pb, abi, meshgen, g, F, L, krylov_mixed, net,
-                               [int(x) for x in args.gain_seeds.split(",")], torch)
+        gain = warm_start_gain(pb, abi, meshgen, warmstart, g, F, L, krylov_mixed, net,
+                               [int(x) for x in args.gain_seeds.split(",")])
         log(f"[bench] gain: {gain}")
+
+    study = None
+    if args.study_draws > 0:
+        log(f"[bench] learned-start study: {args.study_draws} FP64 solves on the 2,500-point mesh at 10 layers")
+        snet, study = warmstart.learned_start_study(n_train=args.study_draws)
+        snet.close()
+        log(f"[bench] study: {json.dumps(study)}")
 
     N = F * L
     head_name = "mixed_learned" if "mixed_learned" in variants else names[-1]
@@ -491,7 +485,8 @@ def main():
     if head["krylov"] == "bicgstab_split":
         colors, _ = pb.face_colors(mesh.neigh)
         f0 = int((colors == 0).sum())
-        algos = {"precond": (split_apply_bytes(nnz, N, F, f0, L, s_bytes),
+        f_top = int((colors == colors.max()).sum())
+        algos = {"precond": (split_apply_bytes(nnz, N, F, f0, L, s_bytes, f_top),
                              "split D-ILU operator (Eisenstat): backward + forward sweep, fused BiCGStab dots",
                              "split_apply")}
     else:
@@ -564,6 +559,7 @@ def main():
                              if "plain" in variants else None),
         "warm_start_training": train_info,
         "warm_start_gain": gain,
+        "learned_start_study": study,
         "fetch": fetch_info,
         "ensemble": ens,
         "pcie_inclusive": pcie,

@@ -79,7 +79,7 @@ class Config(C.Structure):
         ("use_PomLi_probability", C.c_int32),
         ("z0_ustar_coupling", C.c_int32),
         ("rouault_diffusion_coef", C.c_int32),
-        ("reserved_cfg", C.c_int32),
+        ("coarse_tiles", C.c_int32),
     ]
 
 
@@ -173,6 +173,7 @@ class Stats(C.Structure):
         ("susp_resid0", C.c_double),
         ("susp_cold_fallback", C.c_int32),
         ("reserved2", C.c_int32),
+        ("susp_resid_coarse", C.c_double),
     ]
 
     def as_dict(self) -> dict:

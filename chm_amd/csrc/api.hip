@@ -240,6 +240,11 @@ void validate(const pbsm3d_mesh* m, const pbsm3d_config* c) {
         throw Error(PBSM3D_EINVAL, "preconditioner not available on the GPU path");
     if (c->dep_precond != PBSM3D_PC_NONE && c->dep_precond != PBSM3D_PC_JACOBI && c->dep_precond != PBSM3D_PC_MCILU0)
         throw Error(PBSM3D_EINVAL, "deposition preconditioner not available on the GPU path");
+    if (c->coarse_tiles < 0 || c->coarse_tiles > kCoarseMaxTiles)
+        throw Error(PBSM3D_EINVAL, "coarse_tiles must be in [0, 256]");
+    if (c->coarse_tiles > 0 && (c->warm_start != PBSM3D_WS_LEARNED || c->krylov != PBSM3D_KRYLOV_BICGSTAB_SPLIT))
+        throw Error(PBSM3D_EINVAL, "the learned coarse correction needs the learned warm start and the split BiCGStab");
+    if (c->coarse_tiles > m->n_faces) throw Error(PBSM3D_EINVAL, "coarse_tiles exceeds the face count");
     check_neighbours(m->n_faces, m->neigh);
 }
 
@@ -315,6 +320,14 @@ SolveInfo solve_in_position_order(pbsm3d_ctx* c, const pbsm3d_config& cfg, int64
     A.pos = &c->dl_s;
     A.variant = c->spmv_variant[0];
     SolveInfo si;
+    // the learned coarse correction: its basis is the network's prediction, so only when
+    // the learned start was applied (c->susp.x still holds x0 in reference order here)
+    CoarseFn coarse;
+    if (cfg.coarse_tiles > 0 && cfg.warm_start == PBSM3D_WS_LEARNED && c->warm != nullptr) {
+        coarse_setup(c, cfg.coarse_tiles);
+        coarse_prepare(c);
+        coarse = [c](hipStream_t st, const double* r, double* x) { coarse_correct(c, st, r, x); };
+    }
     if (cfg.krylov == PBSM3D_KRYLOV_BICGSTAB_SPLIT) {
         if (cfg.precision == PBSM3D_FP64) {
             CsrOp<double> Ah{N, nullptr, nullptr, nullptr, &c->prof, F, L};
@@ -323,7 +336,8 @@ SolveInfo solve_in_position_order(pbsm3d_ctx* c, const pbsm3d_config& cfg, int64
             Ah.mode = c->dilu_mode[0];
             Ah.prof_class = PBSM3D_K_PRECOND;
             si = bicgstab_split<double>(s, c->bw64, c->ws64, A, Ah, c->bp.p, c->xp.p, c->r_s.p, c->split64[0].p,
-                                        c->split64[1].p, c->split64[2].p, cfg.max_iters, cfg.tol, cfg.inner_rtol);
+                                        c->split64[1].p, c->split64[2].p, cfg.max_iters, cfg.tol, cfg.inner_rtol, 0.0,
+                                        coarse);
         } else {
             CsrOp<float> Ah{N, nullptr, nullptr, nullptr, &c->prof, F, L};
             Ah.co = &c->co;
@@ -333,7 +347,7 @@ SolveInfo solve_in_position_order(pbsm3d_ctx* c, const pbsm3d_config& cfg, int64
             si = bicgstab_split<float>(s, c->bw32, c->ws32, A, Ah, c->bp.p, c->xp.p, c->r_s.p, c->split32[0].p,
                                        c->split32[1].p, c->split32[2].p, cfg.max_iters, cfg.tol,
                                        cfg.inner_rtol > 0 ? cfg.inner_rtol : split_inner_rtol_default(c, cfg),
-                                       cfg.inner_rtol > 0 ? cfg.inner_rtol : kSplitInnerCold);
+                                       cfg.inner_rtol > 0 ? cfg.inner_rtol : kSplitInnerCold, coarse);
         }
     } else if (cfg.precision == PBSM3D_FP64) {
         PcFn<double> M = [c, F, L](hipStream_t st, const double* r, double* z) {
@@ -703,6 +717,7 @@ int pbsm3d_step(pbsm3d_ctx* c, const pbsm3d_forcing* fo, int32_t dt_s, pbsm3d_ou
             stats.susp_resid0 = si.resid0;
             stats.susp_cold_fallback = si.cold_fallback ? 1 : 0;
             stats.susp_refinements = si.refinements;
+            stats.susp_resid_coarse = si.resid_coarse;
             if (!si.converged) throw Error(PBSM3D_ENOCONV, "suspension solver failed to converge");
         }
         PB_HIP(hipEventRecord(c->ev[2], s));

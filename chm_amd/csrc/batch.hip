@@ -711,6 +711,7 @@ struct BatchGroup {
     BatchWork<double> w64, wd;
     DevBuf<double> spmv_part_s, spmv_part_d, fac;
     DevBuf<int32_t> face_member, member_on;
+    bool stepped = false;  // a step has run: the staged forcing and solutions are this step's
     ~BatchGroup() {
         if (ctx) pbsm3d_destroy(ctx);
     }
@@ -985,6 +986,7 @@ void group_step(pbsm3d_batch* B, BatchGroup& g, const pbsm3d_forcing* fo, int32_
     }
     PB_HIP(hipEventRecord(c->ev[4], s));
     PB_HIP(hipStreamSynchronize(s));
+    g.stepped = true;
     for (int m = 0; m < M; ++m) {
         pbsm3d_stats& o = st[g.members[m]];
         o = pbsm3d_stats{};
@@ -1057,6 +1059,7 @@ int pbsm3d_batch_create(pbsm3d_batch** out, int32_t n_members, const pbsm3d_mesh
                                            "PBSM3D_KRYLOV_BICGSTAB_SPLIT, precond / dep_precond PBSM3D_PC_MCILU0)");
             if (c.nLayer < 2 || c.nLayer > 16) throw Error(PBSM3D_EINVAL, "batch members need nLayer in [2, 16]");
             if (!(c.z0_snow > 0) || !(c.settling_scale >= 0)) throw Error(PBSM3D_EINVAL, "bad z0_snow / settling_scale");
+            if (c.coarse_tiles != 0) throw Error(PBSM3D_EINVAL, "the batch has no learned coarse correction");
         }
         std::vector<int> Ls;
         for (int m = 0; m < n_members; ++m)
@@ -1199,6 +1202,30 @@ int pbsm3d_batch_get_state(pbsm3d_batch* B, int32_t m, double* sum_drift, double
             PB_HIP(hipMemcpyAsync(sum_drift, c->sum_drift.p + off, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         if (sum_subl)
             PB_HIP(hipMemcpyAsync(sum_subl, c->sum_subl.p + off, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        PB_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int pbsm3d_batch_training_pairs(pbsm3d_batch* B, int32_t m, float* X, float* Y) {
+    if (!B || m < 0 || m >= B->M || !X || !Y) return PBSM3D_EINVAL;
+    return batch_guard(B, [&] {
+        BatchGroup& g = *B->groups[B->group_of[m]];
+        const int j = B->index_in_group[m];
+        const int64_t off = g.face_off[j], n = g.face_off[j + 1] - off;
+        pbsm3d_ctx* c = g.ctx;
+        if (c->L != 10) throw Error(PBSM3D_EINVAL, "training pairs need nLayer 10");
+        if (!g.stepped) throw Error(PBSM3D_EINVAL, "no batch step has run yet");
+        PB_HIP(hipSetDevice(c->device));
+        const int64_t F = c->F;
+        c->feat.alloc(F * MlpDims::IN);
+        c->pred.alloc(F * MlpDims::OUT);
+        // the union's features (neighbour indices stay inside each member) and targets,
+        // then the member's rows; vw_dir and t are the step's staged forcing
+        training_pairs_into(c, c->forcing_stage.p + 2 * F, c->forcing_stage.p + 4 * F, c->feat.p, c->pred.p);
+        PB_HIP(hipMemcpyAsync(X, c->feat.p + off * MlpDims::IN, n * MlpDims::IN * sizeof(float),
+                              hipMemcpyDeviceToDevice, c->stream));
+        PB_HIP(hipMemcpyAsync(Y, c->pred.p + off * MlpDims::OUT, n * MlpDims::OUT * sizeof(float),
+                              hipMemcpyDeviceToDevice, c->stream));
         PB_HIP(hipStreamSynchronize(c->stream));
     });
 }

@@ -7,6 +7,22 @@
 #include "krylov.hpp"
 #include "mlp.hpp"
 
+namespace pb {
+constexpr int kCoarseAdj = 16;        // tiles one coarse tile may touch, itself included
+constexpr int kCoarseMaxTiles = 256;  // coarse unknowns (Gauss-Jordan inverse in one workgroup)
+// The learned coarse space of the suspension solve: T tiles of faces (index ranges),
+// the basis w (the network's predicted profile), the Galerkin matrix W^T A W and its
+// inverse.
+struct CoarseSpace {
+    int T = 0;
+    bool ready = false;  // G / Ginv hold the current step's coarse system
+    DevBuf<int64_t> tile_off;  // [T + 1] face ranges
+    DevBuf<int32_t> tile_of_face, tile_of_pos, adj;  // [F], [F], [T x kCoarseAdj]
+    DevBuf<double> w, wpos, rref;  // [N] basis (reference / position order), residual in reference order
+    DevBuf<double> G, M, Ginv, c, y;
+};
+}  // namespace pb
+
 struct pbsm3d_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -58,6 +74,7 @@ struct pbsm3d_ctx {
     pb::DevBuf<float> feat, pred;     // [F x 16], [F x 11]
     const double* last_vw_dir = nullptr;  // this step's vw_dir and t, context-owned copies (device)
     const double* last_t = nullptr;
+    pb::CoarseSpace coarse;  // learned coarse correction (cfg.coarse_tiles > 0)
     pb::Profiler prof;
     std::string err;
 
@@ -74,6 +91,15 @@ namespace pb {
 AssembleArgs assemble_args(pbsm3d_ctx* c, const double* const fdev[9], int32_t dt_s);
 // features of the current step into c->feat (needs the assembled step)
 void warm_features(pbsm3d_ctx* c, const double* vw_dir, const double* t);
+// training pairs of the last step over all c->F faces: X [F x 16] features (vw_dir, t the
+// step's forcing on the device), Y [F x 11] targets from the solved x of both systems
+void training_pairs_into(pbsm3d_ctx* c, const double* vw_dir, const double* t, float* X, float* Y);
 // x0 of the suspension (which = 0) or deposition (which = 1) solve from c->pred
 void warm_apply(pbsm3d_ctx* c, int which);
+// learned coarse correction: tiles and adjacency (once per tile count), the step's
+// basis and Galerkin inverse (after the learned start wrote x0), and one correction
+// x_pos += W (W^T A W)^-1 W^T r (r_pos the FP64 residual, both in position order)
+void coarse_setup(pbsm3d_ctx* c, int T);
+void coarse_prepare(pbsm3d_ctx* c);
+void coarse_correct(pbsm3d_ctx* c, hipStream_t s, const double* r_pos, double* x_pos);
 }  // namespace pb

@@ -141,10 +141,12 @@ double dev_norm2(hipStream_t s, GmresWork<double>& w, int64_t n, const double* v
 
 namespace {
 // A start whose residual exceeds ||b|| (that of x0 = 0) is dropped: x is reset
-// to zero and the caller recomputes the residual.  Records ||r0|| / ||b||.
+// to zero and the caller recomputes the residual.  Records ||r0|| / ||b||.  The
+// margin keeps x0 = 0 itself, whose ||r0|| and ||b|| differ only by the order of
+// their two reductions, from counting as a fallback.
 bool drop_worse_start(hipStream_t s, SolveInfo& info, double beta, double nb, double* x, int64_t n) {
     info.resid0 = beta / nb;
-    if (!(info.resid0 > 1.0)) return false;
+    if (!(info.resid0 > 1.0 + 1e-12)) return false;
     PB_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, s));
     info.cold_fallback = true;
     return true;
@@ -406,7 +408,7 @@ void add_into(hipStream_t s, int64_t n, double a, const float* d, double* x) { a
 template <typename T>
 SolveInfo bicgstab_split(hipStream_t s, BicgWork<T>& w, GmresWork<T>& gw, const CsrOp<double>& A64,
                          const CsrOp<T>& Ahat, const double* b, double* x, double* r, T* work1, T* work2, T* work3,
-                         int maxit, double tol, double inner_rtol, double inner_rtol_cold) {
+                         int maxit, double tol, double inner_rtol, double inner_rtol_cold, const CoarseFn& coarse) {
     SolveInfo info;
     const int64_t n = A64.n;
     w.ensure(n);
@@ -429,6 +431,16 @@ SolveInfo bicgstab_split(hipStream_t s, BicgWork<T>& w, GmresWork<T>& gw, const 
             break;
         }
         if (info.iters >= maxit) break;
+        if (coarse) {
+            coarse(s, r, x);
+            beta = residual_norm(s, gw, A64, x, b, r);
+            if (info.cycles == 0) info.resid_coarse = beta / nb;
+            info.resid = beta / nb;
+            if (info.resid <= tol) {
+                info.converged = true;
+                break;
+            }
+        }
         scale_into(s, n, r, 1.0 / beta, work1);  // r / ||r||
         {
             ProfScope ps(Ahat.prof, s, pclass);
@@ -453,10 +465,10 @@ SolveInfo bicgstab_split(hipStream_t s, BicgWork<T>& w, GmresWork<T>& gw, const 
 
 template SolveInfo bicgstab_split<double>(hipStream_t, BicgWork<double>&, GmresWork<double>&, const CsrOp<double>&,
                                           const CsrOp<double>&, const double*, double*, double*, double*, double*,
-                                          double*, int, double, double, double);
+                                          double*, int, double, double, double, const CoarseFn&);
 template SolveInfo bicgstab_split<float>(hipStream_t, BicgWork<float>&, GmresWork<float>&, const CsrOp<double>&,
                                          const CsrOp<float>&, const double*, double*, double*, float*, float*, float*,
-                                         int, double, double, double);
+                                         int, double, double, double, const CoarseFn&);
 
 SolveInfo bicgstab_mixed(hipStream_t s, BicgWork<float>& w, GmresWork<float>& gw, const CsrOp<double>& A64,
                          const CsrOp<float>& A32, const PcFn<float>& M32, const double* b, double* x, double* r,

@@ -77,6 +77,7 @@ struct SolveInfo {
     double resid0 = 1;  // the same for the initial guess
     bool converged = false;
     bool cold_fallback = false;  // the initial guess was worse than x0 = 0 and was dropped
+    double resid_coarse = 0;     // ||b - A x|| / ||b|| after the first coarse correction (0: none)
 };
 
 // Device workspace of one GMRES instance of storage type T.
@@ -142,11 +143,16 @@ SolveInfo bicgstab_solve(hipStream_t s, BicgWork<double>& w, GmresWork<double>& 
 // Ahat (storage T) from zero to a relative inner_rtol, maps the correction back
 // with the right factor and adds it to x in FP64.  Ahat must have .split set;
 // work1, work2 are n-vectors of T.  inner_rtol_cold (> 0) replaces inner_rtol when
-// the given start is dropped for x0 = 0 (a warm start worse than zero).
+// the given start is dropped for x0 = 0 (a warm start worse than zero).  coarse (may be
+// empty): a correction of x from the FP64 residual r, applied at the start of every
+// cycle before the inner run (the learned coarse correction); the residual is then
+// recomputed, and SolveInfo::resid_coarse records it after the first correction.
+using CoarseFn = std::function<void(hipStream_t, const double*, double*)>;
 template <typename T>
 SolveInfo bicgstab_split(hipStream_t s, BicgWork<T>& w, GmresWork<T>& gw, const CsrOp<double>& A64,
                          const CsrOp<T>& Ahat, const double* b, double* x, double* r, T* work1, T* work2, T* work3,
-                         int maxit, double tol, double inner_rtol, double inner_rtol_cold = 0.0);
+                         int maxit, double tol, double inner_rtol, double inner_rtol_cold = 0.0,
+                         const CoarseFn& coarse = CoarseFn());
 // FP64 refinement around FP32 BiCGStab runs (FP64 residuals, FP64 dot accumulation)
 SolveInfo bicgstab_mixed(hipStream_t s, BicgWork<float>& w, GmresWork<float>& gw, const CsrOp<double>& A64,
                          const CsrOp<float>& A32, const PcFn<float>& M32, const double* b, double* x, double* r,

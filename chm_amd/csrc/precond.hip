@@ -340,7 +340,10 @@ __global__ void __launch_bounds__(kBlock) k_eis_bwd(int64_t F, int Lrt, int64_t 
 // o = w - acc, v = D~^-1 o (without t: the left transform b^ = D~ (D~ + L)^-1 b).
 // FUSE: colour 0 (no lower neighbours), whose backward sweep runs first in the
 // same thread and whose t stays in registers.
-template <typename T, int LM, bool EX, bool WITH_T, bool FUSE, bool DOTS, bool NT>
+// PRE_DW: the dot vector is loaded with the rest of the column (more loads in flight, ten
+// more VGPRs) rather than in the final loop.  SV = false: v is not stored (the top colour,
+// whose v no higher colour gathers)
+template <typename T, int LM, bool EX, bool WITH_T, bool FUSE, bool DOTS, bool NT, bool PRE_DW = true, bool SV = true>
 __device__ __forceinline__ void eis_fwd_column(int64_t F, int Lrt, int64_t k, const int32_t* __restrict__ nbp,
                                                const uint8_t* __restrict__ lowmask, const T* __restrict__ lat,
                                                const T* __restrict__ vert, const T* __restrict__ dinv,
@@ -367,7 +370,7 @@ __device__ __forceinline__ void eis_fwd_column(int64_t F, int Lrt, int64_t k, co
         if (z < L) {
             const uint32_t q = (uint32_t)z * Fu + ku;
             wv[z] = w[q];
-            if (DOTS) dwv[z] = dw[q];  // loaded with the rest of the column, not in the final loop
+            if (DOTS && PRE_DW) dwv[z] = dw[q];
             dv[z] = ldc<NT>(&dinv[q]);
             vb[z] = z > 0 ? ldc<NT>(&vert[q]) : T(0);
             T acc = 0;
@@ -426,18 +429,18 @@ __device__ __forceinline__ void eis_fwd_column(int64_t F, int Lrt, int64_t k, co
                 oz = wv[z] - acc;
                 vz = dv[z] * oz;
             }
-            v[q] = vz;
+            if (SV) v[q] = vz;
             o[q] = oz;
             prev = vz;
             if (DOTS) {
-                d1 = fma((double)dwv[z], (double)oz, d1);
+                d1 = fma((double)(PRE_DW ? dwv[z] : dw[q]), (double)oz, d1);
                 d2 = fma((double)oz, (double)oz, d2);
             }
         }
 }
 
 // DOTS: per-block partial sums (dw . o, o . o) into partial[2 * blockIdx.x] (FP64).
-template <typename T, int LM, bool EX, bool XCD, bool WITH_T, bool FUSE, bool DOTS, bool NT>
+template <typename T, int LM, bool EX, bool XCD, bool WITH_T, bool FUSE, bool DOTS, bool NT, bool SV = true>
 __global__ void __launch_bounds__(kBlock) k_eis_fwd(int64_t F, int L, int64_t k0, int64_t k1,
                                                     const int32_t* __restrict__ nbp,
                                                     const uint8_t* __restrict__ lowmask, const T* __restrict__ lat,
@@ -450,7 +453,7 @@ __global__ void __launch_bounds__(kBlock) k_eis_fwd(int64_t F, int L, int64_t k0
     if (stop && *stop != 0.0) return;  // a finished BiCGStab run (uniform over the grid)
     const int64_t k = k0 + logical_block<XCD>((k1 - k0 + kBlock - 1) / kBlock) * kBlock + threadIdx.x;
     double d1 = 0, d2 = 0;
-    if (k < k1) eis_fwd_column<T, LM, EX, WITH_T, FUSE, DOTS, NT>(F, L, k, nbp, lowmask, lat, vert, dinv, E, w, t, tnb, vnb,
+    if (k < k1) eis_fwd_column<T, LM, EX, WITH_T, FUSE, DOTS, NT, true, SV>(F, L, k, nbp, lowmask, lat, vert, dinv, E, w, t, tnb, vnb,
                                                            v, o, dw, d1, d2);
     if constexpr (DOTS) {
         __shared__ double s_red[kBlock / 64];
@@ -484,7 +487,7 @@ __global__ void __launch_bounds__(kBlock) k_eis_bwd_tab(int64_t F, int Lrt, cons
     eis_bwd_column<T, LM, EX, NT>(F, Lrt, k, nbp, lowmask, lat, vert, dinv, w, tnb, t);
 }
 
-template <typename T, int LM, bool EX, bool WITH_T, bool FUSE, bool DOTS, bool NT>
+template <typename T, int LM, bool EX, bool WITH_T, bool FUSE, bool DOTS, bool NT, bool SV = true>
 __global__ void __launch_bounds__(kBlock) k_eis_fwd_tab(int64_t F, int L, const int4* __restrict__ tab,
                                                         const int32_t* __restrict__ nbp,
                                                         const uint8_t* __restrict__ lowmask, const T* __restrict__ lat,
@@ -499,8 +502,8 @@ __global__ void __launch_bounds__(kBlock) k_eis_fwd_tab(int64_t F, int L, const 
     if (stop && stop[(int64_t)e.z * stop_stride] != 0.0) return;  // uniform over the workgroup
     const int64_t k = (int64_t)e.x + threadIdx.x;
     double d1 = 0, d2 = 0;
-    if (k < e.y) eis_fwd_column<T, LM, EX, WITH_T, FUSE, DOTS, NT>(F, L, k, nbp, lowmask, lat, vert, dinv, E, w, t, tnb,
-                                                                  vnb, v, o, dw, d1, d2);
+    if (k < e.y) eis_fwd_column<T, LM, EX, WITH_T, FUSE, DOTS, NT, true, SV>(F, L, k, nbp, lowmask, lat, vert, dinv, E, w,
+                                                                            t, tnb, vnb, v, o, dw, d1, d2);
     if constexpr (DOTS) {
         __shared__ double s_red[kBlock / 64];
         const double a = block_sum(d1, s_red);
@@ -1063,18 +1066,19 @@ bool tune_dilu_xcd(hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFa
 namespace {
 // one forward launch of colour c; with dw the block partials of (dw . o, o . o) go to
 // partial[2 * (nb + block)] and nb advances by the launch's block count
-template <typename T, int LM, bool EX, bool XCD, bool NT, bool WITH_T, bool FUSE>
+// SV = false for the top colour: its v is not stored, no higher colour gathers it
+template <typename T, int LM, bool EX, bool XCD, bool NT, bool WITH_T, bool FUSE, bool SV>
 void launch_fwd(hipStream_t s, dim3 g, int64_t F, int L, const ColorOrder& co, int c, DiluFactors<T>& d, const T* w,
                 T* o, const T* dw, double* partial, int64_t& nb, const double* stop) {
     if (dw) {
-        hipLaunchKernelGGL((k_eis_fwd<T, LM, EX, XCD, WITH_T, FUSE, true, NT>), g, dim3(kBlock), 0, s, F, L, co.off[c],
-                           co.off[c + 1], co.nbp.p, co.lowmask.p, d.lat.p, d.vert.p, d.dinv.p, d.E.p, w, d.t.p,
-                           d.t.p, d.y.p, d.y.p, o, dw, partial + 2 * nb, stop);
+        hipLaunchKernelGGL((k_eis_fwd<T, LM, EX, XCD, WITH_T, FUSE, true, NT, SV>), g, dim3(kBlock), 0, s, F, L,
+                           co.off[c], co.off[c + 1], co.nbp.p, co.lowmask.p, d.lat.p, d.vert.p, d.dinv.p, d.E.p, w,
+                           d.t.p, d.t.p, d.y.p, d.y.p, o, dw, partial + 2 * nb, stop);
         nb += g.x;
     } else {
-        hipLaunchKernelGGL((k_eis_fwd<T, LM, EX, XCD, WITH_T, FUSE, false, NT>), g, dim3(kBlock), 0, s, F, L, co.off[c],
-                           co.off[c + 1], co.nbp.p, co.lowmask.p, d.lat.p, d.vert.p, d.dinv.p, d.E.p, w, d.t.p,
-                           d.t.p, d.y.p, d.y.p, o, dw, partial, stop);
+        hipLaunchKernelGGL((k_eis_fwd<T, LM, EX, XCD, WITH_T, FUSE, false, NT, SV>), g, dim3(kBlock), 0, s, F, L,
+                           co.off[c], co.off[c + 1], co.nbp.p, co.lowmask.p, d.lat.p, d.vert.p, d.dinv.p, d.E.p, w,
+                           d.t.p, d.t.p, d.y.p, d.y.p, o, dw, partial, stop);
     }
 }
 
@@ -1087,16 +1091,21 @@ int64_t split_apply_t(hipStream_t s, const ColorOrder& co, int64_t F, int L, Dil
     // backward sweep, colours top .. 1: t
     for (int c = top; c >= 1; --c) {
         if (co.off[c + 1] == co.off[c]) continue;
-        hipLaunchKernelGGL((k_eis_bwd<T, LM, EX, XCD, NT>), blocks(c), dim3(kBlock), 0, s, F, L, co.off[c], co.off[c + 1],
-                           co.nbp.p, co.lowmask.p, d.lat.p, d.vert.p, d.dinv.p, w, d.t.p, d.t.p, stop);
+        hipLaunchKernelGGL((k_eis_bwd<T, LM, EX, XCD, NT>), blocks(c), dim3(kBlock), 0, s, F, L, co.off[c],
+                           co.off[c + 1], co.nbp.p, co.lowmask.p, d.lat.p, d.vert.p, d.dinv.p, w, d.t.p, d.t.p, stop);
     }
     // colour 0: backward and forward sweeps in one pass
     if (co.off[1] > co.off[0])
-        launch_fwd<T, LM, EX, XCD, NT, true, true>(s, blocks(0), F, L, co, 0, d, w, o, dw, partial, nb, stop);
+        launch_fwd<T, LM, EX, XCD, NT, true, true, true>(s, blocks(0), F, L, co, 0, d, w, o, dw, partial, nb, stop);
     // forward sweep, colours 1 .. top
     for (int c = 1; c <= top; ++c) {
         if (co.off[c + 1] == co.off[c]) continue;
-        launch_fwd<T, LM, EX, XCD, NT, true, false>(s, blocks(c), F, L, co, c, d, w, o, dw, partial, nb, stop);
+        if (c == top)
+            launch_fwd<T, LM, EX, XCD, NT, true, false, false>(s, blocks(c), F, L, co, c, d, w, o, dw, partial, nb,
+                                                               stop);
+        else
+            launch_fwd<T, LM, EX, XCD, NT, true, false, true>(s, blocks(c), F, L, co, c, d, w, o, dw, partial, nb,
+                                                              stop);
     }
     return nb;
 }
@@ -1248,14 +1257,17 @@ void split_apply_batch_t(hipStream_t s, const ColorOrder& co, int64_t F, int L, 
         if (st.nblk[c] == 0) continue;
         const dim3 g((unsigned)st.nblk[c]);
         double* pc = dw ? partial + 2 * st.part_off[c] : nullptr;
-#define PB_FWD_TAB(FUSE, DOTS)                                                                                   \
-    hipLaunchKernelGGL((k_eis_fwd_tab<T, LM, EX, true, FUSE, DOTS, false>), g, dim3(kBlock), 0, s, F, L, st.tab[c].p, \
-                       co.nbp.p, co.lowmask.p, d.lat.p, d.vert.p, d.dinv.p, d.E.p, w, d.t.p, d.t.p, d.y.p, d.y.p, o, dw, \
-                       pc, stop, stop_stride)
+#define PB_FWD_TAB(FUSE, DOTS, SV)                                                                                 \
+    hipLaunchKernelGGL((k_eis_fwd_tab<T, LM, EX, true, FUSE, DOTS, false, SV>), g, dim3(kBlock), 0, s, F, L,         \
+                       st.tab[c].p, co.nbp.p, co.lowmask.p, d.lat.p, d.vert.p, d.dinv.p, d.E.p, w, d.t.p, d.t.p, d.y.p, \
+                       d.y.p, o, dw, pc, stop, stop_stride)
+        // the top colour does not store v (no higher colour gathers it)
         if (c == 0) {
-            if (dw) PB_FWD_TAB(true, true); else PB_FWD_TAB(true, false);
+            if (dw) PB_FWD_TAB(true, true, true); else PB_FWD_TAB(true, false, true);
+        } else if (c == top) {
+            if (dw) PB_FWD_TAB(false, true, false); else PB_FWD_TAB(false, false, false);
         } else {
-            if (dw) PB_FWD_TAB(false, true); else PB_FWD_TAB(false, false);
+            if (dw) PB_FWD_TAB(false, true, true); else PB_FWD_TAB(false, false, true);
         }
 #undef PB_FWD_TAB
     }

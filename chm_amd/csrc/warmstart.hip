@@ -95,6 +95,14 @@ void warm_features(pbsm3d_ctx* c, const double* vw_dir, const double* t) {
                        c->csubl.p, c->area.p, c->elen.p, c->nx.p, c->ny.p, c->neigh.p, vw_dir, t, c->feat.p);
 }
 
+void training_pairs_into(pbsm3d_ctx* c, const double* vw_dir, const double* t, float* X, float* Y) {
+    hipLaunchKernelGGL(k_features, dim3(grid_for(c->F)), dim3(kBlock), 0, c->stream, c->F, c->L, c->fscal.p, c->u_z.p,
+                       c->csubl.p, c->area.p, c->elen.p, c->nx.p, c->ny.p, c->neigh.p, vw_dir, t, X);
+    hipLaunchKernelGGL(k_targets, dim3(grid_for(c->F)), dim3(kBlock), 0, c->stream, c->F, c->L, c->susp.x.p,
+                       c->dep.x.p, Y);
+    PB_HIP(hipGetLastError());
+}
+
 void warm_apply(pbsm3d_ctx* c, int which) {
     hipLaunchKernelGGL(k_apply, dim3(grid_for(c->F)), dim3(kBlock), 0, c->stream, c->F, c->L, c->pred.p,
                        which == 0 ? c->susp.x.p : nullptr, which == 1 ? c->dep.x.p : nullptr);
@@ -206,11 +214,7 @@ int pbsm3d_training_pairs(pbsm3d_ctx* c, float* X, float* Y) {
     if (!c || !X || !Y || c->L != 10 || !c->last_vw_dir) return PBSM3D_EINVAL;
     try {
         PB_HIP(hipSetDevice(c->device));
-        hipLaunchKernelGGL(pb::k_features, dim3(pb::grid_for(c->F)), dim3(pb::kBlock), 0, c->stream, c->F, c->L,
-                           c->fscal.p, c->u_z.p, c->csubl.p, c->area.p, c->elen.p, c->nx.p, c->ny.p, c->neigh.p,
-                           c->last_vw_dir, c->last_t, X);
-        hipLaunchKernelGGL(pb::k_targets, dim3(pb::grid_for(c->F)), dim3(pb::kBlock), 0, c->stream, c->F, c->L,
-                           c->susp.x.p, c->dep.x.p, Y);
+        pb::training_pairs_into(c, c->last_vw_dir, c->last_t, X, Y);
         PB_HIP(hipStreamSynchronize(c->stream));
         return PBSM3D_OK;
     } catch (const pb::Error& e) {

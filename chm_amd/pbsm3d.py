@@ -48,6 +48,7 @@ EXPORTED = (
     "pbsm3d_batch_info",
     "pbsm3d_batch_get_state",
     "pbsm3d_batch_set_state",
+    "pbsm3d_batch_training_pairs",
     "pbsm3d_batch_destroy",
     "pbsm3d_batch_last_error",
     "pbsm3d_wind_config_default",
@@ -77,6 +78,7 @@ EXPORTED = (
     "pbsm3d_mlp_last_error",
     "pbsm3d_training_pairs",
     "pbsm3d_set_warm_start",
+    "pbsm3d_coarse_system",
 )
 
 N_FEATURES, N_TARGETS = 16, 11
@@ -148,6 +150,7 @@ def lib():
         L.pbsm3d_batch_info.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         L.pbsm3d_batch_get_state.argtypes = [vp, C.c_int32, dp, dp]
         L.pbsm3d_batch_set_state.argtypes = [vp, C.c_int32, dp, dp]
+        L.pbsm3d_batch_training_pairs.argtypes = [vp, C.c_int32, vp, vp]
         L.pbsm3d_batch_destroy.argtypes = [vp]
         L.pbsm3d_batch_destroy.restype = None
         L.pbsm3d_batch_last_error.argtypes = [vp]
@@ -200,6 +203,7 @@ def lib():
         L.pbsm3d_mlp_last_error.restype = C.c_char_p
         L.pbsm3d_training_pairs.argtypes = [vp, vp, vp]
         L.pbsm3d_set_warm_start.argtypes = [vp, vp]
+        L.pbsm3d_coarse_system.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_double), C.POINTER(C.c_double)]
         _lib = L
     return _lib
 
@@ -350,6 +354,18 @@ class Context:
     def set_warm_start(self, net: "Mlp | None"):
         self._warm = net  # keep the network alive while attached
         _check(lib().pbsm3d_set_warm_start(self._h, net._h if net is not None else None), self._h)
+
+    def coarse_system(self):
+        """The learned coarse space of the last step (cfg.coarse_tiles = T > 0): tile of each
+        face [F], the basis w [L*F] (reference order) and the Galerkin matrix G [T x T]."""
+        T = int(self.cfg.coarse_tiles)
+        tile = np.zeros(self.F, dtype=np.int32)
+        w = np.zeros(self.F * self.L)
+        G = np.zeros(T * T)
+        _check(lib().pbsm3d_coarse_system(self._h, tile.ctypes.data_as(C.POINTER(C.c_int32)),
+                                           w.ctypes.data_as(C.POINTER(C.c_double)),
+                                           G.ctypes.data_as(C.POINTER(C.c_double))), self._h)
+        return tile, w, G.reshape(T, T)
 
     def get_state(self):
         a, b = np.zeros(self.F), np.zeros(self.F)
@@ -565,6 +581,12 @@ class Batch:
         keep = abi.Keep()
         self._check(lib().pbsm3d_batch_get_state(self._h, m, keep.dptr(a), keep.dptr(b)))
         return a, b
+
+    def training_pairs(self, m: int, X, Y):
+        """Member m's features [F_m x 16] and solved targets [F_m x 11] of the last step
+        into float32 device tensors X, Y (nLayer 10)."""
+        self._check(lib().pbsm3d_batch_training_pairs(self._h, m, C.c_void_p(X.data_ptr()),
+                                                       C.c_void_p(Y.data_ptr())))
 
     def set_state(self, m: int, sum_drift, sum_subl):
         keep = abi.Keep()

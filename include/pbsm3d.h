@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define PBSM3D_ABI_VERSION 6
+#define PBSM3D_ABI_VERSION 7
 
 /* return codes */
 #define PBSM3D_OK 0
@@ -129,7 +129,11 @@ typedef struct pbsm3d_config {
     int32_t rouault_diffusion_coef; /* 0 (PBSM3D.cpp:258, 1164-1172): vertical diffusivity
                                        coefficient 1 / (1 + w^2 / (1.56 u*^2)) per cell in
                                        place of snow_diffusion_const                        */
-    int32_t reserved_cfg;
+    int32_t coarse_tiles;           /* 0 (extension): with the learned warm start and the
+                                       split BiCGStab, T in [1, 256] adds the learned coarse
+                                       correction: at the start of every refinement cycle
+                                       x += W (W^T A W)^-1 W^T (b - A x), W = the network's
+                                       predicted profile on T tiles (face-index ranges)     */
 } pbsm3d_config;
 
 /* Krylov method of the suspension solve (the deposition solve is GMRES(restart)) */
@@ -193,6 +197,8 @@ typedef struct pbsm3d_stats {
     double susp_resid0;           /* ||b - A x0|| / ||b|| of the suspension start   */
     int32_t susp_cold_fallback;   /* 1: the warm start was worse than x0 = 0 and    */
     int32_t reserved2;            /*    the solve restarted from zero               */
+    double susp_resid_coarse;     /* ||b - A x|| / ||b|| after the first learned coarse
+                                     correction (0: not applied)                     */
 } pbsm3d_stats;
 
 typedef struct pbsm3d_ctx pbsm3d_ctx;
@@ -324,6 +330,11 @@ const char* pbsm3d_mlp_last_error(const pbsm3d_mlp* net);
 int pbsm3d_training_pairs(pbsm3d_ctx* ctx, float* X, float* Y);
 /* attach (or detach with NULL) the network used when cfg.warm_start = PBSM3D_WS_LEARNED */
 int pbsm3d_set_warm_start(pbsm3d_ctx* ctx, pbsm3d_mlp* net);
+/* The learned coarse space of the last step that used it (cfg.coarse_tiles = T > 0), for
+ * parity checks (host arrays, each may be NULL): tile_of_face[F], the basis w[nLayer*F]
+ * (the network's predicted profile, reference order) and the Galerkin matrix
+ * G[T*T] = W^T A W (row-major, a tile with w = 0 has G[a][a] = 1). */
+int pbsm3d_coarse_system(pbsm3d_ctx* ctx, int32_t* tile_of_face, double* w, double* G);
 
 #ifdef __cplusplus
 }

@@ -59,6 +59,13 @@ int pbsm3d_batch_info(const pbsm3d_batch* b, int32_t* unions, int64_t* launches,
 int pbsm3d_batch_get_state(pbsm3d_batch* b, int32_t member, double* sum_drift, double* sum_subl);
 int pbsm3d_batch_set_state(pbsm3d_batch* b, int32_t member, const double* sum_drift, const double* sum_subl);
 
+/* Training pairs of the learned warm start from member `member`'s last step (nLayer 10),
+ * as pbsm3d_training_pairs gives them for one context: X [F_m x 16] features and Y
+ * [F_m x 11] targets, device arrays of that member's face count.  Many forcing draws of
+ * one mesh solved as the members of one batch give the network's training set in few
+ * launches. */
+int pbsm3d_batch_training_pairs(pbsm3d_batch* b, int32_t member, float* X, float* Y);
+
 void pbsm3d_batch_destroy(pbsm3d_batch* b);
 /* b NULL: the last pbsm3d_batch_create error */
 const char* pbsm3d_batch_last_error(const pbsm3d_batch* b);

@@ -28,7 +28,7 @@ def test_library_exports_every_declared_symbol():
     assert len(names) >= 10
     for n in names:
         assert hasattr(L, n), n
-    assert L.pbsm3d_abi_version() == 6
+    assert L.pbsm3d_abi_version() == 7
     assert set(pbsm3d.EXPORTED) <= set(names)
 
 
@@ -94,6 +94,7 @@ def test_context_calls_reject_null_context():
     assert L.pbsm3d_profile_classes(None, 1) == abi.PBSM3D_EINVAL
     assert L.pbsm3d_precond_select(None, 0, 1) == abi.PBSM3D_EINVAL
     assert L.pbsm3d_spmv_select(None, 0, 1) == abi.PBSM3D_EINVAL
+    assert L.pbsm3d_batch_training_pairs(None, 0, None, None) == abi.PBSM3D_EINVAL
 
 
 def test_empty_inputs_rejected_without_gpu():
@@ -152,3 +153,26 @@ def test_bad_neighbour_links_rejected_without_gpu():
     # the unmodified mesh colours fine
     assert L.pbsm3d_face_colors(m.n_faces, g0.neigh.ctypes.data_as(C.POINTER(C.c_int32)),
                                 col.ctypes.data_as(C.POINTER(C.c_int32))) >= 3
+
+
+def test_coarse_correction_config_rejected_without_gpu():
+    """coarse_tiles is refused before any device call outside [0, 256], above the face
+    count, without the learned start, or without the split BiCGStab; the batch has no
+    coarse correction."""
+    from chm_amd import pbsm3d
+
+    m = meshgen.synthetic_mesh(60, seed=2)
+    g = pbsm3d.face_geometry(m)
+    split, learned = abi.KRYLOV_BICGSTAB_SPLIT, abi.WS_LEARNED
+    for kw in ({"coarse_tiles": -1, "krylov": split, "warm_start": learned},
+               {"coarse_tiles": 257, "krylov": split, "warm_start": learned},
+               {"coarse_tiles": m.n_faces + 1, "krylov": split, "warm_start": learned},
+               {"coarse_tiles": 4, "krylov": split},
+               {"coarse_tiles": 4, "warm_start": learned}):
+        with pytest.raises(pbsm3d.Pbsm3dError) as e:
+            pbsm3d.Context(g, pbsm3d.default_config(nLayer=10, **kw))
+        assert e.value.code == abi.PBSM3D_EINVAL, kw
+    with pytest.raises(pbsm3d.Pbsm3dError) as e:
+        pbsm3d.Batch([g], [pbsm3d.default_config(nLayer=10, krylov=split, warm_start=learned, coarse_tiles=4)])
+    assert e.value.code == abi.PBSM3D_EINVAL
+    assert pbsm3d.lib().pbsm3d_coarse_system(None, None, None, None) == abi.PBSM3D_EINVAL

@@ -151,3 +151,36 @@ def test_batch_graph_replays_match_stream_launches():
             assert sa[k]["susp_iters"] == sb[k]["susp_iters"] and sa[k]["dep_iters"] == sb[k]["dep_iters"]
             for f in ga[k]:
                 np.testing.assert_array_equal(ga[k][f], gb[k][f], err_msg=f"member {k} {f}")
+
+
+def test_batch_training_pairs_match_member_context():
+    """pbsm3d_batch_training_pairs: three forcing draws of one 10-layer mesh solved as the
+    members of one batch give each member the features and targets that
+    pbsm3d_training_pairs gives for the same draw solved in a context of its own."""
+    import torch
+
+    from chm_amd import pbsm3d as pb
+
+    print("[case] batch training pairs", flush=True)
+    mesh = meshgen.synthetic_mesh(2500, seed=0)
+    g = pb.face_geometry(mesh)
+    F = mesh.n_faces
+    cfg = pb.default_config(nLayer=10, krylov=abi.KRYLOV_BICGSTAB_SPLIT, tol=1e-12)
+    fos = [meshgen.forcing_c2(F, seed=s) for s in (1, 2, 3)]
+    b = pb.Batch([g] * 3, [cfg] * 3)
+    _, st = b.step(fos)
+    ctx = pb.Context(g, cfg)
+    for m, fo in enumerate(fos):
+        assert st[m]["suspension_present"] == 1
+        Xb = torch.empty((F, pb.N_FEATURES), dtype=torch.float32, device="cuda:0")
+        Yb = torch.empty((F, pb.N_TARGETS), dtype=torch.float32, device="cuda:0")
+        b.training_pairs(m, Xb, Yb)
+        ctx.step(fo)
+        Xs, Ys = torch.empty_like(Xb), torch.empty_like(Yb)
+        ctx.training_pairs(Xs, Ys)
+        # features come from the per-face assembly (the same expressions in both); targets
+        # from the two solves at 1e-12 (log concentrations, asinh of the deposition flux)
+        assert torch.allclose(Xb, Xs, rtol=1e-6, atol=1e-6), (m, (Xb - Xs).abs().max().item())
+        assert (Yb - Ys).abs().max().item() < 1e-4, (m, (Yb - Ys).abs().max().item())
+    b.close()
+    ctx.close()
