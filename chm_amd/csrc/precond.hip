@@ -439,7 +439,7 @@ __device__ __forceinline__ void eis_fwd_column(int64_t F, int Lrt, int64_t k, co
         }
 }
 
-// DOTS: per-block partial sums (dw . o, o . o) into partial[2 * blockIdx.x] (FP64).
+// DOTS: per-block partial sums (dw . o, o . o) into partial[2 * logical block] (FP64).
 template <typename T, int LM, bool EX, bool XCD, bool WITH_T, bool FUSE, bool DOTS, bool NT, bool SV = true>
 __global__ void __launch_bounds__(kBlock) k_eis_fwd(int64_t F, int L, int64_t k0, int64_t k1,
                                                     const int32_t* __restrict__ nbp,
@@ -451,17 +451,22 @@ __global__ void __launch_bounds__(kBlock) k_eis_fwd(int64_t F, int L, int64_t k0
                                                     T* __restrict__ o, const T* __restrict__ dw,
                                                     double* __restrict__ partial, const double* __restrict__ stop) {
     if (stop && *stop != 0.0) return;  // a finished BiCGStab run (uniform over the grid)
-    const int64_t k = k0 + logical_block<XCD>((k1 - k0 + kBlock - 1) / kBlock) * kBlock + threadIdx.x;
+    const int64_t lb = logical_block<XCD>((k1 - k0 + kBlock - 1) / kBlock);
+    const int64_t k = k0 + lb * kBlock + threadIdx.x;
     double d1 = 0, d2 = 0;
-    if (k < k1) eis_fwd_column<T, LM, EX, WITH_T, FUSE, DOTS, NT, true, SV>(F, L, k, nbp, lowmask, lat, vert, dinv, E, w, t, tnb, vnb,
-                                                           v, o, dw, d1, d2);
+    if (k < k1)
+        eis_fwd_column<T, LM, EX, WITH_T, FUSE, DOTS, NT, true, SV>(F, L, k, nbp, lowmask, lat, vert, dinv, E, w, t,
+                                                                    tnb, vnb, v, o, dw, d1, d2);
     if constexpr (DOTS) {
+        // partials indexed by the logical block: the fixed-order sum sees the same faces in
+        // the same order with and without the XCD workgroup mapping (blocks past the colour
+        // add zeros), so the tuner's choice of mapping does not change the iterates
         __shared__ double s_red[kBlock / 64];
         const double a = block_sum(d1, s_red);
         const double b = block_sum(d2, s_red);
         if (threadIdx.x == 0) {
-            partial[2 * blockIdx.x] = a;
-            partial[2 * blockIdx.x + 1] = b;
+            partial[2 * lb] = a;
+            partial[2 * lb + 1] = b;
         }
     }
 }
@@ -571,7 +576,7 @@ __global__ void __launch_bounds__(1024) k_eis_bwd_lp(int64_t F, int L, int64_t k
 }
 
 // FUSE: colour 0 (no lower neighbours), both sweeps in one launch with t kept in
-// LDS.  DOTS: partial[2 * blockIdx.x] = (dw . o, o . o) over the workgroup (FP64).
+// LDS.  DOTS: partial[2 * logical block] = (dw . o, o . o) over the workgroup (FP64).
 template <typename T, bool XCD, bool FUSE, bool DOTS, bool NT>
 __global__ void __launch_bounds__(1024) k_eis_fwd_lp(int64_t F, int L, int64_t k0, int64_t k1,
                                                      const int32_t* __restrict__ nbp,
@@ -594,7 +599,8 @@ __global__ void __launch_bounds__(1024) k_eis_fwd_lp(int64_t F, int L, int64_t k
     T* sa = st + LC;
     T* sb = sa + LC;
     const int lane = threadIdx.x, z = threadIdx.y;
-    const int64_t k = k0 + logical_block<XCD>((k1 - k0 + kLpCols - 1) / kLpCols) * kLpCols + lane;
+    const int64_t lb = logical_block<XCD>((k1 - k0 + kLpCols - 1) / kLpCols);
+    const int64_t k = k0 + lb * kLpCols + lane;
     const bool valid = k < k1;
     const int64_t N = F * L;
     const int64_t q = (int64_t)z * F + k;
@@ -680,8 +686,8 @@ __global__ void __launch_bounds__(1024) k_eis_fwd_lp(int64_t F, int L, int64_t k
                 a += s_red[2 * zz];
                 b += s_red[2 * zz + 1];
             }
-            partial[2 * blockIdx.x] = a;
-            partial[2 * blockIdx.x + 1] = b;
+            partial[2 * lb] = a;
+            partial[2 * lb + 1] = b;
         }
     }
 }
@@ -1087,7 +1093,9 @@ int64_t split_apply_t(hipStream_t s, const ColorOrder& co, int64_t F, int L, Dil
                       const T* dw, double* partial, const double* stop) {
     int64_t nb = 0;
     const int top = co.ncolor - 1;
-    auto blocks = [&](int c) { return dim3((unsigned)launch_blocks<XCD>((co.off[c + 1] - co.off[c] + kBlock - 1) / kBlock)); };
+    // grids rounded to a multiple of the XCD count with or without the XCD mapping: the dot
+    // partials then have the same layout in both (the extra workgroups add zeros)
+    auto blocks = [&](int c) { return dim3((unsigned)launch_blocks<true>((co.off[c + 1] - co.off[c] + kBlock - 1) / kBlock)); };
     // backward sweep, colours top .. 1: t
     for (int c = top; c >= 1; --c) {
         if (co.off[c + 1] == co.off[c]) continue;
@@ -1168,8 +1176,8 @@ int64_t split_apply_lp(hipStream_t s, const ColorOrder& co, int64_t F, int L, Di
     const int top = co.ncolor - 1;
     const dim3 blk(kLpCols, (unsigned)L);
     const size_t lc = (size_t)L * kLpCols * sizeof(T);
-    auto blocks = [&](int c) {
-        return dim3((unsigned)launch_blocks<XCD>((co.off[c + 1] - co.off[c] + kLpCols - 1) / kLpCols));
+    auto blocks = [&](int c) {  // rounded as in split_apply_t
+        return dim3((unsigned)launch_blocks<true>((co.off[c + 1] - co.off[c] + kLpCols - 1) / kLpCols));
     };
     for (int c = top; c >= 1; --c) {
         if (co.off[c + 1] == co.off[c]) continue;

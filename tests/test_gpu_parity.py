@@ -364,7 +364,10 @@ def test_every_spmv_launch_shape(pb, oracle, mesh_c1, precision):
 @pytest.mark.parametrize("precision", [abi.FP64, abi.MIXED])
 def test_every_split_sweep_mode(pb, oracle, mesh_c1, precision):
     """The split operator's eight sweep modes (XCD workgroup mapping, non-temporal factor
-    loads, column or layer-parallel sweeps) each solve to the oracle."""
+    loads, column or layer-parallel sweeps) each solve to the oracle, and the four column
+    modes (0-3) give bit-identical iterations and fields: the mapping and the load hint
+    change neither the arithmetic nor the order of the dot-product partial sums, so the
+    tuner's choice among them does not move the iterates."""
     _print_case("every_split_sweep_mode", precision=precision)
     F = mesh_c1.n_faces
     fo = meshgen.forcing_c1(F, seed=0)
@@ -375,6 +378,7 @@ def test_every_split_sweep_mode(pb, oracle, mesh_c1, precision):
     ctx.step(fo)
     ptun = ctx.precond_tuning(which)
     assert 0 <= ptun["mode"] < pb.PRECOND_MODES and all(ms > 0 for ms in ptun["ms"]), ptun
+    column = []
     for m in range(pb.PRECOND_MODES):
         ctx.precond_select(which, m)
         ctx.set_state(np.zeros(F), np.zeros(F))
@@ -382,6 +386,12 @@ def test_every_split_sweep_mode(pb, oracle, mesh_c1, precision):
         assert st["susp_resid"] <= 1e-12
         for k in ("conc", "Qsusp", "qdep", "drift_mass"):
             assert rel_err(gg[k], og[k]) < 1e-8, (m, k, rel_err(gg[k], og[k]))
+        if m < 4:
+            column.append((st["susp_iters"], st["dep_iters"], gg))
+    for m, (si, di, gg) in enumerate(column[1:], 1):
+        assert (si, di) == column[0][:2], (m, si, di, column[0][:2])
+        for k in ("conc", "Qsusp", "qdep", "drift_mass"):
+            np.testing.assert_array_equal(gg[k], column[0][2][k], err_msg=f"mode {m} {k}")
     with pytest.raises(pb.Pbsm3dError):
         ctx.precond_select(which, pb.PRECOND_MODES)
     # profiling restricted to one kernel class brackets that class only
