@@ -261,6 +261,39 @@ def warm_start_gain(pb, abi, meshgen, warmstart, g, F, L, krylov_mixed, net, see
     return out
 
 
+def inner_target_sweep(pb, abi, meshgen, g, F, L, net, seeds, torch):
+    """Total suspension-solve time and iterations of the mixed split BiCGStab over held-out
+    draws for each candidate inner target, from x0 = 0 and from the learned start (the
+    selection behind the library's defaults; seeds disjoint from the timed forcing, the
+    training draws and the warm-start comparison draws)."""
+    dev = "cuda:0"
+    draws = [{k: torch.as_tensor(v, device=dev) for k, v in meshgen.forcing_c2(F, seed=sd).items()} for sd in seeds]
+    warm0 = {k: torch.as_tensor(v, device=dev) for k, v in meshgen.forcing_c2(F, seed=0).items()}
+    outs = {k: torch.zeros(F, dtype=torch.float64, device=dev) for k in abi.OUT_FIELDS}
+    res = {"seeds": list(seeds)}
+    for label, ws, targets in (("cold", abi.WS_ZERO, (1e-3, 1.5e-3, 2e-3, 3e-3, 5e-3)),
+                               ("learned", abi.WS_LEARNED, (1e-4, 1.5e-4, 2e-4, 3e-4, 5e-4, 1e-3))):
+        rows = []
+        for tgt in targets:
+            cfg = pb.default_config(nLayer=L, precision=abi.MIXED, krylov=abi.KRYLOV_BICGSTAB_SPLIT,
+                                    warm_start=ws, inner_rtol=tgt)
+            ctx = pb.Context(g, cfg)
+            if ws == abi.WS_LEARNED:
+                ctx.set_warm_start(net)
+            ctx.step(warm0, outs=outs, on_device=True)
+            its = ms = 0.0
+            for d in draws:
+                _, st = ctx.step(d, outs=outs, on_device=True)
+                its += st["susp_iters"]
+                ms += st["ms_susp_solve"]
+            ctx.close()
+            rows.append({"inner_rtol": tgt, "susp_iters": int(its), "ms_susp_solve": round(ms, 2)})
+            log(f"[bench] inner sweep {label} {tgt:g}: {int(its)} iterations, {ms:.1f} ms")
+        res[label] = rows
+        res[f"best_{label}"] = min(rows, key=lambda r: r["ms_susp_solve"])["inner_rtol"]
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -291,6 +324,9 @@ def main():
     ap.add_argument("--dilu-mode", type=int, default=-1, choices=list(range(-1, 8)),
                     help="pin the D-ILU sweep mode: bit 0 XCD workgroup mapping; split operator only: bit 1 "
                          "non-temporal factor loads, bit 2 layer-parallel sweeps (default: tuned)")
+    ap.add_argument("--inner-sweep-seeds", default="",
+                    help="e.g. 200-211: time the mixed split solve's inner-target candidates over these "
+                         "held-out draws, cold and learned (off by default)")
     ap.add_argument("--study-draws", type=int, default=20000,
                     help="learned-start study: FP64 solves of forcing draws on the 2,500-point mesh at 10 layers "
                          "that train its network, before cold vs learned on 16 held-out draws (0: skip)")
@@ -465,6 +501,12 @@ def main():
                                [int(x) for x in args.gain_seeds.split(",")])
         log(f"[bench] gain: {gain}")
 
+    sweep = None
+    if net is not None and args.inner_sweep_seeds:
+        a, b = (int(x) for x in args.inner_sweep_seeds.split("-"))
+        sweep = inner_target_sweep(pb, abi, meshgen, g, F, L, net, range(a, b + 1), torch)
+        log(f"[bench] inner sweep: {json.dumps(sweep)}")
+
     study = None
     if args.study_draws > 0:
         log(f"[bench] learned-start study: {args.study_draws} FP64 solves on the 2,500-point mesh at 10 layers")
@@ -560,6 +602,7 @@ def main():
         "warm_start_training": train_info,
         "warm_start_gain": gain,
         "learned_start_study": study,
+        "inner_target_sweep": sweep,
         "fetch": fetch_info,
         "ensemble": ens,
         "pcie_inclusive": pcie,

@@ -866,7 +866,7 @@ void group_suspension_solve(pbsm3d_batch* B, BatchGroup& g, BatchWork<T>& w, Dil
     const pbsm3d_config& cfg = c->cfg;
     const int M = (int)g.members.size();
     w.ensure(g.susp);
-    constexpr double kCold = 2e-3, kLearned = 5e-4;  // the single solve's defaults (api.hip)
+    constexpr double kCold = 2e-3, kLearned = 2e-4;  // the single solve's defaults (api.hip)
     std::vector<double> inner(M), cold(M);
     for (int m = 0; m < M; ++m) {
         if (cfg.inner_rtol > 0) {

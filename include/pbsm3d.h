@@ -114,7 +114,7 @@ typedef struct pbsm3d_config {
     int32_t krylov;               /* PBSM3D_KRYLOV_GMRES (reference) or _BICGSTAB  */
     double inner_rtol;            /* MIXED: relative residual each FP32 inner solve
                                      aims for; 0 = default (split BiCGStab: 2e-3
-                                     from x0 = 0, 5e-4 from the learned start;
+                                     from x0 = 0, 2e-4 from the learned start;
                                      1e-4 otherwise)                               */
     /* reference keys that are off by default */
     int32_t use_PomLi_probability;  /* 0 (PBSM3D.cpp:124, 848-865): c_salt scaled by the
