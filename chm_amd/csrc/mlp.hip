@@ -58,6 +58,45 @@ struct GemmArgs {
     int ksplit;   // K range per blockIdx.z
 };
 
+// One 16-byte chunk of a 64 x 32 operand tile into LDS T[64][TK + 8] (T[row][k]).
+// kmajor = 0: element (row, k) at X[row * ld + k] (k contiguous; chunk = 8 k of one row);
+// kmajor = 1: element (row, k) at X[k * ld + row] (row contiguous; chunk = 8 rows of one k).
+// rows: the matrix's row count (M or N); r0 / k0: the tile origin; ke: end of the K range.
+__device__ __forceinline__ void stage_chunk(const uint16_t* __restrict__ X, int ld, int kmajor, int rows, int r0,
+                                            int k0, int ke, int tid, uint16_t (*T)[TK + 8]) {
+    typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
+    if (!kmajor) {
+        const int r = tid >> 2, c = (tid & 3) * 8;  // 64 rows x 4 chunks
+        const int gr = r0 + r, gk = k0 + c;
+        u16x8 v = {0, 0, 0, 0, 0, 0, 0, 0};
+        const size_t off = (size_t)gr * ld + gk;
+        if (gr < rows && gk + 8 <= ke && ((uintptr_t)(X + off) & 15) == 0) {
+            v = *reinterpret_cast<const u16x8*>(X + off);
+        } else if (gr < rows) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                if (gk + j < ke) v[j] = X[off + j];
+        }
+        *reinterpret_cast<u16x8*>(&T[r][c]) = v;
+    } else {
+        const int kk = tid >> 3, r = (tid & 7) * 8;  // 32 k x 8 chunks of 8 rows
+        const int gk = k0 + kk, gr = r0 + r;
+        u16x8 v = {0, 0, 0, 0, 0, 0, 0, 0};
+        const size_t off = (size_t)gk * ld + gr;
+        if (gk < ke) {
+            if (gr + 8 <= rows && ((uintptr_t)(X + off) & 15) == 0) {
+                v = *reinterpret_cast<const u16x8*>(X + off);
+            } else {
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    if (gr + j < rows) v[j] = X[off + j];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) T[r + j][kk] = v[j];
+    }
+}
+
 __global__ void __launch_bounds__(256) k_gemm_bf16(GemmArgs g) {
     __shared__ __align__(16) uint16_t As[TM][TK + 8];
     __shared__ __align__(16) uint16_t Bs[TN][TK + 8];
@@ -72,36 +111,13 @@ __global__ void __launch_bounds__(256) k_gemm_bf16(GemmArgs g) {
 #pragma unroll
         for (int b = 0; b < 2; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
     for (int k0 = kb; k0 < ke; k0 += TK) {
-        for (int e = tid; e < TM * TK; e += 256) {
-            int r, c;
-            if (g.transA) {  // stored [K x M]: walk m fastest for coalescing
-                r = e % TM;
-                c = e / TM;
-            } else {
-                r = e / TK;
-                c = e % TK;
-            }
-            const int gm = m0 + r, gk = k0 + c;
-            uint16_t v = 0;
-            if (gm < g.M && gk < ke)
-                v = g.transA ? g.A[(size_t)gk * g.lda + gm] : g.A[(size_t)gm * g.lda + gk];
-            As[r][c] = v;
-        }
-        for (int e = tid; e < TK * TN; e += 256) {
-            int r, c;
-            if (g.transB) {  // stored [N x K]: walk k fastest
-                r = e % TK;
-                c = e / TK;
-            } else {
-                r = e / TN;
-                c = e % TN;
-            }
-            const int gk = k0 + r, gn = n0 + c;
-            uint16_t v = 0;
-            if (gk < ke && gn < g.N)
-                v = g.transB ? g.B[(size_t)gn * g.ldb + gk] : g.B[(size_t)gk * g.ldb + gn];
-            Bs[c][r] = v;
-        }
+        // each thread stages one 8-element (16-byte) chunk of A and one of B: 64 x 32 = 256
+        // chunks per operand.  A chunk runs along the operand's contiguous dimension (k for
+        // A [M x K] / B [N x K], m or n for the transposed storage) and is loaded with one
+        // vector load when it lies inside the matrix, element by element (zero-filled)
+        // otherwise; the LDS layout (As[m][k], Bs[n][k]) is the same either way.
+        stage_chunk(g.A, g.lda, g.transA, g.M, m0, k0, ke, tid, As);
+        stage_chunk(g.B, g.ldb, g.transB ^ 1, g.N, n0, k0, ke, tid, Bs);
         __syncthreads();
 #pragma unroll
         for (int tm = 0; tm < 2; ++tm) {
@@ -143,8 +159,17 @@ __global__ void k_sum_slabs(int M, int N, int splits, const float* __restrict__ 
                             int ldc) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)M * N) return;
+    const size_t mn = (size_t)M * N;
     float acc = 0.f;
-    for (int z = 0; z < splits; ++z) acc += slab[(size_t)z * M * N + t];
+    int z = 0;
+    for (; z + 8 <= splits; z += 8) {  // eight loads in flight, added in slab order
+        float v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = slab[(size_t)(z + u) * mn + t];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc += v[u];
+    }
+    for (; z < splits; ++z) acc += slab[(size_t)z * mn + t];
     C[(t / N) * ldc + t % N] = acc;
 }
 
@@ -215,13 +240,43 @@ __global__ void k_mse_grad(int64_t n, const float* __restrict__ Yp, const float*
     if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
-// column sums of a bf16 matrix (rows x cols) into f32 (bias gradients)
-__global__ void k_colsum(int64_t rows, int cols, const uint16_t* __restrict__ D, float* __restrict__ out) {
+// Column sums of a bf16 matrix (rows x cols, cols dividing 256) into f32 (bias gradients),
+// in two fixed-order passes: workgroup b sums rows [b R, (b + 1) R) with the 256 / cols
+// threads of each column reading interleaved rows (a row's columns are read together, so
+// the loads are coalesced; eight loads in flight per thread), then one workgroup per
+// column adds the per-workgroup sums (each thread a strided subset, then a fixed-order
+// block sum).
+constexpr int kColsumBlocks = 512;
+__global__ void __launch_bounds__(256) k_colsum_part(int64_t rows, int cols, int64_t R,
+                                                     const uint16_t* __restrict__ D, float* __restrict__ part) {
+    __shared__ float red[256];
+    const int c = threadIdx.x % cols, lanes = 256 / cols, sub = threadIdx.x / cols;
+    const int64_t r0 = (int64_t)blockIdx.x * R, r1 = min(rows, r0 + R);
+    float s = 0.f;
+    int64_t r = r0 + sub;
+    for (; r + 7 * lanes < r1; r += 8 * lanes) {
+        uint16_t v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = D[(r + u * lanes) * cols + c];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += __uint_as_float((uint32_t)v[u] << 16);
+    }
+    for (; r < r1; r += lanes) s += __uint_as_float((uint32_t)D[r * cols + c] << 16);
+    red[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x < cols) {
+        float t = 0.f;
+        for (int k = 0; k < lanes; ++k) t += red[k * cols + threadIdx.x];
+        part[(int64_t)blockIdx.x * cols + threadIdx.x] = t;
+    }
+}
+__global__ void __launch_bounds__(256) k_colsum_final(int nb, int cols, const float* __restrict__ part,
+                                                      float* __restrict__ out) {
     __shared__ float red[256];
     const int c = blockIdx.x;
-    float s = 0.f;
-    for (int64_t r = threadIdx.x; r < rows; r += blockDim.x) s += __uint_as_float((uint32_t)D[r * cols + c] << 16);
-    red[threadIdx.x] = s;
+    float t = 0.f;
+    for (int b = threadIdx.x; b < nb; b += 256) t += part[(int64_t)b * cols + c];
+    red[threadIdx.x] = t;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
         if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
@@ -245,20 +300,33 @@ __global__ void __launch_bounds__(256) k_loss_accum(int64_t nb, const float* __r
     if (threadIdx.x == 0) *acc += red[0] * scale;
 }
 
-__global__ void k_adam(int64_t n, float* __restrict__ p, float* __restrict__ m, float* __restrict__ v,
-                       const float* __restrict__ g, float lr, float b1, float b2, float eps, float c1, float c2,
-                       uint16_t* __restrict__ pb, const uint8_t* __restrict__ keep) {
+// Adam over all eight parameter tensors (four weight matrices, four bias vectors) in one
+// launch: element t belongs to segment k with off[k] <= t < off[k + 1]
+struct AdamSegs {
+    float* p[8];
+    float* m[8];
+    float* v[8];
+    const float* g[8];
+    uint16_t* pb[8];
+    const uint8_t* keep[8];
+    int64_t off[9];
+};
+__global__ void k_adam_all(AdamSegs a, float lr, float b1, float b2, float eps, float c1, float c2) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    if (keep && !keep[t]) return;
-    const float gt = g[t];
-    const float mt = b1 * m[t] + (1.f - b1) * gt;
-    const float vt = b2 * v[t] + (1.f - b2) * gt * gt;
-    m[t] = mt;
-    v[t] = vt;
-    const float pt = p[t] - lr * (mt / c1) / (sqrtf(vt / c2) + eps);
-    p[t] = pt;
-    if (pb) pb[t] = to_bf16(pt);
+    if (t >= a.off[8]) return;
+    int k = 0;
+#pragma unroll
+    for (int j = 1; j < 8; ++j) k += t >= a.off[j] ? 1 : 0;
+    const int64_t i = t - a.off[k];
+    if (a.keep[k] && !a.keep[k][i]) return;
+    const float gt = a.g[k][i];
+    const float mt = b1 * a.m[k][i] + (1.f - b1) * gt;
+    const float vt = b2 * a.v[k][i] + (1.f - b2) * gt * gt;
+    a.m[k][i] = mt;
+    a.v[k][i] = vt;
+    const float pt = a.p[k][i] - lr * (mt / c1) / (sqrtf(vt / c2) + eps);
+    a.p[k][i] = pt;
+    if (a.pb[k]) a.pb[k][i] = to_bf16(pt);
 }
 
 __global__ void k_to_bf16(int64_t n, const float* __restrict__ a, uint16_t* __restrict__ b) {
@@ -465,6 +533,7 @@ void Mlp::ensure(int64_t n) {
     idx.alloc(n);
     partial.alloc(g1(n * MlpDims::OUTP) + 1);
     if (!splitws.p) splitws.alloc((size_t)kMaxSplits * MlpDims::H * MlpDims::H);
+    if (!colpart.p) colpart.alloc((size_t)kColsumBlocks * MlpDims::H);
     loss_dev.alloc(1);
 }
 
@@ -506,7 +575,9 @@ double Mlp::backward_prepared(int64_t n, float* dY_f32, double* loss_acc) {
     hipStream_t s = stream;
     const unsigned nb = g1(n * MlpDims::OUTP);
     hipLaunchKernelGGL(k_mse_grad, dim3(nb), dim3(256), 0, s, n, Yp.p, Tn.p, dYb.p, dY_f32, partial.p);
-    const int splits = (int)std::min<int64_t>(kMaxSplits, std::max<int64_t>(1, n / 2048));
+    // weight gradients: K = n samples split into up to 64 slabs of >= 512 rows, so that the
+    // 128 x 128 outputs (4 tiles) still give 256 workgroups at a batch of 32,768
+    const int splits = (int)std::min<int64_t>(kMaxSplits, std::max<int64_t>(1, n / 512));
     const uint16_t* dcur = dYb.p;  // gradient w.r.t. this layer's pre-activation (bf16)
     for (int l = 3; l >= 0; --l) {
         const int di = MlpDims::din(l), dn = MlpDims::dout(l);
@@ -523,7 +594,12 @@ double Mlp::backward_prepared(int64_t n, float* dY_f32, double* loss_acc) {
         gw.C = gW[l].p;
         gw.ldc = di;
         gemm(s, gw, splits, splitws.p);
-        hipLaunchKernelGGL(k_colsum, dim3(dn), dim3(256), 0, s, n, dn, dcur, gb[l].p);
+        {
+            const int64_t R = std::max<int64_t>(64, (n + kColsumBlocks - 1) / kColsumBlocks);
+            const int nbk = (int)((n + R - 1) / R);
+            hipLaunchKernelGGL(k_colsum_part, dim3(nbk), dim3(256), 0, s, n, dn, R, dcur, colpart.p);
+            hipLaunchKernelGGL(k_colsum_final, dim3(dn), dim3(256), 0, s, nbk, dn, colpart.p, gb[l].p);
+        }
         if (l > 0) {  // dZ_{l-1} = (dZ W) * [Z_{l-1} > 0]   (M=n, N=din, K=dout)
             GemmArgs gd{};
             gd.M = (int)n;
@@ -558,13 +634,19 @@ void Mlp::adam_step(float lr) {
     ++t_adam;
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
     const float c1 = 1.f - std::pow(b1, (float)t_adam), c2 = 1.f - std::pow(b2, (float)t_adam);
+    AdamSegs a{};
+    a.off[0] = 0;
     for (int l = 0; l < 4; ++l) {
         const int64_t nw = (int64_t)MlpDims::din(l) * MlpDims::dout(l), nbias = MlpDims::dout(l);
-        hipLaunchKernelGGL(k_adam, dim3(g1(nw)), dim3(256), 0, stream, nw, W[l].p, mW[l].p, vW[l].p, gW[l].p, lr, b1,
-                           b2, eps, c1, c2, Wb[l].p, l == 3 ? keepW4.p : (const uint8_t*)nullptr);
-        hipLaunchKernelGGL(k_adam, dim3(g1(nbias)), dim3(256), 0, stream, nbias, bvec[l].p, mb[l].p, vb[l].p, gb[l].p,
-                           lr, b1, b2, eps, c1, c2, (uint16_t*)nullptr, l == 3 ? keepb4.p : (const uint8_t*)nullptr);
+        const int kw = 2 * l, kb = 2 * l + 1;
+        a.p[kw] = W[l].p, a.m[kw] = mW[l].p, a.v[kw] = vW[l].p, a.g[kw] = gW[l].p, a.pb[kw] = Wb[l].p;
+        a.keep[kw] = l == 3 ? keepW4.p : nullptr;
+        a.off[kw + 1] = a.off[kw] + nw;
+        a.p[kb] = bvec[l].p, a.m[kb] = mb[l].p, a.v[kb] = vb[l].p, a.g[kb] = gb[l].p, a.pb[kb] = nullptr;
+        a.keep[kb] = l == 3 ? keepb4.p : nullptr;
+        a.off[kb + 1] = a.off[kb] + nbias;
     }
+    hipLaunchKernelGGL(k_adam_all, dim3(g1(a.off[8])), dim3(256), 0, stream, a, lr, b1, b2, eps, c1, c2);
 }
 
 void Mlp::prepare(const float* X, const float* Y, int64_t n, const int64_t* rows) {

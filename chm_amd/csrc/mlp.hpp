@@ -59,6 +59,7 @@ class Mlp {
     DevBuf<uint16_t> Xb, Ab[3], dZb[3], dYb;
     DevBuf<float> Z[3], Yp, Tn, partial;
     DevBuf<float> splitws;  // split-K partial products of the weight gradients
+    DevBuf<float> colpart;  // per-workgroup column sums of the bias gradients
     DevBuf<double> loss_dev;  // the epoch's summed batch losses (training)
     DevBuf<int64_t> idx;
     int64_t t_adam = 0;
