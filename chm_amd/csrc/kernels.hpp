@@ -170,6 +170,7 @@ struct ColorOrder {
     DevBuf<int32_t> nbp;       // [3F] j-major over positions: position of neighbour j, -1 if none
     DevBuf<uint8_t> rev;       // [3F] j-major over positions: edge of neighbour j that points back
     DevBuf<uint8_t> lowmask;   // [F] over positions: bit j = neighbour j has a lower colour
+    DevBuf<uint8_t> edge;      // [F] over positions: slot j holds the neighbour of edge (edge >> 2j) & 3
     std::vector<int32_t> h_color;
 };
 // Zeroes elements [from, n) of a buffer (the pad slots below); at set-up only.

@@ -51,6 +51,7 @@ __global__ void __launch_bounds__(kBlock) k_dilu_factor(int64_t F, int L, int64_
                                                         const int32_t* __restrict__ nbp,
                                                         const uint8_t* __restrict__ rev,
                                                         const uint8_t* __restrict__ lowmask,
+                                                        const uint8_t* __restrict__ edge,
                                                         const int32_t* __restrict__ rp, const T* __restrict__ val,
                                                         const uint16_t* __restrict__ slots, T* __restrict__ lat,
                                                         T* __restrict__ vert, T* __restrict__ dinv,
@@ -61,6 +62,7 @@ __global__ void __launch_bounds__(kBlock) k_dilu_factor(int64_t F, int L, int64_
     const int64_t i = faces[k];
     const uint32_t si = slots[i];
     const uint32_t lm = lowmask[k];
+    const uint32_t ek = edge[k];  // slot f holds the neighbour of edge (ek >> 2f) & 3
     int32_t np_[3], nf[3];
     uint32_t sn[3];
     int rv[3];
@@ -81,7 +83,7 @@ __global__ void __launch_bounds__(kBlock) k_dilu_factor(int64_t F, int L, int64_
         diag[q] = d;
 #pragma unroll
         for (int f = 0; f < 3; ++f) {
-            const T a_in = np_[f] >= 0 ? val[base + nib(si, f + 1)] : T(0);
+            const T a_in = np_[f] >= 0 ? val[base + nib(si, (int)((ek >> (2 * f)) & 3u) + 1)] : T(0);
             lat[f * N + q] = a_in;
             if ((lm >> f) & 1u) {
                 const int64_t rn = (int64_t)z * F + nf[f];
@@ -942,6 +944,17 @@ int face_colors(int64_t F, const int32_t* nb, int32_t* color) {
     return nc;
 }
 
+// Position order of the multicolour D-ILU.  Colours are laid out one after the other.
+// Within a colour, faces are ordered by connected component of the face graph (the members
+// of a batch union stay contiguous) and face index, and then -- for the middle colours, which
+// have both lower- and higher-colour neighbours -- regrouped inside blocks of 1024 faces by
+// how many of their neighbours carry a lower colour.  Each face's three neighbour slots hold its lower-colour neighbours first,
+// then the higher-colour ones, then the absent ones.  The forward sweep reads the lateral
+// couplings of the lower slots and the backward sweep those of the higher slots; with the
+// faces of one count together, the cache lines of a slot's coupling array hold only lower or
+// only higher couplings over each run, so each sweep fetches its own couplings and not the
+// other sweep's.  Faces of one colour are not coupled to each other, so their order does not
+// change the factorisation.
 void build_color_order(ColorOrder& co, int64_t F, const int32_t* nb, hipStream_t s) {
     std::vector<int32_t> color = color_faces(F, nb);
     int nc = 0;
@@ -950,34 +963,86 @@ void build_color_order(ColorOrder& co, int64_t F, const int32_t* nb, hipStream_t
     co.off.assign(nc + 1, 0);
     for (int64_t i = 0; i < F; ++i) co.off[color[i] + 1]++;
     for (int c = 0; c < nc; ++c) co.off[c + 1] += co.off[c];
-    std::vector<int32_t> faces(F), pos(F);
-    std::vector<int64_t> fill(co.off.begin(), co.off.end() - 1);
-    for (int64_t i = 0; i < F; ++i) {
-        const int64_t k = fill[color[i]]++;
-        faces[k] = (int32_t)i;
-        pos[i] = (int32_t)k;
+    // connected components, numbered in the order of their smallest face index
+    std::vector<int32_t> comp(F, -1), stack;
+    int32_t ncomp = 0;
+    for (int64_t i0 = 0; i0 < F; ++i0) {
+        if (comp[i0] >= 0) continue;
+        comp[i0] = ncomp;
+        stack.assign(1, (int32_t)i0);
+        while (!stack.empty()) {
+            const int32_t u = stack.back();
+            stack.pop_back();
+            for (int f = 0; f < 3; ++f) {
+                const int32_t w = nb[3 * (int64_t)u + f];
+                if (w >= 0 && comp[w] < 0) {
+                    comp[w] = ncomp;
+                    stack.push_back(w);
+                }
+            }
+        }
+        ++ncomp;
     }
-    std::vector<int32_t> nbp(3 * F, -1);
-    std::vector<uint8_t> rev(3 * F, 0), low(F, 0);
-    for (int64_t k = 0; k < F; ++k) {
-        const int64_t i = faces[k];
+    std::vector<uint8_t> nlo(F, 0);
+    for (int64_t i = 0; i < F; ++i)
         for (int f = 0; f < 3; ++f) {
             const int32_t n = nb[3 * i + f];
+            if (n >= 0 && color[n] < color[i]) nlo[i]++;
+        }
+    // the regrouping by lower-neighbour count stays inside blocks of kGroupBlock consecutive
+    // faces of a colour: a block covers a compact patch of the mesh, so the neighbours its
+    // faces gather stay within one patch as well
+    constexpr int64_t kGroupBlock = 1024;
+    std::vector<int32_t> faces(F), pos(F), blk(F);
+    for (int64_t i = 0; i < F; ++i) faces[i] = (int32_t)i;
+    std::stable_sort(faces.begin(), faces.end(), [&](int32_t a, int32_t b) {
+        if (color[a] != color[b]) return color[a] < color[b];
+        if (comp[a] != comp[b]) return comp[a] < comp[b];
+        return a < b;
+    });
+    for (int64_t k = 0; k < F; ++k) blk[faces[k]] = (int32_t)((k - co.off[color[faces[k]]]) / kGroupBlock);
+    std::stable_sort(faces.begin(), faces.end(), [&](int32_t a, int32_t b) {
+        if (color[a] != color[b]) return color[a] < color[b];
+        if (comp[a] != comp[b]) return comp[a] < comp[b];
+        if (blk[a] != blk[b]) return blk[a] < blk[b];
+        const bool mid = color[a] > 0 && color[a] < nc - 1;
+        if (mid && nlo[a] != nlo[b]) return nlo[a] < nlo[b];
+        return a < b;
+    });
+    for (int64_t k = 0; k < F; ++k) pos[faces[k]] = (int32_t)k;
+    std::vector<int32_t> nbp(3 * F, -1);
+    std::vector<uint8_t> rev(3 * F, 0), low(F, 0), edge(F, 0);
+    for (int64_t k = 0; k < F; ++k) {
+        const int64_t i = faces[k];
+        // slot order: lower-colour neighbours, higher-colour ones, absent ones (edge order within)
+        int e[3], ne = 0;
+        for (int pass = 0; pass < 3; ++pass)
+            for (int f = 0; f < 3; ++f) {
+                const int32_t n = nb[3 * i + f];
+                const int cls = n < 0 ? 2 : (color[n] < color[i] ? 0 : 1);
+                if (cls == pass) e[ne++] = f;
+            }
+        for (int sl = 0; sl < 3; ++sl) {
+            const int f = e[sl];
+            edge[k] |= (uint8_t)(f << (2 * sl));
+            const int32_t n = nb[3 * i + f];
             if (n < 0) continue;
-            nbp[f * F + k] = pos[n];
+            nbp[sl * F + k] = pos[n];
             for (int g = 0; g < 3; ++g)
-                if (nb[3 * (int64_t)n + g] == (int32_t)i) rev[f * F + k] = (uint8_t)g;
-            if (color[n] < color[i]) low[k] |= (uint8_t)(1u << f);
+                if (nb[3 * (int64_t)n + g] == (int32_t)i) rev[sl * F + k] = (uint8_t)g;
+            if (color[n] < color[i]) low[k] |= (uint8_t)(1u << sl);
         }
     }
     co.faces.alloc(F);
     co.nbp.alloc(3 * F);
     co.rev.alloc(3 * F);
     co.lowmask.alloc(F);
+    co.edge.alloc(F);
     PB_HIP(hipMemcpyAsync(co.faces.p, faces.data(), F * sizeof(int32_t), hipMemcpyHostToDevice, s));
     PB_HIP(hipMemcpyAsync(co.nbp.p, nbp.data(), 3 * F * sizeof(int32_t), hipMemcpyHostToDevice, s));
     PB_HIP(hipMemcpyAsync(co.rev.p, rev.data(), 3 * F, hipMemcpyHostToDevice, s));
     PB_HIP(hipMemcpyAsync(co.lowmask.p, low.data(), F, hipMemcpyHostToDevice, s));
+    PB_HIP(hipMemcpyAsync(co.edge.p, edge.data(), F, hipMemcpyHostToDevice, s));
     PB_HIP(hipStreamSynchronize(s));
     co.h_color = std::move(color);
 }
@@ -989,7 +1054,8 @@ void dilu_factor(hipStream_t s, const ColorOrder& co, int64_t F, int L, const in
         const int64_t k0 = co.off[c], k1 = co.off[c + 1];
         if (k1 == k0) continue;
         hipLaunchKernelGGL((k_dilu_factor<T>), dim3(grid_for(k1 - k0)), dim3(kBlock), 0, s, F, L, k0, k1, co.faces.p,
-                           co.nbp.p, co.rev.p, co.lowmask.p, rp, val, slots, d.lat.p, d.vert.p, d.dinv.p, d.diag.p, d.E.p);
+                           co.nbp.p, co.rev.p, co.lowmask.p, co.edge.p, rp, val, slots, d.lat.p, d.vert.p, d.dinv.p,
+                           d.diag.p, d.E.p);
     }
 }
 
