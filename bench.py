@@ -137,6 +137,9 @@ def host_io_rate(pb, g, cfg, net, forcing, steps, spmv_shape=-1, dilu_mode=-1):
     return steps / dt
 
 
+PROFILE_EVERY = 8  # timed steps: one launch in eight of a reported class is bracketed
+
+
 def run_variant(pb, g, dev_forcing, dev_outs, cfg, net, steps, warmup, torch, spmv_shape=-1, dilu_mode=-1):
     ctx = pb.Context(g, cfg)
     _pin(pb, ctx, cfg, spmv_shape, dilu_mode)
@@ -145,9 +148,10 @@ def run_variant(pb, g, dev_forcing, dev_outs, cfg, net, steps, warmup, torch, sp
     for _ in range(warmup):
         ctx.step(dev_forcing, outs=dev_outs, on_device=True)
     torch.cuda.synchronize()
-    # the timed steps bracket only the classes whose rooflines are reported (each bracket
-    # adds device idle time); two further steps bracket every class for the breakdown
-    ctx.profile(True, classes=timed_classes(pb, cfg))
+    # the timed steps bracket only the classes whose rooflines are reported, and only every
+    # PROFILE_EVERY-th launch of them (each bracket's two event records leave the device idle
+    # for ~6 us); two further steps bracket every launch of every class for the breakdown
+    ctx.profile(True, classes=timed_classes(pb, cfg), every=PROFILE_EVERY)
     ctx.profile_read(reset=True)
     stats = []
     torch.cuda.synchronize()
@@ -161,7 +165,7 @@ def run_variant(pb, g, dev_forcing, dev_outs, cfg, net, steps, warmup, torch, sp
     t1 = time.perf_counter()
     step_ms = [1000.0 * (b - a) for a, b in zip(marks, marks[1:])]
     prof = ctx.profile_read(reset=True)
-    ctx.profile(True)
+    ctx.profile(True, every=1)
     for _ in range(BREAKDOWN_STEPS):
         ctx.step(dev_forcing, outs=dev_outs, on_device=True)
     torch.cuda.synchronize()
@@ -548,7 +552,7 @@ def main():
         rooflines[kname] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": tr, "kernel": f"{kname} ({what})",
                             "algorithmic_bytes": algo, "ms_avg": round(k["ms_avg"], 5), "launches": k["launches"],
-                            "ms_total": round(k["ms_total"], 3)}
+                            "ms_total": round(k["ms_total"], 3), "sampled_every": PROFILE_EVERY}
         if tr:
             rooflines[kname]["traffic_source"] = traffic["source"]
     dom = max(rooflines, key=lambda n: rooflines[n]["ms_total"]) if rooflines else None

@@ -826,6 +826,13 @@ int pbsm3d_profile_classes(pbsm3d_ctx* c, uint32_t mask) {
     return PBSM3D_OK;
 }
 
+int pbsm3d_profile_sample(pbsm3d_ctx* c, int every) {
+    if (!c || every < 1) return PBSM3D_EINVAL;
+    c->prof.every = every;
+    for (int64_t& n : c->prof.seen) n = 0;
+    return PBSM3D_OK;
+}
+
 int pbsm3d_profile_read(pbsm3d_ctx* c, int kernel, int64_t* launches, double* ms_total, int reset) {
     if (!c || kernel < 0 || kernel >= PBSM3D_K_COUNT) return PBSM3D_EINVAL;
     if (launches) *launches = c->prof.count[kernel];

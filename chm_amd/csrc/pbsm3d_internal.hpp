@@ -92,6 +92,8 @@ struct ColumnFactors {
 struct Profiler {
     bool on = false;
     uint32_t mask = ~0u;  // bit k: class k is bracketed (each pair of events costs a few us)
+    int every = 1;        // bracket only every every-th launch of each class
+    int64_t seen[PBSM3D_K_COUNT] = {};
     std::vector<hipEvent_t> pool;
     std::vector<int> ids;
     size_t used = 0;
@@ -102,6 +104,7 @@ struct Profiler {
     }
     bool begin(hipStream_t s, int id) {
         if (!on || !((mask >> id) & 1u)) return false;
+        if (seen[id]++ % every != 0) return false;
         while (pool.size() < used + 2) {
             hipEvent_t e;
             PB_HIP(hipEventCreate(&e));

@@ -30,6 +30,7 @@ EXPORTED = (
     "pbsm3d_profile_enable",
     "pbsm3d_profile_read",
     "pbsm3d_profile_classes",
+    "pbsm3d_profile_sample",
     "pbsm3d_spmv_tuning",
     "pbsm3d_spmv_select",
     "pbsm3d_face_colors",
@@ -136,6 +137,7 @@ def lib():
         L.pbsm3d_profile_enable.argtypes = [vp, C.c_int]
         L.pbsm3d_profile_read.argtypes = [vp, C.c_int, C.POINTER(C.c_int64), dp, C.c_int]
         L.pbsm3d_profile_classes.argtypes = [vp, C.c_uint32]
+        L.pbsm3d_profile_sample.argtypes = [vp, C.c_int]
         L.pbsm3d_spmv_tuning.argtypes = [vp, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_float), C.c_int32]
         L.pbsm3d_spmv_select.argtypes = [vp, C.c_int, C.c_int]
         L.pbsm3d_face_colors.argtypes = [C.c_int64, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
@@ -303,12 +305,14 @@ class Context:
                self._h)
         return rp, col, val, rhs
 
-    def profile(self, enable: bool = True, classes=None):
+    def profile(self, enable: bool = True, classes=None, every: int = 1):
         """Turn HIP-event timing of the kernel classes on or off; classes: the names
-        (KERNEL_CLASSES) to bracket, None = all."""
+        (KERNEL_CLASSES) to bracket, None = all; every: bracket only every every-th launch
+        of each class."""
         mask = (1 << len(KERNEL_CLASSES)) - 1 if classes is None else sum(
             1 << KERNEL_CLASSES.index(n) for n in classes)
         _check(lib().pbsm3d_profile_classes(self._h, mask), self._h)
+        _check(lib().pbsm3d_profile_sample(self._h, every), self._h)
         _check(lib().pbsm3d_profile_enable(self._h, 1 if enable else 0), self._h)
 
     def profile_read(self, reset: bool = True) -> dict:

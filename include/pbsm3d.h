@@ -261,6 +261,10 @@ int pbsm3d_profile_read(pbsm3d_ctx* ctx, int kernel, int64_t* launches, double* 
  * microseconds of device idle time, so a timing run brackets only the classes
  * it reports. */
 int pbsm3d_profile_classes(pbsm3d_ctx* ctx, uint32_t mask);
+/* Brackets only every `every`-th launch of each class (default 1: all of them); the counts
+ * and times read back are those of the bracketed launches, so launches are homogeneous
+ * samples of their class while the idle time of the event records drops by that factor. */
+int pbsm3d_profile_sample(pbsm3d_ctx* ctx, int every);
 
 /* Launch shape of the suspension SpMV (no counterpart in the reference).  With
  * the multicolour D-ILU preconditioner the suspension solve runs in the
