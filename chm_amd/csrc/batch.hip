@@ -712,6 +712,7 @@ struct BatchGroup {
     DevBuf<double> spmv_part_s, spmv_part_d, fac;
     DevBuf<int32_t> face_member, member_on;
     bool stepped = false;  // a step has run: the staged forcing and solutions are this step's
+    bool pairs_ready = false;  // c->feat / c->pred hold this step's training pairs of the union
     ~BatchGroup() {
         if (ctx) pbsm3d_destroy(ctx);
     }
@@ -891,6 +892,7 @@ void group_step(pbsm3d_batch* B, BatchGroup& g, const pbsm3d_forcing* fo, int32_
     const int L = c->L;
     const int64_t N = F * L;
     const int M = (int)g.members.size();
+    g.stepped = g.pairs_ready = false;  // until this step completes
     PB_HIP(hipSetDevice(c->device));
     PB_HIP(hipEventRecord(c->ev[0], s));
     const double* fdev[9];
@@ -987,6 +989,7 @@ void group_step(pbsm3d_batch* B, BatchGroup& g, const pbsm3d_forcing* fo, int32_
     PB_HIP(hipEventRecord(c->ev[4], s));
     PB_HIP(hipStreamSynchronize(s));
     g.stepped = true;
+    g.pairs_ready = false;
     for (int m = 0; m < M; ++m) {
         pbsm3d_stats& o = st[g.members[m]];
         o = pbsm3d_stats{};
@@ -1217,11 +1220,14 @@ int pbsm3d_batch_training_pairs(pbsm3d_batch* B, int32_t m, float* X, float* Y) 
         if (!g.stepped) throw Error(PBSM3D_EINVAL, "no batch step has run yet");
         PB_HIP(hipSetDevice(c->device));
         const int64_t F = c->F;
-        c->feat.alloc(F * MlpDims::IN);
-        c->pred.alloc(F * MlpDims::OUT);
-        // the union's features (neighbour indices stay inside each member) and targets,
-        // then the member's rows; vw_dir and t are the step's staged forcing
-        training_pairs_into(c, c->forcing_stage.p + 2 * F, c->forcing_stage.p + 4 * F, c->feat.p, c->pred.p);
+        if (!g.pairs_ready) {
+            // the union's features (neighbour indices stay inside each member) and targets,
+            // once per step; vw_dir and t are the step's staged forcing
+            c->feat.alloc(F * MlpDims::IN);
+            c->pred.alloc(F * MlpDims::OUT);
+            training_pairs_into(c, c->forcing_stage.p + 2 * F, c->forcing_stage.p + 4 * F, c->feat.p, c->pred.p);
+            g.pairs_ready = true;
+        }
         PB_HIP(hipMemcpyAsync(X, c->feat.p + off * MlpDims::IN, n * MlpDims::IN * sizeof(float),
                               hipMemcpyDeviceToDevice, c->stream));
         PB_HIP(hipMemcpyAsync(Y, c->pred.p + off * MlpDims::OUT, n * MlpDims::OUT * sizeof(float),

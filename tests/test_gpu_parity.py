@@ -398,9 +398,18 @@ def test_every_split_sweep_mode(pb, oracle, mesh_c1, precision):
     ctx.profile(True, classes=("precond",))
     ctx.step(fo)
     prof = ctx.profile_read(reset=True)
-    ctx.profile(False)
     assert prof["precond"][0] > 0 and prof["precond"][1] > 0, prof
     assert all(n == 0 for k, (n, _) in prof.items() if k != "precond"), prof
+    # sampling: one launch in four of the class is bracketed (the same step, so the same
+    # number of launches), with an average duration of the same size
+    ctx.profile(True, classes=("precond",), every=4)
+    ctx.step(fo)
+    prof4 = ctx.profile_read(reset=True)
+    ctx.profile(False)
+    n1, n4 = prof["precond"][0], prof4["precond"][0]
+    assert n4 == (n1 + 3) // 4, (n1, n4)
+    avg1, avg4 = prof["precond"][1] / n1, prof4["precond"][1] / n4
+    assert 0.5 < avg4 / avg1 < 2.0, (avg1, avg4)
 
 
 def test_c3_mixed_precision_96_steps(pb, oracle, mesh_c1):
