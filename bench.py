@@ -340,8 +340,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU design is described in "
-              f"DESIGN.md and no multi-GPU figure is produced.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): the step is built and measured for "
+              f"one MI355X, and no multi-GPU figure is produced.", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
