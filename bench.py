@@ -137,6 +137,30 @@ def host_io_rate(pb, g, cfg, net, forcing, steps, spmv_shape=-1, dilu_mode=-1):
     return steps / dt
 
 
+def stream_probe(torch, mb: int = 1000, reps: int = 5) -> dict:
+    """GB/s (read + written bytes, best of reps) of two plain streaming kernels on `mb` MB
+    operands: torch's device-to-device copy (1 read : 1 write) and c = a + b (2 : 1)."""
+    n = mb * 1000000 // 4
+    a = torch.empty(n, dtype=torch.float32, device="cuda:0").fill_(1.0)
+    b = torch.empty_like(a).fill_(2.0)
+    c = torch.empty_like(a)
+    out = {}
+    for name, fn, nbytes in (("copy", lambda: c.copy_(a), 2 * 4 * n),
+                             ("add", lambda: torch.add(a, b, out=c), 3 * 4 * n)):
+        fn()
+        best = 0.0
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            e1.synchronize()
+            best = max(best, nbytes / (e0.elapsed_time(e1) * 1e-3) / 1e9)
+        out[name] = round(best, 1)
+    del a, b, c
+    return out
+
+
 PROFILE_EVERY = 8  # timed steps: one launch in eight of a reported class is bracketed
 
 
@@ -557,6 +581,11 @@ def main():
             rooflines[kname]["traffic_source"] = traffic["source"]
     dom = max(rooflines, key=lambda n: rooflines[n]["ms_total"]) if rooflines else None
     roofline = rooflines.get(dom)
+    if roofline is not None:
+        # context for frac: what plain streaming kernels reach on this device (1 GB operands)
+        stream = stream_probe(torch)
+        roofline["measured_stream_gbs"] = stream
+        roofline["frac_of_measured_stream"] = round(roofline["achieved"] / max(stream.values()), 4)
     # BiCGStab vector work (p; alpha + s; omega + x, r; the rho / stop stage), from the untimed
     # steps with every class bracketed: 15 n-vectors of traffic per iteration, suspension
     # (n = N, the inner precision) and deposition (n = F, FP64)
