@@ -325,7 +325,6 @@ SolveInfo solve_in_position_order(pbsm3d_ctx* c, const pbsm3d_config& cfg, int64
     // the learned start was applied (c->susp.x still holds x0 in reference order here)
     CoarseFn coarse;
     if (cfg.coarse_tiles > 0 && cfg.warm_start == PBSM3D_WS_LEARNED && c->warm != nullptr) {
-        coarse_setup(c, cfg.coarse_tiles);
         coarse_prepare(c);
         coarse = [c](hipStream_t st, const double* r, double* x) { coarse_correct(c, st, r, x); };
     }
@@ -559,6 +558,7 @@ int pbsm3d_create(pbsm3d_ctx** out, const pbsm3d_mesh* m, const pbsm3d_config* c
             c->xp.alloc(N);
         }
         if (cfg->dep_precond == PBSM3D_PC_MCILU0) c->dl_d.alloc(F);
+        if (cfg->coarse_tiles > 0) coarse_setup(c.get(), cfg->coarse_tiles, m->neigh);  // tiles + adjacency
         c->invdiag_d.alloc(F);
         c->r_s.alloc(N);
         c->r_d.alloc(F);

@@ -189,27 +189,25 @@ __global__ void __launch_bounds__(kBlock) k_coarse_fill(int64_t n, double v, dou
 
 }  // namespace
 
-void coarse_setup(pbsm3d_ctx* c, int T) {
+void coarse_setup(pbsm3d_ctx* c, int T, const int32_t* nb) {
     CoarseSpace& cs = c->coarse;
     if (cs.T == T) return;
     if (T < 1 || T > kCoarseMaxTiles) throw Error(PBSM3D_EINVAL, "coarse_tiles must be in [0, 256]");
     const int64_t F = c->F, N = F * c->L;
     if (T > F) throw Error(PBSM3D_EINVAL, "coarse_tiles exceeds the face count");
+    if ((int64_t)c->co.h_faces.size() != F) throw Error(PBSM3D_EINVAL, "the coarse correction needs the D-ILU order");
     hipStream_t s = c->stream;
     std::vector<int64_t> off(T + 1);
     for (int a = 0; a <= T; ++a) off[a] = (int64_t)a * F / T;
     std::vector<int32_t> tof(F);
     for (int a = 0; a < T; ++a) std::fill(tof.begin() + off[a], tof.begin() + off[a + 1], a);
-    // tile adjacency through the face neighbours (device neigh is j-major [j F + i])
-    std::vector<int32_t> nb(3 * F), faces(F);
-    PB_HIP(hipMemcpyAsync(nb.data(), c->neigh.p, 3 * F * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    PB_HIP(hipMemcpyAsync(faces.data(), c->co.faces.p, F * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    PB_HIP(hipStreamSynchronize(s));
+    // tile adjacency through the face neighbours (host mesh array, face-major [3 i + j])
+    const std::vector<int32_t>& faces = c->co.h_faces;
     std::vector<std::vector<int32_t>> adj(T);
     for (int a = 0; a < T; ++a) adj[a].push_back(a);
     for (int64_t i = 0; i < F; ++i)
         for (int j = 0; j < 3; ++j) {
-            const int32_t n = nb[j * F + i];
+            const int32_t n = nb[3 * i + j];
             if (n < 0) continue;
             auto& v = adj[tof[i]];
             if (std::find(v.begin(), v.end(), tof[n]) == v.end()) v.push_back(tof[n]);

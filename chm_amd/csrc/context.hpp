@@ -96,10 +96,10 @@ void warm_features(pbsm3d_ctx* c, const double* vw_dir, const double* t);
 void training_pairs_into(pbsm3d_ctx* c, const double* vw_dir, const double* t, float* X, float* Y);
 // x0 of the suspension (which = 0) or deposition (which = 1) solve from c->pred
 void warm_apply(pbsm3d_ctx* c, int which);
-// learned coarse correction: tiles and adjacency (once per tile count), the step's
+// learned coarse correction: tiles and adjacency (at create, from the host mesh), the step's
 // basis and Galerkin inverse (after the learned start wrote x0), and one correction
 // x_pos += W (W^T A W)^-1 W^T r (r_pos the FP64 residual, both in position order)
-void coarse_setup(pbsm3d_ctx* c, int T);
+void coarse_setup(pbsm3d_ctx* c, int T, const int32_t* neigh_face_major);
 void coarse_prepare(pbsm3d_ctx* c);
 void coarse_correct(pbsm3d_ctx* c, hipStream_t s, const double* r_pos, double* x_pos);
 }  // namespace pb

@@ -172,6 +172,7 @@ struct ColorOrder {
     DevBuf<uint8_t> lowmask;   // [F] over positions: bit j = neighbour j has a lower colour
     DevBuf<uint8_t> edge;      // [F] over positions: slot j holds the neighbour of edge (edge >> 2j) & 3
     std::vector<int32_t> h_color;
+    std::vector<int32_t> h_faces;  // host copy of faces: position -> face index
 };
 // Zeroes elements [from, n) of a buffer (the pad slots below); at set-up only.
 template <typename T>

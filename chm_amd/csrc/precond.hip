@@ -1045,6 +1045,7 @@ void build_color_order(ColorOrder& co, int64_t F, const int32_t* nb, hipStream_t
     PB_HIP(hipMemcpyAsync(co.edge.p, edge.data(), F, hipMemcpyHostToDevice, s));
     PB_HIP(hipStreamSynchronize(s));
     co.h_color = std::move(color);
+    co.h_faces = std::move(faces);
 }
 
 template <typename T>

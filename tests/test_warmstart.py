@@ -207,3 +207,41 @@ def test_learned_coarse_correction(trained_small, precision):
     o2, _ = ref.step(fo, want_fields=True)
     for k in ("conc", "Qsusp", "qdep", "drift_mass"):
         assert rel_err(out[k], o2[k]) < 1e-8, (k, rel_err(out[k], o2[k]))
+
+
+def _tile_degree(neigh, T):
+    """Largest number of tiles (itself included) that one face-index tile touches."""
+    F = neigh.shape[0]
+    tof = np.repeat(np.arange(T), np.diff((np.arange(T + 1) * F) // T))
+    i, j = np.nonzero(neigh >= 0)
+    pairs = np.unique(np.stack([tof[i], tof[neigh[i, j]]], axis=1), axis=0)
+    others = pairs[pairs[:, 0] != pairs[:, 1], 0]
+    return int((np.bincount(others, minlength=T) + 1).max())
+
+
+@pytest.mark.gpu
+def test_coarse_tiles_checked_at_create():
+    """The coarse tiles and their adjacency are built when the context is created, so a tile
+    count the mesh cannot carry fails at pbsm3d_create and not at the first step: on the
+    Hilbert-ordered 2,500-point mesh each of 32 face-index tiles touches at most 15 others
+    and the context is made; with the faces renumbered by a random permutation the same 32
+    tiles touch nearly all others and creation fails with PBSM3D_EINVAL."""
+    from chm_amd import pbsm3d as pb
+
+    print("[case] coarse tiles checked at create", flush=True)
+    T = 32
+    mesh = meshgen.synthetic_mesh(2500, seed=0)
+    cfg = pb.default_config(nLayer=10, krylov=abi.KRYLOV_BICGSTAB_SPLIT, warm_start=abi.WS_LEARNED, coarse_tiles=T)
+    assert _tile_degree(mesh.neigh, T) <= 16
+    pb.Context(pb.face_geometry(mesh), cfg).close()
+    F = mesh.n_faces
+    perm = np.random.default_rng(3).permutation(F)
+    inv = np.empty(F, dtype=np.int64)
+    inv[perm] = np.arange(F)
+    nb = mesh.neigh[perm]
+    nb = np.where(nb >= 0, inv[np.maximum(nb, 0)], -1).astype(np.int32)
+    shuffled = meshgen.Mesh(mesh.vertex, mesh.elem[perm], nb, None if mesh.area is None else mesh.area[perm])
+    assert _tile_degree(shuffled.neigh, T) > 16
+    with pytest.raises(pb.Pbsm3dError) as e:
+        pb.Context(pb.face_geometry(shuffled), cfg)
+    assert e.value.code == abi.PBSM3D_EINVAL and "coarse tile" in str(e.value)
