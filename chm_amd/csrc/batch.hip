@@ -370,17 +370,48 @@ void build_segments(Segments& sg, const ColorOrder& co, const std::vector<int32_
         }
     }
     sg.sweep.total = off;
-    // element chunks, member-major: member m, layer z, colour c
+    // element chunks, member-major: the runs of consecutive rows (position order: tiles of
+    // kTile positions, each tile layer by layer) that belong to one member, walked in row
+    // order and cut at kChunk rows; a tile inside one member's run of a colour is one run
+    std::vector<int32_t> member_of(F);
+    for (int c = 0; c < nc; ++c)
+        for (int m = 0; m < M; ++m) {
+            const int64_t k0 = kb[(size_t)c * M + m];
+            std::fill(member_of.begin() + k0, member_of.begin() + k0 + cnt[(size_t)c * M + m], (int32_t)m);
+        }
+    std::vector<std::vector<int4>> per(M);
+    auto add_run = [&](int m, int64_t q0, int64_t q1) {
+        std::vector<int4>& v = per[m];
+        while (q0 < q1) {
+            if (!v.empty() && v.back().y == q0 && v.back().y - v.back().x < kChunk) {
+                const int64_t e = std::min<int64_t>(q1, (int64_t)v.back().x + kChunk);
+                v.back().y = (int)e;
+                q0 = e;
+            } else {
+                const int64_t e = std::min<int64_t>(q1, q0 + kChunk);
+                v.push_back(make_int4((int)q0, (int)e, m, 0));
+                q0 = e;
+            }
+        }
+    };
+    for (int64_t k0 = 0; k0 < F; k0 += kTile) {
+        const int64_t w = std::min<int64_t>(kTile, F - k0);
+        for (int z = 0; z < L; ++z) {
+            const int64_t q0 = pos_row(k0, z, F, L);
+            for (int64_t j = 0; j < w;) {
+                const int m = member_of[k0 + j];
+                int64_t e = j + 1;
+                while (e < w && member_of[k0 + e] == m) ++e;
+                add_run(m, q0 + j, q0 + e);
+                j = e;
+            }
+        }
+    }
     std::vector<int4> ch;
     std::vector<int64_t> cb(M + 1, 0);
     for (int m = 0; m < M; ++m) {
         cb[m] = (int64_t)ch.size();
-        for (int z = 0; z < L; ++z)
-            for (int c = 0; c < nc; ++c) {
-                const int64_t q0 = (int64_t)z * F + kb[(size_t)c * M + m], n = cnt[(size_t)c * M + m];
-                for (int64_t k = 0; k < n; k += kChunk)
-                    ch.push_back(make_int4((int)(q0 + k), (int)(q0 + std::min<int64_t>(k + kChunk, n)), m, 0));
-            }
+        ch.insert(ch.end(), per[m].begin(), per[m].end());
     }
     cb[M] = (int64_t)ch.size();
     sg.nchunk = (int64_t)ch.size();

@@ -169,14 +169,16 @@ __global__ void __launch_bounds__(kBlock) k_coarse_solve(int T, const double* __
     y[i] = acc;
 }
 
-// x += W y in position order: x[z F + k] += wpos[z F + k] y[tile of the face at position k]
+// x += W y in position order: x[q] += wpos[q] y[tile of the face at position k] for the
+// rows q = pos_row(k, z) of position k
 __global__ void __launch_bounds__(kBlock) k_coarse_prolong(int64_t F, int L, const int32_t* __restrict__ tile_of_pos,
                                                            const double* __restrict__ wpos,
                                                            const double* __restrict__ y, double* __restrict__ x) {
     for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < F; k += (int64_t)gridDim.x * blockDim.x) {
         const double yk = y[tile_of_pos[k]];
+        const int64_t qb = pos_base64(k, L), qs = pos_stride64(k, F);
         for (int z = 0; z < L; ++z) {
-            const int64_t q = (int64_t)z * F + k;
+            const int64_t q = qb + (int64_t)z * qs;
             x[q] = fma(wpos[q], yk, x[q]);
         }
     }

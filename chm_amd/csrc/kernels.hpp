@@ -49,6 +49,39 @@ __device__ __forceinline__ int64_t logical_block(int64_t nb) {
     return (b % kXcd) * per + b / kXcd;
 }
 
+// Row layout of everything held in position order (the D-ILU factors, the position-order
+// operator and the Krylov vectors of the position-order solves).  Positions k are cut into
+// tiles of kTile = 64 (the last tile holds F mod 64 positions when that is not zero) and
+// a tile's rows are stored layer by layer: row (k, z) sits at pos_base(k) + z * pos_stride(k).
+// A column kernel's wave (64 consecutive positions of one tile) then reads each array as
+// one run of 64 L rows, and the layers of a face column lie 64 rows apart instead of F.
+// With every column in its own layer plane 4 F bytes apart (q = z F + k), a colour launch
+// touched K arrays x L planes per wave and streamed at 55-75 % of the rate it reaches with
+// the tiles.  The map is a bijection onto [0, F L); for L = 1 it is the identity.
+constexpr int kTileShift = 6;
+constexpr int kTile = 1 << kTileShift;
+__host__ __device__ __forceinline__ uint32_t pos_base(uint32_t k, uint32_t L) {
+    return (((k >> kTileShift) * L) << kTileShift) + (k & (kTile - 1));
+}
+__host__ __device__ __forceinline__ uint32_t pos_stride(uint32_t k, uint32_t F) {
+    return (k >> kTileShift) < (F >> kTileShift) ? (uint32_t)kTile : (F & (kTile - 1));
+}
+// the same in 64-bit arithmetic (kernels that do not assume 3 F L < 2^32)
+__host__ __device__ __forceinline__ int64_t pos_base64(int64_t k, int64_t L) {
+    return (((k >> kTileShift) * L) << kTileShift) + (k & (kTile - 1));
+}
+__host__ __device__ __forceinline__ int64_t pos_stride64(int64_t k, int64_t F) {
+    return (k >> kTileShift) < (F >> kTileShift) ? (int64_t)kTile : (F & (kTile - 1));
+}
+__host__ __device__ __forceinline__ int64_t pos_row(int64_t k, int z, int64_t F, int64_t L) {
+    return pos_base64(k, L) + (int64_t)z * pos_stride64(k, F);
+}
+// First position of the grid of a column launch over positions [k0, k1): k0 rounded down to
+// its tile, so that every wave stays inside one tile (threads below k0 do no work).
+__host__ __device__ __forceinline__ int64_t tile_floor(int64_t k0) { return k0 & ~(int64_t)(kTile - 1); }
+// workgroups of B threads of a column launch over positions [k0, k1)
+inline int64_t col_blocks(int64_t k0, int64_t k1, int B = kBlock) { return (k1 - tile_floor(k0) + B - 1) / B; }
+
 struct AssembleArgs {
     int64_t F;
     int L;
@@ -161,8 +194,9 @@ void pc_jacobi(hipStream_t s, int64_t n, const double* invdiag, const T* r, T* z
 // order ILU(0) only modifies the diagonal, so the factors are the strict lower
 // and upper parts of A plus d~_r = a_rr - sum_{k<r} a_rk a_kr / d~_k.
 // Faces get a colour-major position k (colour by colour, ascending face id
-// inside a colour); the factors live in position order so that every colour
-// phase reads and writes contiguous memory.
+// inside a colour); the factors live in position order (rows at pos_base(k) +
+// z pos_stride(k), tiles of 64 positions) so that every colour phase reads and
+// writes contiguous runs.
 struct ColorOrder {
     int ncolor = 0;
     std::vector<int64_t> off;  // positions of colour c: [off[c], off[c+1])
@@ -179,7 +213,7 @@ template <typename T>
 inline void zero_tail(DevBuf<T>& b, size_t from) {
     if (b.p && b.n > from) PB_HIP(hipMemset(b.p + from, 0, (b.n - from) * sizeof(T)));
 }
-// D-ILU factors in position order; index q = z*F + k.  lat, y and t carry one zero pad slot
+// D-ILU factors in position order; row q = pos_base(k) + z pos_stride(k).  lat, y and t carry one zero pad slot
 // past their last entry (lat[3N], y[N], t[N]): the split operator's sweeps read an absent
 // neighbour there instead of branching round the load.
 template <typename T>
@@ -210,7 +244,7 @@ int face_colors(int64_t F, const int32_t* neigh_fm, int32_t* color);
 template <typename T>
 void dilu_factor(hipStream_t s, const ColorOrder& co, int64_t F, int L, const int32_t* rp, const T* val,
                  const uint16_t* slots, DiluFactors<T>& d);
-// z = M^-1 r.  pos: r and z in position order (q = z*F + k), else in the reference order.
+// z = M^-1 r.  pos: r and z in position order (tiled rows), else in the reference order.
 template <typename T>
 void dilu_apply(hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFactors<T>& d, const T* r, T* z,
                 bool pos = false, bool xcd = false);
@@ -230,7 +264,7 @@ template <typename T>
 void split_op(int which, hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFactors<T>& d, const T* in, T* out,
               int mode);
 // split_op(0) with per-block partial sums partial[2b] = dw.out, partial[2b+1] = out.out; returns
-// the number of blocks b (at most F / kBlock + 8 per colour).  stop (may be NULL): the
+// the number of blocks b (at most F / kBlock + 9 per colour).  stop (may be NULL): the
 // launches return at entry when *stop != 0 (a BiCGStab run that has finished).
 template <typename T>
 int64_t split_apply_dots(hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFactors<T>& d, const T* in, T* out,
@@ -287,7 +321,7 @@ template <typename T>
 int tune_pos_spmv(hipStream_t s, const ColorOrder& co, int64_t F, int L, const DiluFactors<T>& d, const T* x, T* y,
                   float* ms_out);
 inline int64_t pos_spmv_blocks(int64_t F, int L) { return ((F + kBlock - 1) / kBlock) * L; }
-// out[z*F + k] = in[z*F + face(k)]  /  out[z*F + face(k)] = in[z*F + k]
+// out[pos_row(k, z)] = in[z*F + face(k)]  /  out[z*F + face(k)] = in[pos_row(k, z)]
 template <typename Ti, typename To>
 void to_pos(hipStream_t s, const ColorOrder& co, int64_t F, int L, const Ti* in, To* out);
 template <typename Ti, typename To>

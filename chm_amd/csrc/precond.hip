@@ -12,7 +12,9 @@
 //
 // Layout: faces are given colour-major positions k, and the factor data
 // (lateral and vertical couplings, 1/d~, the intermediate y) are stored at
-// q = z*F + k, so a colour phase touches one contiguous stretch per layer.
+// row pos_base(k) + z pos_stride(k): tiles of 64 positions, each tile's rows
+// layer by layer, so a wave of a colour phase touches one run of 64 L rows
+// per array.
 // The factorisation gathers each row from the CSR once per step; the apply
 // gathers r and scatters z once each.  The same kernels serve the 2-D
 // deposition system (one layer).
@@ -66,6 +68,7 @@ __global__ void __launch_bounds__(kBlock) k_dilu_factor(int64_t F, int L, int64_
     int32_t np_[3], nf[3];
     uint32_t sn[3];
     int rv[3];
+    int64_t nq[3], ns[3];  // rows of the lower neighbours' columns: nq + z ns
 #pragma unroll
     for (int f = 0; f < 3; ++f) {
         np_[f] = nbp[f * F + k];
@@ -73,11 +76,14 @@ __global__ void __launch_bounds__(kBlock) k_dilu_factor(int64_t F, int L, int64_
         nf[f] = low ? faces[np_[f]] : 0;
         sn[f] = low ? slots[nf[f]] : 0u;
         rv[f] = low ? rev[f * F + k] : 0;
+        nq[f] = low ? pos_base64(np_[f], L) : 0;
+        ns[f] = low ? pos_stride64(np_[f], F) : 0;
     }
+    const int64_t qb = pos_base64(k, L), qs = pos_stride64(k, F);
     T dprev = 0;
     for (int z = 0; z < L; ++z) {
         const int64_t r = (int64_t)z * F + i;
-        const int64_t q = (int64_t)z * F + k;
+        const int64_t q = qb + (int64_t)z * qs;
         const int64_t base = (int64_t)rp[r] + (z > 0 ? 1 : 0);
         T d = val[base + nib(si, 0)];
         diag[q] = d;
@@ -89,7 +95,7 @@ __global__ void __launch_bounds__(kBlock) k_dilu_factor(int64_t F, int L, int64_
                 const int64_t rn = (int64_t)z * F + nf[f];
                 const int64_t bn = (int64_t)rp[rn] + (z > 0 ? 1 : 0);
                 const T a_ni = val[bn + nib(sn[f], rv[f] + 1)];
-                d -= a_in * a_ni * dinv[(int64_t)z * F + np_[f]];
+                d -= a_in * a_ni * dinv[nq[f] + (int64_t)z * ns[f]];
             }
         }
         const T a_ib = z > 0 ? val[rp[r]] : T(0);            // (r, r-F): first entry of row r
@@ -108,7 +114,7 @@ __global__ void __launch_bounds__(kBlock) k_dilu_factor(int64_t F, int L, int64_
 }
 
 // Forward sweep of one colour: y = D~^-1 (r - L y).  POS: r is in position
-// order (q = z*F + k) rather than in the reference order (z*F + face).
+// order (tiled rows) rather than in the reference order (z*F + face).
 // Neighbours of a face carry other colours, so the y values this launch reads
 // (ynb, at neighbour positions) and writes (y, own positions) are disjoint
 // elements of one array; the two restrict pointers let the loads run ahead
@@ -126,22 +132,31 @@ __global__ void __launch_bounds__(kBlock) k_dilu_fwd(int64_t F, int L, int64_t k
                                                      const T* __restrict__ vert, const T* __restrict__ dinv,
                                                      const T* __restrict__ rin, const T* __restrict__ ynb,
                                                      T* __restrict__ y, T* __restrict__ zout) {
-    const int64_t k = k0 + logical_block<XCD>((k1 - k0 + kBlock - 1) / kBlock) * kBlock + threadIdx.x;
-    if (k >= k1) return;
+    const int64_t kb = tile_floor(k0);
+    const int64_t k = kb + logical_block<XCD>((k1 - kb + kBlock - 1) / kBlock) * kBlock + threadIdx.x;
+    if (k < k0 || k >= k1) return;
     const int64_t N = F * L;
     const int64_t i = POS ? k : (int64_t)faces[k];
     const uint32_t lm = lowmask[k];
+    const int64_t qb = pos_base64(k, L), qs = pos_stride64(k, F);
     int32_t np_[3];
+    int64_t nq[3], ns[3];
 #pragma unroll
-    for (int f = 0; f < 3; ++f) np_[f] = ((lm >> f) & 1u) ? nbp[f * F + k] : -1;
+    for (int f = 0; f < 3; ++f) {
+        np_[f] = ((lm >> f) & 1u) ? nbp[f * F + k] : -1;
+        nq[f] = np_[f] >= 0 ? pos_base64(np_[f], L) : 0;
+        ns[f] = np_[f] >= 0 ? pos_stride64(np_[f], F) : 0;
+    }
+    // row z of r / z: position order (POS) or the reference order
+    auto io = [&](int z) { return POS ? qb + (int64_t)z * qs : (int64_t)z * F + i; };
     auto independent = [&](int z) {
-        const int64_t q = (int64_t)z * F + k;
-        T acc = rin[(int64_t)z * F + i];
+        const int64_t q = qb + (int64_t)z * qs;
+        T acc = rin[io(z)];
 #pragma unroll
         for (int f = 0; f < 3; ++f)
         {
             const bool on = np_[f] >= 0;
-            acc -= *(on ? &lat[f * N + q] : zero_ptr<T>()) * *(on ? &ynb[(int64_t)z * F + np_[f]] : zero_ptr<T>());
+            acc -= *(on ? &lat[f * N + q] : zero_ptr<T>()) * *(on ? &ynb[nq[f] + (int64_t)z * ns[f]] : zero_ptr<T>());
         }
         return acc;
     };
@@ -150,7 +165,7 @@ __global__ void __launch_bounds__(kBlock) k_dilu_fwd(int64_t F, int L, int64_t k
 #pragma unroll
         for (int z = 0; z < LM; ++z)
             if (z < L) {
-                const int64_t q = (int64_t)z * F + k;
+                const int64_t q = qb + (int64_t)z * qs;
                 sv[z] = independent(z);
                 vv[z] = z > 0 ? vert[q] : T(0);
                 dv[z] = dinv[q];
@@ -164,7 +179,7 @@ __global__ void __launch_bounds__(kBlock) k_dilu_fwd(int64_t F, int L, int64_t k
                 if (TOP)
                     sv[z] = prev;
                 else
-                    y[(int64_t)z * F + k] = prev;
+                    y[qb + (int64_t)z * qs] = prev;
             }
         if (TOP) {
             T next = 0;
@@ -172,14 +187,14 @@ __global__ void __launch_bounds__(kBlock) k_dilu_fwd(int64_t F, int L, int64_t k
             for (int z = LM - 1; z >= 0; --z)
                 if (z < L) {
                     next = sv[z] - dv[z] * (T(0) + va[z] * next);
-                    if (!POS) y[(int64_t)z * F + k] = next;
-                    zout[(int64_t)z * F + i] = next;
+                    if (!POS) y[qb + (int64_t)z * qs] = next;
+                    zout[io(z)] = next;
                 }
         }
     } else {
         T prev = 0;
         for (int z = 0; z < L; ++z) {
-            const int64_t q = (int64_t)z * F + k;
+            const int64_t q = qb + (int64_t)z * qs;
             T acc = independent(z);
             if (z > 0) acc -= vert[q] * prev;
             prev = acc * dinv[q];
@@ -188,12 +203,12 @@ __global__ void __launch_bounds__(kBlock) k_dilu_fwd(int64_t F, int L, int64_t k
         if (TOP) {
             T next = 0;
             for (int z = L - 1; z >= 0; --z) {
-                const int64_t q = (int64_t)z * F + k;
+                const int64_t q = qb + (int64_t)z * qs;
                 T acc = 0;
                 if (z < L - 1) acc += vert[N + q] * next;
                 next = y[q] - dinv[q] * acc;
                 if (!POS) y[q] = next;
-                zout[(int64_t)z * F + i] = next;
+                zout[io(z)] = next;
             }
         }
     }
@@ -211,25 +226,31 @@ __global__ void __launch_bounds__(kBlock) k_dilu_bwd(int64_t F, int L, int64_t k
                                                      const T* __restrict__ vert, const T* __restrict__ dinv,
                                                      T* __restrict__ y, const T* __restrict__ done,
                                                      T* __restrict__ zout) {
-    const int64_t k = k0 + logical_block<XCD>((k1 - k0 + kBlock - 1) / kBlock) * kBlock + threadIdx.x;
-    if (k >= k1) return;
+    const int64_t kb = tile_floor(k0);
+    const int64_t k = kb + logical_block<XCD>((k1 - kb + kBlock - 1) / kBlock) * kBlock + threadIdx.x;
+    if (k < k0 || k >= k1) return;
     const int64_t N = F * L;
     const int64_t i = POS ? k : (int64_t)faces[k];
     const uint32_t lm = lowmask[k];
+    const int64_t qb = pos_base64(k, L), qs = pos_stride64(k, F);
     int32_t np_[3];
+    int64_t nq[3], ns[3];
 #pragma unroll
     for (int f = 0; f < 3; ++f) {
         const int32_t n = nbp[f * F + k];
         np_[f] = (n >= 0 && !((lm >> f) & 1u)) ? n : -1;
+        nq[f] = np_[f] >= 0 ? pos_base64(np_[f], L) : 0;
+        ns[f] = np_[f] >= 0 ? pos_stride64(np_[f], F) : 0;
     }
+    auto io = [&](int z) { return POS ? qb + (int64_t)z * qs : (int64_t)z * F + i; };
     auto independent = [&](int z) {
-        const int64_t q = (int64_t)z * F + k;
+        const int64_t q = qb + (int64_t)z * qs;
         T acc = 0;
 #pragma unroll
         for (int f = 0; f < 3; ++f)
         {
             const bool on = np_[f] >= 0;
-            acc += *(on ? &lat[f * N + q] : zero_ptr<T>()) * *(on ? &done[(int64_t)z * F + np_[f]] : zero_ptr<T>());
+            acc += *(on ? &lat[f * N + q] : zero_ptr<T>()) * *(on ? &done[nq[f] + (int64_t)z * ns[f]] : zero_ptr<T>());
         }
         return acc;
     };
@@ -238,7 +259,7 @@ __global__ void __launch_bounds__(kBlock) k_dilu_bwd(int64_t F, int L, int64_t k
 #pragma unroll
         for (int z = 0; z < LM; ++z)
             if (z < L) {
-                const int64_t q = (int64_t)z * F + k;
+                const int64_t q = qb + (int64_t)z * qs;
                 av[z] = independent(z);
                 vv[z] = z < L - 1 ? vert[N + q] : T(0);
                 dv[z] = dinv[q];
@@ -249,18 +270,18 @@ __global__ void __launch_bounds__(kBlock) k_dilu_bwd(int64_t F, int L, int64_t k
         for (int z = LM - 1; z >= 0; --z)
             if (z < L) {
                 next = yv[z] - dv[z] * (av[z] + vv[z] * next);
-                if (!POS) y[(int64_t)z * F + k] = next;
-                zout[(int64_t)z * F + i] = next;
+                if (!POS) y[qb + (int64_t)z * qs] = next;
+                zout[io(z)] = next;
             }
     } else {
         T next = 0;
         for (int z = L - 1; z >= 0; --z) {
-            const int64_t q = (int64_t)z * F + k;
+            const int64_t q = qb + (int64_t)z * qs;
             T acc = independent(z);
             if (z < L - 1) acc += vert[N + q] * next;
             next = y[q] - dinv[q] * acc;
             if (!POS) y[q] = next;
-            zout[(int64_t)z * F + i] = next;
+            zout[io(z)] = next;
         }
     }
 }
@@ -290,25 +311,29 @@ __device__ __forceinline__ void eis_bwd_column(int64_t F, int Lrt, int64_t k, co
     const int L = EX ? LM : Lrt;  // EX: the layer count is the unroll bound, no guards
     const uint32_t Fu = (uint32_t)F, ku = (uint32_t)k;
     const uint32_t N = Fu * (uint32_t)L;
+    const uint32_t qb = pos_base(ku, (uint32_t)L), qs = pos_stride(ku, Fu);
     const uint32_t lm = lowmask[k];
     int32_t np_[3];
+    uint32_t nq[3], ns[3];  // rows of the neighbour columns: nq + z ns
 #pragma unroll
     for (int f = 0; f < 3; ++f) {
         const int32_t n = nbp[f * F + k];
         np_[f] = (n >= 0 && !((lm >> f) & 1u)) ? n : -1;
+        nq[f] = np_[f] >= 0 ? pos_base((uint32_t)np_[f], (uint32_t)L) : 0u;
+        ns[f] = np_[f] >= 0 ? pos_stride((uint32_t)np_[f], Fu) : 0u;
     }
     T av[LM], va[LM], dv[LM], wv[LM];
 #pragma unroll
     for (int z = 0; z < LM; ++z)
         if (z < L) {
-            const uint32_t q = (uint32_t)z * Fu + ku;
+            const uint32_t q = qb + (uint32_t)z * qs;
             T acc = 0;
 #pragma unroll
             for (int f = 0; f < 3; ++f)
             {
                 const bool on = np_[f] >= 0;
                 acc += ldc<NT>(&lat[on ? (uint32_t)f * N + q : 3 * N]) *
-                       tnb[on ? (uint32_t)z * Fu + (uint32_t)np_[f] : N];
+                       tnb[on ? nq[f] + (uint32_t)z * ns[f] : N];
             }
             av[z] = acc;
             va[z] = z < L - 1 ? ldc<NT>(&vert[N + q]) : T(0);
@@ -320,7 +345,7 @@ __device__ __forceinline__ void eis_bwd_column(int64_t F, int Lrt, int64_t k, co
     for (int z = LM - 1; z >= 0; --z)
         if (z < L) {
             next = dv[z] * (wv[z] - (av[z] + va[z] * next));
-            t[(uint32_t)z * Fu + ku] = next;
+            t[qb + (uint32_t)z * qs] = next;
         }
 }
 
@@ -332,8 +357,9 @@ __global__ void __launch_bounds__(kBlock) k_eis_bwd(int64_t F, int Lrt, int64_t 
                                                     const T* __restrict__ w, const T* __restrict__ tnb,
                                                     T* __restrict__ t, const double* __restrict__ stop) {
     if (stop && *stop != 0.0) return;  // a finished BiCGStab run (uniform over the grid)
-    const int64_t k = k0 + logical_block<XCD>((k1 - k0 + kBlock - 1) / kBlock) * kBlock + threadIdx.x;
-    if (k >= k1) return;
+    const int64_t kb = tile_floor(k0);
+    const int64_t k = kb + logical_block<XCD>((k1 - kb + kBlock - 1) / kBlock) * kBlock + threadIdx.x;
+    if (k < k0 || k >= k1) return;
     eis_bwd_column<T, LM, EX, NT>(F, Lrt, k, nbp, lowmask, lat, vert, dinv, w, tnb, t);
 }
 
@@ -357,20 +383,24 @@ __device__ __forceinline__ void eis_fwd_column(int64_t F, int Lrt, int64_t k, co
     const int L = EX ? LM : Lrt;  // EX: the layer count is the unroll bound, no guards
     const uint32_t Fu = (uint32_t)F, ku = (uint32_t)k;
     const uint32_t N = Fu * (uint32_t)L;
+    const uint32_t qb = pos_base(ku, (uint32_t)L), qs = pos_stride(ku, Fu);
     const uint32_t lm = lowmask[k];
     int32_t lo[3], up[3];
+    uint32_t nq[3], ns[3];  // rows of the neighbour columns (lower or upper): nq + z ns
 #pragma unroll
     for (int f = 0; f < 3; ++f) {
         const int32_t n = nbp[f * F + k];
         const bool low = (lm >> f) & 1u;
         lo[f] = (n >= 0 && low) ? n : -1;
         up[f] = (n >= 0 && !low) ? n : -1;
+        nq[f] = n >= 0 ? pos_base((uint32_t)n, (uint32_t)L) : 0u;
+        ns[f] = n >= 0 ? pos_stride((uint32_t)n, Fu) : 0u;
     }
     T wv[LM], dv[LM], vb[LM], lv[LM], tv[LM], ev[LM], dwv[LM];
 #pragma unroll
     for (int z = 0; z < LM; ++z)
         if (z < L) {
-            const uint32_t q = (uint32_t)z * Fu + ku;
+            const uint32_t q = qb + (uint32_t)z * qs;
             wv[z] = w[q];
             if (DOTS && PRE_DW) dwv[z] = dw[q];
             dv[z] = ldc<NT>(&dinv[q]);
@@ -382,7 +412,7 @@ __device__ __forceinline__ void eis_fwd_column(int64_t F, int Lrt, int64_t k, co
                 {
                     const bool on = lo[f] >= 0;
                     acc += ldc<NT>(&lat[on ? (uint32_t)f * N + q : 3 * N]) *
-                           vnb[on ? (uint32_t)z * Fu + (uint32_t)lo[f] : N];
+                           vnb[on ? nq[f] + (uint32_t)z * ns[f] : N];
                 }
             }
             lv[z] = acc;
@@ -397,14 +427,14 @@ __device__ __forceinline__ void eis_fwd_column(int64_t F, int Lrt, int64_t k, co
 #pragma unroll
         for (int z = 0; z < LM; ++z)
             if (z < L) {
-                const uint32_t q = (uint32_t)z * Fu + ku;
+                const uint32_t q = qb + (uint32_t)z * qs;
                 T acc = 0;
 #pragma unroll
                 for (int f = 0; f < 3; ++f)
                 {
                     const bool on = up[f] >= 0;
                     acc += ldc<NT>(&lat[on ? (uint32_t)f * N + q : 3 * N]) *
-                           tnb[on ? (uint32_t)z * Fu + (uint32_t)up[f] : N];
+                           tnb[on ? nq[f] + (uint32_t)z * ns[f] : N];
                 }
                 av[z] = acc;
                 va[z] = z < L - 1 ? ldc<NT>(&vert[N + q]) : T(0);
@@ -421,7 +451,7 @@ __device__ __forceinline__ void eis_fwd_column(int64_t F, int Lrt, int64_t k, co
 #pragma unroll
     for (int z = 0; z < LM; ++z)
         if (z < L) {
-            const uint32_t q = (uint32_t)z * Fu + ku;
+            const uint32_t q = qb + (uint32_t)z * qs;
             const T acc = lv[z] + vb[z] * prev;
             T oz, vz;
             if (WITH_T) {
@@ -453,10 +483,11 @@ __global__ void __launch_bounds__(kBlock) k_eis_fwd(int64_t F, int L, int64_t k0
                                                     T* __restrict__ o, const T* __restrict__ dw,
                                                     double* __restrict__ partial, const double* __restrict__ stop) {
     if (stop && *stop != 0.0) return;  // a finished BiCGStab run (uniform over the grid)
-    const int64_t lb = logical_block<XCD>((k1 - k0 + kBlock - 1) / kBlock);
-    const int64_t k = k0 + lb * kBlock + threadIdx.x;
+    const int64_t kb = tile_floor(k0);
+    const int64_t lb = logical_block<XCD>((k1 - kb + kBlock - 1) / kBlock);
+    const int64_t k = kb + lb * kBlock + threadIdx.x;
     double d1 = 0, d2 = 0;
-    if (k < k1)
+    if (k >= k0 && k < k1)
         eis_fwd_column<T, LM, EX, WITH_T, FUSE, DOTS, NT, true, SV>(F, L, k, nbp, lowmask, lat, vert, dinv, E, w, t,
                                                                     tnb, vnb, v, o, dw, d1, d2);
     if constexpr (DOTS) {
@@ -550,17 +581,18 @@ __global__ void __launch_bounds__(1024) k_eis_bwd_lp(int64_t F, int L, int64_t k
     T* sa = reinterpret_cast<T*>(lp_smem);
     T* sb = sa + L * kLpCols;
     const int lane = threadIdx.x, z = threadIdx.y;
-    const int64_t k = k0 + logical_block<XCD>((k1 - k0 + kLpCols - 1) / kLpCols) * kLpCols + lane;
-    const bool valid = k < k1;
+    const int64_t kb = tile_floor(k0);
+    const int64_t k = kb + logical_block<XCD>((k1 - kb + kLpCols - 1) / kLpCols) * kLpCols + lane;
+    const bool valid = k >= k0 && k < k1;
     const int64_t N = F * L;
     if (valid) {
         const uint32_t lm = lowmask[k];
-        const int64_t q = (int64_t)z * F + k;
+        const int64_t q = pos_row(k, z, F, L);
         T acc = 0;
 #pragma unroll
         for (int f = 0; f < 3; ++f) {
             const int32_t n = nbp[f * F + k];
-            if (n >= 0 && !((lm >> f) & 1u)) acc += ldc<NT>(&lat[f * N + q]) * tnb[(int64_t)z * F + n];
+            if (n >= 0 && !((lm >> f) & 1u)) acc += ldc<NT>(&lat[f * N + q]) * tnb[pos_row(n, z, F, L)];
         }
         const T dv = ldc<NT>(&dinv[q]);
         const T va = z < L - 1 ? ldc<NT>(&vert[N + q]) : T(0);
@@ -572,7 +604,7 @@ __global__ void __launch_bounds__(1024) k_eis_bwd_lp(int64_t F, int L, int64_t k
         T next = 0;
         for (int zz = L - 1; zz >= 0; --zz) {
             next = sa[zz * kLpCols + lane] - sb[zz * kLpCols + lane] * next;
-            t[(int64_t)zz * F + k] = next;
+            t[pos_row(k, zz, F, L)] = next;
         }
     }
 }
@@ -601,11 +633,12 @@ __global__ void __launch_bounds__(1024) k_eis_fwd_lp(int64_t F, int L, int64_t k
     T* sa = st + LC;
     T* sb = sa + LC;
     const int lane = threadIdx.x, z = threadIdx.y;
-    const int64_t lb = logical_block<XCD>((k1 - k0 + kLpCols - 1) / kLpCols);
-    const int64_t k = k0 + lb * kLpCols + lane;
-    const bool valid = k < k1;
+    const int64_t kb = tile_floor(k0);
+    const int64_t lb = logical_block<XCD>((k1 - kb + kLpCols - 1) / kLpCols);
+    const int64_t k = kb + lb * kLpCols + lane;
+    const bool valid = k >= k0 && k < k1;
     const int64_t N = F * L;
-    const int64_t q = (int64_t)z * F + k;
+    const int64_t q = valid ? pos_row(k, z, F, L) : 0;
     const int s = z * kLpCols + lane;
     T wv = 0, dv = 0, lz = 0, ev = 0, g = 0;
     if (valid) {
@@ -619,7 +652,7 @@ __global__ void __launch_bounds__(1024) k_eis_fwd_lp(int64_t F, int L, int64_t k
 #pragma unroll
             for (int f = 0; f < 3; ++f) {
                 const int32_t n = nbp[f * F + k];
-                if (n >= 0 && !((lm >> f) & 1u)) acc += ldc<NT>(&lat[f * N + q]) * tnb[(int64_t)z * F + n];
+                if (n >= 0 && !((lm >> f) & 1u)) acc += ldc<NT>(&lat[f * N + q]) * tnb[pos_row(n, z, F, L)];
             }
             const T uz = z < L - 1 ? ldc<NT>(&vert[N + q]) : T(0);
             sa[s] = dv * (wv - acc);
@@ -629,7 +662,7 @@ __global__ void __launch_bounds__(1024) k_eis_fwd_lp(int64_t F, int L, int64_t k
 #pragma unroll
             for (int f = 0; f < 3; ++f) {
                 const int32_t n = nbp[f * F + k];
-                if (n >= 0 && ((lm >> f) & 1u)) acc += ldc<NT>(&lat[f * N + q]) * vnb[(int64_t)z * F + n];
+                if (n >= 0 && ((lm >> f) & 1u)) acc += ldc<NT>(&lat[f * N + q]) * vnb[pos_row(n, z, F, L)];
             }
             const T tv = t[q];
             g = wv + ev * tv - acc;
@@ -710,17 +743,17 @@ __global__ void __launch_bounds__(kBlock) k_pos_spmv(int64_t F, int L, const int
     const int64_t N = F * L;
     double rr = 0.0;
     if (k < F) {
-        const int64_t q = (int64_t)z * F + k;
-        const int64_t lay = (int64_t)z * F;
+        const int64_t qs = pos_stride64(k, F);
+        const int64_t q = pos_base64(k, L) + (int64_t)z * qs;
         T acc = diag[q] * x[q];
-        if (z > 0) acc = fma(vert[q], x[q - F], acc);
+        if (z > 0) acc = fma(vert[q], x[q - qs], acc);
 #pragma unroll
         for (int f = 0; f < 3; ++f) {
             const int32_t n = nbp[f * F + k];
             const bool on = n >= 0;
-            acc = fma(*(on ? &lat[f * N + q] : zero_ptr<T>()), *(on ? &x[lay + n] : zero_ptr<T>()), acc);
+            acc = fma(*(on ? &lat[f * N + q] : zero_ptr<T>()), *(on ? &x[pos_row(n, z, F, L)] : zero_ptr<T>()), acc);
         }
-        if (z < L - 1) acc = fma(vert[N + q], x[q + F], acc);
+        if (z < L - 1) acc = fma(vert[N + q], x[q + qs], acc);
         if (RESID) {
             const T res = b[q] - acc;
             y[q] = res;
@@ -766,23 +799,29 @@ __global__ void __launch_bounds__(B) k_pos_spmv_col(int64_t F, int L, const int3
     if (k < F) {
         const int64_t N = F * L;
         int32_t np_[3];
+        int64_t nq[3], ns[3];
 #pragma unroll
-        for (int f = 0; f < 3; ++f) np_[f] = nbp[f * F + k];
+        for (int f = 0; f < 3; ++f) {
+            np_[f] = nbp[f * F + k];
+            nq[f] = np_[f] >= 0 ? pos_base64(np_[f], L) : 0;
+            ns[f] = np_[f] >= 0 ? pos_stride64(np_[f], F) : 0;
+        }
+        const int64_t qb = pos_base64(k, L), qs = pos_stride64(k, F);
         T xv[LM];
 #pragma unroll
         for (int z = 0; z < LM; ++z)
-            if (z < L) xv[z] = x[(int64_t)z * F + k];
+            if (z < L) xv[z] = x[qb + (int64_t)z * qs];
 #pragma unroll
         for (int z = 0; z < LM; ++z)
             if (z < L) {
-                const int64_t q = (int64_t)z * F + k;
+                const int64_t q = qb + (int64_t)z * qs;
                 T acc = diag[q] * xv[z];
                 if (z > 0) acc = fma(vert[q], xv[z > 0 ? z - 1 : 0], acc);
 #pragma unroll
                 for (int f = 0; f < 3; ++f)
                 {
                     const bool on = np_[f] >= 0;
-                    acc = fma(*(on ? &lat[f * N + q] : zero_ptr<T>()), *(on ? &x[(int64_t)z * F + np_[f]] : zero_ptr<T>()),
+                    acc = fma(*(on ? &lat[f * N + q] : zero_ptr<T>()), *(on ? &x[nq[f] + (int64_t)z * ns[f]] : zero_ptr<T>()),
                               acc);
                 }
                 if (z < L - 1) acc = fma(vert[N + q], xv[z + 1 < LM ? z + 1 : z], acc);
@@ -808,17 +847,17 @@ __global__ void __launch_bounds__(B) k_pos_spmv_row(int64_t F, int L, const int3
     if (k < F) {
         const int z = blockIdx.y;
         const int64_t N = F * L;
-        const int64_t q = (int64_t)z * F + k;
-        const int64_t lay = (int64_t)z * F;
+        const int64_t qs = pos_stride64(k, F);
+        const int64_t q = pos_base64(k, L) + (int64_t)z * qs;
         T acc = diag[q] * x[q];
-        if (z > 0) acc = fma(vert[q], x[q - F], acc);
+        if (z > 0) acc = fma(vert[q], x[q - qs], acc);
 #pragma unroll
         for (int f = 0; f < 3; ++f) {
             const int32_t n = nbp[f * F + k];
             const bool on = n >= 0;
-            acc = fma(*(on ? &lat[f * N + q] : zero_ptr<T>()), *(on ? &x[lay + n] : zero_ptr<T>()), acc);
+            acc = fma(*(on ? &lat[f * N + q] : zero_ptr<T>()), *(on ? &x[pos_row(n, z, F, L)] : zero_ptr<T>()), acc);
         }
-        if (z < L - 1) acc = fma(vert[N + q], x[q + F], acc);
+        if (z < L - 1) acc = fma(vert[N + q], x[q + qs], acc);
         y[q] = acc;
         if (DOTS) {
             wy = (double)w[q] * (double)acc;
@@ -828,22 +867,23 @@ __global__ void __launch_bounds__(B) k_pos_spmv_row(int64_t F, int L, const int3
     block_dots<B, DOTS>(wy, yy, partial);
 }
 
-// reference order (z*F + face) <-> position order (z*F + k), with a type change
+// reference order (z*F + face) <-> position order (tiled rows), with a type change;
+// blockIdx.y is the layer (of L = gridDim.y)
 template <typename Ti, typename To>
 __global__ void __launch_bounds__(kBlock) k_to_pos(int64_t F, const int32_t* __restrict__ faces,
                                                    const Ti* __restrict__ in, To* __restrict__ out) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= F) return;
-    const int64_t lay = (int64_t)blockIdx.y * F;
-    out[lay + k] = (To)in[lay + faces[k]];
+    const int z = blockIdx.y;
+    out[pos_row(k, z, F, gridDim.y)] = (To)in[(int64_t)z * F + faces[k]];
 }
 template <typename Ti, typename To>
 __global__ void __launch_bounds__(kBlock) k_from_pos(int64_t F, const int32_t* __restrict__ faces,
                                                      const Ti* __restrict__ in, To* __restrict__ out) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= F) return;
-    const int64_t lay = (int64_t)blockIdx.y * F;
-    out[lay + faces[k]] = (To)in[lay + k];
+    const int z = blockIdx.y;
+    out[(int64_t)z * F + faces[k]] = (To)in[pos_row(k, z, F, gridDim.y)];
 }
 
 __global__ void __launch_bounds__(kBlock) k_narrow(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
@@ -1067,7 +1107,7 @@ void dilu_apply_t(hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFac
     for (int c = 0; c <= top; ++c) {
         const int64_t k0 = co.off[c], k1 = co.off[c + 1];
         if (k1 == k0) continue;
-        const dim3 g((unsigned)launch_blocks<XCD>((k1 - k0 + kBlock - 1) / kBlock));
+        const dim3 g((unsigned)launch_blocks<XCD>(col_blocks(k0, k1)));
         if (c == top)
             hipLaunchKernelGGL((k_dilu_fwd<T, POS, LM, true, XCD>), g, dim3(kBlock), 0, s, F, L, k0, k1, co.faces.p,
                                co.nbp.p, co.lowmask.p, d.lat.p, d.vert.p, d.dinv.p, r, d.y.p, d.y.p, z);
@@ -1080,7 +1120,7 @@ void dilu_apply_t(hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFac
     for (int c = top - 1; c >= 0; --c) {
         const int64_t k0 = co.off[c], k1 = co.off[c + 1];
         if (k1 == k0) continue;
-        const dim3 g((unsigned)launch_blocks<XCD>((k1 - k0 + kBlock - 1) / kBlock));
+        const dim3 g((unsigned)launch_blocks<XCD>(col_blocks(k0, k1)));
         hipLaunchKernelGGL((k_dilu_bwd<T, POS, LM, XCD>), g, dim3(kBlock), 0, s, F, L, k0, k1, co.faces.p, co.nbp.p,
                            co.lowmask.p, d.lat.p, d.vert.p, d.dinv.p, d.y.p, done, z);
     }
@@ -1162,7 +1202,7 @@ int64_t split_apply_t(hipStream_t s, const ColorOrder& co, int64_t F, int L, Dil
     const int top = co.ncolor - 1;
     // grids rounded to a multiple of the XCD count with or without the XCD mapping: the dot
     // partials then have the same layout in both (the extra workgroups add zeros)
-    auto blocks = [&](int c) { return dim3((unsigned)launch_blocks<true>((co.off[c + 1] - co.off[c] + kBlock - 1) / kBlock)); };
+    auto blocks = [&](int c) { return dim3((unsigned)launch_blocks<true>(col_blocks(co.off[c], co.off[c + 1]))); };
     // backward sweep, colours top .. 1: t
     for (int c = top; c >= 1; --c) {
         if (co.off[c + 1] == co.off[c]) continue;
@@ -1188,7 +1228,7 @@ template <typename T, int LM, bool XCD, bool NT>
 void split_lsolve_t(hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFactors<T>& d, const T* b, T* bhat) {
     for (int c = 0; c < co.ncolor; ++c) {
         if (co.off[c + 1] == co.off[c]) continue;
-        const dim3 g((unsigned)launch_blocks<XCD>((co.off[c + 1] - co.off[c] + kBlock - 1) / kBlock));
+        const dim3 g((unsigned)launch_blocks<XCD>(col_blocks(co.off[c], co.off[c + 1])));
         hipLaunchKernelGGL((k_eis_fwd<T, LM, false, XCD, false, false, false, NT>), g, dim3(kBlock), 0, s, F, L, co.off[c],
                            co.off[c + 1], co.nbp.p, co.lowmask.p, d.lat.p, d.vert.p, d.dinv.p, d.E.p, b, d.t.p,
                            d.t.p, d.y.p, d.y.p, bhat, (const T*)nullptr, (double*)nullptr, (const double*)nullptr);
@@ -1198,7 +1238,7 @@ template <typename T, int LM, bool XCD, bool NT>
 void split_rsolve_t(hipStream_t s, const ColorOrder& co, int64_t F, int L, DiluFactors<T>& d, const T* xhat, T* x) {
     for (int c = co.ncolor - 1; c >= 0; --c) {
         if (co.off[c + 1] == co.off[c]) continue;
-        const dim3 g((unsigned)launch_blocks<XCD>((co.off[c + 1] - co.off[c] + kBlock - 1) / kBlock));
+        const dim3 g((unsigned)launch_blocks<XCD>(col_blocks(co.off[c], co.off[c + 1])));
         hipLaunchKernelGGL((k_eis_bwd<T, LM, false, XCD, NT>), g, dim3(kBlock), 0, s, F, L, co.off[c], co.off[c + 1],
                            co.nbp.p, co.lowmask.p, d.lat.p, d.vert.p, d.dinv.p, xhat, x, x, (const double*)nullptr);
     }
@@ -1244,7 +1284,7 @@ int64_t split_apply_lp(hipStream_t s, const ColorOrder& co, int64_t F, int L, Di
     const dim3 blk(kLpCols, (unsigned)L);
     const size_t lc = (size_t)L * kLpCols * sizeof(T);
     auto blocks = [&](int c) {  // rounded as in split_apply_t
-        return dim3((unsigned)launch_blocks<true>((co.off[c + 1] - co.off[c] + kLpCols - 1) / kLpCols));
+        return dim3((unsigned)launch_blocks<true>(col_blocks(co.off[c], co.off[c + 1], kLpCols)));
     };
     for (int c = top; c >= 1; --c) {
         if (co.off[c + 1] == co.off[c]) continue;
