@@ -76,8 +76,8 @@ __global__ void __launch_bounds__(kBlock) k_dilu_factor(int64_t F, int L, int64_
         nf[f] = low ? faces[np_[f]] : 0;
         sn[f] = low ? slots[nf[f]] : 0u;
         rv[f] = low ? rev[f * F + k] : 0;
-        nq[f] = low ? pos_base64(np_[f], L) : 0;
-        ns[f] = low ? pos_stride64(np_[f], F) : 0;
+        nq[f] = pos_base64(max(np_[f], 0), L);  // rows of an absent neighbour: never read
+        ns[f] = pos_stride64(max(np_[f], 0), F);
     }
     const int64_t qb = pos_base64(k, L), qs = pos_stride64(k, F);
     T dprev = 0;
@@ -144,8 +144,8 @@ __global__ void __launch_bounds__(kBlock) k_dilu_fwd(int64_t F, int L, int64_t k
 #pragma unroll
     for (int f = 0; f < 3; ++f) {
         np_[f] = ((lm >> f) & 1u) ? nbp[f * F + k] : -1;
-        nq[f] = np_[f] >= 0 ? pos_base64(np_[f], L) : 0;
-        ns[f] = np_[f] >= 0 ? pos_stride64(np_[f], F) : 0;
+        nq[f] = pos_base64(max(np_[f], 0), L);  // branch-free: the loads stay together
+        ns[f] = pos_stride64(max(np_[f], 0), F);
     }
     // row z of r / z: position order (POS) or the reference order
     auto io = [&](int z) { return POS ? qb + (int64_t)z * qs : (int64_t)z * F + i; };
@@ -239,8 +239,8 @@ __global__ void __launch_bounds__(kBlock) k_dilu_bwd(int64_t F, int L, int64_t k
     for (int f = 0; f < 3; ++f) {
         const int32_t n = nbp[f * F + k];
         np_[f] = (n >= 0 && !((lm >> f) & 1u)) ? n : -1;
-        nq[f] = np_[f] >= 0 ? pos_base64(np_[f], L) : 0;
-        ns[f] = np_[f] >= 0 ? pos_stride64(np_[f], F) : 0;
+        nq[f] = pos_base64(max(np_[f], 0), L);  // branch-free: the loads stay together
+        ns[f] = pos_stride64(max(np_[f], 0), F);
     }
     auto io = [&](int z) { return POS ? qb + (int64_t)z * qs : (int64_t)z * F + i; };
     auto independent = [&](int z) {
@@ -319,8 +319,10 @@ __device__ __forceinline__ void eis_bwd_column(int64_t F, int Lrt, int64_t k, co
     for (int f = 0; f < 3; ++f) {
         const int32_t n = nbp[f * F + k];
         np_[f] = (n >= 0 && !((lm >> f) & 1u)) ? n : -1;
-        nq[f] = np_[f] >= 0 ? pos_base((uint32_t)np_[f], (uint32_t)L) : 0u;
-        ns[f] = np_[f] >= 0 ? pos_stride((uint32_t)np_[f], Fu) : 0u;
+        // branch-free (an absent neighbour's rows are never read): a branch here would
+        // hold each index load behind the previous one
+        nq[f] = pos_base((uint32_t)max(n, 0), (uint32_t)L);
+        ns[f] = pos_stride((uint32_t)max(n, 0), Fu);
     }
     T av[LM], va[LM], dv[LM], wv[LM];
 #pragma unroll
@@ -393,8 +395,8 @@ __device__ __forceinline__ void eis_fwd_column(int64_t F, int Lrt, int64_t k, co
         const bool low = (lm >> f) & 1u;
         lo[f] = (n >= 0 && low) ? n : -1;
         up[f] = (n >= 0 && !low) ? n : -1;
-        nq[f] = n >= 0 ? pos_base((uint32_t)n, (uint32_t)L) : 0u;
-        ns[f] = n >= 0 ? pos_stride((uint32_t)n, Fu) : 0u;
+        nq[f] = pos_base((uint32_t)max(n, 0), (uint32_t)L);  // branch-free, as in eis_bwd_column
+        ns[f] = pos_stride((uint32_t)max(n, 0), Fu);
     }
     T wv[LM], dv[LM], vb[LM], lv[LM], tv[LM], ev[LM], dwv[LM];
 #pragma unroll
@@ -803,8 +805,8 @@ __global__ void __launch_bounds__(B) k_pos_spmv_col(int64_t F, int L, const int3
 #pragma unroll
         for (int f = 0; f < 3; ++f) {
             np_[f] = nbp[f * F + k];
-            nq[f] = np_[f] >= 0 ? pos_base64(np_[f], L) : 0;
-            ns[f] = np_[f] >= 0 ? pos_stride64(np_[f], F) : 0;
+            nq[f] = pos_base64(max(np_[f], 0), L);
+            ns[f] = pos_stride64(max(np_[f], 0), F);
         }
         const int64_t qb = pos_base64(k, L), qs = pos_stride64(k, F);
         T xv[LM];
