@@ -51,9 +51,10 @@ def split_apply_bytes(nnz: int, n: int, f: int, f0: int, L: int, s: int, f_top: 
     t and the dot vector and writes v and the result (10 vectors); colour 0 (f0 faces)
     does both sweeps in one pass with t in registers (6 vectors); the top colour (f_top
     faces) does not write v, which no higher colour reads (one vector less); the
-    neighbour positions and colour mask once per sweep (13 bytes per face and sweep)."""
+    neighbour positions and colour mask once per sweep (13 bytes per face and sweep),
+    except in the top colour's backward sweep, which has no lateral terms."""
     n0 = f0 * L
-    return (nnz - n) * s + (10 * (n - n0) + 6 * n0 - f_top * L) * s + 26 * f - 13 * f0
+    return (nnz - n) * s + (10 * (n - n0) + 6 * n0 - f_top * L) * s + 26 * f - 13 * f0 - 13 * f_top
 
 
 def spmv_dots_bytes(nnz: int, n: int, f: int, s: int) -> int:

@@ -298,8 +298,10 @@ __global__ void __launch_bounds__(kBlock) k_dilu_bwd(int64_t F, int L, int64_t k
 // ---------------------------------------------------------------------------
 
 // t = D~^-1 (w - U t) for one colour; tnb reads t at the (higher-colour)
-// neighbour positions, t is written at the colour's own positions
-template <typename T, int LM, bool EX, bool NT>
+// neighbour positions, t is written at the colour's own positions.  LAT = false: the
+// highest colour, whose neighbours all carry lower colours, so U has only the coupling
+// above and the column reads no neighbour data at all.
+template <typename T, int LM, bool EX, bool NT, bool LAT = true>
 __device__ __forceinline__ void eis_bwd_column(int64_t F, int Lrt, int64_t k, const int32_t* __restrict__ nbp,
                                                const uint8_t* __restrict__ lowmask, const T* __restrict__ lat,
                                                const T* __restrict__ vert, const T* __restrict__ dinv,
@@ -312,12 +314,12 @@ __device__ __forceinline__ void eis_bwd_column(int64_t F, int Lrt, int64_t k, co
     const uint32_t Fu = (uint32_t)F, ku = (uint32_t)k;
     const uint32_t N = Fu * (uint32_t)L;
     const uint32_t qb = pos_base(ku, (uint32_t)L), qs = pos_stride(ku, Fu);
-    const uint32_t lm = lowmask[k];
+    const uint32_t lm = LAT ? lowmask[k] : 7u;
     int32_t np_[3];
     uint32_t nq[3], ns[3];  // rows of the neighbour columns: nq + z ns
 #pragma unroll
     for (int f = 0; f < 3; ++f) {
-        const int32_t n = nbp[f * F + k];
+        const int32_t n = LAT ? nbp[f * F + k] : -1;
         np_[f] = (n >= 0 && !((lm >> f) & 1u)) ? n : -1;
         // branch-free (an absent neighbour's rows are never read): a branch here would
         // hold each index load behind the previous one
@@ -330,12 +332,14 @@ __device__ __forceinline__ void eis_bwd_column(int64_t F, int Lrt, int64_t k, co
         if (z < L) {
             const uint32_t q = qb + (uint32_t)z * qs;
             T acc = 0;
+            if (LAT) {
 #pragma unroll
-            for (int f = 0; f < 3; ++f)
-            {
-                const bool on = np_[f] >= 0;
-                acc += ldc<NT>(&lat[on ? (uint32_t)f * N + q : 3 * N]) *
-                       tnb[on ? nq[f] + (uint32_t)z * ns[f] : N];
+                for (int f = 0; f < 3; ++f)
+                {
+                    const bool on = np_[f] >= 0;
+                    acc += ldc<NT>(&lat[on ? (uint32_t)f * N + q : 3 * N]) *
+                           tnb[on ? nq[f] + (uint32_t)z * ns[f] : N];
+                }
             }
             av[z] = acc;
             va[z] = z < L - 1 ? ldc<NT>(&vert[N + q]) : T(0);
@@ -351,7 +355,7 @@ __device__ __forceinline__ void eis_bwd_column(int64_t F, int Lrt, int64_t k, co
         }
 }
 
-template <typename T, int LM, bool EX, bool XCD, bool NT>
+template <typename T, int LM, bool EX, bool XCD, bool NT, bool LAT = true>
 __global__ void __launch_bounds__(kBlock) k_eis_bwd(int64_t F, int Lrt, int64_t k0, int64_t k1,
                                                     const int32_t* __restrict__ nbp,
                                                     const uint8_t* __restrict__ lowmask, const T* __restrict__ lat,
@@ -362,7 +366,7 @@ __global__ void __launch_bounds__(kBlock) k_eis_bwd(int64_t F, int Lrt, int64_t 
     const int64_t kb = tile_floor(k0);
     const int64_t k = kb + logical_block<XCD>((k1 - kb + kBlock - 1) / kBlock) * kBlock + threadIdx.x;
     if (k < k0 || k >= k1) return;
-    eis_bwd_column<T, LM, EX, NT>(F, Lrt, k, nbp, lowmask, lat, vert, dinv, w, tnb, t);
+    eis_bwd_column<T, LM, EX, NT, LAT>(F, Lrt, k, nbp, lowmask, lat, vert, dinv, w, tnb, t);
 }
 
 // Forward sweep of one colour: acc = L v (lower neighbours from vnb, the cell
@@ -512,7 +516,7 @@ __global__ void __launch_bounds__(kBlock) k_eis_fwd(int64_t F, int L, int64_t k0
 // faces [tab[b].x, tab[b].y) of one colour of member tab[b].z and returns at once when
 // that member's run has stopped (stop[member * stop_stride] != 0).  No workgroup spans
 // two members, so the DOTS partials of a workgroup belong to one member.
-template <typename T, int LM, bool EX, bool NT>
+template <typename T, int LM, bool EX, bool NT, bool LAT = true>
 __global__ void __launch_bounds__(kBlock) k_eis_bwd_tab(int64_t F, int Lrt, const int4* __restrict__ tab,
                                                         const int32_t* __restrict__ nbp,
                                                         const uint8_t* __restrict__ lowmask, const T* __restrict__ lat,
@@ -524,7 +528,7 @@ __global__ void __launch_bounds__(kBlock) k_eis_bwd_tab(int64_t F, int Lrt, cons
     if (stop && stop[(int64_t)e.z * stop_stride] != 0.0) return;  // uniform over the workgroup
     const int64_t k = (int64_t)e.x + threadIdx.x;
     if (k >= e.y) return;
-    eis_bwd_column<T, LM, EX, NT>(F, Lrt, k, nbp, lowmask, lat, vert, dinv, w, tnb, t);
+    eis_bwd_column<T, LM, EX, NT, LAT>(F, Lrt, k, nbp, lowmask, lat, vert, dinv, w, tnb, t);
 }
 
 template <typename T, int LM, bool EX, bool WITH_T, bool FUSE, bool DOTS, bool NT, bool SV = true>
@@ -1205,11 +1209,15 @@ int64_t split_apply_t(hipStream_t s, const ColorOrder& co, int64_t F, int L, Dil
     // grids rounded to a multiple of the XCD count with or without the XCD mapping: the dot
     // partials then have the same layout in both (the extra workgroups add zeros)
     auto blocks = [&](int c) { return dim3((unsigned)launch_blocks<true>(col_blocks(co.off[c], co.off[c + 1]))); };
-    // backward sweep, colours top .. 1: t
+    // backward sweep, colours top .. 1: t (the top colour has no upper lateral neighbours)
     for (int c = top; c >= 1; --c) {
         if (co.off[c + 1] == co.off[c]) continue;
-        hipLaunchKernelGGL((k_eis_bwd<T, LM, EX, XCD, NT>), blocks(c), dim3(kBlock), 0, s, F, L, co.off[c],
-                           co.off[c + 1], co.nbp.p, co.lowmask.p, d.lat.p, d.vert.p, d.dinv.p, w, d.t.p, d.t.p, stop);
+        if (c == top)
+            hipLaunchKernelGGL((k_eis_bwd<T, LM, EX, XCD, NT, false>), blocks(c), dim3(kBlock), 0, s, F, L, co.off[c],
+                               co.off[c + 1], co.nbp.p, co.lowmask.p, d.lat.p, d.vert.p, d.dinv.p, w, d.t.p, d.t.p, stop);
+        else
+            hipLaunchKernelGGL((k_eis_bwd<T, LM, EX, XCD, NT>), blocks(c), dim3(kBlock), 0, s, F, L, co.off[c],
+                               co.off[c + 1], co.nbp.p, co.lowmask.p, d.lat.p, d.vert.p, d.dinv.p, w, d.t.p, d.t.p, stop);
     }
     // colour 0: backward and forward sweeps in one pass
     if (co.off[1] > co.off[0])
@@ -1366,9 +1374,14 @@ void split_apply_batch_t(hipStream_t s, const ColorOrder& co, int64_t F, int L, 
     const int top = co.ncolor - 1;
     for (int c = top; c >= 1; --c) {
         if (st.nblk[c] == 0) continue;
-        hipLaunchKernelGGL((k_eis_bwd_tab<T, LM, EX, false>), dim3((unsigned)st.nblk[c]), dim3(kBlock), 0, s, F, L,
-                           st.tab[c].p, co.nbp.p, co.lowmask.p, d.lat.p, d.vert.p, d.dinv.p, w, d.t.p, d.t.p, stop,
-                           stop_stride);
+        if (c == top)
+            hipLaunchKernelGGL((k_eis_bwd_tab<T, LM, EX, false, false>), dim3((unsigned)st.nblk[c]), dim3(kBlock), 0, s,
+                               F, L, st.tab[c].p, co.nbp.p, co.lowmask.p, d.lat.p, d.vert.p, d.dinv.p, w, d.t.p, d.t.p,
+                               stop, stop_stride);
+        else
+            hipLaunchKernelGGL((k_eis_bwd_tab<T, LM, EX, false>), dim3((unsigned)st.nblk[c]), dim3(kBlock), 0, s, F, L,
+                               st.tab[c].p, co.nbp.p, co.lowmask.p, d.lat.p, d.vert.p, d.dinv.p, w, d.t.p, d.t.p, stop,
+                               stop_stride);
     }
     for (int c = 0; c <= top; ++c) {
         if (st.nblk[c] == 0) continue;
