@@ -164,7 +164,7 @@ class Stats(C.Structure):
         ("susp_rhs_max", C.c_double),
         ("dep_rhs_max", C.c_double),
         ("susp_refinements", C.c_int32),
-        ("reserved", C.c_int32),
+        ("susp_restarts", C.c_int32),
         ("ms_assembly", C.c_double),
         ("ms_susp_solve", C.c_double),
         ("ms_post", C.c_double),

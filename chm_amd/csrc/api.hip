@@ -347,7 +347,8 @@ SolveInfo solve_in_position_order(pbsm3d_ctx* c, const pbsm3d_config& cfg, int64
             si = bicgstab_split<float>(s, c->bw32, c->ws32, A, Ah, c->bp.p, c->xp.p, c->r_s.p, c->split32[0].p,
                                        c->split32[1].p, c->split32[2].p, cfg.max_iters, cfg.tol,
                                        cfg.inner_rtol > 0 ? cfg.inner_rtol : split_inner_rtol_default(c, cfg),
-                                       cfg.inner_rtol > 0 ? cfg.inner_rtol : kSplitInnerCold, coarse);
+                                       cfg.inner_rtol > 0 ? cfg.inner_rtol : kSplitInnerCold, coarse,
+                                       c->restart_eps32);
         }
     } else if (cfg.precision == PBSM3D_FP64) {
         PcFn<double> M = [c, F, L](hipStream_t st, const double* r, double* z) {
@@ -367,7 +368,7 @@ SolveInfo solve_in_position_order(pbsm3d_ctx* c, const pbsm3d_config& cfg, int64
         };
         if (bicg)
             si = bicgstab_mixed(s, c->bw32, c->ws32, A, A32, M, c->bp.p, c->xp.p, c->r_s.p, c->r32.p, c->d32.p,
-                                cfg.max_iters, cfg.tol, cfg.inner_rtol > 0 ? cfg.inner_rtol : 1e-4);
+                                cfg.max_iters, cfg.tol, cfg.inner_rtol > 0 ? cfg.inner_rtol : 1e-4, c->restart_eps32);
         else
             si = mixed_solve(s, c->ws32, A, A32, M, c->bp.p, c->xp.p, c->r_s.p, c->r32.p, c->d32.p, cfg.restart,
                              cfg.max_iters, cfg.tol, cfg.inner_rtol > 0 ? cfg.inner_rtol : 1e-4);
@@ -718,6 +719,7 @@ int pbsm3d_step(pbsm3d_ctx* c, const pbsm3d_forcing* fo, int32_t dt_s, pbsm3d_ou
             stats.susp_resid0 = si.resid0;
             stats.susp_cold_fallback = si.cold_fallback ? 1 : 0;
             stats.susp_refinements = si.refinements;
+            stats.susp_restarts = si.restarts;
             stats.susp_resid_coarse = si.resid_coarse;
             if (!si.converged) throw Error(PBSM3D_ENOCONV, "suspension solver failed to converge");
         }
@@ -875,6 +877,12 @@ int pbsm3d_precond_select(pbsm3d_ctx* c, int which, int mode) {
     if (!c || (which != 0 && which != 1) || mode < -1 || mode >= kSplitModes) return PBSM3D_EINVAL;
     if (mode > 1 && c->cfg.krylov != PBSM3D_KRYLOV_BICGSTAB_SPLIT) return PBSM3D_EINVAL;  // split sweeps only
     c->dilu_mode[which] = mode;
+    return PBSM3D_OK;
+}
+
+int pbsm3d_bicg_restart(pbsm3d_ctx* c, double eps) {
+    if (!c || !(eps >= 0.0 && eps < 1.0)) return PBSM3D_EINVAL;
+    c->restart_eps32 = eps;
     return PBSM3D_OK;
 }
 

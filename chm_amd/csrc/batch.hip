@@ -33,7 +33,8 @@ namespace {
 // per-member scalars of the batched BiCGStab, BS_COUNT doubles per member
 enum BatchSc {
     BS_RHO_PREV = 0, BS_RHO, BS_ALPHA, BS_OMEGA, BS_RHV, BS_SS, BS_ST, BS_TT, BS_RR, BS_FLAG, BS_STOP, BS_STOPABS,
-    BS_ITERS, BS_COUNT
+    BS_ITERS, BS_RHNN, BS_RESTART, BS_NRESTART,  // shadow-residual restarts as in bicg_stage
+    BS_COUNT
 };
 enum StageMode { ST_START = 0, ST_ALPHA, ST_OMEGA, ST_RHO };
 constexpr int kChunk = 1024;       // elements per workgroup of the element-wise kernels (4 per thread)
@@ -62,14 +63,19 @@ __global__ void __launch_bounds__(kBlock) k_seg_dots2(const int4* __restrict__ c
     }
 }
 
-// p = r + beta (p - omega v), beta = (rho / rho_prev) (alpha / omega) of the member
+// p = r + beta (p - omega v), beta = (rho / rho_prev) (alpha / omega) of the member; after a
+// restart of the member's shadow residual, p = r and rh = r
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_seg_p(const int4* __restrict__ ch, const T* __restrict__ r,
                                                   const T* __restrict__ v, const double* __restrict__ sc,
-                                                  T* __restrict__ p) {
+                                                  T* __restrict__ p, T* __restrict__ rh) {
     const int4 e = ch[blockIdx.x];
     const double* m = sc + (int64_t)e.z * BS_COUNT;
     if (m[BS_STOP] != 0.0) return;
+    if (m[BS_RESTART] != 0.0) {
+        for (int64_t q = (int64_t)e.x + threadIdx.x; q < e.y; q += kBlock) p[q] = rh[q] = r[q];
+        return;
+    }
     const double omega = m[BS_OMEGA];
     const double beta = (m[BS_RHO] / m[BS_RHO_PREV]) * (m[BS_ALPHA] / omega);
     for (int64_t q = (int64_t)e.x + threadIdx.x; q < e.y; q += kBlock)
@@ -218,7 +224,7 @@ __global__ void __launch_bounds__(kStage) k_seg_stage(int mode, const int64_t* _
                                                       const int64_t* __restrict__ op_rng,
                                                       const double* __restrict__ pairs, const double* __restrict__ ssp,
                                                       double* __restrict__ sc, const double* __restrict__ target,
-                                                      double* __restrict__ host_slot) {
+                                                      double* __restrict__ host_slot, double restart_eps) {
     __shared__ double red[kStage / 64];
     __shared__ double bc;
     const int m = blockIdx.x;
@@ -243,6 +249,9 @@ __global__ void __launch_bounds__(kStage) k_seg_stage(int mode, const int64_t* _
                     S[BS_OMEGA] = 1;
                     S[BS_FLAG] = 0;
                     S[BS_ITERS] = 0;
+                    S[BS_RHNN] = a;  // rh = r
+                    S[BS_RESTART] = 0;
+                    S[BS_NRESTART] = 0;
                     const double tg = target[m], rr = sqrt(b);
                     S[BS_STOPABS] = tg < 0 ? 0.0 : tg * rr;
                     S[BS_STOP] = (tg < 0 || rr <= S[BS_STOPABS] || !isfinite(rr)) ? 1.0 : 0.0;
@@ -254,6 +263,14 @@ __global__ void __launch_bounds__(kStage) k_seg_stage(int mode, const int64_t* _
                     if (!isfinite(b)) S[BS_FLAG] = 3;
                     const double rr = sqrt(b);
                     if (S[BS_FLAG] != 0.0 || rr <= S[BS_STOPABS] || !isfinite(rr)) S[BS_STOP] = 1;
+                    const bool restart =
+                        S[BS_STOP] == 0.0 && restart_eps > 0.0 && fabs(a) < restart_eps * sqrt(S[BS_RHNN] * b);
+                    S[BS_RESTART] = restart ? 1.0 : 0.0;
+                    if (restart) {
+                        S[BS_RHO] = b;
+                        S[BS_RHNN] = b;
+                        S[BS_NRESTART] += 1.0;
+                    }
                 }
             }
         } else {
@@ -516,20 +533,21 @@ void enqueue_iteration(hipStream_t s, Segments& sg, const ColorOrder& co, DiluFa
     if (first)
         PB_HIP(hipMemcpyAsync(w.p.p, w.r.p, sg.N * sizeof(T), hipMemcpyDeviceToDevice, s));
     else
-        hipLaunchKernelGGL((k_seg_p<T>), dim3(nch), dim3(kBlock), 0, s, sg.chunks.p, w.r.p, w.v.p, w.sc.p, w.p.p);
+        hipLaunchKernelGGL((k_seg_p<T>), dim3(nch), dim3(kBlock), 0, s, sg.chunks.p, w.r.p, w.v.p, w.sc.p, w.p.p,
+                           w.rh.p);
     split_apply_batch<T>(s, co, sg.F, sg.L, d, sg.sweep, w.p.p, w.v.p, w.rh.p, w.op_part.p, stop, BS_COUNT);
     hipLaunchKernelGGL(k_seg_stage, dim3(M), dim3(kStage), 0, s, (int)ST_ALPHA, sg.chunk_begin.p, sg.op_rng.p,
-                       w.op_part.p, (const double*)nullptr, w.sc.p, (const double*)nullptr, (double*)nullptr);
+                       w.op_part.p, (const double*)nullptr, w.sc.p, (const double*)nullptr, (double*)nullptr, 0.0);
     hipLaunchKernelGGL((k_seg_s<T>), dim3(nch), dim3(kBlock), 0, s, sg.chunks.p, w.v.p, w.r.p, w.sc.p, w.sv.p,
                        w.ss_part.p);
     split_apply_batch<T>(s, co, sg.F, sg.L, d, sg.sweep, w.sv.p, w.t.p, w.sv.p, w.op_part.p, stop, BS_COUNT);
     hipLaunchKernelGGL(k_seg_stage, dim3(M), dim3(kStage), 0, s, (int)ST_OMEGA, sg.chunk_begin.p, sg.op_rng.p,
-                       w.op_part.p, w.ss_part.p, w.sc.p, (const double*)nullptr, (double*)nullptr);
+                       w.op_part.p, w.ss_part.p, w.sc.p, (const double*)nullptr, (double*)nullptr, 0.0);
     hipLaunchKernelGGL((k_seg_xr<T>), dim3(nch), dim3(kBlock), 0, s, sg.chunks.p, w.p.p, w.sv.p, w.sv.p, w.t.p,
                        w.rh.p, w.sc.p, x, w.r.p, w.xr_part.p);
     hipLaunchKernelGGL(k_seg_stage, dim3(M), dim3(kStage), 0, s, (int)ST_RHO, sg.chunk_begin.p, sg.op_rng.p,
                        w.xr_part.p, (const double*)nullptr, w.sc.p, (const double*)nullptr,
-                       w.hsc_dev + (size_t)M * BS_COUNT * slot);
+                       w.hsc_dev + (size_t)M * BS_COUNT * slot, sizeof(T) == 4 ? kBicgRestartEps32 : 0.0);
     PB_HIP(hipGetLastError());
 }
 
@@ -548,18 +566,19 @@ inline int64_t iteration_launches(const Segments& sg, const ColorOrder& co) {
 template <typename T>
 void batch_bicg_run(hipStream_t s, Segments& sg, const ColorOrder& co, DiluFactors<T>& d, BatchWork<T>& w, const T* b,
                     T* x, const std::vector<double>& target, int maxit, std::vector<int>& iters, RunCount& rc,
-                    bool graphs) {
+                    bool graphs, std::vector<int>& restarts) {
     const int M = sg.M;
     const int64_t n = sg.N;
     const unsigned nch = (unsigned)sg.nchunk;
     iters.assign(M, 0);
+    restarts.assign(M, 0);
     if (maxit <= 0 || nch == 0) return;
     PB_HIP(hipMemcpyAsync(w.r.p, b, n * sizeof(T), hipMemcpyDeviceToDevice, s));
     PB_HIP(hipMemcpyAsync(w.rh.p, b, n * sizeof(T), hipMemcpyDeviceToDevice, s));
     PB_HIP(hipMemcpyAsync(w.tgt.p, target.data(), M * sizeof(double), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL((k_seg_dots2<T>), dim3(nch), dim3(kBlock), 0, s, sg.chunks.p, w.rh.p, w.r.p, w.xr_part.p);
     hipLaunchKernelGGL(k_seg_stage, dim3(M), dim3(kStage), 0, s, (int)ST_START, sg.chunk_begin.p, sg.op_rng.p,
-                       w.xr_part.p, (const double*)nullptr, w.sc.p, w.tgt.p, w.hsc_dev);
+                       w.xr_part.p, (const double*)nullptr, w.sc.p, w.tgt.p, w.hsc_dev, 0.0);
     PB_HIP(hipGetLastError());
     PB_HIP(hipStreamSynchronize(s));
     auto all_stopped = [&](const double* slot) {
@@ -625,11 +644,14 @@ void batch_bicg_run(hipStream_t s, Segments& sg, const ColorOrder& co, DiluFacto
     std::vector<double> h((size_t)M * BS_COUNT);
     PB_HIP(hipMemcpyAsync(h.data(), w.sc.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, s));
     PB_HIP(hipStreamSynchronize(s));
-    for (int m = 0; m < M; ++m) iters[m] = (int)h[(size_t)m * BS_COUNT + BS_ITERS];
+    for (int m = 0; m < M; ++m) {
+        iters[m] = (int)h[(size_t)m * BS_COUNT + BS_ITERS];
+        restarts[m] = (int)h[(size_t)m * BS_COUNT + BS_NRESTART];
+    }
 }
 
 struct MemberSolve {
-    int iters = 0, refinements = 0;
+    int iters = 0, refinements = 0, restarts = 0;
     double resid = 0, resid0 = 1;
     bool converged = false, cold_fallback = false;
 };
@@ -675,7 +697,7 @@ void batch_split_solve(hipStream_t s, Segments& sg, const ColorOrder& co, DiluFa
     }
     for (int m = 0; m < M; ++m) out[m].resid0 = nb[m] > 0 ? beta[m] / nb[m] : 0.0;
     std::vector<double> f(M), target(M), a(M);
-    std::vector<int> it_run;
+    std::vector<int> it_run, rs_run;
     while (true) {
         bool any = false;
         for (int m = 0; m < M; ++m) {
@@ -698,7 +720,7 @@ void batch_split_solve(hipStream_t s, Segments& sg, const ColorOrder& co, DiluFa
         hipLaunchKernelGGL((k_seg_scale<T>), dim3(nch), dim3(kBlock), 0, s, sg.chunks.p, r, w.fac.p, w1);
         split_op<T>(1, s, co, sg.F, sg.L, dT, w1, w2, 0);  // left transform of r / ||r||
         PB_HIP(hipMemsetAsync(w3, 0, n * sizeof(T), s));
-        batch_bicg_run<T>(s, sg, co, dT, w, w2, w3, target, left, it_run, rc, graphs);
+        batch_bicg_run<T>(s, sg, co, dT, w, w2, w3, target, left, it_run, rc, graphs, rs_run);
         split_op<T>(2, s, co, sg.F, sg.L, dT, w3, w1, 0);  // correction back to the unknowns
         upload_factors(s, w.fac, a);
         hipLaunchKernelGGL((k_seg_axpy<T>), dim3(nch), dim3(kBlock), 0, s, sg.chunks.p, w.fac.p, w1, x);
@@ -707,6 +729,7 @@ void batch_split_solve(hipStream_t s, Segments& sg, const ColorOrder& co, DiluFa
             if (target[m] >= 0) {
                 out[m].iters += it_run[m];
                 out[m].refinements += 1;
+                out[m].restarts += rs_run[m];
             }
         beta = residual();
     }
@@ -1033,6 +1056,7 @@ void group_step(pbsm3d_batch* B, BatchGroup& g, const pbsm3d_forcing* fo, int32_
         o.susp_rhs_max = smax[m];
         o.dep_rhs_max = dmax[m];
         o.susp_refinements = sres[m].refinements;
+        o.susp_restarts = sres[m].restarts;
         o.susp_resid0 = sres[m].resid0;
         o.susp_cold_fallback = sres[m].cold_fallback ? 1 : 0;
     }

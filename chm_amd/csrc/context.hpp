@@ -61,6 +61,8 @@ struct pbsm3d_ctx {
     // D-ILU sweep mode per precision: bit 0 the XCD workgroup mapping, bit 1 (split
     // operator only) the non-temporal factor loads; -1 until tuned
     int dilu_mode[2] = {-1, -1};
+    // shadow-residual restart threshold of the FP32 BiCGStab runs (bicg_stage; 0: never)
+    double restart_eps32 = pb::kBicgRestartEps32;
     float dilu_tune_ms[2][pb::kSplitModes] = {};
     pb::DevBuf<float> val32, r32, d32;
     pb::GmresWork<double> ws64, wd64;

@@ -337,11 +337,16 @@ void axpy_dots(hipStream_t s, int64_t n, double a, const T* x, const T* y, T* ou
 // made on the device too: once sc[SC_STOP] is set every later BiCGStab kernel of the run
 // returns at entry, so the host can queue iteration k+1 before it reads the result of k.
 enum BicgSc {
-    SC_RHO_PREV = 0, SC_RHO, SC_ALPHA, SC_OMEGA, SC_RHV, SC_SS, SC_ST, SC_TT, SC_RR, SC_FLAG, SC_STOP, SC_COUNT
+    SC_RHO_PREV = 0, SC_RHO, SC_ALPHA, SC_OMEGA, SC_RHV, SC_SS, SC_ST, SC_TT, SC_RR, SC_FLAG, SC_STOP,
+    SC_RHNN,      // rh.rh, the shadow residual's squared norm
+    SC_RESTART,   // 1: the last stage restarted the shadow residual (the next p kernel sets rh = p = r)
+    SC_NRESTART,  // restarts in this run
+    SC_COUNT
 };
-// p = r + beta (p - omega v) with beta from the scalars of the previous step
+// p = r + beta (p - omega v) with beta from the scalars of the previous step; after a
+// restart of the shadow residual (sc[SC_RESTART]): p = r and rh = r
 template <typename T>
-void bicg_p_dev(hipStream_t s, int64_t n, const T* r, const T* v, const double* sc, T* p);
+void bicg_p_dev(hipStream_t s, int64_t n, const T* r, const T* v, const double* sc, T* p, T* rh);
 // alpha = rho / (rh.v) and s = r - alpha v, partial[b] = s.s (nblk blocks).  dots: the
 // operator's dots_nb per-block pairs (rh.v, v.v); every workgroup sums them in the same fixed
 // order and workgroup 0 records rh.v and alpha in sc (no single-workgroup stage launch).
@@ -361,9 +366,11 @@ void bicg_xr_dev(hipStream_t s, int64_t n, const T* ph, const T* sh, const T* sv
 // rr, and sc[SC_STOP] set when a breakdown was flagged (sc[SC_FLAG] 1: rh.v = 0, 2: omega = 0,
 // 3: non-finite residual), sqrt(rr) <= stop_abs or rr is not finite.  Stage 1 does nothing once
 // sc[SC_STOP] is set.  host_slot (may be NULL): device-visible pinned host memory that
-// receives sc[0 .. SC_COUNT) after the update.
+// receives sc[0 .. SC_COUNT) after the update.  restart_eps > 0: stage 1 restarts the shadow
+// residual when |rh.r| < restart_eps ||rh|| ||r|| (a near breakdown of the Lanczos recurrence):
+// rho = r.r, sc[SC_RESTART] = 1, and the next p kernel sets rh = p = r.
 void bicg_stage(hipStream_t s, const double* partial, int64_t nblk, int stride, int stage, double* sc,
-                double stop_abs = 0.0, double* host_slot = nullptr);
+                double stop_abs = 0.0, double* host_slot = nullptr, double restart_eps = 0.0);
 
 // conversions / vector updates
 template <typename T>

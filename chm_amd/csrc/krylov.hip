@@ -284,7 +284,8 @@ int64_t spmv_dots(hipStream_t s, const CsrOp<T>& A, const T* x, T* y, const T* w
 // split operator does).  stop_rel: stop_abs is relative to the starting ||r||.
 template <typename T>
 int bicg_run(hipStream_t s, BicgWork<T>& w, const CsrOp<T>& A, const PcFn<T>& M, const T* b, T* x, bool x_zero,
-             double stop_abs, int maxit, double* res_out, bool stop_rel = false) {
+             double stop_abs, int maxit, double* res_out, bool stop_rel = false, double restart_eps = 0.0,
+             int* restarts = nullptr) {
     const int64_t n = A.n;
     const int nblk = reduce_blocks(n);
     // The s and x/r kernels: every workgroup first sums the operator's dot partials itself,
@@ -324,7 +325,7 @@ int bicg_run(hipStream_t s, BicgWork<T>& w, const CsrOp<T>& A, const PcFn<T>& M,
             PB_HIP(hipMemcpyAsync(w.p.p, w.r.p, n * sizeof(T), hipMemcpyDeviceToDevice, s));
         } else {
             ProfScope ps(A.prof, s, PBSM3D_K_ORTHO);
-            bicg_p_dev<T>(s, n, w.r.p, w.v.p, sc, w.p.p);
+            bicg_p_dev<T>(s, n, w.r.p, w.v.p, sc, w.p.p, w.rh.p);
         }
         if (pc) {
             ProfScope ps(A.prof, s, PBSM3D_K_PRECOND);
@@ -353,7 +354,7 @@ int bicg_run(hipStream_t s, BicgWork<T>& w, const CsrOp<T>& A, const PcFn<T>& M,
             bicg_xr_dev<T>(s, n, ph, sh, w.s.p, w.t.p, w.rh.p, sc, x, w.r.p, w.xr_partial.p, nvec, w.partial.p, nb,
                            w.ss_partial.p, nvec);
             // rho, rr, the stop test, and the scalars into host slot it % 2 (no copy launch)
-            bicg_stage(s, w.xr_partial.p, nvec, 2, 1, sc, stop_abs, w.hsc_dev + SC_COUNT * (it & 1));
+            bicg_stage(s, w.xr_partial.p, nvec, 2, 1, sc, stop_abs, w.hsc_dev + SC_COUNT * (it & 1), restart_eps);
         }
         PB_HIP(hipEventRecord(w.ev[it & 1], s));
     };
@@ -365,7 +366,10 @@ int bicg_run(hipStream_t s, BicgWork<T>& w, const CsrOp<T>& A, const PcFn<T>& M,
         const double* hk = h + SC_COUNT * (it & 1);
         rr = std::sqrt(hk[SC_RR]);
         *res_out = rr;
-        if (hk[SC_STOP] != 0.0 || it >= maxit) return it;
+        if (hk[SC_STOP] != 0.0 || it >= maxit) {
+            if (restarts) *restarts += (int)hk[SC_NRESTART];
+            return it;
+        }
     }
 }
 }  // namespace
@@ -408,7 +412,8 @@ void add_into(hipStream_t s, int64_t n, double a, const float* d, double* x) { a
 template <typename T>
 SolveInfo bicgstab_split(hipStream_t s, BicgWork<T>& w, GmresWork<T>& gw, const CsrOp<double>& A64,
                          const CsrOp<T>& Ahat, const double* b, double* x, double* r, T* work1, T* work2, T* work3,
-                         int maxit, double tol, double inner_rtol, double inner_rtol_cold, const CoarseFn& coarse) {
+                         int maxit, double tol, double inner_rtol, double inner_rtol_cold, const CoarseFn& coarse,
+                         double restart_eps32) {
     SolveInfo info;
     const int64_t n = A64.n;
     w.ensure(n);
@@ -450,7 +455,8 @@ SolveInfo bicgstab_split(hipStream_t s, BicgWork<T>& w, GmresWork<T>& gw, const 
         double stop = 0.5 * tol * nb / beta;
         if (stop < inner_rtol) stop = inner_rtol;
         double est = 0;
-        info.iters += bicg_run<T>(s, w, Ahat, PcFn<T>(), work2, work3, true, stop, maxit - info.iters, &est, true);
+        info.iters += bicg_run<T>(s, w, Ahat, PcFn<T>(), work2, work3, true, stop, maxit - info.iters, &est, true,
+                                  sizeof(T) == 4 ? restart_eps32 : 0.0, &info.restarts);
         {
             ProfScope ps(Ahat.prof, s, pclass);
             split_op<T>(2, s, *Ahat.co, Ahat.F, Ahat.L, *Ahat.split, work3, work1, Ahat.mode);  // d = (D~+U)^-1 dhat
@@ -465,14 +471,14 @@ SolveInfo bicgstab_split(hipStream_t s, BicgWork<T>& w, GmresWork<T>& gw, const 
 
 template SolveInfo bicgstab_split<double>(hipStream_t, BicgWork<double>&, GmresWork<double>&, const CsrOp<double>&,
                                           const CsrOp<double>&, const double*, double*, double*, double*, double*,
-                                          double*, int, double, double, double, const CoarseFn&);
+                                          double*, int, double, double, double, const CoarseFn&, double);
 template SolveInfo bicgstab_split<float>(hipStream_t, BicgWork<float>&, GmresWork<float>&, const CsrOp<double>&,
                                          const CsrOp<float>&, const double*, double*, double*, float*, float*, float*,
-                                         int, double, double, double, const CoarseFn&);
+                                         int, double, double, double, const CoarseFn&, double);
 
 SolveInfo bicgstab_mixed(hipStream_t s, BicgWork<float>& w, GmresWork<float>& gw, const CsrOp<double>& A64,
                          const CsrOp<float>& A32, const PcFn<float>& M32, const double* b, double* x, double* r,
-                         float* r32, float* d32, int maxit, double tol, double inner_rtol) {
+                         float* r32, float* d32, int maxit, double tol, double inner_rtol, double restart_eps32) {
     SolveInfo info;
     w.ensure(A64.n);
     const double nb = norm_of(s, gw, A64.n, b);
@@ -495,7 +501,8 @@ SolveInfo bicgstab_mixed(hipStream_t s, BicgWork<float>& w, GmresWork<float>& gw
         double stop = 0.5 * tol * nb / beta;
         if (stop < inner_rtol) stop = inner_rtol;
         double est = 0;
-        info.iters += bicg_run<float>(s, w, A32, M32, r32, d32, true, stop, maxit - info.iters, &est);
+        info.iters += bicg_run<float>(s, w, A32, M32, r32, d32, true, stop, maxit - info.iters, &est, false,
+                                      restart_eps32, &info.restarts);
         axpy_f32_into_f64(s, A64.n, beta, d32, x);
         info.cycles += 1;
         info.refinements += 1;

@@ -78,6 +78,7 @@ struct SolveInfo {
     bool converged = false;
     bool cold_fallback = false;  // the initial guess was worse than x0 = 0 and was dropped
     double resid_coarse = 0;     // ||b - A x|| / ||b|| after the first coarse correction (0: none)
+    int restarts = 0;            // shadow-residual restarts of the FP32 BiCGStab runs
 };
 
 // Device workspace of one GMRES instance of storage type T.
@@ -135,6 +136,13 @@ struct BicgWork {
     void ensure(int64_t n_);
 };
 
+// Default restart threshold of the shadow residual in FP32 BiCGStab runs: the vectors carry
+// 24-bit mantissas, and once |rh.r| < 1e-5 ||rh|| ||r|| rho keeps about two significant digits,
+// alpha = rho / rh.v swings by orders of magnitude and the residual can wander for hundreds of
+// iterations (a near breakdown of the Lanczos recurrence); restarting with rh = r removes it.
+// FP64 runs (the deposition solve) converge through ratios of 1e-10 and never restart.
+constexpr double kBicgRestartEps32 = 1e-5;
+
 SolveInfo bicgstab_solve(hipStream_t s, BicgWork<double>& w, GmresWork<double>& gw, const CsrOp<double>& A,
                          const PcFn<double>& M, const double* b, double* x, int maxit, double tol);
 // BiCGStab on the split-preconditioned system (Eisenstat's trick) inside FP64
@@ -147,15 +155,18 @@ SolveInfo bicgstab_solve(hipStream_t s, BicgWork<double>& w, GmresWork<double>& 
 // empty): a correction of x from the FP64 residual r, applied at the start of every
 // cycle before the inner run (the learned coarse correction); the residual is then
 // recomputed, and SolveInfo::resid_coarse records it after the first correction.
+// restart_eps32: shadow-residual restart threshold of the runs when T = float (bicg_stage;
+// FP64 runs never restart).
 using CoarseFn = std::function<void(hipStream_t, const double*, double*)>;
 template <typename T>
 SolveInfo bicgstab_split(hipStream_t s, BicgWork<T>& w, GmresWork<T>& gw, const CsrOp<double>& A64,
                          const CsrOp<T>& Ahat, const double* b, double* x, double* r, T* work1, T* work2, T* work3,
                          int maxit, double tol, double inner_rtol, double inner_rtol_cold = 0.0,
-                         const CoarseFn& coarse = CoarseFn());
+                         const CoarseFn& coarse = CoarseFn(), double restart_eps32 = kBicgRestartEps32);
 // FP64 refinement around FP32 BiCGStab runs (FP64 residuals, FP64 dot accumulation)
 SolveInfo bicgstab_mixed(hipStream_t s, BicgWork<float>& w, GmresWork<float>& gw, const CsrOp<double>& A64,
                          const CsrOp<float>& A32, const PcFn<float>& M32, const double* b, double* x, double* r,
-                         float* r32, float* d32, int maxit, double tol, double inner_rtol);
+                         float* r32, float* d32, int maxit, double tol, double inner_rtol,
+                         double restart_eps32 = kBicgRestartEps32);
 
 }  // namespace pb

@@ -404,11 +404,25 @@ __global__ void __launch_bounds__(kBlock) k_axpy_dots(int64_t n, double a, const
 template <typename T>
 using Vec4 = T __attribute__((ext_vector_type(4)));
 
-// p = r + beta (p - omega v), beta = (rho / rho_prev) (alpha / omega) of the previous step
+// p = r + beta (p - omega v), beta = (rho / rho_prev) (alpha / omega) of the previous step;
+// after a restart of the shadow residual, p = r and rh = r
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_bicg_p_dev(int64_t n, const T* __restrict__ r, const T* __restrict__ v,
-                                                       const double* __restrict__ sc, T* __restrict__ p) {
+                                                       const double* __restrict__ sc, T* __restrict__ p,
+                                                       T* __restrict__ rh) {
     if (sc[SC_STOP] != 0.0) return;  // uniform over the grid
+    if (sc[SC_RESTART] != 0.0) {     // uniform over the grid
+        const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+        const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        const int64_t n4 = n / 4;
+        for (int64_t i = t0; i < n4; i += stride) {
+            const Vec4<T> rv = reinterpret_cast<const Vec4<T>*>(r)[i];
+            reinterpret_cast<Vec4<T>*>(p)[i] = rv;
+            reinterpret_cast<Vec4<T>*>(rh)[i] = rv;
+        }
+        for (int64_t k = 4 * n4 + t0; k < n; k += stride) p[k] = rh[k] = r[k];
+        return;
+    }
     const double omega = sc[SC_OMEGA];
     const double beta = (sc[SC_RHO] / sc[SC_RHO_PREV]) * (sc[SC_ALPHA] / omega);
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -644,7 +658,8 @@ __global__ void __launch_bounds__(kBlock) k_bicg_xr_dev(int64_t n, const T* __re
 constexpr int kStageThreads = 1024;
 __global__ void __launch_bounds__(kStageThreads) k_bicg_stage(const double* __restrict__ partial, int64_t nblk,
                                                               int stride, int stage, double* __restrict__ sc,
-                                                              double stop_abs, double* __restrict__ host_slot) {
+                                                              double stop_abs, double* __restrict__ host_slot,
+                                                              double restart_eps) {
     __shared__ double s_red[kStageThreads / 64];
     if (stage != 0 && sc[SC_STOP] != 0.0) return;  // uniform over the block
     double a, b;  // rh.r, r.r
@@ -663,6 +678,9 @@ __global__ void __launch_bounds__(kStageThreads) k_bicg_stage(const double* __re
         sc[SC_OMEGA] = 1;
         sc[SC_FLAG] = 0;
         sc[SC_STOP] = 0;
+        sc[SC_RHNN] = a;  // rh = r
+        sc[SC_RESTART] = 0;
+        sc[SC_NRESTART] = 0;
     } else {  // the new residual of an iteration, and the stop test
         sc[SC_RHO_PREV] = sc[SC_RHO];
         sc[SC_RHO] = a;
@@ -670,6 +688,15 @@ __global__ void __launch_bounds__(kStageThreads) k_bicg_stage(const double* __re
         if (!isfinite(b)) sc[SC_FLAG] = 3;
         const double rr = sqrt(b);
         if (sc[SC_FLAG] != 0.0 || rr <= stop_abs || !isfinite(rr)) sc[SC_STOP] = 1;
+        // near breakdown: rh is nearly orthogonal to r, so rho carries few significant digits
+        // and alpha = rho / rh.v swings; take rh = r (rho = r.r) and restart the directions
+        const bool restart = sc[SC_STOP] == 0.0 && restart_eps > 0.0 && fabs(a) < restart_eps * sqrt(sc[SC_RHNN] * b);
+        sc[SC_RESTART] = restart ? 1.0 : 0.0;
+        if (restart) {
+            sc[SC_RHO] = b;
+            sc[SC_RHNN] = b;
+            sc[SC_NRESTART] += 1.0;
+        }
     }
     if (host_slot)  // the scalars straight into the host-visible slot the host waits on
         for (int k = 0; k < SC_COUNT; ++k) host_slot[k] = sc[k];
@@ -810,10 +837,10 @@ void check_vec4_aligned(std::initializer_list<const T*> ptrs) {
             throw Error(PBSM3D_EINVAL, "BiCGStab vectors must be aligned to four elements");
 }
 template <typename T>
-void bicg_p_dev(hipStream_t s, int64_t n, const T* r, const T* v, const double* sc, T* p) {
-    check_vec4_aligned<T>({r, v, p});
+void bicg_p_dev(hipStream_t s, int64_t n, const T* r, const T* v, const double* sc, T* p, T* rh) {
+    check_vec4_aligned<T>({r, v, p, rh});
     hipLaunchKernelGGL((k_bicg_p_dev<T>), dim3(grid_for((n + 3) / 4, kBlock, 8192)), dim3(kBlock), 0, s, n, r, v, sc,
-                       p);
+                       p, rh);
 }
 template <typename T>
 void bicg_s_dev(hipStream_t s, int64_t n, const T* v, const T* r, double* sc, T* s_out, double* partial, int nblk,
@@ -834,11 +861,11 @@ void bicg_xr_dev(hipStream_t s, int64_t n, const T* ph, const T* sh, const T* sv
                        dots, dots_nb, ss_part, ss_nb);
 }
 void bicg_stage(hipStream_t s, const double* partial, int64_t nblk, int stride, int stage, double* sc,
-                double stop_abs, double* host_slot) {
+                double stop_abs, double* host_slot, double restart_eps) {
     if (stride == 2 && reinterpret_cast<uintptr_t>(partial) % 16 != 0)
         throw Error(PBSM3D_EINVAL, "dot pairs must be 16-byte aligned");
     hipLaunchKernelGGL(k_bicg_stage, dim3(1), dim3(kStageThreads), 0, s, partial, nblk, stride, stage, sc, stop_abs,
-                       host_slot);
+                       host_slot, restart_eps);
 }
 
 template <typename T>
@@ -879,7 +906,7 @@ void copy_f64(hipStream_t s, int64_t n, const double* a, double* b) {
     template void spmv_col<T>(hipStream_t, int64_t, int, const int32_t*, const T*, const int32_t*, const uint16_t*, \
                               const T*, const T*, T*, double*);                                                    \
     template void axpy_dots<T>(hipStream_t, int64_t, double, const T*, const T*, T*, const T*, double*, int);      \
-    template void bicg_p_dev<T>(hipStream_t, int64_t, const T*, const T*, const double*, T*);                     \
+    template void bicg_p_dev<T>(hipStream_t, int64_t, const T*, const T*, const double*, T*, T*);                     \
     template void bicg_s_dev<T>(hipStream_t, int64_t, const T*, const T*, double*, T*, double*, int,             \
                                 const double*, int64_t);                                                          \
     template void bicg_xr_dev<T>(hipStream_t, int64_t, const T*, const T*, const T*, const T*, const T*, double*, \

@@ -36,6 +36,7 @@ EXPORTED = (
     "pbsm3d_face_colors",
     "pbsm3d_precond_tuning",
     "pbsm3d_precond_select",
+    "pbsm3d_bicg_restart",
     "pbsm3d_fetch_config_default",
     "pbsm3d_fetch_create",
     "pbsm3d_fetch_run",
@@ -143,6 +144,7 @@ def lib():
         L.pbsm3d_face_colors.argtypes = [C.c_int64, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
         L.pbsm3d_precond_tuning.argtypes = [vp, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_float)]
         L.pbsm3d_precond_select.argtypes = [vp, C.c_int, C.c_int]
+        L.pbsm3d_bicg_restart.argtypes = [vp, C.c_double]
         L.pbsm3d_batch_create.argtypes = [C.POINTER(vp), C.c_int32, C.POINTER(abi.Mesh), C.POINTER(abi.Config),
                                           C.c_int]
         L.pbsm3d_batch_step.argtypes = [vp, C.POINTER(abi.Forcing), C.c_int32, C.POINTER(abi.Out),
@@ -345,6 +347,11 @@ class Context:
         """Fix the D-ILU sweep mode of the FP64 (0) / FP32 (1) solve (0 .. 1, or 0 .. 7 with
         the split operator); -1 re-tunes."""
         _check(lib().pbsm3d_precond_select(self._h, which, mode), self._h)
+
+    def bicg_restart(self, eps: float):
+        """Shadow-residual restart threshold of the FP32 BiCGStab runs (default 1e-5, 0: off);
+        stats["susp_restarts"] counts the restarts of a step."""
+        _check(lib().pbsm3d_bicg_restart(self._h, float(eps)), self._h)
 
     def spmv_select(self, which: int, variant: int):
         """Fix the SpMV launch shape of the FP64 (0) / FP32 (1) operator; -1 re-tunes."""

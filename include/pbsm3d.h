@@ -188,7 +188,7 @@ typedef struct pbsm3d_stats {
     double susp_rhs_max;          /* ||b||_inf (getRhsMax)                          */
     double dep_rhs_max;
     int32_t susp_refinements;     /* outer FP64 refinement steps (MIXED)            */
-    int32_t reserved;
+    int32_t susp_restarts;        /* shadow-residual restarts of the FP32 BiCGStab  */
     double ms_assembly;           /* HIP-event times of the last step, ms           */
     double ms_susp_solve;
     double ms_post;
@@ -293,6 +293,13 @@ int pbsm3d_spmv_select(pbsm3d_ctx* ctx, int which, int variant);
 #define PBSM3D_PRECOND_MODES 8
 int pbsm3d_precond_tuning(const pbsm3d_ctx* ctx, int which, int32_t* mode, float* ms);
 int pbsm3d_precond_select(pbsm3d_ctx* ctx, int which, int mode);
+
+/* Restart threshold of the shadow residual in the FP32 BiCGStab runs (MIXED with
+ * PBSM3D_KRYLOV_BICGSTAB or _BICGSTAB_SPLIT; no counterpart in the reference, whose
+ * GMRES has no shadow vector): when |rh.r| < eps ||rh|| ||r||, a near breakdown in
+ * which rho keeps few significant digits, the run continues with rh = p = r.  Default
+ * 1e-5; 0 disables it; eps must lie in [0, 1).  stats.susp_restarts counts them. */
+int pbsm3d_bicg_restart(pbsm3d_ctx* ctx, double eps);
 
 /* Face colours of the multicolour D-ILU preconditioner (host computation, no
  * GPU needed; no counterpart in the reference): neighbouring faces never share

@@ -94,6 +94,7 @@ def test_context_calls_reject_null_context():
     assert L.pbsm3d_profile_classes(None, 1) == abi.PBSM3D_EINVAL
     assert L.pbsm3d_precond_select(None, 0, 1) == abi.PBSM3D_EINVAL
     assert L.pbsm3d_spmv_select(None, 0, 1) == abi.PBSM3D_EINVAL
+    assert L.pbsm3d_bicg_restart(None, 1e-5) == abi.PBSM3D_EINVAL
     assert L.pbsm3d_batch_training_pairs(None, 0, None, None) == abi.PBSM3D_EINVAL
 
 

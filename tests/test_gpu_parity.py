@@ -361,6 +361,35 @@ def test_every_spmv_launch_shape(pb, oracle, mesh_c1, precision):
         ctx.precond_select(which, 2)
 
 
+@pytest.mark.parametrize("krylov", [abi.KRYLOV_BICGSTAB_SPLIT, abi.KRYLOV_BICGSTAB])
+def test_shadow_residual_restart(pb, oracle, mesh_c1, krylov):
+    """The FP32 BiCGStab's shadow-residual restart (rh = p = r when |rh.r| < eps ||rh|| ||r||):
+    with eps = 0 no run restarts; with eps = 0.05 the runs restart whenever r has turned
+    nearly orthogonal to rh -- more often than not after a few iterations -- and the
+    refinement still reaches 1e-12 with fields that match the oracle; the default (1e-5)
+    matches it too, and thresholds outside [0, 1) are refused."""
+    _print_case("shadow_residual_restart", krylov=krylov)
+    F = mesh_c1.n_faces
+    fo = meshgen.forcing_c1(F, seed=0)
+    g, ctx, orc = _pair(pb, oracle, mesh_c1, L=10, precision=abi.MIXED, tol=1e-12, krylov=krylov, max_iters=4000)
+    og, _ = orc.step(fo)
+    res = {}
+    for eps in (0.0, 0.05, 1e-5):
+        ctx.bicg_restart(eps)
+        ctx.set_state(np.zeros(F), np.zeros(F))
+        gg, st = ctx.step(fo, want_fields=True)
+        assert st["susp_resid"] <= 1e-12, (eps, st)
+        for k in ("conc", "Qsusp", "qdep", "drift_mass"):
+            assert rel_err(gg[k], og[k]) < 1e-8, (eps, k, rel_err(gg[k], og[k]))
+        res[eps] = (st["susp_iters"], st["susp_restarts"])
+    print("[restart] (iterations, restarts) per threshold", res, flush=True)
+    assert res[0.0][1] == 0
+    assert res[0.05][1] > 0, res
+    for bad in (-1e-3, 1.0, float("nan")):
+        with pytest.raises(pb.Pbsm3dError):
+            ctx.bicg_restart(bad)
+
+
 @pytest.mark.parametrize("precision", [abi.FP64, abi.MIXED])
 def test_every_split_sweep_mode(pb, oracle, mesh_c1, precision):
     """The split operator's eight sweep modes (XCD workgroup mapping, non-temporal factor
