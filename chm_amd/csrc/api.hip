@@ -273,9 +273,11 @@ float ms_between(hipEvent_t a, hipEvent_t b) {
 
 // Default inner target of the FP32 split BiCGStab, selected on 36 forcing draws of the
 // 999,968-face mesh at 10 layers (seeds 200-235, disjoint from the timed, training and
-// comparison draws), total suspension-solve time: from x0 = 0, 1e-3 1506 ms, 1.5e-3 1499,
-// 2e-3 1473, 3e-3 1462, 5e-3 1467 (flat from 2e-3 up); from the learned start, which begins
-// near 0.2 ||b||, 1e-4 1435, 1.5e-4 1430, 2e-4 1426, 3e-4 1444, 5e-4 1517, 1e-3 1498.
+// comparison draws), total suspension-solve time: from x0 = 0, 1e-3 1382 ms, 1.5e-3 1468,
+// 2e-3 1365, 3e-3 1351, 5e-3 1383 (flat within the draw-to-draw noise from 2e-3 up); from the
+// learned start, which begins near 0.2 ||b||, 1e-4 1373, 1.5e-4 1371, 2e-4 1327, 3e-4 1435,
+// 5e-4 1375, 1e-3 1369.  (An earlier build, before the tiled rows and the shadow-residual
+// restart: 1506, 1499, 1473, 1462, 1467 and 1435, 1430, 1426, 1444, 1517, 1498.)
 constexpr double kSplitInnerCold = 2e-3, kSplitInnerLearned = 2e-4;
 static double split_inner_rtol_default(const pbsm3d_ctx* c, const pbsm3d_config& cfg) {
     return cfg.warm_start == PBSM3D_WS_LEARNED && c->warm != nullptr ? kSplitInnerLearned : kSplitInnerCold;
