@@ -56,6 +56,34 @@ def test_batch_matches_members_one_at_a_time(precision, oracle):
     assert (b.get_state(2)[0] == 0).all()
 
 
+@pytest.mark.parametrize("precision", [abi.FP64, abi.MIXED])
+def test_batch_tiny_members_straddle_tiles(precision):
+    """Members of 26, 63, 65 and 768 faces at one layer count form one union whose member
+    runs start and end inside the 64-position tiles of the position order, so the
+    element chunks of the per-member vector kernels are cut at member boundaries inside
+    tiles: two steps of the batch equal the members' own contexts (1e-12 solves)."""
+    from chm_amd import pbsm3d as pb
+
+    print(f"[case] batch tiny members precision={precision}", flush=True)
+    members = [ensemble.Member(seed=2000 + k, n_points=n, n_layer=10, z0=0.01, settling_scale=1.0, wind_scale=1.0)
+               for k, n in enumerate((18, 38, 39, 392))]
+    meshes = [meshgen.synthetic_mesh(n, seed=0) for n in (18, 38, 39, 392)]
+    assert [m.n_faces for m in meshes] == [26, 63, 65, 768]
+    b = ensemble.build_batch(members, meshes, precision=precision, tol=1e-12)
+    ctxs = [pb.Context(pb.face_geometry(mesh), ensemble.member_config(m, precision, tol=1e-12))
+            for m, mesh in zip(members, meshes)]
+    assert b.info()["unions"] == 1
+    for step in range(2):
+        fos = [meshgen.forcing_c1(mesh.n_faces, seed=10 + step) for mesh in meshes]
+        got, st = b.step(fos, want_fields=True)
+        for k, (ctx, fo) in enumerate(zip(ctxs, fos)):
+            want, wst = ctx.step(fo, want_fields=True)
+            assert st[k]["suspension_present"] == wst["suspension_present"] == 1
+            assert st[k]["susp_resid"] <= 1e-12 and st[k]["dep_resid"] <= 1e-12, (k, st[k])
+            for f in ("conc", "Qsusp", "qdep", "drift_mass"):
+                assert rel_err(got[k][f], want[f]) < 1e-8, (step, k, f, rel_err(got[k][f], want[f]))
+
+
 def test_batch_member_without_suspension():
     """A calm member (no suspension, hence no deposition) beside windy ones keeps the
     -9999 defaults and x = 0, as a lone module would; the windy members still solve."""

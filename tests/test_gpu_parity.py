@@ -361,6 +361,32 @@ def test_every_spmv_launch_shape(pb, oracle, mesh_c1, precision):
         ctx.precond_select(which, 2)
 
 
+@pytest.mark.parametrize("L", [7, 10])
+@pytest.mark.parametrize("n_points", [18, 38, 39, 392])
+def test_tile_boundaries(pb, oracle, n_points, L):
+    """Meshes whose face counts sit on either side of the 64-position tiles of the position
+    order -- 26 faces (one partial tile), 63, 65 (a full tile and a tile of one) and 768
+    (twelve full tiles) -- at a layer count the column sweeps unroll exactly (10) and one
+    they guard (7): the mixed and FP64 split BiCGStab and FP64 GMRES with the position-order
+    SpMV and D-ILU reach 1e-12 and match the oracle."""
+    mesh = meshgen.synthetic_mesh(n_points, seed=0)
+    F = mesh.n_faces
+    _print_case("tile_boundaries", F=F, L=L)
+    fo = meshgen.forcing_c1(F, seed=1)
+    og = None
+    for precision, krylov in ((abi.MIXED, abi.KRYLOV_BICGSTAB_SPLIT), (abi.FP64, abi.KRYLOV_BICGSTAB_SPLIT),
+                              (abi.FP64, abi.KRYLOV_GMRES)):
+        g, ctx, orc = _pair(pb, oracle, mesh, L=L, precision=precision, tol=1e-12, krylov=krylov)
+        if og is None:
+            og, _ = orc.step(fo)
+        gg, st = ctx.step(fo, want_fields=True)
+        assert st["suspension_present"] == 1 and st["deposition_present"] == 1, st
+        assert st["susp_resid"] <= 1e-12 and st["dep_resid"] <= 1e-12, (precision, krylov, st)
+        for k in ("conc", "Qsusp", "qdep", "drift_mass"):
+            assert rel_err(gg[k], og[k]) < 1e-8, (precision, krylov, k, rel_err(gg[k], og[k]))
+        ctx.close()
+
+
 @pytest.mark.parametrize("krylov", [abi.KRYLOV_BICGSTAB_SPLIT, abi.KRYLOV_BICGSTAB])
 def test_shadow_residual_restart(pb, oracle, mesh_c1, krylov):
     """The FP32 BiCGStab's shadow-residual restart (rh = p = r when |rh.r| < eps ||rh|| ||r||):
