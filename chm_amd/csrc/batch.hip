@@ -372,8 +372,10 @@ void build_segments(Segments& sg, const ColorOrder& co, const std::vector<int32_
         for (int m = 0; m < M; ++m) {
             const int64_t k0 = kb[(size_t)c * M + m], n = cnt[(size_t)c * M + m];
             const int64_t b0 = (int64_t)tab.size();
-            for (int64_t k = 0; k < n; k += kBlock)
-                tab.push_back(make_int4((int)(k0 + k), (int)(k0 + std::min<int64_t>(k + kBlock, n)), m, 0));
+            // workgroups start on tile boundaries (a), so each wave stays inside one tile
+            if (n > 0)
+                for (int64_t a = tile_floor(k0); a < k0 + n; a += kBlock)
+                    tab.push_back(make_int4((int)std::max(a, k0), (int)std::min(a + kBlock, k0 + n), m, (int)a));
             rng[((size_t)m * kMaxColors + c) * 2] = off + b0;
             rng[((size_t)m * kMaxColors + c) * 2 + 1] = off + (int64_t)tab.size();
         }

@@ -271,8 +271,9 @@ int64_t split_apply_dots(hipStream_t s, const ColorOrder& co, int64_t F, int L, 
                          const T* dw, double* partial, int mode, const double* stop = nullptr);
 // Workgroup tables of a batch of independent systems held as one union (batch.hip):
 // the union's faces are member-major, so each colour's positions are too, and the
-// sweeps of colour c run as nblk[c] workgroups, tab[c][b] = {k0, k1, member, 0} (faces
-// [k0, k1) of one member, at most kBlock).  The forward launches of one application
+// sweeps of colour c run as nblk[c] workgroups, tab[c][b] = {k0, k1, member, a} (faces
+// [k0, k1) of one member, inside [a, a + kBlock) with a a multiple of kTile: thread t takes
+// position a + t).  The forward launches of one application
 // (colour 0 first, then 1 .. top) write their DOTS partial pairs at pair offset
 // part_off[c] + b.
 constexpr int kMaxColors = 8;

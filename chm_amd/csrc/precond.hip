@@ -515,7 +515,9 @@ __global__ void __launch_bounds__(kBlock) k_eis_fwd(int64_t F, int L, int64_t k0
 // faces consecutive, so each colour's positions are member-major): workgroup b takes
 // faces [tab[b].x, tab[b].y) of one colour of member tab[b].z and returns at once when
 // that member's run has stopped (stop[member * stop_stride] != 0).  No workgroup spans
-// two members, so the DOTS partials of a workgroup belong to one member.
+// two members, so the DOTS partials of a workgroup belong to one member.  Thread t of the
+// workgroup takes position tab[b].w + t (tab[b].w a tile boundary, so each wave stays in
+// one tile) when it lies in [tab[b].x, tab[b].y).
 template <typename T, int LM, bool EX, bool NT, bool LAT = true>
 __global__ void __launch_bounds__(kBlock) k_eis_bwd_tab(int64_t F, int Lrt, const int4* __restrict__ tab,
                                                         const int32_t* __restrict__ nbp,
@@ -526,8 +528,8 @@ __global__ void __launch_bounds__(kBlock) k_eis_bwd_tab(int64_t F, int Lrt, cons
                                                         int stop_stride) {
     const int4 e = tab[blockIdx.x];
     if (stop && stop[(int64_t)e.z * stop_stride] != 0.0) return;  // uniform over the workgroup
-    const int64_t k = (int64_t)e.x + threadIdx.x;
-    if (k >= e.y) return;
+    const int64_t k = (int64_t)e.w + threadIdx.x;
+    if (k < e.x || k >= e.y) return;
     eis_bwd_column<T, LM, EX, NT, LAT>(F, Lrt, k, nbp, lowmask, lat, vert, dinv, w, tnb, t);
 }
 
@@ -544,9 +546,9 @@ __global__ void __launch_bounds__(kBlock) k_eis_fwd_tab(int64_t F, int L, const 
                                                         int stop_stride) {
     const int4 e = tab[blockIdx.x];
     if (stop && stop[(int64_t)e.z * stop_stride] != 0.0) return;  // uniform over the workgroup
-    const int64_t k = (int64_t)e.x + threadIdx.x;
+    const int64_t k = (int64_t)e.w + threadIdx.x;
     double d1 = 0, d2 = 0;
-    if (k < e.y) eis_fwd_column<T, LM, EX, WITH_T, FUSE, DOTS, NT, true, SV>(F, L, k, nbp, lowmask, lat, vert, dinv, E, w,
+    if (k >= e.x && k < e.y) eis_fwd_column<T, LM, EX, WITH_T, FUSE, DOTS, NT, true, SV>(F, L, k, nbp, lowmask, lat, vert, dinv, E, w,
                                                                             t, tnb, vnb, v, o, dw, d1, d2);
     if constexpr (DOTS) {
         __shared__ double s_red[kBlock / 64];
