@@ -52,9 +52,13 @@ def split_apply_bytes(nnz: int, n: int, f: int, f0: int, L: int, s: int, f_top: 
     does both sweeps in one pass with t in registers (6 vectors); the top colour (f_top
     faces) does not write v, which no higher colour reads (one vector less); the
     neighbour positions and colour mask once per sweep (13 bytes per face and sweep),
-    except in the top colour's backward sweep, which has no lateral terms."""
+    except in the top colour's backward sweep, which has no lateral terms.  The dot vector is
+    a separate array only in the first of BiCGStab's two applications (r^ . v); in the second
+    (s . t) it is the input s itself, read once: the class average counts it for half of the
+    rows."""
     n0 = f0 * L
-    return (nnz - n) * s + (10 * (n - n0) + 6 * n0 - f_top * L) * s + 26 * f - 13 * f0 - 13 * f_top
+    return ((nnz - n) * s + (10 * (n - n0) + 6 * n0 - f_top * L) * s + 26 * f - 13 * f0 - 13 * f_top
+            - (n * s) // 2)
 
 
 def spmv_dots_bytes(nnz: int, n: int, f: int, s: int) -> int:
