@@ -166,7 +166,11 @@ def stream_probe(torch, mb: int = 1000, reps: int = 5) -> dict:
     return out
 
 
-PROFILE_EVERY = 8  # timed steps: one launch in eight of a reported class is bracketed
+# timed steps: one launch in seven of a reported class is bracketed.  The period is odd on
+# purpose: BiCGStab applies the operator twice per iteration (the second time with its input
+# as the dot vector, one array less to read), so an even period would sample one of the two
+# applications only; an odd one alternates between them.
+PROFILE_EVERY = 7
 
 
 def run_variant(pb, g, dev_forcing, dev_outs, cfg, net, steps, warmup, torch, spmv_shape=-1, dilu_mode=-1):
