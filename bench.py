@@ -61,6 +61,14 @@ def split_apply_bytes(nnz: int, n: int, f: int, f0: int, L: int, s: int, f_top: 
             - (n * s) // 2)
 
 
+def bicg_vectors_per_iteration(preconditioned: bool) -> int:
+    """n-vectors of HBM traffic of one BiCGStab iteration's vector kernels: p (r, v, p in;
+    p out), s (r, v in; s out), x / r (p^, s^, s, t, r^, x in; x, r out).  With the split
+    operator there is no separate preconditioner, p^ is p and s^ is s -- one array each, read
+    once -- so the x / r kernel moves 7 vectors instead of 8."""
+    return 4 + 3 + (8 if preconditioned else 7)
+
+
 def spmv_dots_bytes(nnz: int, n: int, f: int, s: int) -> int:
     """The SpMV of the BiCGStab iteration, which also reads the vector w of its fused
     dot products (w.y, y.y)."""
@@ -596,13 +604,12 @@ def main():
         roofline["measured_stream_gbs"] = stream
         roofline["frac_of_measured_stream"] = round(roofline["achieved"] / max(stream.values()), 4)
     # BiCGStab vector work (p; alpha + s; omega + x, r; the rho / stop stage), from the untimed
-    # steps with every class bracketed: 14 n-vectors of traffic per iteration (p: r, v, p in, p
-    # out; s: r, v in, s out; x / r: p, s, t, r^, x in, x, r out -- s is passed as both the
-    # preconditioned and the plain s, the same array, read once from HBM), suspension
-    # (n = N, the inner precision) and deposition (n = F, FP64)
+    # steps with every class bracketed: bicg_vectors_per_iteration n-vectors of traffic per
+    # iteration, suspension (n = N, the inner precision) and deposition (n = F, FP64)
     bd = head.get(f"kernel_breakdown_{BREAKDOWN_STEPS}_steps", {}).get("ortho")
     if head["krylov"] == "bicgstab_split" and bd and bd["ms_total"] > 0:
-        algo = BREAKDOWN_STEPS * 14 * (head["susp_iters"] * N * s_bytes + head["dep_iters"] * F * 8)
+        nvs = bicg_vectors_per_iteration(False)
+        algo = BREAKDOWN_STEPS * nvs * (head["susp_iters"] * N * s_bytes + head["dep_iters"] * F * 8)
         achieved = algo / (bd["ms_total"] * 1e-3) / 1e9
         rooflines["ortho"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                               "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
@@ -611,8 +618,8 @@ def main():
                               "algorithmic_bytes": algo, "ms_total": bd["ms_total"], "brackets": bd["launches"],
                               "source": f"kernel_breakdown_{BREAKDOWN_STEPS}_steps (HIP events, every class bracketed)"}
         tv = traffic.get(f"bicg_vectors_fp{8 * s_bytes}", {}).get("bytes_per_iteration")
-        if tv:  # measured HBM bytes of one suspension iteration's vector kernels, against 14 N s
-            rooflines["ortho"]["traffic_per_suspension_iteration"] = {"measured": tv, "algorithmic": 14 * N * s_bytes,
+        if tv:  # measured HBM bytes of one suspension iteration's vector kernels, against nvs N s
+            rooflines["ortho"]["traffic_per_suspension_iteration"] = {"measured": tv, "algorithmic": nvs * N * s_bytes,
                                                                       "source": traffic["source"]}
 
     cpu = cpu1 = None
